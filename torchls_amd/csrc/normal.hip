@@ -1,0 +1,501 @@
+// Deterministic fused normal matvec y = J^T (J x) + lambda x for gfx950.
+//
+// Replaces warp_normal.py (_factor_kernel 69-103, _block_factor_kernel 26-66,
+// _finish 106-122).  Instead of one float atomic per observation column, every
+// workgroup reduces a tile of observations in a fixed order:
+//
+//   1. each thread takes observations (tid + k*256) of the tile, gathers x for
+//      every slot, forms q = sum_s J_s x_s and the slot contributions J_s^T q
+//      (registers; Jacobian components are read coalesced from the packed
+//      component-major layout);
+//   2. per slot, contributions are staged in LDS in the tile's order by that
+//      slot's variable index (precomputed rank), and each precomputed segment
+//      (a run of one variable) is summed serially by one thread;
+//   3. segment sums go either to a partial slot in HBM (SPARSE) or into a
+//      per-workgroup dense LDS accumulator of the whole variable type (DENSE,
+//      used when the type is small, e.g. cameras), which the persistent
+//      workgroup writes out once;
+//   4. a finish kernel adds each variable's partials in plan order plus
+//      lambda * x.
+//
+// Every float operation therefore happens in an order fixed by the plan, so
+// results are bitwise reproducible run to run and inside hipGraph replays.
+#include <algorithm>
+
+#include "common.hpp"
+
+using namespace tlh;
+
+namespace {
+
+template <int D0, int D1, int D2>
+struct Dims {
+  static constexpr int ns = 1 + (D1 > 0) + (D2 > 0);
+  static constexpr int sum = D0 + D1 + D2;
+  static constexpr int max = D0 > D1 ? (D0 > D2 ? D0 : D2) : (D1 > D2 ? D1 : D2);
+  static constexpr int d(int s) { return s == 0 ? D0 : (s == 1 ? D1 : D2); }
+  static constexpr int off(int s) { return s == 0 ? 0 : (s == 1 ? D0 : D0 + D1); }
+};
+
+// Sum the segments of one slot of tile `tile` from the LDS stage (entries of
+// `dim` values in the slot's tile order) into partials or the dense accumulator.
+template <typename Real, int D>
+__device__ __forceinline__ void reduce_segments(const SlotArgs<Real>& s, int tile, int len,
+                                                const Real* stage, Real* acc) {
+  const int sb = s.seg_ptr[tile];
+  const int se = s.seg_ptr[tile + 1];
+  for (int k = sb + (int)threadIdx.x; k < se; k += TLH_THREADS) {
+    const int a = s.seg_start[k];
+    const int b = (k + 1 < se) ? s.seg_start[k + 1] : len;
+    Real sum[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) sum[c] = stage[a * D + c];
+    for (int i = a + 1; i < b; ++i) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) sum[c] += stage[i * D + c];
+    }
+    const int target = s.seg_target[k];
+    if (s.mode == TLH_REDUCE_DENSE) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) acc[target * D + c] += sum[c];
+    } else {
+      Real* out = s.partials + (int64_t)target * D;
+#pragma unroll
+      for (int c = 0; c < D; ++c) out[c] = sum[c];
+    }
+  }
+}
+
+// Runtime-dimension variant used by the generic kernel.
+template <typename Real>
+__device__ __forceinline__ void reduce_segments_rt(const SlotArgs<Real>& s, int D, int tile,
+                                                   int len, const Real* stage, Real* acc) {
+  const int sb = s.seg_ptr[tile];
+  const int se = s.seg_ptr[tile + 1];
+  for (int k = sb + (int)threadIdx.x; k < se; k += TLH_THREADS) {
+    const int a = s.seg_start[k];
+    const int b = (k + 1 < se) ? s.seg_start[k + 1] : len;
+    const int target = s.seg_target[k];
+    for (int c = 0; c < D; ++c) {
+      Real sum = stage[a * D + c];
+      for (int i = a + 1; i < b; ++i) sum += stage[i * D + c];
+      if (s.mode == TLH_REDUCE_DENSE)
+        acc[target * D + c] += sum;
+      else
+        s.partials[(int64_t)target * D + c] = sum;
+    }
+  }
+}
+
+template <typename Real>
+__device__ __forceinline__ Real* dense_regions(const GroupArgs<Real>& g, Real* base, Real* acc[3]) {
+  Real* p = base;
+  for (int s = 0; s < TLH_MAX_SLOTS; ++s) {
+    if (s < g.nslots && g.slot[s].mode == TLH_REDUCE_DENSE) {
+      acc[s] = p;
+      p += (int64_t)g.slot[s].type_count * g.slot[s].dim;
+    } else {
+      acc[s] = nullptr;
+    }
+  }
+  return p;
+}
+
+template <typename Real>
+__device__ __forceinline__ void zero_dense(const GroupArgs<Real>& g, Real* acc[3]) {
+  for (int s = 0; s < g.nslots; ++s) {
+    if (acc[s] == nullptr) continue;
+    const int n = g.slot[s].type_count * g.slot[s].dim;
+    for (int i = threadIdx.x; i < n; i += TLH_THREADS) acc[s][i] = Real(0);
+  }
+}
+
+template <typename Real>
+__device__ __forceinline__ void flush_dense(const GroupArgs<Real>& g, Real* acc[3]) {
+  for (int s = 0; s < g.nslots; ++s) {
+    if (acc[s] == nullptr) continue;
+    const int64_t n = (int64_t)g.slot[s].type_count * g.slot[s].dim;
+    Real* out = g.slot[s].partials + (int64_t)(g.slot[s].dense_block + blockIdx.x) * n;
+    for (int64_t i = threadIdx.x; i < n; i += TLH_THREADS) out[i] = acc[s][i];
+  }
+}
+
+// Specialised kernel: residual rows R and slot dims D0..D2 known at compile time.
+template <typename Real, int R, int D0, int D1, int D2, int ITEMS>
+__global__ void __launch_bounds__(TLH_THREADS) normal_tile_kernel(GroupArgs<Real> g) {
+  using S = Dims<D0, D1, D2>;
+  constexpr int NS = S::ns;
+  constexpr int DSUM = S::sum;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Real* stage = reinterpret_cast<Real*>(smem_raw);
+  Real* acc[3];
+  dense_regions(g, stage + TLH_THREADS * ITEMS * S::max, acc);
+  zero_dense(g, acc);
+
+  const int t0 = blockIdx.x * g.tiles_per_wg;
+  const int t1 = min(t0 + g.tiles_per_wg, g.ntiles);
+  for (int t = t0; t < t1; ++t) {
+    const int start = g.tile_start[t];
+    const int len = g.tile_start[t + 1] - start;
+    Real cont[ITEMS][DSUM];
+    int rk[ITEMS][NS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+      const int pos = (int)threadIdx.x + it * TLH_THREADS;
+      const int64_t n = start + min(pos, len - 1);
+      Real xv[DSUM];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const int d = S::d(s);
+        const int64_t v = g.slot[s].index[n];
+        const Real* xs = g.slot[s].x + v * d;
+#pragma unroll
+        for (int c = 0; c < d; ++c) xv[S::off(s) + c] = xs[c];
+        rk[it][s] = g.slot[s].rank ? (int)g.slot[s].rank[n] : pos;
+      }
+      Real jv[R][DSUM];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+#pragma unroll
+          for (int c = 0; c < S::d(s); ++c) {
+            const int comp = R * S::off(s) + r * S::d(s) + c;
+            jv[r][S::off(s) + c] = g.packed[(int64_t)comp * g.stride + n];
+          }
+        }
+      }
+      Real q[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        Real a = Real(0);
+#pragma unroll
+        for (int c = 0; c < DSUM; ++c) a += jv[r][c] * xv[c];
+        q[r] = a;
+      }
+#pragma unroll
+      for (int c = 0; c < DSUM; ++c) {
+        Real a = Real(0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) a += jv[r][c] * q[r];
+        cont[it][c] = a;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int d = S::d(s);
+      __syncthreads();
+#pragma unroll
+      for (int it = 0; it < ITEMS; ++it) {
+        const int pos = (int)threadIdx.x + it * TLH_THREADS;
+        if (pos < len) {
+#pragma unroll
+          for (int c = 0; c < d; ++c) stage[rk[it][s] * d + c] = cont[it][S::off(s) + c];
+        }
+      }
+      __syncthreads();
+      if (s == 0) reduce_segments<Real, D0>(g.slot[0], t, len, stage, acc[0]);
+      if constexpr (D1 > 0) {
+        if (s == 1) reduce_segments<Real, (D1 > 0 ? D1 : 1)>(g.slot[1], t, len, stage, acc[1]);
+      }
+      if constexpr (D2 > 0) {
+        if (s == 2) reduce_segments<Real, (D2 > 0 ? D2 : 1)>(g.slot[2], t, len, stage, acc[2]);
+      }
+    }
+  }
+  __syncthreads();
+  flush_dense(g, acc);
+}
+
+// Generic kernel: any rows (<= 16) and slot dims (<= 16); re-reads J per slot.
+constexpr int GEN_RMAX = 16;
+constexpr int GEN_DSUM = 48;
+
+template <typename Real>
+__global__ void __launch_bounds__(TLH_THREADS)
+    normal_tile_generic(GroupArgs<Real> g, int dmax, int cap) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Real* stage = reinterpret_cast<Real*>(smem_raw);
+  Real* acc[3];
+  // The stage holds one slot of a full tile (cap entries of dmax values).
+  dense_regions(g, stage + (int64_t)cap * dmax, acc);
+  zero_dense(g, acc);
+  const int R = g.rows;
+  int off[TLH_MAX_SLOTS + 1];
+  off[0] = 0;
+  for (int s = 0; s < TLH_MAX_SLOTS; ++s) off[s + 1] = off[s] + (s < g.nslots ? g.slot[s].dim : 0);
+
+  const int t0 = blockIdx.x * g.tiles_per_wg;
+  const int t1 = min(t0 + g.tiles_per_wg, g.ntiles);
+  for (int t = t0; t < t1; ++t) {
+    const int start = g.tile_start[t];
+    const int len = g.tile_start[t + 1] - start;
+    for (int s = 0; s < g.nslots; ++s) {
+      const int d = g.slot[s].dim;
+      __syncthreads();
+      for (int pos = threadIdx.x; pos < len; pos += TLH_THREADS) {
+        const int64_t n = start + pos;
+        Real q[GEN_RMAX];
+        for (int r = 0; r < R; ++r) q[r] = Real(0);
+        for (int u = 0; u < g.nslots; ++u) {
+          const int du = g.slot[u].dim;
+          const Real* xs = g.slot[u].x + (int64_t)g.slot[u].index[n] * du;
+          for (int r = 0; r < R; ++r)
+            for (int c = 0; c < du; ++c)
+              q[r] += g.packed[(int64_t)(R * off[u] + r * du + c) * g.stride + n] * xs[c];
+        }
+        const int rk = g.slot[s].rank ? (int)g.slot[s].rank[n] : pos;
+        for (int c = 0; c < d; ++c) {
+          Real a = Real(0);
+          for (int r = 0; r < R; ++r) a += g.packed[(int64_t)(R * off[s] + r * d + c) * g.stride + n] * q[r];
+          stage[rk * d + c] = a;
+        }
+      }
+      __syncthreads();
+      reduce_segments_rt(g.slot[s], d, t, len, stage, acc[s]);
+    }
+  }
+  __syncthreads();
+  flush_dense(g, acc);
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(TLH_THREADS)
+    finish_kernel(Real* __restrict__ y, const Real* __restrict__ x, Real damping,
+                  const Real* __restrict__ partials, const int32_t* __restrict__ csr_ptr,
+                  const int32_t* __restrict__ csr_idx, int64_t count, int dim, int dense_blocks) {
+  const int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
+  if (i >= count * dim) return;
+  const int64_t v = i / dim;
+  const int c = (int)(i - v * dim);
+  Real sum = Real(0);
+  for (int b = 0; b < dense_blocks; ++b) sum += partials[((int64_t)b * count + v) * dim + c];
+  const int k1 = csr_ptr[v + 1];
+  for (int k = csr_ptr[v]; k < k1; ++k) sum += partials[(int64_t)csr_idx[k] * dim + c];
+  y[i] = sum + damping * x[i];
+}
+
+template <typename In, typename Out>
+__global__ void __launch_bounds__(TLH_THREADS)
+    pack_kernel(Out* __restrict__ packed, int64_t stride, const int64_t* __restrict__ order,
+                int64_t count, int rows, int nslots, const In* b0, const In* b1, const In* b2,
+                int d0, int d1, int d2) {
+  const int64_t n = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
+  if (n >= count) return;
+  const int64_t o = order[n];
+  const In* src[3] = {b0, b1, b2};
+  const int dims[3] = {d0, d1, d2};
+  int comp = 0;
+  for (int s = 0; s < nslots; ++s) {
+    const int k = rows * dims[s];
+    const In* p = src[s] + o * k;
+    for (int e = 0; e < k; ++e) packed[(int64_t)(comp + e) * stride + n] = (Out)p[e];
+    comp += k;
+  }
+}
+
+// ------------------------------------------------------------------- host side
+template <typename Real>
+GroupArgs<Real> make_args(const tlh_operator_t* op, const tlh_group_t& g, const void* x) {
+  GroupArgs<Real> a{};
+  a.packed = static_cast<const Real*>(g.packed);
+  a.stride = g.stride;
+  a.tile_start = g.tile_start;
+  a.ntiles = g.ntiles;
+  a.tiles_per_wg = g.tiles_per_wg;
+  a.rows = g.rows;
+  a.nslots = g.nslots;
+  for (int s = 0; s < g.nslots; ++s) {
+    const tlh_slot_t& sl = g.slot[s];
+    const tlh_type_t& ty = op->types[sl.type];
+    SlotArgs<Real>& o = a.slot[s];
+    o.x = static_cast<const Real*>(x) + ty.offset;
+    o.partials = static_cast<Real*>(ty.partials);
+    o.index = sl.index;
+    o.rank = sl.rank;
+    o.seg_ptr = sl.seg_ptr;
+    o.seg_start = sl.seg_start;
+    o.seg_target = sl.seg_target;
+    o.dim = sl.dim;
+    o.mode = sl.mode;
+    o.dense_block = sl.dense_block;
+    o.type_count = (int32_t)ty.count;
+  }
+  return a;
+}
+
+template <typename Real>
+int64_t dense_bytes(const tlh_operator_t* op, const tlh_group_t& g) {
+  int64_t b = 0;
+  for (int s = 0; s < g.nslots; ++s)
+    if (g.slot[s].mode == TLH_REDUCE_DENSE)
+      b += (int64_t)op->types[g.slot[s].type].count * g.slot[s].dim * (int64_t)sizeof(Real);
+  return b;
+}
+
+template <typename Real>
+constexpr int items_for() {
+  return sizeof(Real) == 4 ? 4 : 2;
+}
+
+template <typename Real, int R, int D0, int D1, int D2>
+int launch_special(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
+                   hipStream_t stream) {
+  constexpr int ITEMS = items_for<Real>();
+  if (g.tile_cap > TLH_THREADS * ITEMS) return TLH_EINVAL;
+  const int64_t lds = (int64_t)TLH_THREADS * ITEMS * Dims<D0, D1, D2>::max * sizeof(Real) +
+                      dense_bytes<Real>(op, g);
+  auto kernel = normal_tile_kernel<Real, R, D0, D1, D2, ITEMS>;
+  TLH_RETURN_IF_ERROR(allow_lds((const void*)kernel, lds));
+  hipLaunchKernelGGL(kernel, dim3(g.nwg), dim3(TLH_THREADS), (size_t)lds, stream,
+                     make_args<Real>(op, g, x));
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+template <typename Real>
+int64_t generic_lds(const tlh_operator_t* op, const tlh_group_t& g) {
+  int dmax = 0;
+  for (int s = 0; s < g.nslots; ++s) dmax = std::max(dmax, (int)g.slot[s].dim);
+  return (int64_t)g.tile_cap * dmax * sizeof(Real) + dense_bytes<Real>(op, g);
+}
+
+template <typename Real>
+int launch_generic(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
+                   hipStream_t stream) {
+  int dmax = 0, dsum = 0;
+  for (int s = 0; s < g.nslots; ++s) {
+    dmax = std::max(dmax, (int)g.slot[s].dim);
+    dsum += g.slot[s].dim;
+  }
+  if (g.rows > GEN_RMAX || dsum > GEN_DSUM) return TLH_ENOTFOUND;
+  const int64_t lds = generic_lds<Real>(op, g);
+  auto kernel = normal_tile_generic<Real>;
+  TLH_RETURN_IF_ERROR(allow_lds((const void*)kernel, lds));
+  hipLaunchKernelGGL(kernel, dim3(g.nwg), dim3(TLH_THREADS), (size_t)lds, stream,
+                     make_args<Real>(op, g, x), dmax, (int)g.tile_cap);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+// Shapes with a specialised kernel; everything else takes the generic kernel.
+template <typename Real>
+int launch_group(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
+                 hipStream_t stream) {
+  const int R = g.rows;
+  const int d0 = g.slot[0].dim;
+  const int d1 = g.nslots > 1 ? g.slot[1].dim : 0;
+  const int d2 = g.nslots > 2 ? g.slot[2].dim : 0;
+  if (d2 == 0 && g.tile_cap <= TLH_THREADS * items_for<Real>()) {
+    if (R == 2 && d0 == 6 && d1 == 3) return launch_special<Real, 2, 6, 3, 0>(op, g, x, stream);
+    if (R == 2 && d0 == 3 && d1 == 6) return launch_special<Real, 2, 3, 6, 0>(op, g, x, stream);
+    if (R == 6 && d0 == 6 && d1 == 6) return launch_special<Real, 6, 6, 6, 0>(op, g, x, stream);
+    if (R == 3 && d0 == 3 && d1 == 0) return launch_special<Real, 3, 3, 0, 0>(op, g, x, stream);
+    if (R == 6 && d0 == 6 && d1 == 0) return launch_special<Real, 6, 6, 0, 0>(op, g, x, stream);
+  }
+  return launch_generic<Real>(op, g, x, stream);
+}
+
+template <typename Real>
+int normal_matvec_impl(const tlh_operator_t* op, const void* x, void* y, double damping,
+                       hipStream_t stream) {
+  for (int gi = 0; gi < op->ngroups; ++gi) {
+    const tlh_group_t& g = op->groups[gi];
+    if (g.count == 0 || g.ntiles == 0) continue;
+    const int rc = launch_group<Real>(op, g, x, stream);
+    if (rc) return rc;
+  }
+  for (int ti = 0; ti < op->ntypes; ++ti) {
+    const tlh_type_t& t = op->types[ti];
+    const int64_t total = t.count * t.dim;
+    if (total == 0) continue;
+    hipLaunchKernelGGL(finish_kernel<Real>, dim3((unsigned)ceil_div(total, TLH_THREADS)),
+                       dim3(TLH_THREADS), 0, stream, static_cast<Real*>(y) + t.offset,
+                       static_cast<const Real*>(x) + t.offset, (Real)damping,
+                       static_cast<const Real*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim,
+                       t.dense_blocks);
+    TLH_RETURN_IF_LAUNCH_FAILED();
+  }
+  return 0;
+}
+
+template <typename In, typename Out>
+int pack_impl(const tlh_group_t& g, const void* const* blocks, hipStream_t stream) {
+  const In* b[3] = {nullptr, nullptr, nullptr};
+  int d[3] = {0, 0, 0};
+  for (int s = 0; s < g.nslots; ++s) {
+    b[s] = static_cast<const In*>(blocks[s]);
+    d[s] = g.slot[s].dim;
+  }
+  hipLaunchKernelGGL((pack_kernel<In, Out>), dim3((unsigned)ceil_div(g.count, TLH_THREADS)),
+                     dim3(TLH_THREADS), 0, stream, static_cast<Out*>(g.packed), g.stride, g.order,
+                     g.count, g.rows, g.nslots, b[0], b[1], b[2], d[0], d[1], d[2]);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+int check_op(const tlh_operator_t* op) {
+  if (!op || (op->dtype != TLH_F32 && op->dtype != TLH_F64)) return TLH_EINVAL;
+  if (op->ngroups < 0 || op->ntypes < 1 || !op->types) return TLH_EINVAL;
+  for (int gi = 0; gi < op->ngroups; ++gi) {
+    const tlh_group_t& g = op->groups[gi];
+    if (g.nslots < 1 || g.nslots > TLH_MAX_SLOTS) return TLH_EINVAL;
+    for (int s = 0; s < g.nslots; ++s)
+      if (g.slot[s].type < 0 || g.slot[s].type >= op->ntypes) return TLH_EINVAL;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int tlh_device_info(int device, tlh_device_info_t* out) {
+  if (!out) return TLH_EINVAL;
+  hipDeviceProp_t prop;
+  TLH_RETURN_IF_ERROR(hipGetDeviceProperties(&prop, device));
+  out->cu_count = prop.multiProcessorCount;
+  int lds = 0;
+  TLH_RETURN_IF_ERROR(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+  out->lds_per_block = lds;
+  out->warp_size = prop.warpSize;
+  int arch = 0;
+  for (const char* c = prop.gcnArchName; *c && *c != ':'; ++c)
+    if (*c >= '0' && *c <= '9') arch = arch * 10 + (*c - '0');
+  out->gcn_arch = arch;
+  return 0;
+}
+
+extern "C" int64_t tlh_group_lds_bytes(const tlh_operator_t* op, int group) {
+  if (check_op(op) || group < 0 || group >= op->ngroups) return -1;
+  const tlh_group_t& g = op->groups[group];
+  int dmax = 0;
+  for (int s = 0; s < g.nslots; ++s) dmax = std::max(dmax, (int)g.slot[s].dim);
+  if (op->dtype == TLH_F32) {
+    const int64_t special = (int64_t)TLH_THREADS * items_for<float>() * dmax * 4 + dense_bytes<float>(op, g);
+    return std::max(special, generic_lds<float>(op, g));
+  }
+  const int64_t special = (int64_t)TLH_THREADS * items_for<double>() * dmax * 8 + dense_bytes<double>(op, g);
+  return std::max(special, generic_lds<double>(op, g));
+}
+
+extern "C" int tlh_pack_blocks(const tlh_operator_t* op, int group, const void* const* blocks,
+                               int in_dtype, tlh_stream_t stream) {
+  if (int rc = check_op(op)) return rc;
+  if (group < 0 || group >= op->ngroups || !blocks) return TLH_EINVAL;
+  const tlh_group_t& g = op->groups[group];
+  if (g.count == 0) return 0;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (in_dtype == TLH_F32 && op->dtype == TLH_F32) return pack_impl<float, float>(g, blocks, s);
+  if (in_dtype == TLH_F64 && op->dtype == TLH_F32) return pack_impl<double, float>(g, blocks, s);
+  if (in_dtype == TLH_F64 && op->dtype == TLH_F64) return pack_impl<double, double>(g, blocks, s);
+  if (in_dtype == TLH_F32 && op->dtype == TLH_F64) return pack_impl<float, double>(g, blocks, s);
+  return TLH_EINVAL;
+}
+
+extern "C" int tlh_normal_matvec(const tlh_operator_t* op, const void* x, void* y, double damping,
+                                 tlh_stream_t stream) {
+  if (int rc = check_op(op)) return rc;
+  if (!x || !y) return TLH_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (op->dtype == TLH_F32) return normal_matvec_impl<float>(op, x, y, damping, s);
+  return normal_matvec_impl<double>(op, x, y, damping, s);
+}

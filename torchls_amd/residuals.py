@@ -1,0 +1,51 @@
+"""Residual templates of the benchmark problems, with analytic Jacobians.
+
+Single-observation functions work with ``Problem.add_residual`` (vmapped like
+any user residual); the ``*_batched`` helpers linearize a whole batch at once
+on the device and are what the synthetic benchmarks use.
+
+Reprojection follows examples/named_bundle_adjustment.py:15-17 with a
+world-to-camera SE(3) pose updated on the right, tangent [v; w]
+(geometry.py:130-137):  d(Rp+t)/dv = R, d(Rp+t)/dw = -R[p]x, d(Rp+t)/dp = R.
+"""
+
+from __future__ import annotations
+
+import torch
+from torch import Tensor
+
+from .geometry import skew
+
+
+def reprojection(camera: Tensor, point: Tensor, observation: Tensor) -> Tensor:
+    local = camera[:3, :3] @ point + camera[:3, 3]
+    return local[:2] / local[2] - observation
+
+
+def reprojection_jacobian(camera: Tensor, point: Tensor, observation: Tensor):
+    """(d r / d camera-tangent [2,6], d r / d point [2,3]) for one observation."""
+    R = camera[:3, :3]
+    local = R @ point + camera[:3, 3]
+    x, y, z = local[0], local[1], local[2]
+    zero = torch.zeros_like(z)
+    D = torch.stack((1 / z, zero, -x / (z * z), zero, 1 / z, -y / (z * z))).reshape(2, 3)
+    DR = D @ R
+    return torch.cat((DR, -(DR @ skew(point))), dim=-1), DR
+
+
+def reprojection_batched(cameras: Tensor, points: Tensor, cam_idx: Tensor, pt_idx: Tensor, uv: Tensor):
+    """Residuals [N,2] and blocks [N,2,6], [N,2,3] for all observations at once."""
+    R = cameras[cam_idx, :3, :3]
+    p = points[pt_idx]
+    local = (R @ p[..., None])[..., 0] + cameras[cam_idx, :3, 3]
+    x, y, z = local.unbind(-1)
+    res = torch.stack((x / z, y / z), -1) - uv
+    inv = 1 / z
+    D = torch.zeros(z.shape + (2, 3), dtype=z.dtype, device=z.device)
+    D[:, 0, 0] = inv
+    D[:, 0, 2] = -x * inv * inv
+    D[:, 1, 1] = inv
+    D[:, 1, 2] = -y * inv * inv
+    DR = D @ R
+    Jc = torch.cat((DR, -(DR @ skew(p))), dim=-1)
+    return res, Jc, DR

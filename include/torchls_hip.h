@@ -127,6 +127,185 @@ int tlh_pack_blocks(const tlh_operator_t* op, int group, const void* const* bloc
 int tlh_normal_matvec(const tlh_operator_t* op, const void* x, void* y, double damping,
                       tlh_stream_t stream);
 
+/* Variable diagonal blocks in float64: out[t] is [count_t, dim_t, dim_t] (full,
+ * symmetric).  op must be an all-SPARSE plan sharing the normal plan's order and
+ * tiles (same packed blocks); each type's `partials` is a float64 workspace of
+ * [sparse_partials, dim*(dim+1)/2].  lds_limit: bytes of LDS per workgroup.
+ * Replaces: WarpDiagonalPlan.diagonal_blocks (warp_diagonal.py:123-179),
+ * BlockJacobian.diagonal_blocks (sparse.py:104-121). */
+int tlh_diagonal_blocks(const tlh_operator_t* op, double* const* out, int64_t lds_limit,
+                        tlh_stream_t stream);
+
+/* ---------------------------------------------------------------- block Jacobi */
+#define TLH_MAX_TYPES 8
+
+/* Block-Jacobi preconditioner: per type, float64 inverse blocks of (D + lambda I)
+ * stored as the row-major upper triangle [count][dim*(dim+1)/2]. */
+typedef struct tlh_block_diag {
+  int32_t ntypes;
+  int32_t dim[TLH_MAX_TYPES];
+  int64_t count[TLH_MAX_TYPES];
+  int64_t offset[TLH_MAX_TYPES];
+  const double* inv[TLH_MAX_TYPES];
+} tlh_block_diag_t;
+
+/* inverse = (blocks + damping I)^-1 through a float64 Cholesky per block
+ * (dim <= 8); *status |= 1 when a block is not positive definite (the analogue
+ * of torch.linalg.cholesky's LinAlgError, solvers.py:267-272, 595-596).
+ * Replaces: block_jacobi's factorisation (solvers.py:259-285). */
+int tlh_block_inverse(const double* blocks, int64_t count, int dim, double damping,
+                      double* inverse, int32_t* status, tlh_stream_t stream);
+
+/* ---------------------------------------------------------------- PCG */
+#define TLH_PCG_MAX_BLOCKS 1024
+#define TLH_PCG_RZ 0        /* r.z of the current iterate (state[0] of warp_pcg) */
+#define TLH_PCG_THR2 1      /* threshold^2 (state[1]) */
+#define TLH_PCG_CURVATURE 2 /* p.Ap */
+#define TLH_PCG_NEW_RZ 3
+#define TLH_PCG_NORM2 4     /* ||r||^2 after the step */
+#define TLH_PCG_TRUE2 5     /* ||rhs - A x||^2 */
+#define TLH_PCG_DRIFT2 6    /* ||(rhs - A x) - r||^2 */
+#define TLH_PCG_AUX 7       /* free slot for tlh_dot */
+#define TLH_PCG_NSCALARS 8
+
+/* Device-resident PCG state (warp_pcg.py:131-132): control = {iterations,
+ * status, stop_at, spare}; status 0 active, 1 recurrence tolerance, 2
+ * nonfinite, 3 nonpositive curvature (warp_pcg.py:78-79).  partials holds
+ * 2 * TLH_PCG_MAX_BLOCKS doubles; counter must start at zero (kernels reset it). */
+typedef struct tlh_pcg_state {
+  int32_t* control;
+  double* scalars;
+  double* partials;
+  uint32_t* counter;
+} tlh_pcg_state_t;
+
+/* scalars[slot] = a . b accumulated in float64 (deterministic).
+ * Replaces: torch.dot in warp_pcg.py:139,147-148 and solvers.py:198-246. */
+int tlh_dot(const void* a, const void* b, int64_t n, int dtype, const tlh_pcg_state_t* st,
+            int slot, tlh_stream_t stream);
+
+/* r = src, x = 0 (when x != NULL), z = M^-1 r, p = z, scalars[RZ] = r.z.
+ * pc == NULL means no preconditioner.  Replaces warp_pcg.py:130-132, 169-172, 210-212. */
+int tlh_pcg_reset(const tlh_block_diag_t* pc, const void* src, void* x, void* r, void* z, void* p,
+                  int64_t n, int dtype, const tlh_pcg_state_t* st, tlh_stream_t stream);
+
+/* Guarded step x += alpha p, r -= alpha Ap (alpha = rz / p.Ap in float64), then
+ * mode 0: z = M^-1 r with pc; 1: z = r; 2: step only (caller applies M and dots);
+ * modes 0/1 also set scalars[NEW_RZ] = r.z and scalars[NORM2] = r.r.
+ * Replaces warp_pcg._step (33-50) fused with the preconditioner and two dots. */
+int tlh_pcg_step(const tlh_block_diag_t* pc, int mode, void* x, void* r, void* z, const void* p,
+                 const void* ap, int64_t n, int dtype, const tlh_pcg_state_t* st,
+                 tlh_stream_t stream);
+
+/* Guarded p = z + (new_rz / rz) p, then the status update of warp_pcg._finish.
+ * Replaces warp_pcg._direction (53-67) and _finish (70-92). */
+int tlh_pcg_direction(void* p, const void* z, int64_t n, int dtype, const tlh_pcg_state_t* st,
+                      tlh_stream_t stream);
+
+/* true_r = rhs - ax; scalars[TRUE2] = ||true_r||^2, scalars[DRIFT2] = ||true_r - r||^2.
+ * Replaces the true-residual check of warp_pcg.py:189-204. */
+int tlh_pcg_true_residual(const void* rhs, const void* ax, const void* r, void* true_r, int64_t n,
+                          int dtype, const tlh_pcg_state_t* st, tlh_stream_t stream);
+
+/* ---------------------------------------------------------------- LM float64 reductions */
+
+/* Partial sums of g = J^T v for one group of an all-SPARSE plan, in float64, read
+ * from the reference-layout blocks (blocks[s]: [count, rows, dim_s], block_dtype)
+ * and v ([count*rows] in original observation order, v_dtype).  Each type's
+ * `partials` must be a float64 [sparse_partials, dim] workspace.  Call once per
+ * group, then tlh_rmatvec_finish_f64.
+ * Replaces: BlockJacobian.rmatvec (sparse.py:43-55), Linearization.gradient
+ * (problem.py:40-42). */
+int tlh_rmatvec_f64(const tlh_operator_t* op, int group, const void* const* blocks, int block_dtype,
+                    const void* v, int v_dtype, int64_t lds_limit, tlh_stream_t stream);
+
+/* out[offset_t + ...] = sum of each variable's rmatvec partials (float64 [size]). */
+int tlh_rmatvec_finish_f64(const tlh_operator_t* op, double* out, tlh_stream_t stream);
+
+/* out2[0] = ||J_g x||^2 of one residual group in float64 (slots summed before
+ * squaring); index[s] are the group's int64 variable indices (original order),
+ * offsets[s] the first tangent column of each slot's type.  partials holds
+ * 2*TLH_PCG_MAX_BLOCKS doubles, counter one zeroed uint32.
+ * Replaces: BlockJacobian.squared_matvec_norm (sparse.py:57-81). */
+int tlh_squared_matvec_norm(int64_t count, int rows, int nslots, const int32_t* dims,
+                            const int64_t* offsets, const int64_t* const* index,
+                            const void* const* blocks, int block_dtype, const void* x, int x_dtype,
+                            double* out2, double* partials, uint32_t* counter, tlh_stream_t stream);
+
+/* y = J_g x for one group, y in original observation order [count*rows] (float64
+ * accumulation, y_dtype output).  Replaces: BlockJacobian.matvec (sparse.py:31-41). */
+int tlh_block_matvec(int64_t count, int rows, int nslots, const int32_t* dims, const int64_t* offsets,
+                     const int64_t* const* index, const void* const* blocks, int block_dtype,
+                     const void* x, int x_dtype, void* y, int y_dtype, tlh_stream_t stream);
+
+/* out2 = { sum w*rho(||after_i||^2), sum w*[rho(||before_i||^2) - rho(||after_i||^2)] }
+ * over `count` residual blocks of `rows` rows (before may be NULL: zeros).
+ * loss 0 = squared norm, 1 = Huber(delta) with the divided-difference reduction;
+ * weights (float64, stride weight_stride, may be NULL).
+ * Replaces: AnalyzedLeastSquaresProblem.cost/_evaluate_trial (problem.py:301-348),
+ * HuberLoss.value/_reduction (losses.py:80-107). */
+int tlh_trial_objective(const void* before, int before_dtype, const void* after, int after_dtype,
+                        int64_t count, int rows, int loss, double delta, const double* weights,
+                        int weight_stride, double* out2, double* partials, uint32_t* counter,
+                        tlh_stream_t stream);
+
+/* ---------------------------------------------------------------- Schur complement */
+#define TLH_SCHUR_MATVEC 0  /* camera partials of J_c^T (J_c x - J_p C^-1 W^T x) */
+#define TLH_SCHUR_RHS 1     /* camera partials of -W C^-1 b_p */
+#define TLH_SCHUR_BACKSUB 2 /* outp = C^-1 (b_p - W^T x) */
+
+/* One pass of the implicit Schur operator over the W-group `wgroup` of a Schur
+ * plan (2 slots, slot `eslot` eliminated and primary: sorted, every eliminated
+ * variable inside one tile).  xc: retained vector (its type offset is taken
+ * from the operator); bp/outp: eliminated vectors starting at that type;
+ * cinv: float64 upper triangles of (D_e + lambda I)^-1.  MATVEC/RHS leave
+ * retained-type partials for tlh_normal_matvec-style finishing.
+ * Replaces: DampedSchurSystem.matvec / reduced_rhs / backsubstitute
+ * (schur.py:285-328) and the CSR W, W^T products (schur.py:270-283, 290, 296, 328). */
+int tlh_schur_apply(const tlh_operator_t* op, int wgroup, int eslot, int mode, const void* xc,
+                    const void* bp, void* outp, const double* cinv, tlh_stream_t stream);
+
+/* Retained preconditioner blocks M = D + lambda I - sum_edges W_e C^-1 W_e^T (full
+ * float64 [count, dim, dim]) over an all-SPARSE Schur plan (float64 partials
+ * [sparse_partials, dim*(dim+1)/2] in the retained type's `partials`).
+ * Replaces: DampedSchurSystem.preconditioner (schur.py:330-378). */
+int tlh_schur_precond(const tlh_operator_t* op, int wgroup, int eslot, const double* cinv,
+                      const double* D, double damping, double* M, int64_t lds_limit,
+                      tlh_stream_t stream);
+
+/* Subtract W C^-1 W^T from a dense float64 reduced matrix M (leading dimension
+ * ld, holding blockdiag(D + lambda I)): one thread per camera pair key
+ * (key_ci <= key_cj) summing its observation pairs pair_i/pair_j
+ * [pair_ptr[k], pair_ptr[k+1]) in order (deterministic, no atomics).
+ * Replaces: DampedSchurSystem.dense_matrix (schur.py:380-397). */
+int tlh_schur_dense(const tlh_operator_t* op, int wgroup, int eslot, const double* cinv,
+                    const int32_t* pair_ptr, const int32_t* pair_i, const int32_t* pair_j,
+                    const int32_t* key_ci, const int32_t* key_cj, int64_t npairs, double* M,
+                    int64_t ld, tlh_stream_t stream);
+
+/* Finish for a subset of types: y[t] = sum of partials + base_scale * base[t] for
+ * t in [type_begin, type_end) (retained types of a Schur plan).  x-like vectors
+ * are indexed by each type's offset.  Used after tlh_schur_apply and the normal
+ * tile kernels of retained-only groups. */
+int tlh_finish_types(const tlh_operator_t* op, int type_begin, int type_end, const void* base,
+                     double base_scale, void* y, tlh_stream_t stream);
+
+/* Normal-matvec tile pass of a single group (no finish): adds that group's
+ * J^T J x contributions to its types' partials. */
+int tlh_group_tiles(const tlh_operator_t* op, int group, const void* x, tlh_stream_t stream);
+
+/* ---------------------------------------------------------------- dense Cholesky */
+/* In-place lower Cholesky A = L L^T (row-major, leading dimension ld, float64):
+ * blocked right-looking, 64-wide panels, MFMA (v_mfma_f64_16x16x4_f64) trailing
+ * updates.  Only the lower triangle is read and written meaningfully; the upper
+ * triangle of the diagonal blocks is left undefined.  *status |= 1 on a
+ * non-positive pivot.  Replaces: torch.linalg.cholesky (solvers.py:517). */
+int tlh_cholesky_f64(double* A, int64_t n, int64_t ld, int32_t* status, tlh_stream_t stream);
+
+/* b <- (L L^T)^-1 b with the factor from tlh_cholesky_f64 (float64, in place).
+ * Replaces: torch.cholesky_solve (solvers.py:518). */
+int tlh_cholesky_solve_f64(const double* L, int64_t n, int64_t ld, double* b, tlh_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

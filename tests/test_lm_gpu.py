@@ -1,0 +1,77 @@
+"""LM solves through Problem.solve() on the HIP backend vs the float64 oracle LM."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle.ba import BAProblem
+from oracle.lm import Termination, solve as oracle_solve
+from torchls_amd import ConjugateGradientConfig, HuberLoss, Problem, SE3Var, TerminationConfig, generators
+from torchls_amd.residuals import reprojection
+
+pytestmark = pytest.mark.gpu
+
+
+def plain_reprojection(camera, point, observation):
+    local = camera[:3, :3] @ point + camera[:3, 3]
+    return local[:2] / local[2] - observation
+
+
+def build(prob, device, fn=reprojection, dtype=torch.float64):
+    P = Problem(
+        {"cams": torch.tensor(prob.cameras_init, device=device, dtype=dtype),
+         "points": torch.tensor(prob.points_init, device=device, dtype=dtype)},
+        manifolds={"cams": SE3Var},
+    )
+    P.add_residual(fn, ("cams", torch.tensor(prob.cam_idx)), ("points", torch.tensor(prob.pt_idx)),
+                   torch.tensor(prob.observations, device=device, dtype=dtype), loss=HuberLoss(1e-2))
+    return P
+
+
+def oracle(prob, iters):
+    o = BAProblem(prob.cam_idx, prob.pt_idx, prob.observations, prob.n_cameras, prob.n_points, huber_delta=1e-2)
+    return oracle_solve(o, (prob.cameras_init, prob.points_init), term=Termination(max_iterations=iters))
+
+
+@pytest.mark.parametrize("fn", [reprojection, plain_reprojection])
+def test_float64_matches_oracle_trajectory(device, fn):
+    print(f"lm float64 fn={fn.__name__}", flush=True)
+    prob = generators.ba_small()
+    P = build(prob, device, fn)
+    sol, s = P.solve(return_summary=True, termination=TerminationConfig(max_iterations=20))
+    _, ref = oracle(prob, 20)
+    assert s.termination_reason == ref.termination_reason
+    assert (s.iterations, s.accepted_steps, s.rejected_steps) == (ref.iterations, ref.accepted_steps, ref.rejected_steps)
+    assert s.lambda_history == pytest.approx(ref.lambda_history, rel=1e-12)
+    assert s.linear_iterations_history == ref.linear_iterations_history
+    assert np.allclose(s.cost_history, ref.cost_history, rtol=1e-9, atol=0)
+    assert math.isclose(s.final_cost, ref.final_cost, rel_tol=1e-9)
+    assert sol["cams"].shape == (16, 4, 4) and P.parameters["points"] is sol["points"]
+
+
+def test_mixed_precision_final_cost(device):
+    prob = generators.ba_small()
+    P = build(prob, device)
+    cg = ConjugateGradientConfig(matvec_dtype=torch.float32)
+    _, s = P.solve(return_summary=True, conjugate_gradient=cg, termination=TerminationConfig(max_iterations=20))
+    _, ref = oracle(prob, 20)
+    assert abs(s.final_cost - ref.final_cost) <= 1e-6 * ref.final_cost
+
+
+def test_readme_triangulation(device):
+    def reprojection_guarded(point, camera, observed_uv):
+        xyz = camera[:3, :3] @ point + camera[:3, 3]
+        valid = xyz[2] > 1e-6
+        z = torch.where(valid, xyz[2], torch.ones_like(xyz[2]))
+        error = xyz[:2] / z - observed_uv
+        return torch.where(valid, error, torch.zeros_like(error))
+
+    cameras = torch.eye(4, dtype=torch.float64).repeat(2, 1, 1)
+    cameras[1, 0, 3] = -1.0
+    obs = torch.tensor([[0.2, 0.1], [0.0, 0.1]], dtype=torch.float64)
+    P = Problem({"points": torch.tensor([[0.8, 0.4, 4.0]], dtype=torch.float64, device=device)})
+    P.add_residual(reprojection_guarded, ("points", torch.tensor([0, 0])), cameras.to(device), obs.to(device))
+    sol = P.solve()
+    assert torch.allclose(sol["points"].cpu(), torch.tensor([[1.0, 0.5, 5.0]], dtype=torch.float64), atol=1e-8)

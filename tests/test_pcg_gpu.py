@@ -1,0 +1,113 @@
+"""GPU parity of the HIP PCG (warp_pcg control flow) against the float64 oracle."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle.pcg import Config, block_jacobi as np_block_jacobi, pcg_batched
+from tests.helpers import ba_jacobian, to_block_jacobian
+from torchls_amd import generators
+from torchls_amd.hip_diagonal import HipBlockJacobi, HipDiagonalPlan
+from torchls_amd.hip_normal import HipNormalPlan
+from torchls_amd.hip_pcg import hip_pcg
+from torchls_amd.solvers import ConjugateGradientConfig
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(device, dtype=torch.float64, seed=5, lam=1e-2):
+    prob = generators.bundle_adjustment(16, 500, 8, seed=seed)
+    J, r = ba_jacobian(prob, delta=1e-2)
+    BJ = to_block_jacobian(J, device, torch.float64)
+    op = HipNormalPlan(BJ, dtype=dtype).bind(BJ)
+    diag = HipDiagonalPlan(BJ).diagonal_blocks(BJ)
+    g = J.rmatvec(r.reshape(-1))
+    return J, BJ, op, diag, g, lam
+
+
+CONFIGS = [
+    dict(),
+    dict(residual_check_interval=5, graph_batch_iterations=3),
+    dict(restart_strategy="periodic", residual_check_interval=7),
+    dict(max_iterations=9),
+    dict(residual_check_interval=0, graph_batch_iterations=4),
+    dict(rtol=1e-12, max_iterations=60, residual_check_interval=10),
+]
+
+
+@pytest.mark.parametrize("cfg", range(len(CONFIGS)))
+def test_matches_oracle_control_flow(device, cfg):
+    print(f"pcg control flow cfg={CONFIGS[cfg]}", flush=True)
+    J, BJ, op, diag, g, lam = _system(device)
+    config = ConjugateGradientConfig(**CONFIGS[cfg])
+    pre = HipBlockJacobi(BJ.layout, diag, lam)
+    res = hip_pcg(lambda v: op.normal_matvec(v, lam), torch.tensor(-g, device=device), preconditioner=pre,
+                  config=config)
+    ocfg = Config(**{k: getattr(config, k) for k in Config.__dataclass_fields__})
+    ref = pcg_batched(lambda v: J.normal_matvec(v, lam), -g,
+                      np_block_jacobi([d.cpu().numpy() for d in diag], J.layout, lam), ocfg)
+    assert (res.iterations, res.restarts, res.residual_checks, res.reason, res.converged) == (
+        ref.iterations, ref.restarts, ref.residual_checks, ref.reason, ref.converged)
+    x = res.solution.cpu().numpy()
+    assert np.linalg.norm(x - ref.solution) <= 1e-9 * np.linalg.norm(ref.solution)
+    assert math.isclose(res.residual_norm, ref.residual_norm, rel_tol=1e-6, abs_tol=1e-14)
+
+
+def test_graph_replay_matches_eager(device):
+    J, BJ, op, diag, g, lam = _system(device)
+    pre = HipBlockJacobi(BJ.layout, diag, lam)
+    rhs = torch.tensor(-g, device=device)
+    cfg = ConjugateGradientConfig(residual_check_interval=10, graph_batch_iterations=4)
+    a = hip_pcg(lambda v: op.normal_matvec(v, lam), rhs, preconditioner=pre, config=cfg, use_graph=True)
+    b = hip_pcg(lambda v: op.normal_matvec(v, lam), rhs, preconditioner=pre, config=cfg, use_graph=False)
+    c = hip_pcg(lambda v: op.normal_matvec(v, lam), rhs, preconditioner=pre, config=cfg, use_graph=True)
+    assert torch.equal(a.solution, b.solution) and torch.equal(a.solution, c.solution)
+    assert a.iterations == b.iterations == c.iterations
+
+
+@pytest.mark.parametrize("kind", ["none", "callable"])
+def test_other_preconditioners(device, kind):
+    J, BJ, op, diag, g, lam = _system(device)
+    rhs = torch.tensor(-g, device=device)
+    if kind == "none":
+        pre, npre = None, None
+    else:
+        dvec = torch.cat([d.diagonal(dim1=-2, dim2=-1).reshape(-1) for d in diag]) + lam
+        pre = lambda v: v / dvec  # noqa: E731
+        dn = dvec.cpu().numpy()
+        npre = lambda v: v / dn  # noqa: E731
+    cfg = ConjugateGradientConfig(max_iterations=300)
+    res = hip_pcg(lambda v: op.normal_matvec(v, lam), rhs, preconditioner=pre, config=cfg)
+    ocfg = Config(max_iterations=300)
+    ref = pcg_batched(lambda v: J.normal_matvec(v, lam), -g, npre, ocfg)
+    assert (res.iterations, res.restarts, res.reason) == (ref.iterations, ref.restarts, ref.reason)
+    assert np.linalg.norm(res.solution.cpu().numpy() - ref.solution) <= 1e-8 * np.linalg.norm(ref.solution)
+
+
+def test_float32_mixed(device):
+    J, BJ, op32, diag, g, lam = _system(device, dtype=torch.float32)
+    pre = HipBlockJacobi(BJ.layout, diag, lam)
+    res = hip_pcg(lambda v: op32.normal_matvec(v, lam), torch.tensor(-g, device=device, dtype=torch.float32),
+                  preconditioner=pre, config=ConjugateGradientConfig(rtol=1e-5))
+    A = J.to_dense()
+    exact = np.linalg.solve(A.T @ A + lam * np.eye(J.n), -g)
+    assert res.reason == "residual_tolerance"
+    assert np.linalg.norm(res.solution.cpu().numpy() - exact) <= 1e-3 * np.linalg.norm(exact)
+
+
+def test_failure_reasons(device):
+    J, BJ, op, diag, g, lam = _system(device)
+    rhs = torch.tensor(-g, device=device)
+    cfg = ConjugateGradientConfig()
+    nan = rhs.clone()
+    nan[3] = float("nan")
+    r = hip_pcg(lambda v: op.normal_matvec(v, lam), nan, preconditioner=None, config=cfg)
+    assert r.reason == "nonfinite" and not r.converged and r.iterations == 0
+    z = hip_pcg(lambda v: op.normal_matvec(v, lam), torch.zeros_like(rhs), preconditioner=None, config=cfg)
+    assert z.reason == "residual_tolerance" and z.converged and z.iterations == 0
+    neg = hip_pcg(lambda v: -op.normal_matvec(v, lam), rhs, preconditioner=None, config=cfg)
+    ref = pcg_batched(lambda v: -J.normal_matvec(v, lam), -g, None, Config())
+    assert neg.reason == ref.reason == "nonpositive_curvature"
+    assert neg.iterations == ref.iterations

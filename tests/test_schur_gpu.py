@@ -1,0 +1,108 @@
+"""GPU parity of the implicit HIP Schur system and the MFMA Cholesky vs the oracle."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle.ba import BAProblem
+from oracle.lm import Termination, solve as oracle_solve
+from oracle.schur import SchurSystem
+from oracle.sparse import NpGroup, NpJacobian
+from tests.helpers import rel_err
+from torchls_amd import HuberLoss, Problem, SE3Var, TerminationConfig, generators
+from torchls_amd.cholesky import cholesky_f64, cholesky_solve_f64
+from torchls_amd.hip_diagonal import LinAlgError
+from torchls_amd.residuals import reprojection
+from torchls_amd.schur import HipSchurPlan
+
+pytestmark = pytest.mark.gpu
+
+
+def camera_prior(camera, reference):
+    return (camera[:3, :] - reference[:3, :]).reshape(-1)
+
+
+def point_prior(point, reference):
+    return point - reference
+
+
+def build(prob, device, priors=False):
+    P = Problem({"cams": torch.tensor(prob.cameras_init, device=device),
+                 "points": torch.tensor(prob.points_init, device=device)}, manifolds={"cams": SE3Var})
+    P.add_residual(reprojection, ("cams", torch.tensor(prob.cam_idx)), ("points", torch.tensor(prob.pt_idx)),
+                   torch.tensor(prob.observations, device=device), loss=HuberLoss(1e-2))
+    if priors:
+        P.add_residual(camera_prior, ("cams", 0), torch.tensor(prob.cameras_true[0], device=device))
+        P.add_residual(point_prior, ("points", 0), torch.tensor(prob.points_true[0], device=device))
+    return P
+
+
+def to_np(jacobian, layout):
+    groups = [NpGroup([b.double().cpu().numpy() for b in g.blocks], list(g.type_indices),
+                      [i.cpu().numpy() for i in g.variable_indices]) for g in jacobian.groups]
+    return NpJacobian(groups, [(e.count, e.dim, e.offset) for e in layout])
+
+
+@pytest.mark.parametrize("priors", [False, True])
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-11), (torch.float32, 2e-5)])
+def test_schur_operators(device, priors, dtype, tol):
+    print(f"schur operators priors={priors} dtype={dtype}", flush=True)
+    prob = generators.bundle_adjustment(16, 500, 8, seed=21)
+    P = build(prob, device, priors)
+    an = P._analyze()
+    lin = an.linearize(P.parameters)
+    diag = tuple(d.double() for d in lin.jacobian.diagonal_blocks())
+    lam = 1e-2
+    plan = HipSchurPlan(an, "points", device, dtype=dtype)
+    system = plan.linearize(lin.jacobian, diag).damped(lam)
+    J = to_np(lin.jacobian, an.layout)
+    ref = SchurSystem(J, [d.cpu().numpy() for d in diag], 1, lam)
+    rng = np.random.default_rng(3)
+    rhs = rng.standard_normal(J.n)
+    rr = system.reduced_rhs(torch.tensor(rhs, device=device))
+    assert rel_err(rr.cpu().numpy(), ref.reduced_rhs(rhs)) <= tol
+    x = rng.standard_normal(ref.dim)
+    y = system.matvec(torch.tensor(x, device=device, dtype=dtype))
+    assert rel_err(y.cpu().numpy(), ref.matvec(x)) <= tol
+    full = system.backsubstitute(torch.tensor(x, device=device, dtype=dtype))
+    assert rel_err(full.cpu().numpy(), ref.backsubstitute(x)) <= tol
+    for a, b in zip(system.preconditioner_blocks(), ref.preconditioner_blocks()):
+        assert rel_err(a.cpu().numpy(), b) <= tol
+    M = system.dense_matrix().cpu().numpy()
+    Md = ref.dense_matrix()
+    assert rel_err(np.tril(M), np.tril(Md)) <= tol
+
+
+@pytest.mark.parametrize("n", [5, 70, 200, 1536])
+def test_cholesky(device, n):
+    print(f"cholesky n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n + 3))
+    M = A @ A.T + n * 1e-3 * np.eye(n)
+    L = cholesky_f64(torch.tensor(M, device=device)).cpu().numpy()
+    Lr = np.linalg.cholesky(M)
+    assert rel_err(np.tril(L), Lr) <= 1e-12
+    b = rng.standard_normal(n)
+    x = cholesky_solve_f64(torch.tensor(M, device=device), torch.tensor(b, device=device)).cpu().numpy()
+    assert rel_err(x, np.linalg.solve(M, b)) <= 1e-10
+    with pytest.raises(LinAlgError):
+        cholesky_f64(torch.tensor(-M, device=device))
+
+
+@pytest.mark.parametrize("solver", ["schur_cg", "schur_dense"])
+def test_schur_lm_matches_oracle(device, solver):
+    print(f"schur lm {solver}", flush=True)
+    prob = generators.ba_small()
+    P = build(prob, device)
+    _, s = P.solve(return_summary=True, linear_solver=solver, eliminate="points",
+                   termination=TerminationConfig(max_iterations=20))
+    o = BAProblem(prob.cam_idx, prob.pt_idx, prob.observations, prob.n_cameras, prob.n_points, huber_delta=1e-2)
+    _, ref = oracle_solve(o, (prob.cameras_init, prob.points_init), linear_solver=solver, eliminate=1,
+                          term=Termination(max_iterations=20))
+    assert s.termination_reason == ref.termination_reason
+    assert (s.iterations, s.accepted_steps, s.rejected_steps) == (ref.iterations, ref.accepted_steps, ref.rejected_steps)
+    assert math.isclose(s.final_cost, ref.final_cost, rel_tol=1e-9)
+    if solver == "schur_cg":
+        assert s.linear_iterations_history == ref.linear_iterations_history

@@ -91,6 +91,26 @@ class Operator(ctypes.Structure):
     ]
 
 
+TLH_MAX_TYPES = 8
+TLH_PCG_MAX_BLOCKS = 1024
+PCG_RZ, PCG_THR2, PCG_CURVATURE, PCG_NEW_RZ, PCG_NORM2, PCG_TRUE2, PCG_DRIFT2, PCG_AUX = range(8)
+PCG_NSCALARS = 8
+
+
+class BlockDiag(ctypes.Structure):
+    _fields_ = [
+        ("ntypes", i32),
+        ("dim", i32 * TLH_MAX_TYPES),
+        ("count", i64 * TLH_MAX_TYPES),
+        ("offset", i64 * TLH_MAX_TYPES),
+        ("inv", c_void_p * TLH_MAX_TYPES),
+    ]
+
+
+class PcgStateDesc(ctypes.Structure):
+    _fields_ = [("control", c_void_p), ("scalars", c_void_p), ("partials", c_void_p), ("counter", c_void_p)]
+
+
 class DeviceInfo(ctypes.Structure):
     _fields_ = [("cu_count", i32), ("lds_per_block", i32), ("warp_size", i32), ("gcn_arch", i32)]
 
@@ -107,6 +127,79 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.POINTER(Operator), c_void_p, c_void_p, ctypes.c_double, c_void_p],
     ),
+    "tlh_diagonal_blocks": (
+        ctypes.c_int,
+        [ctypes.POINTER(Operator), ctypes.POINTER(c_void_p), i64, c_void_p],
+    ),
+    "tlh_block_inverse": (
+        ctypes.c_int,
+        [c_void_p, i64, ctypes.c_int, ctypes.c_double, c_void_p, c_void_p, c_void_p],
+    ),
+    "tlh_dot": (
+        ctypes.c_int,
+        [c_void_p, c_void_p, i64, ctypes.c_int, ctypes.POINTER(PcgStateDesc), ctypes.c_int, c_void_p],
+    ),
+    "tlh_pcg_reset": (
+        ctypes.c_int,
+        [ctypes.POINTER(BlockDiag), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, i64, ctypes.c_int,
+         ctypes.POINTER(PcgStateDesc), c_void_p],
+    ),
+    "tlh_pcg_step": (
+        ctypes.c_int,
+        [ctypes.POINTER(BlockDiag), ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, i64,
+         ctypes.c_int, ctypes.POINTER(PcgStateDesc), c_void_p],
+    ),
+    "tlh_pcg_direction": (
+        ctypes.c_int,
+        [c_void_p, c_void_p, i64, ctypes.c_int, ctypes.POINTER(PcgStateDesc), c_void_p],
+    ),
+    "tlh_pcg_true_residual": (
+        ctypes.c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, i64, ctypes.c_int, ctypes.POINTER(PcgStateDesc), c_void_p],
+    ),
+    "tlh_rmatvec_f64": (
+        ctypes.c_int,
+        [ctypes.POINTER(Operator), ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int, c_void_p, ctypes.c_int,
+         i64, c_void_p],
+    ),
+    "tlh_rmatvec_finish_f64": (ctypes.c_int, [ctypes.POINTER(Operator), c_void_p, c_void_p]),
+    "tlh_squared_matvec_norm": (
+        ctypes.c_int,
+        [i64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(i32), ctypes.POINTER(i64), ctypes.POINTER(c_void_p),
+         ctypes.POINTER(c_void_p), ctypes.c_int, c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
+    "tlh_block_matvec": (
+        ctypes.c_int,
+        [i64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(i32), ctypes.POINTER(i64), ctypes.POINTER(c_void_p),
+         ctypes.POINTER(c_void_p), ctypes.c_int, c_void_p, ctypes.c_int, c_void_p, ctypes.c_int, c_void_p],
+    ),
+    "tlh_trial_objective": (
+        ctypes.c_int,
+        [c_void_p, ctypes.c_int, c_void_p, ctypes.c_int, i64, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+         c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
+    "tlh_schur_apply": (
+        ctypes.c_int,
+        [ctypes.POINTER(Operator), ctypes.c_int, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+         c_void_p],
+    ),
+    "tlh_schur_precond": (
+        ctypes.c_int,
+        [ctypes.POINTER(Operator), ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, ctypes.c_double, c_void_p, i64,
+         c_void_p],
+    ),
+    "tlh_schur_dense": (
+        ctypes.c_int,
+        [ctypes.POINTER(Operator), ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+         c_void_p, i64, c_void_p, i64, c_void_p],
+    ),
+    "tlh_finish_types": (
+        ctypes.c_int,
+        [ctypes.POINTER(Operator), ctypes.c_int, ctypes.c_int, c_void_p, ctypes.c_double, c_void_p, c_void_p],
+    ),
+    "tlh_group_tiles": (ctypes.c_int, [ctypes.POINTER(Operator), ctypes.c_int, c_void_p, c_void_p]),
+    "tlh_cholesky_f64": (ctypes.c_int, [c_void_p, i64, i64, c_void_p, c_void_p]),
+    "tlh_cholesky_solve_f64": (ctypes.c_int, [c_void_p, i64, i64, c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -1,0 +1,34 @@
+"""Dense float64 Cholesky factor/solve of the reduced camera system (HIP, MFMA)."""
+
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from .hip_diagonal import LinAlgError
+
+
+def cholesky_f64(M: torch.Tensor) -> torch.Tensor:
+    """Lower Cholesky factor of a symmetric positive definite float64 matrix (lower
+    triangle meaningful).  Raises LinAlgError on a non-positive pivot."""
+    if M.ndim != 2 or M.shape[0] != M.shape[1] or M.dtype != torch.float64 or M.device.type != "cuda":
+        raise ValueError("cholesky_f64 expects a square float64 GPU matrix")
+    L = M.contiguous().clone()
+    n = L.shape[0]
+    status = torch.zeros(1, dtype=torch.int32, device=M.device)
+    lib = _lib.load()
+    _lib.check(lib.tlh_cholesky_f64(L.data_ptr(), n, n, status.data_ptr(), _lib.stream_handle(M.device)),
+               "tlh_cholesky_f64")
+    if int(status.item()) != 0:
+        raise LinAlgError("matrix is not positive definite")
+    return L
+
+
+def cholesky_solve_f64(M: torch.Tensor, rhs: torch.Tensor) -> torch.Tensor:
+    """Solve M x = rhs for SPD M through cholesky_f64 and two triangular solves."""
+    L = cholesky_f64(M)
+    x = rhs.to(torch.float64).contiguous().clone()
+    lib = _lib.load()
+    _lib.check(lib.tlh_cholesky_solve_f64(L.data_ptr(), L.shape[0], L.shape[0], x.data_ptr(),
+                                          _lib.stream_handle(M.device)), "tlh_cholesky_solve_f64")
+    return x

@@ -1,0 +1,373 @@
+// float64 reductions of the Levenberg-Marquardt loop.
+//
+//  * tlh_rmatvec_f64: g = J^T v from the linearization's own blocks (reference
+//    layout, any float dtype) in float64, deterministic (all-SPARSE tile plan).
+//    This is the outer gradient that is recomputed in float64 every LM
+//    iteration even when the PCG runs on float32 blocks.
+//    Replaces Linearization.gradient / BlockJacobian.rmatvec (problem.py:40-42,
+//    sparse.py:43-55).
+//  * tlh_squared_matvec_norm: ||J x||^2 accumulated in float64, slots summed
+//    before squaring (sparse.py:57-81).
+//  * tlh_trial_objective: 0.5 sum rho(||r_new||^2) and the trial reduction
+//    0.5 sum [rho(||r_old||^2) - rho(||r_new||^2)] evaluated through expanded
+//    differences and the Huber divided difference (problem.py:315-348,
+//    losses.py:80-107), weights optional.
+#include <math.h>
+
+#include "common.hpp"
+
+using namespace tlh;
+
+namespace {
+
+constexpr int RED_BLOCKS = TLH_PCG_MAX_BLOCKS;
+
+__device__ __forceinline__ void block_sum2(double (&v)[2], double* smem) {
+  smem[threadIdx.x] = v[0];
+  smem[TLH_THREADS + threadIdx.x] = v[1];
+  __syncthreads();
+  for (int w = TLH_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      smem[threadIdx.x] += smem[threadIdx.x + w];
+      smem[TLH_THREADS + threadIdx.x] += smem[TLH_THREADS + threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  v[0] = smem[0];
+  v[1] = smem[TLH_THREADS];
+  __syncthreads();
+}
+
+// Fixed-order grid reduction of two sums; the last block writes out[0..1].
+__device__ __forceinline__ void grid_sum2(double (&v)[2], double* partials, uint32_t* counter, double* out,
+                                          double* smem) {
+  __shared__ int last;
+  block_sum2(v, smem);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = v[0];
+    partials[RED_BLOCKS + blockIdx.x] = v[1];
+    const uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = t == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  double a[2] = {0.0, 0.0};
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += TLH_THREADS) {
+    a[0] += __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    a[1] += __hip_atomic_load(&partials[RED_BLOCKS + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  block_sum2(a, smem);
+  if (threadIdx.x == 0) {
+    out[0] = a[0];
+    out[1] = a[1];
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+inline int red_grid(int64_t work) {
+  int64_t g = ceil_div(work, TLH_THREADS);
+  return (int)(g < 1 ? 1 : (g > RED_BLOCKS ? RED_BLOCKS : g));
+}
+
+template <typename T>
+__device__ __forceinline__ double ld(const void* p, int64_t i) {
+  return (double)static_cast<const T*>(p)[i];
+}
+
+__device__ __forceinline__ double load_any(const void* p, int64_t i, int dtype) {
+  return dtype == TLH_F64 ? ld<double>(p, i) : ld<float>(p, i);
+}
+
+// ------------------------------------------------------------ rmatvec (f64)
+struct RmatvecArgs {
+  const void* blocks[TLH_MAX_SLOTS];  // reference layout [count, rows, dim]
+  const void* v;                      // [count * rows] in original observation order
+  const int64_t* order;
+  const int32_t* tile_start;
+  int32_t ntiles;
+  int32_t rows;
+  int32_t nslots;
+  int32_t block_dtype;
+  int32_t v_dtype;
+  int32_t dims[TLH_MAX_SLOTS];
+  const uint16_t* rank[TLH_MAX_SLOTS];
+  const int32_t* seg_ptr[TLH_MAX_SLOTS];
+  const int32_t* seg_start[TLH_MAX_SLOTS];
+  const int32_t* seg_target[TLH_MAX_SLOTS];
+  double* partials[TLH_MAX_SLOTS];
+};
+
+__global__ void __launch_bounds__(TLH_THREADS) rmatvec_tile_kernel(RmatvecArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* stage = reinterpret_cast<double*>(smem_raw);
+  const int t = blockIdx.x;
+  const int start = a.tile_start[t];
+  const int len = a.tile_start[t + 1] - start;
+  const int R = a.rows;
+  for (int s = 0; s < a.nslots; ++s) {
+    const int D = a.dims[s];
+    __syncthreads();
+    for (int pos = threadIdx.x; pos < len; pos += TLH_THREADS) {
+      const int64_t n = start + pos;
+      const int64_t o = a.order[n];
+      const int rk = a.rank[s] ? (int)a.rank[s][n] : pos;
+      double vr[16];
+      for (int r = 0; r < R; ++r) vr[r] = load_any(a.v, o * R + r, a.v_dtype);
+      for (int c = 0; c < D; ++c) {
+        double acc = 0.0;
+        for (int r = 0; r < R; ++r) acc += load_any(a.blocks[s], (o * R + r) * D + c, a.block_dtype) * vr[r];
+        stage[rk * D + c] = acc;
+      }
+    }
+    __syncthreads();
+    const int sb = a.seg_ptr[s][t], se = a.seg_ptr[s][t + 1];
+    for (int k = sb + (int)threadIdx.x; k < se; k += TLH_THREADS) {
+      const int lo = a.seg_start[s][k];
+      const int hi = (k + 1 < se) ? a.seg_start[s][k + 1] : len;
+      double* dst = a.partials[s] + (int64_t)a.seg_target[s][k] * D;
+      for (int c = 0; c < D; ++c) {
+        double sum = stage[lo * D + c];
+        for (int i = lo + 1; i < hi; ++i) sum += stage[i * D + c];
+        dst[c] = sum;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(TLH_THREADS)
+    rmatvec_finish_kernel(double* __restrict__ out, const double* __restrict__ partials,
+                          const int32_t* __restrict__ csr_ptr, const int32_t* __restrict__ csr_idx,
+                          int64_t count, int dim) {
+  const int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
+  if (i >= count * dim) return;
+  const int64_t v = i / dim;
+  const int c = (int)(i - v * dim);
+  double sum = 0.0;
+  for (int k = csr_ptr[v]; k < csr_ptr[v + 1]; ++k) sum += partials[(int64_t)csr_idx[k] * dim + c];
+  out[i] = sum;
+}
+
+// ------------------------------------------------------- ||J x||^2 (float64)
+struct RawGroup {
+  int64_t count;
+  int32_t rows;
+  int32_t nslots;
+  int32_t dims[TLH_MAX_SLOTS];
+  int64_t offsets[TLH_MAX_SLOTS];
+  const int64_t* index[TLH_MAX_SLOTS];
+  const void* blocks[TLH_MAX_SLOTS];
+};
+
+__global__ void __launch_bounds__(TLH_THREADS)
+    sqnorm_kernel(RawGroup g, int block_dtype, const void* x, int x_dtype, double* out, double* partials,
+                  uint32_t* counter) {
+  __shared__ double smem[2 * TLH_THREADS];
+  double v[2] = {0.0, 0.0};
+  const int R = g.rows;
+  for (int64_t o = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; o < g.count;
+       o += (int64_t)gridDim.x * TLH_THREADS) {
+    double q[16];
+    for (int r = 0; r < R; ++r) q[r] = 0.0;
+    for (int s = 0; s < g.nslots; ++s) {
+      const int D = g.dims[s];
+      const int64_t col = g.offsets[s] + g.index[s][o] * D;
+      for (int c = 0; c < D; ++c) {
+        const double xv = load_any(x, col + c, x_dtype);
+        for (int r = 0; r < R; ++r) q[r] += load_any(g.blocks[s], (o * R + r) * D + c, block_dtype) * xv;
+      }
+    }
+    for (int r = 0; r < R; ++r) v[0] += q[r] * q[r];
+  }
+  grid_sum2(v, partials, counter, out, smem);
+}
+
+// y[o*R + r] = sum_s J_s[o] x_s in float64, written in the output dtype (sparse.py:31-41).
+__global__ void __launch_bounds__(TLH_THREADS)
+    block_matvec_kernel(RawGroup g, int block_dtype, const void* x, int x_dtype, void* y, int y_dtype) {
+  const int R = g.rows;
+  const int64_t o = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
+  if (o >= g.count) return;
+  double q[16];
+  for (int r = 0; r < R; ++r) q[r] = 0.0;
+  for (int s = 0; s < g.nslots; ++s) {
+    const int D = g.dims[s];
+    const int64_t col = g.offsets[s] + g.index[s][o] * D;
+    for (int c = 0; c < D; ++c) {
+      const double xv = load_any(x, col + c, x_dtype);
+      for (int r = 0; r < R; ++r) q[r] += load_any(g.blocks[s], (o * R + r) * D + c, block_dtype) * xv;
+    }
+  }
+  for (int r = 0; r < R; ++r) {
+    if (y_dtype == TLH_F64)
+      static_cast<double*>(y)[o * R + r] = q[r];
+    else
+      static_cast<float*>(y)[o * R + r] = (float)q[r];
+  }
+}
+
+// ---------------------------------------------------- trial objective (f64)
+__device__ __forceinline__ double huber_value(double s, double delta) {
+  const double norm = sqrt(fmax(s, 2.2250738585072014e-308));
+  return s > delta * delta ? delta * (2.0 * norm - delta) : s;
+}
+
+// Divided difference of the Huber loss between two squared norms (losses.py:95-107).
+__device__ __forceinline__ double huber_reduction(double before, double after, double diff, double delta) {
+  const double tiny = 2.2250738585072014e-308;
+  const double a = sqrt(before), b = sqrt(after);
+  const double low = fmin(a, b), high = fmax(a, b);
+  const double shortfall = fmax(delta - low, 0.0);
+  const double crossing = (2.0 * delta - shortfall * (shortfall / fmax(high - low, tiny))) / fmax(a + b, tiny);
+  double coef;
+  if (high <= delta)
+    coef = 1.0;
+  else if (low >= delta)
+    coef = 2.0 * delta / fmax(a + b, tiny);
+  else
+    coef = crossing;
+  return diff * coef;
+}
+
+__global__ void __launch_bounds__(TLH_THREADS)
+    trial_kernel(const void* before, int before_dtype, const void* after, int after_dtype, int64_t count, int rows,
+                 int loss, double delta, const double* weights, int weight_stride, double* out, double* partials,
+                 uint32_t* counter) {
+  __shared__ double smem[2 * TLH_THREADS];
+  double v[2] = {0.0, 0.0};
+  for (int64_t o = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; o < count;
+       o += (int64_t)gridDim.x * TLH_THREADS) {
+    double s_old = 0.0, s_new = 0.0, diff = 0.0;
+    for (int r = 0; r < rows; ++r) {
+      const double a = before ? load_any(before, o * rows + r, before_dtype) : 0.0;
+      const double b = load_any(after, o * rows + r, after_dtype);
+      s_old += a * a;
+      s_new += b * b;
+      diff += (a - b) * (a + b);
+    }
+    double value, change;
+    if (loss == 1) {
+      value = huber_value(s_new, delta);
+      change = huber_reduction(s_old, s_new, diff, delta);
+    } else {
+      value = s_new;
+      change = diff;
+    }
+    if (weights) {
+      const double w = weights[o * weight_stride];
+      value *= w;
+      change *= w;
+    }
+    v[0] += value;
+    v[1] += change;
+  }
+  grid_sum2(v, partials, counter, out, smem);
+}
+
+}  // namespace
+
+extern "C" int tlh_rmatvec_f64(const tlh_operator_t* op, int group, const void* const* blocks, int block_dtype,
+                               const void* v, int v_dtype, int64_t lds_limit, tlh_stream_t stream) {
+  if (!op || group < 0 || group >= op->ngroups || !blocks || !v) return TLH_EINVAL;
+  const tlh_group_t& g = op->groups[group];
+  if (g.count == 0) return 0;
+  if (g.rows > 16) return TLH_EINVAL;
+  RmatvecArgs a{};
+  int dmax = 1;
+  for (int s = 0; s < g.nslots; ++s) {
+    const tlh_slot_t& sl = g.slot[s];
+    if (sl.mode != TLH_REDUCE_SPARSE) return TLH_EINVAL;
+    a.blocks[s] = blocks[s];
+    a.dims[s] = sl.dim;
+    a.rank[s] = sl.rank;
+    a.seg_ptr[s] = sl.seg_ptr;
+    a.seg_start[s] = sl.seg_start;
+    a.seg_target[s] = sl.seg_target;
+    a.partials[s] = static_cast<double*>(op->types[sl.type].partials);
+    dmax = sl.dim > dmax ? sl.dim : dmax;
+  }
+  a.v = v;
+  a.order = g.order;
+  a.tile_start = g.tile_start;
+  a.ntiles = g.ntiles;
+  a.rows = g.rows;
+  a.nslots = g.nslots;
+  a.block_dtype = block_dtype;
+  a.v_dtype = v_dtype;
+  const int64_t lds = (int64_t)g.tile_cap * dmax * 8;
+  if (lds > lds_limit) return TLH_ELDS;
+  TLH_RETURN_IF_ERROR(allow_lds((const void*)rmatvec_tile_kernel, lds));
+  hipLaunchKernelGGL(rmatvec_tile_kernel, dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds,
+                     reinterpret_cast<hipStream_t>(stream), a);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+extern "C" int tlh_rmatvec_finish_f64(const tlh_operator_t* op, double* out, tlh_stream_t stream) {
+  if (!op || !out) return TLH_EINVAL;
+  for (int ti = 0; ti < op->ntypes; ++ti) {
+    const tlh_type_t& t = op->types[ti];
+    const int64_t total = t.count * t.dim;
+    if (total == 0) continue;
+    hipLaunchKernelGGL(rmatvec_finish_kernel, dim3((unsigned)ceil_div(total, TLH_THREADS)), dim3(TLH_THREADS), 0,
+                       reinterpret_cast<hipStream_t>(stream), out + t.offset,
+                       static_cast<const double*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim);
+    TLH_RETURN_IF_LAUNCH_FAILED();
+  }
+  return 0;
+}
+
+extern "C" int tlh_squared_matvec_norm(int64_t count, int rows, int nslots, const int32_t* dims,
+                                       const int64_t* offsets, const int64_t* const* index,
+                                       const void* const* blocks, int block_dtype, const void* x, int x_dtype,
+                                       double* out2, double* partials, uint32_t* counter, tlh_stream_t stream) {
+  if (nslots < 1 || nslots > TLH_MAX_SLOTS || rows < 1 || rows > 16 || !out2 || !partials || !counter)
+    return TLH_EINVAL;
+  RawGroup g{};
+  g.count = count;
+  g.rows = rows;
+  g.nslots = nslots;
+  for (int s = 0; s < nslots; ++s) {
+    g.dims[s] = dims[s];
+    g.offsets[s] = offsets[s];
+    g.index[s] = index[s];
+    g.blocks[s] = blocks[s];
+  }
+  hipLaunchKernelGGL(sqnorm_kernel, dim3(red_grid(count)), dim3(TLH_THREADS), 0,
+                     reinterpret_cast<hipStream_t>(stream), g, block_dtype, x, x_dtype, out2, partials, counter);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+extern "C" int tlh_trial_objective(const void* before, int before_dtype, const void* after, int after_dtype,
+                                   int64_t count, int rows, int loss, double delta, const double* weights,
+                                   int weight_stride, double* out2, double* partials, uint32_t* counter,
+                                   tlh_stream_t stream) {
+  if (!after || !out2 || !partials || !counter || rows < 1 || loss < 0 || loss > 1) return TLH_EINVAL;
+  hipLaunchKernelGGL(trial_kernel, dim3(red_grid(count)), dim3(TLH_THREADS), 0,
+                     reinterpret_cast<hipStream_t>(stream), before, before_dtype, after, after_dtype, count, rows,
+                     loss, delta, weights, weight_stride, out2, partials, counter);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+extern "C" int tlh_block_matvec(int64_t count, int rows, int nslots, const int32_t* dims, const int64_t* offsets,
+                                const int64_t* const* index, const void* const* blocks, int block_dtype,
+                                const void* x, int x_dtype, void* y, int y_dtype, tlh_stream_t stream) {
+  if (nslots < 1 || nslots > TLH_MAX_SLOTS || rows < 1 || rows > 16 || !y || !x) return TLH_EINVAL;
+  if (count == 0) return 0;
+  RawGroup g{};
+  g.count = count;
+  g.rows = rows;
+  g.nslots = nslots;
+  for (int s = 0; s < nslots; ++s) {
+    g.dims[s] = dims[s];
+    g.offsets[s] = offsets[s];
+    g.index[s] = index[s];
+    g.blocks[s] = blocks[s];
+  }
+  hipLaunchKernelGGL(block_matvec_kernel, dim3((unsigned)ceil_div(count, TLH_THREADS)), dim3(TLH_THREADS), 0,
+                     reinterpret_cast<hipStream_t>(stream), g, block_dtype, x, x_dtype, y, y_dtype);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}

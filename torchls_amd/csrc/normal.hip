@@ -275,6 +275,40 @@ __global__ void __launch_bounds__(TLH_THREADS)
   y[i] = sum + damping * x[i];
 }
 
+// Finish for types with many dense blocks: a workgroup owns 32 consecutive
+// columns; its 8 column lanes each sum every 8th dense block (fixed stride),
+// then the 8 lane sums are added in lane order.  Same result for every run.
+constexpr int FIN_COLS = 32;
+constexpr int FIN_LANES = TLH_THREADS / FIN_COLS;
+
+template <typename Real>
+__global__ void __launch_bounds__(TLH_THREADS)
+    finish_dense_kernel(Real* __restrict__ y, const Real* __restrict__ x, Real damping,
+                        const Real* __restrict__ partials, const int32_t* __restrict__ csr_ptr,
+                        const int32_t* __restrict__ csr_idx, int64_t count, int dim, int dense_blocks) {
+  __shared__ Real lane_sum[FIN_LANES][FIN_COLS];
+  const int col = threadIdx.x % FIN_COLS;
+  const int lane = threadIdx.x / FIN_COLS;
+  const int64_t total = count * dim;
+  const int64_t i = (int64_t)blockIdx.x * FIN_COLS + col;
+  Real sum = Real(0);
+  if (i < total) {
+    for (int b = lane; b < dense_blocks; b += FIN_LANES) sum += partials[(int64_t)b * total + i];
+  }
+  lane_sum[lane][col] = sum;
+  __syncthreads();
+  if (lane == 0 && i < total) {
+    Real acc = lane_sum[0][col];
+#pragma unroll
+    for (int l = 1; l < FIN_LANES; ++l) acc += lane_sum[l][col];
+    const int64_t v = i / dim;
+    const int c = (int)(i - v * dim);
+    const int k1 = csr_ptr[v + 1];
+    for (int k = csr_ptr[v]; k < k1; ++k) acc += partials[(int64_t)csr_idx[k] * dim + c];
+    y[i] = acc + damping * x[i];
+  }
+}
+
 template <typename In, typename Out>
 __global__ void __launch_bounds__(TLH_THREADS)
     pack_kernel(Out* __restrict__ packed, int64_t stride, const int64_t* __restrict__ order,
@@ -397,6 +431,10 @@ int launch_group(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
 }
 
 template <typename Real>
+int finish_impl(const tlh_operator_t* op, int t0, int t1, const void* x, double damping, void* y,
+                hipStream_t stream);
+
+template <typename Real>
 int normal_matvec_impl(const tlh_operator_t* op, const void* x, void* y, double damping,
                        hipStream_t stream) {
   for (int gi = 0; gi < op->ngroups; ++gi) {
@@ -405,15 +443,28 @@ int normal_matvec_impl(const tlh_operator_t* op, const void* x, void* y, double 
     const int rc = launch_group<Real>(op, g, x, stream);
     if (rc) return rc;
   }
-  for (int ti = 0; ti < op->ntypes; ++ti) {
+  return finish_impl<Real>(op, 0, op->ntypes, x, damping, y, stream);
+}
+
+template <typename Real>
+int finish_impl(const tlh_operator_t* op, int t0, int t1, const void* x, double damping, void* y,
+                hipStream_t stream) {
+  for (int ti = t0; ti < t1; ++ti) {
     const tlh_type_t& t = op->types[ti];
     const int64_t total = t.count * t.dim;
     if (total == 0) continue;
-    hipLaunchKernelGGL(finish_kernel<Real>, dim3((unsigned)ceil_div(total, TLH_THREADS)),
-                       dim3(TLH_THREADS), 0, stream, static_cast<Real*>(y) + t.offset,
-                       static_cast<const Real*>(x) + t.offset, (Real)damping,
-                       static_cast<const Real*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim,
-                       t.dense_blocks);
+    if (t.dense_blocks >= 2 * FIN_LANES)
+      hipLaunchKernelGGL(finish_dense_kernel<Real>, dim3((unsigned)ceil_div(total, FIN_COLS)),
+                         dim3(TLH_THREADS), 0, stream, static_cast<Real*>(y) + t.offset,
+                         static_cast<const Real*>(x) + t.offset, (Real)damping,
+                         static_cast<const Real*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim,
+                         t.dense_blocks);
+    else
+      hipLaunchKernelGGL(finish_kernel<Real>, dim3((unsigned)ceil_div(total, TLH_THREADS)),
+                         dim3(TLH_THREADS), 0, stream, static_cast<Real*>(y) + t.offset,
+                         static_cast<const Real*>(x) + t.offset, (Real)damping,
+                         static_cast<const Real*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim,
+                         t.dense_blocks);
     TLH_RETURN_IF_LAUNCH_FAILED();
   }
   return 0;
@@ -498,4 +549,23 @@ extern "C" int tlh_normal_matvec(const tlh_operator_t* op, const void* x, void* 
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (op->dtype == TLH_F32) return normal_matvec_impl<float>(op, x, y, damping, s);
   return normal_matvec_impl<double>(op, x, y, damping, s);
+}
+
+extern "C" int tlh_finish_types(const tlh_operator_t* op, int type_begin, int type_end, const void* base,
+                                double base_scale, void* y, tlh_stream_t stream) {
+  if (int rc = check_op(op)) return rc;
+  if (type_begin < 0 || type_end > op->ntypes || type_begin > type_end || !base || !y) return TLH_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (op->dtype == TLH_F32) return finish_impl<float>(op, type_begin, type_end, base, base_scale, y, s);
+  return finish_impl<double>(op, type_begin, type_end, base, base_scale, y, s);
+}
+
+extern "C" int tlh_group_tiles(const tlh_operator_t* op, int group, const void* x, tlh_stream_t stream) {
+  if (int rc = check_op(op)) return rc;
+  if (group < 0 || group >= op->ngroups || !x) return TLH_EINVAL;
+  const tlh_group_t& g = op->groups[group];
+  if (g.count == 0 || g.ntiles == 0) return 0;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (op->dtype == TLH_F32) return launch_group<float>(op, g, x, s);
+  return launch_group<double>(op, g, x, s);
 }

@@ -1,0 +1,461 @@
+// Device-resident PCG with float64 scalars and a fixed-order reduction.
+//
+// Replaces warp_pcg.py (_step 33-50, _direction 53-67, _finish 70-92) and the
+// three torch.dot calls per iteration (warp_pcg.py:139,147-148), plus the
+// block-Jacobi factorisation/apply of solvers.py:259-285 and
+// preconditioning.py:12-17 (closed-form float64 Cholesky inverse per block).
+//
+// State lives on the device (control = {iterations, status, stop_at};
+// scalars = {rz, thr^2, curvature, new_rz, ||r||^2, ...}) so one iteration can
+// be captured in a hipGraph and replayed; the host reads control only between
+// replay batches, like warp_pcg._run.  Vectors are float32 or float64; every
+// dot product accumulates in float64, and alpha/beta are float64.
+//
+// Reductions: each block sums its grid-stride slice in a fixed order, writes a
+// partial, and the last block to finish (ticket counter) adds the partials in
+// block order -- deterministic, no float atomics.
+#include <math.h>
+
+#include "common.hpp"
+
+using namespace tlh;
+
+namespace {
+
+constexpr int MAX_BLOCKS = TLH_PCG_MAX_BLOCKS;
+
+__device__ __forceinline__ bool finite(double v) { return isfinite(v); }
+
+__device__ __forceinline__ bool frozen(const int32_t* control) {
+  return control[1] != 0 || control[0] >= control[2];
+}
+
+// Block-wide fixed-order tree sum of NV values; result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* smem) {
+  for (int k = 0; k < NV; ++k) smem[k * TLH_THREADS + threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = TLH_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < NV; ++k) smem[k * TLH_THREADS + threadIdx.x] += smem[k * TLH_THREADS + threadIdx.x + w];
+    __syncthreads();
+  }
+  for (int k = 0; k < NV; ++k) v[k] = smem[k * TLH_THREADS];
+  __syncthreads();
+}
+
+// Publish this block's NV sums; the last block returns true with the grid totals in v.
+template <int NV>
+__device__ __forceinline__ bool grid_sum(double (&v)[NV], double* partials, uint32_t* counter,
+                                         double* smem) {
+  __shared__ int last;
+  block_sum<NV>(v, smem);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < NV; ++k) partials[k * MAX_BLOCKS + blockIdx.x] = v[k];
+    const uint32_t ticket =
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = ticket == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return false;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  double acc[NV];
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += TLH_THREADS)
+    for (int k = 0; k < NV; ++k)
+      acc[k] += __hip_atomic_load(&partials[k * MAX_BLOCKS + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  block_sum<NV>(acc, smem);
+  for (int k = 0; k < NV; ++k) v[k] = acc[k];
+  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+inline int grid_for(int64_t work) {
+  const int64_t g = ceil_div(work, TLH_THREADS);
+  return (int)(g < 1 ? 1 : (g > MAX_BLOCKS ? MAX_BLOCKS : g));
+}
+
+// ---------------------------------------------------------------- block inverse
+template <int D>
+__global__ void __launch_bounds__(TLH_THREADS)
+    block_inverse_kernel(const double* __restrict__ blocks, int64_t count, double damping,
+                         double* __restrict__ inverse, int32_t* __restrict__ status) {
+  const int64_t v = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
+  if (v >= count) return;
+  const double* A = blocks + v * D * D;
+  double L[D][D];
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    double s = A[j * D + j] + damping;
+#pragma unroll
+    for (int k = 0; k < j; ++k) s -= L[j][k] * L[j][k];
+    if (!(s > 0.0) || !finite(s)) ok = false;
+    const double d = sqrt(s > 0.0 ? s : 1.0);
+    L[j][j] = d;
+    const double inv_d = 1.0 / d;
+#pragma unroll
+    for (int i = j + 1; i < D; ++i) {
+      double t = A[i * D + j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k];
+      L[i][j] = t * inv_d;
+    }
+  }
+  // M = L^-1 (lower), then inverse = M^T M.
+  double M[D][D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) M[i][j] = 0.0;
+    M[j][j] = 1.0 / L[j][j];
+#pragma unroll
+    for (int i = j + 1; i < D; ++i) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = j; k < i; ++k) t -= L[i][k] * M[k][j];
+      M[i][j] = t / L[i][i];
+    }
+  }
+  double* out = inverse + v * (D * (D + 1) / 2);
+  int e = 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+#pragma unroll
+    for (int j = i; j < D; ++j) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = j; k < D; ++k) t += M[k][i] * M[k][j];
+      out[e++] = t;
+    }
+  }
+  if (!ok) __hip_atomic_fetch_or(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// z_block = inverse_block * r_block (float64 arithmetic), symmetric packed upper.
+template <typename Real>
+__device__ __forceinline__ void apply_block(const double* inv, int d, const Real* r, Real* z, double& rz,
+                                            double& rr) {
+  double rv[16];
+  for (int i = 0; i < d; ++i) rv[i] = (double)r[i];
+  for (int i = 0; i < d; ++i) {
+    double t = 0.0;
+    for (int j = 0; j < d; ++j) {
+      const int a = i < j ? i : j, b = i < j ? j : i;
+      t += inv[a * d - a * (a - 1) / 2 + (b - a)] * rv[j];
+    }
+    const Real zi = (Real)t;
+    z[i] = zi;
+    rz += rv[i] * (double)zi;
+    rr += rv[i] * rv[i];
+  }
+}
+
+__device__ __forceinline__ int find_type(const tlh_block_diag_t& pc, int64_t b, int64_t& local) {
+  int t = 0;
+  int64_t start = 0;
+  while (t + 1 < pc.ntypes && b >= start + pc.count[t]) {
+    start += pc.count[t];
+    ++t;
+  }
+  local = b - start;
+  return t;
+}
+
+// ---------------------------------------------------------------- PCG kernels
+template <typename Real>
+__global__ void __launch_bounds__(TLH_THREADS)
+    dot_kernel(const Real* __restrict__ a, const Real* __restrict__ b, int64_t n, double* scalars, int slot,
+               double* partials, uint32_t* counter) {
+  __shared__ double smem[TLH_THREADS];
+  double v[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * TLH_THREADS)
+    v[0] += (double)a[i] * (double)b[i];
+  if (grid_sum<1>(v, partials, counter, smem) && threadIdx.x == 0) scalars[slot] = v[0];
+}
+
+// Step (x += alpha p, r -= alpha Ap; warp_pcg._step), then z = M^-1 r and the
+// float64 sums r.z and r.r (warp_pcg.py:146-148), fused.  mode:
+// 0 = block-Jacobi (pc), 1 = identity (z = r), 2 = step only (caller applies M).
+template <typename Real>
+__global__ void __launch_bounds__(TLH_THREADS)
+    step_kernel(tlh_block_diag_t pc, int mode, Real* __restrict__ x, Real* __restrict__ r, Real* __restrict__ z,
+                const Real* __restrict__ p, const Real* __restrict__ ap, int64_t n, const int32_t* control,
+                double* scalars, double* partials, uint32_t* counter) {
+  __shared__ double smem[2 * TLH_THREADS];
+  const double c = scalars[TLH_PCG_CURVATURE];
+  const double rz = scalars[TLH_PCG_RZ];
+  const bool move = !frozen(control) && finite(c) && finite(rz) && c > 0.0 && rz > 0.0;
+  const double alpha = move ? rz / c : 0.0;
+  double v[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * TLH_THREADS;
+  if (mode == 0) {
+    int64_t blocks = 0;
+    for (int t = 0; t < pc.ntypes; ++t) blocks += pc.count[t];
+    for (int64_t bidx = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; bidx < blocks; bidx += stride) {
+      int64_t local;
+      const int t = find_type(pc, bidx, local);
+      const int d = pc.dim[t];
+      const int64_t e0 = pc.offset[t] + local * d;
+      if (move) {
+        for (int k = 0; k < d; ++k) {
+          x[e0 + k] = (Real)((double)x[e0 + k] + alpha * (double)p[e0 + k]);
+          r[e0 + k] = (Real)((double)r[e0 + k] - alpha * (double)ap[e0 + k]);
+        }
+      }
+      apply_block<Real>(pc.inv[t] + local * (d * (d + 1) / 2), d, r + e0, z + e0, v[0], v[1]);
+    }
+  } else {
+    for (int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; i < n; i += stride) {
+      if (move) {
+        x[i] = (Real)((double)x[i] + alpha * (double)p[i]);
+        r[i] = (Real)((double)r[i] - alpha * (double)ap[i]);
+      }
+      if (mode == 1) {
+        const double ri = (double)r[i];
+        z[i] = r[i];
+        v[0] += ri * ri;
+        v[1] += ri * ri;
+      }
+    }
+  }
+  if (mode == 2) return;
+  if (grid_sum<2>(v, partials, counter, smem) && threadIdx.x == 0) {
+    scalars[TLH_PCG_NEW_RZ] = v[0];
+    scalars[TLH_PCG_NORM2] = v[1];
+  }
+}
+
+// p = z + beta p (warp_pcg._direction); the last block then applies the
+// status machine of warp_pcg._finish (70-92).
+template <typename Real>
+__global__ void __launch_bounds__(TLH_THREADS)
+    direction_kernel(Real* __restrict__ p, const Real* __restrict__ z, int64_t n, int32_t* control,
+                     double* scalars, double* partials, uint32_t* counter) {
+  __shared__ double smem[TLH_THREADS];
+  const bool stopped = frozen(control);
+  const double c = scalars[TLH_PCG_CURVATURE];
+  const double rz = scalars[TLH_PCG_RZ];
+  const double nrz = scalars[TLH_PCG_NEW_RZ];
+  const bool move = !stopped && finite(c) && finite(rz) && finite(nrz) && c > 0.0 && rz > 0.0;
+  const double beta = move ? nrz / rz : 0.0;
+  if (move) {
+    for (int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * TLH_THREADS)
+      p[i] = (Real)((double)z[i] + beta * (double)p[i]);
+  }
+  double v[1] = {0.0};
+  if (grid_sum<1>(v, partials, counter, smem) && threadIdx.x == 0 && !stopped) {
+    const double norm2 = scalars[TLH_PCG_NORM2];
+    control[0] += 1;
+    if (!finite(c) || !finite(rz) || !finite(norm2))
+      control[1] = 2;
+    else if (c <= 0.0 || rz <= 0.0)
+      control[1] = 3;
+    else if (norm2 <= scalars[TLH_PCG_THR2])
+      control[1] = 1;
+    else if (!finite(nrz))
+      control[1] = 2;
+    scalars[TLH_PCG_RZ] = nrz;
+  }
+}
+
+// r = src; optionally x = 0; z = M^-1 r; p = z; rz = r.z (warp_pcg.py:130-132, 169-172, 210-212).
+template <typename Real>
+__global__ void __launch_bounds__(TLH_THREADS)
+    reset_kernel(tlh_block_diag_t pc, int identity, const Real* __restrict__ src, Real* __restrict__ x,
+                 Real* __restrict__ r, Real* __restrict__ z, Real* __restrict__ p, int64_t n, double* scalars,
+                 double* partials, uint32_t* counter) {
+  __shared__ double smem[2 * TLH_THREADS];
+  double v[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * TLH_THREADS;
+  if (!identity) {
+    int64_t blocks = 0;
+    for (int t = 0; t < pc.ntypes; ++t) blocks += pc.count[t];
+    for (int64_t bidx = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; bidx < blocks; bidx += stride) {
+      int64_t local;
+      const int t = find_type(pc, bidx, local);
+      const int d = pc.dim[t];
+      const int64_t e0 = pc.offset[t] + local * d;
+      for (int k = 0; k < d; ++k) {
+        r[e0 + k] = src[e0 + k];
+        if (x) x[e0 + k] = Real(0);
+      }
+      apply_block<Real>(pc.inv[t] + local * (d * (d + 1) / 2), d, r + e0, z + e0, v[0], v[1]);
+      for (int k = 0; k < d; ++k) p[e0 + k] = z[e0 + k];
+    }
+  } else {
+    for (int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; i < n; i += stride) {
+      const Real ri = src[i];
+      r[i] = ri;
+      z[i] = ri;
+      p[i] = ri;
+      if (x) x[i] = Real(0);
+      v[0] += (double)ri * (double)ri;
+    }
+  }
+  if (grid_sum<2>(v, partials, counter, smem) && threadIdx.x == 0) scalars[TLH_PCG_RZ] = v[0];
+}
+
+// true_r = rhs - Ax; ||true_r||^2 and ||true_r - r||^2 (warp_pcg.py:189-204).
+template <typename Real>
+__global__ void __launch_bounds__(TLH_THREADS)
+    true_residual_kernel(const Real* __restrict__ rhs, const Real* __restrict__ ax, const Real* __restrict__ r,
+                         Real* __restrict__ true_r, int64_t n, double* scalars, double* partials,
+                         uint32_t* counter) {
+  __shared__ double smem[2 * TLH_THREADS];
+  double v[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * TLH_THREADS) {
+    const Real t = rhs[i] - ax[i];
+    true_r[i] = t;
+    const double dt = (double)t, dd = (double)t - (double)r[i];
+    v[0] += dt * dt;
+    v[1] += dd * dd;
+  }
+  if (grid_sum<2>(v, partials, counter, smem) && threadIdx.x == 0) {
+    scalars[TLH_PCG_TRUE2] = v[0];
+    scalars[TLH_PCG_DRIFT2] = v[1];
+  }
+}
+
+bool state_ok(const tlh_pcg_state_t* st) {
+  return st && st->control && st->scalars && st->partials && st->counter;
+}
+
+int64_t precond_blocks(const tlh_block_diag_t* pc) {
+  int64_t b = 0;
+  for (int t = 0; t < pc->ntypes; ++t) b += pc->count[t];
+  return b;
+}
+
+template <typename Real>
+int step_impl(const tlh_block_diag_t* pc, int mode, void* x, void* r, void* z, const void* p, const void* ap,
+              int64_t n, const tlh_pcg_state_t* st, hipStream_t s) {
+  tlh_block_diag_t empty{};
+  const int64_t work = mode == 0 ? precond_blocks(pc) : n;
+  hipLaunchKernelGGL(step_kernel<Real>, dim3(grid_for(work)), dim3(TLH_THREADS), 0, s, mode == 0 ? *pc : empty,
+                     mode, (Real*)x, (Real*)r, (Real*)z, (const Real*)p, (const Real*)ap, n, st->control,
+                     st->scalars, st->partials, st->counter);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+template <typename Real>
+int reset_impl(const tlh_block_diag_t* pc, const void* src, void* x, void* r, void* z, void* p, int64_t n,
+               const tlh_pcg_state_t* st, hipStream_t s) {
+  tlh_block_diag_t empty{};
+  const int64_t work = pc ? precond_blocks(pc) : n;
+  hipLaunchKernelGGL(reset_kernel<Real>, dim3(grid_for(work)), dim3(TLH_THREADS), 0, s, pc ? *pc : empty,
+                     pc ? 0 : 1, (const Real*)src, (Real*)x, (Real*)r, (Real*)z, (Real*)p, n, st->scalars,
+                     st->partials, st->counter);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+bool precond_ok(const tlh_block_diag_t* pc, int64_t n) {
+  if (!pc) return true;
+  if (pc->ntypes < 1 || pc->ntypes > TLH_MAX_TYPES) return false;
+  int64_t total = 0;
+  for (int t = 0; t < pc->ntypes; ++t) {
+    if (pc->dim[t] < 1 || pc->dim[t] > 16 || !pc->inv[t]) return false;
+    total += pc->count[t] * pc->dim[t];
+  }
+  return total == n;
+}
+
+}  // namespace
+
+extern "C" int tlh_block_inverse(const double* blocks, int64_t count, int dim, double damping, double* inverse,
+                                 int32_t* status, tlh_stream_t stream) {
+  if (count < 0 || !status || (count && (!blocks || !inverse))) return TLH_EINVAL;
+  if (count == 0) return 0;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const dim3 grid((unsigned)ceil_div(count, TLH_THREADS)), block(TLH_THREADS);
+  switch (dim) {
+#define TLH_INV_CASE(D)                                                                               \
+  case D:                                                                                             \
+    hipLaunchKernelGGL(block_inverse_kernel<D>, grid, block, 0, s, blocks, count, damping, inverse, status); \
+    break;
+    TLH_INV_CASE(1)
+    TLH_INV_CASE(2)
+    TLH_INV_CASE(3)
+    TLH_INV_CASE(4)
+    TLH_INV_CASE(5)
+    TLH_INV_CASE(6)
+    TLH_INV_CASE(7)
+    TLH_INV_CASE(8)
+#undef TLH_INV_CASE
+    default:
+      return TLH_ENOTFOUND;
+  }
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+extern "C" int tlh_dot(const void* a, const void* b, int64_t n, int dtype, const tlh_pcg_state_t* st, int slot,
+                       tlh_stream_t stream) {
+  if (!state_ok(st) || slot < 0 || slot >= TLH_PCG_NSCALARS || n < 0) return TLH_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const dim3 grid(grid_for(n)), block(TLH_THREADS);
+  if (dtype == TLH_F32)
+    hipLaunchKernelGGL(dot_kernel<float>, grid, block, 0, s, (const float*)a, (const float*)b, n, st->scalars,
+                       slot, st->partials, st->counter);
+  else if (dtype == TLH_F64)
+    hipLaunchKernelGGL(dot_kernel<double>, grid, block, 0, s, (const double*)a, (const double*)b, n, st->scalars,
+                       slot, st->partials, st->counter);
+  else
+    return TLH_EINVAL;
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+extern "C" int tlh_pcg_reset(const tlh_block_diag_t* pc, const void* src, void* x, void* r, void* z, void* p,
+                             int64_t n, int dtype, const tlh_pcg_state_t* st, tlh_stream_t stream) {
+  if (!state_ok(st) || !precond_ok(pc, n) || !src || !r || !z || !p) return TLH_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == TLH_F32) return reset_impl<float>(pc, src, x, r, z, p, n, st, s);
+  if (dtype == TLH_F64) return reset_impl<double>(pc, src, x, r, z, p, n, st, s);
+  return TLH_EINVAL;
+}
+
+extern "C" int tlh_pcg_step(const tlh_block_diag_t* pc, int mode, void* x, void* r, void* z, const void* p,
+                            const void* ap, int64_t n, int dtype, const tlh_pcg_state_t* st, tlh_stream_t stream) {
+  if (!state_ok(st) || mode < 0 || mode > 2 || (mode == 0 && (!pc || !precond_ok(pc, n)))) return TLH_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == TLH_F32) return step_impl<float>(pc, mode, x, r, z, p, ap, n, st, s);
+  if (dtype == TLH_F64) return step_impl<double>(pc, mode, x, r, z, p, ap, n, st, s);
+  return TLH_EINVAL;
+}
+
+extern "C" int tlh_pcg_direction(void* p, const void* z, int64_t n, int dtype, const tlh_pcg_state_t* st,
+                                 tlh_stream_t stream) {
+  if (!state_ok(st)) return TLH_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const dim3 grid(grid_for(n)), block(TLH_THREADS);
+  if (dtype == TLH_F32)
+    hipLaunchKernelGGL(direction_kernel<float>, grid, block, 0, s, (float*)p, (const float*)z, n, st->control,
+                       st->scalars, st->partials, st->counter);
+  else if (dtype == TLH_F64)
+    hipLaunchKernelGGL(direction_kernel<double>, grid, block, 0, s, (double*)p, (const double*)z, n, st->control,
+                       st->scalars, st->partials, st->counter);
+  else
+    return TLH_EINVAL;
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+extern "C" int tlh_pcg_true_residual(const void* rhs, const void* ax, const void* r, void* true_r, int64_t n,
+                                     int dtype, const tlh_pcg_state_t* st, tlh_stream_t stream) {
+  if (!state_ok(st)) return TLH_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const dim3 grid(grid_for(n)), block(TLH_THREADS);
+  if (dtype == TLH_F32)
+    hipLaunchKernelGGL(true_residual_kernel<float>, grid, block, 0, s, (const float*)rhs, (const float*)ax,
+                       (const float*)r, (float*)true_r, n, st->scalars, st->partials, st->counter);
+  else if (dtype == TLH_F64)
+    hipLaunchKernelGGL(true_residual_kernel<double>, grid, block, 0, s, (const double*)rhs, (const double*)ax,
+                       (const double*)r, (double*)true_r, n, st->scalars, st->partials, st->counter);
+  else
+    return TLH_EINVAL;
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}

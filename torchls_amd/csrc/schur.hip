@@ -1,0 +1,561 @@
+// Implicit Schur complement of one independent variable type (points) on gfx950.
+//
+// Replaces DampedSchurSystem (schur.py:263-397) without storing the cross
+// blocks W or any CSR matrix.  With e the eliminated type, C_p = D_p + lambda I
+// and its float64 inverse precomputed per variable:
+//
+//   matvec   S x  = J_c^T J_c x + lambda x - W C^-1 W^T x            (schur.py:303-328)
+//   rhs      b_c' = b_c - W C^-1 b_p                                 (schur.py:285-290)
+//   backsub  x_p  = C^-1 (b_p - W^T x_c)                             (schur.py:292-301)
+//   precond  M_c  = D_c + lambda I - sum_edges W_e C_p^-1 W_e^T      (schur.py:353-378)
+//   dense    M    = blockdiag(D_c + lambda I) - sum_p W_p C_p^-1 W_p^T (schur.py:380-397)
+//
+// The W-group (the residual group holding a retained slot and the eliminated
+// slot, e.g. reprojection) is sorted by the eliminated variable with every
+// variable's observations inside one tile, so one pass over its Jacobian
+// evaluates W C^-1 W^T x: per observation u = J_c x_c; per point (one
+// segment owner) t_p = sum J_p^T u and s_p = C_p^-1 t_p in float64; per
+// observation q = u - J_p s_p and the camera contribution J_c^T q, reduced
+// deterministically exactly like the normal matvec (DENSE LDS accumulator or
+// SPARSE partials + finish).  Groups holding only retained slots are added by
+// the normal tile kernels of csrc/normal.hip.
+#include <algorithm>
+
+#include "common.hpp"
+
+using namespace tlh;
+
+namespace {
+
+enum SchurMode { SCHUR_MATVEC = 0, SCHUR_RHS = 1, SCHUR_BACKSUB = 2 };
+
+template <typename Real>
+struct SchurArgs {
+  const Real* packed;
+  int64_t stride;
+  const int32_t* tile_start;
+  int32_t ntiles;
+  int32_t tiles_per_wg;
+  const int32_t* idx_c;
+  const int32_t* idx_p;
+  const uint16_t* rank_c;
+  const int32_t* seg_ptr_c;
+  const int32_t* seg_start_c;
+  const int32_t* seg_target_c;
+  int32_t mode_c;
+  int32_t dense_block_c;
+  int32_t count_c;
+  Real* part_c;
+  const int32_t* seg_ptr_p;
+  const int32_t* seg_start_p;
+  const Real* xc;      // retained input (x_c), offset to the retained type
+  const Real* bp;      // eliminated right-hand side, offset to the eliminated type
+  Real* outp;          // back-substituted eliminated update, offset to the eliminated type
+  const double* cinv;  // [count_p][DP*(DP+1)/2] upper triangle of (D_p + lambda I)^-1
+};
+
+template <int D>
+__device__ __forceinline__ double sym(const double* m, int i, int j) {
+  const int a = i < j ? i : j, b = i < j ? j : i;
+  return m[a * D - a * (a - 1) / 2 + (b - a)];
+}
+
+template <typename Real, int R, int DC, int DP, bool CFIRST, int ITEMS, int MODE>
+__global__ void __launch_bounds__(TLH_THREADS) schur_tile_kernel(SchurArgs<Real> a) {
+  constexpr int OC = CFIRST ? 0 : R * DP;
+  constexpr int OP = CFIRST ? R * DC : 0;
+  constexpr int T = TLH_THREADS * ITEMS;
+  constexpr int DMAX = DC > DP ? DC : DP;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Real* stage = reinterpret_cast<Real*>(smem_raw);
+  Real* sstage = stage + T * DMAX;
+  Real* acc = sstage + T * DP;
+  const bool dense = MODE != SCHUR_BACKSUB && a.mode_c == TLH_REDUCE_DENSE;
+  if (dense)
+    for (int i = threadIdx.x; i < a.count_c * DC; i += TLH_THREADS) acc[i] = Real(0);
+
+  const int t0 = blockIdx.x * a.tiles_per_wg;
+  const int t1 = min(t0 + a.tiles_per_wg, a.ntiles);
+  for (int t = t0; t < t1; ++t) {
+    const int start = a.tile_start[t];
+    const int len = a.tile_start[t + 1] - start;
+    Real jc[ITEMS][R][DC], jp[ITEMS][R][DP], u[ITEMS][R];
+    int rk[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+      const int pos = (int)threadIdx.x + it * TLH_THREADS;
+      const int64_t n = start + min(pos, len - 1);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+#pragma unroll
+        for (int c = 0; c < DC; ++c) jc[it][r][c] = a.packed[(int64_t)(OC + r * DC + c) * a.stride + n];
+#pragma unroll
+        for (int c = 0; c < DP; ++c) jp[it][r][c] = a.packed[(int64_t)(OP + r * DP + c) * a.stride + n];
+      }
+      rk[it] = a.rank_c ? (int)a.rank_c[n] : min(pos, len - 1);
+      if (MODE != SCHUR_RHS) {
+        const Real* x = a.xc + (int64_t)a.idx_c[n] * DC;
+        Real xv[DC];
+#pragma unroll
+        for (int c = 0; c < DC; ++c) xv[c] = x[c];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          Real s = Real(0);
+#pragma unroll
+          for (int c = 0; c < DC; ++c) s += jc[it][r][c] * xv[c];
+          u[it][r] = s;
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) u[it][r] = Real(0);
+      }
+    }
+    // point contributions J_p^T u (identity order: the tile is sorted by point)
+    __syncthreads();
+    if (MODE != SCHUR_RHS) {
+#pragma unroll
+      for (int it = 0; it < ITEMS; ++it) {
+        const int pos = (int)threadIdx.x + it * TLH_THREADS;
+        if (pos < len) {
+#pragma unroll
+          for (int c = 0; c < DP; ++c) {
+            Real s = Real(0);
+#pragma unroll
+            for (int r = 0; r < R; ++r) s += jp[it][r][c] * u[it][r];
+            stage[pos * DP + c] = s;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // one owner per point: s_p = C_p^-1 (...) in float64
+    {
+      const int sb = a.seg_ptr_p[t], se = a.seg_ptr_p[t + 1];
+      for (int k = sb + (int)threadIdx.x; k < se; k += TLH_THREADS) {
+        const int lo = a.seg_start_p[k];
+        const int hi = (k + 1 < se) ? a.seg_start_p[k + 1] : len;
+        const int64_t v = a.idx_p[start + lo];
+        double tp[DP];
+#pragma unroll
+        for (int c = 0; c < DP; ++c) tp[c] = 0.0;
+        if (MODE != SCHUR_RHS) {
+          for (int i = lo; i < hi; ++i)
+#pragma unroll
+            for (int c = 0; c < DP; ++c) tp[c] += (double)stage[i * DP + c];
+        }
+        if (MODE != SCHUR_MATVEC) {
+#pragma unroll
+          for (int c = 0; c < DP; ++c) tp[c] = (double)a.bp[v * DP + c] - tp[c];
+          // RHS: tp = b_p; BACKSUB: tp = b_p - W^T x
+        }
+        const double* m = a.cinv + v * (DP * (DP + 1) / 2);
+        double sp[DP];
+#pragma unroll
+        for (int i = 0; i < DP; ++i) {
+          double s = 0.0;
+#pragma unroll
+          for (int j = 0; j < DP; ++j) s += sym<DP>(m, i, j) * tp[j];
+          sp[i] = s;
+        }
+        if (MODE == SCHUR_BACKSUB) {
+#pragma unroll
+          for (int c = 0; c < DP; ++c) a.outp[v * DP + c] = (Real)sp[c];
+        } else {
+          for (int i = lo; i < hi; ++i)
+#pragma unroll
+            for (int c = 0; c < DP; ++c) sstage[i * DP + c] = (Real)sp[c];
+        }
+      }
+    }
+    if (MODE == SCHUR_BACKSUB) continue;
+    __syncthreads();
+    Real cc[ITEMS][DC];
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+      const int pos = min((int)threadIdx.x + it * TLH_THREADS, len - 1);
+      Real sv[DP];
+#pragma unroll
+      for (int c = 0; c < DP; ++c) sv[c] = sstage[pos * DP + c];
+      Real q[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        Real s = u[it][r];
+#pragma unroll
+        for (int c = 0; c < DP; ++c) s -= jp[it][r][c] * sv[c];
+        q[r] = s;
+      }
+#pragma unroll
+      for (int c = 0; c < DC; ++c) {
+        Real s = Real(0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) s += jc[it][r][c] * q[r];
+        cc[it][c] = s;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+      const int pos = (int)threadIdx.x + it * TLH_THREADS;
+      if (pos < len) {
+#pragma unroll
+        for (int c = 0; c < DC; ++c) stage[rk[it] * DC + c] = cc[it][c];
+      }
+    }
+    __syncthreads();
+    const int sb = a.seg_ptr_c[t], se = a.seg_ptr_c[t + 1];
+    for (int k = sb + (int)threadIdx.x; k < se; k += TLH_THREADS) {
+      const int lo = a.seg_start_c[k];
+      const int hi = (k + 1 < se) ? a.seg_start_c[k + 1] : len;
+      Real sum[DC];
+#pragma unroll
+      for (int c = 0; c < DC; ++c) sum[c] = stage[lo * DC + c];
+      for (int i = lo + 1; i < hi; ++i)
+#pragma unroll
+        for (int c = 0; c < DC; ++c) sum[c] += stage[i * DC + c];
+      const int target = a.seg_target_c[k];
+      if (dense) {
+#pragma unroll
+        for (int c = 0; c < DC; ++c) acc[target * DC + c] += sum[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < DC; ++c) a.part_c[(int64_t)target * DC + c] = sum[c];
+      }
+    }
+  }
+  if (dense) {
+    __syncthreads();
+    const int64_t nn = (int64_t)a.count_c * DC;
+    Real* out = a.part_c + (int64_t)(a.dense_block_c + blockIdx.x) * nn;
+    for (int64_t i = threadIdx.x; i < nn; i += TLH_THREADS) out[i] = acc[i];
+  }
+}
+
+// ------------------------------------------------ preconditioner edge blocks
+// Per tile (all-SPARSE plan): each (camera, point) edge head forms
+// W_e = sum_{obs in edge} J_c^T J_p (float64), then Q = W_e C_p^-1 W_e^T
+// (upper triangle); values are staged by camera rank and summed per segment
+// into float64 partials [P][DC*(DC+1)/2].
+template <typename Real>
+__global__ void __launch_bounds__(TLH_THREADS)
+    schur_precond_kernel(SchurArgs<Real> a, int R, int DC, int DP, int cfirst, int chunk) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* stage = reinterpret_cast<double*>(smem_raw);
+  const int OC = cfirst ? 0 : R * DP;
+  const int OP = cfirst ? R * DC : 0;
+  const int W = DC * (DC + 1) / 2;
+  double* part = reinterpret_cast<double*>(a.part_c);
+  const int t = blockIdx.x;
+  if (t >= a.ntiles) return;
+  const int start = a.tile_start[t];
+  const int len = a.tile_start[t + 1] - start;
+  for (int c0 = 0; c0 < W; c0 += chunk) {
+    const int cw = min(chunk, W - c0);
+    __syncthreads();
+    for (int pos = threadIdx.x; pos < len; pos += TLH_THREADS) {
+      const int64_t n = start + pos;
+      const int rk = a.rank_c ? (int)a.rank_c[n] : pos;
+      const bool head = pos == 0 || a.idx_c[n] != a.idx_c[n - 1] || a.idx_p[n] != a.idx_p[n - 1];
+      double vals[8];
+      for (int k = 0; k < cw; ++k) vals[k] = 0.0;
+      if (head) {
+        double w[8][8];
+        for (int i = 0; i < DC; ++i)
+          for (int j = 0; j < DP; ++j) w[i][j] = 0.0;
+        for (int m = pos; m < len; ++m) {
+          const int64_t nm = start + m;
+          if (m > pos && (a.idx_c[nm] != a.idx_c[n] || a.idx_p[nm] != a.idx_p[n])) break;
+          for (int r = 0; r < R; ++r)
+            for (int i = 0; i < DC; ++i) {
+              const double ci = (double)a.packed[(int64_t)(OC + r * DC + i) * a.stride + nm];
+              for (int j = 0; j < DP; ++j) w[i][j] += ci * (double)a.packed[(int64_t)(OP + r * DP + j) * a.stride + nm];
+            }
+        }
+        const double* m = a.cinv + (int64_t)a.idx_p[n] * (DP * (DP + 1) / 2);
+        // wm = W C^-1
+        double wm[8][8];
+        for (int i = 0; i < DC; ++i)
+          for (int j = 0; j < DP; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < DP; ++k) {
+              const int lo = k < j ? k : j, hi = k < j ? j : k;
+              s += w[i][k] * m[lo * DP - lo * (lo - 1) / 2 + (hi - lo)];
+            }
+            wm[i][j] = s;
+          }
+        int e = 0, k = 0;
+        for (int i = 0; i < DC; ++i)
+          for (int j = i; j < DC; ++j, ++e) {
+            if (e < c0 || e >= c0 + cw) continue;
+            double s = 0.0;
+            for (int l = 0; l < DP; ++l) s += wm[i][l] * w[j][l];
+            vals[e - c0] = s;
+            (void)k;
+          }
+      }
+      for (int k = 0; k < cw; ++k) stage[rk * cw + k] = vals[k];
+    }
+    __syncthreads();
+    const int sb = a.seg_ptr_c[t], se = a.seg_ptr_c[t + 1];
+    for (int k = sb + (int)threadIdx.x; k < se; k += TLH_THREADS) {
+      const int lo = a.seg_start_c[k];
+      const int hi = (k + 1 < se) ? a.seg_start_c[k + 1] : len;
+      double* dst = part + (int64_t)a.seg_target_c[k] * W + c0;
+      for (int c = 0; c < cw; ++c) {
+        double s = stage[lo * cw + c];
+        for (int i = lo + 1; i < hi; ++i) s += stage[i * cw + c];
+        dst[c] = s;
+      }
+    }
+  }
+}
+
+// M_c = D_c + lambda I - sum of the camera's edge partials (full symmetric).
+__global__ void __launch_bounds__(TLH_THREADS)
+    schur_precond_finish(double* __restrict__ M, const double* __restrict__ D, double damping,
+                         const double* __restrict__ partials, const int32_t* __restrict__ csr_ptr,
+                         const int32_t* __restrict__ csr_idx, int64_t count, int dim) {
+  const int W = dim * (dim + 1) / 2;
+  const int64_t e = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
+  if (e >= count * W) return;
+  const int64_t v = e / W;
+  int k = (int)(e - v * W), i = 0, row = dim;
+  while (k >= row) {
+    k -= row;
+    ++i;
+    --row;
+  }
+  const int j = i + k;
+  const int kk = (int)(e - v * W);
+  double s = 0.0;
+  for (int p = csr_ptr[v]; p < csr_ptr[v + 1]; ++p) s += partials[(int64_t)csr_idx[p] * W + kk];
+  const double val = D[v * dim * dim + i * dim + j] + (i == j ? damping : 0.0) - s;
+  M[v * dim * dim + i * dim + j] = val;
+  M[v * dim * dim + j * dim + i] = val;
+}
+
+// ------------------------------------------------------- dense reduced matrix
+// One thread per unique camera pair (ci <= cj): sum over the pair's
+// observation pairs (i in edge(ci,p), j in edge(cj,p)) of
+// (J_c^T J_p)_i C_p^-1 (J_c^T J_p)_j^T, subtract from block (ci, cj) of M
+// (and its transpose).  M must hold blockdiag(D_c + lambda I) beforehand.
+template <typename Real>
+__global__ void __launch_bounds__(TLH_THREADS)
+    schur_dense_kernel(SchurArgs<Real> a, int R, int DC, int DP, int cfirst, const int32_t* __restrict__ pair_ptr,
+                       const int32_t* __restrict__ pair_i, const int32_t* __restrict__ pair_j,
+                       const int32_t* __restrict__ pair_key_ci, const int32_t* __restrict__ pair_key_cj,
+                       int64_t npairs, double* __restrict__ M, int64_t ld) {
+  const int64_t key = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
+  if (key >= npairs) return;
+  const int OC = cfirst ? 0 : R * DP;
+  const int OP = cfirst ? R * DC : 0;
+  double acc[8][8];
+  for (int i = 0; i < DC; ++i)
+    for (int j = 0; j < DC; ++j) acc[i][j] = 0.0;
+  for (int e = pair_ptr[key]; e < pair_ptr[key + 1]; ++e) {
+    const int64_t ni = pair_i[e], nj = pair_j[e];
+    double wi[8][8], wj[8][8];
+    for (int c = 0; c < DC; ++c)
+      for (int p = 0; p < DP; ++p) {
+        double si = 0.0, sj = 0.0;
+        for (int r = 0; r < R; ++r) {
+          si += (double)a.packed[(int64_t)(OC + r * DC + c) * a.stride + ni] *
+                (double)a.packed[(int64_t)(OP + r * DP + p) * a.stride + ni];
+          sj += (double)a.packed[(int64_t)(OC + r * DC + c) * a.stride + nj] *
+                (double)a.packed[(int64_t)(OP + r * DP + p) * a.stride + nj];
+        }
+        wi[c][p] = si;
+        wj[c][p] = sj;
+      }
+    const double* m = a.cinv + (int64_t)a.idx_p[ni] * (DP * (DP + 1) / 2);
+    for (int c = 0; c < DC; ++c) {
+      double t[8];
+      for (int p = 0; p < DP; ++p) {
+        double s = 0.0;
+        for (int q = 0; q < DP; ++q) {
+          const int lo = q < p ? q : p, hi = q < p ? p : q;
+          s += wi[c][q] * m[lo * DP - lo * (lo - 1) / 2 + (hi - lo)];
+        }
+        t[p] = s;
+      }
+      for (int d = 0; d < DC; ++d) {
+        double s = 0.0;
+        for (int p = 0; p < DP; ++p) s += t[p] * wj[d][p];
+        acc[c][d] += s;
+      }
+    }
+  }
+  const int64_t ci = pair_key_ci[key], cj = pair_key_cj[key];
+  for (int c = 0; c < DC; ++c)
+    for (int d = 0; d < DC; ++d) {
+      M[(ci * DC + c) * ld + cj * DC + d] -= acc[c][d];
+      if (ci != cj) M[(cj * DC + d) * ld + ci * DC + c] -= acc[c][d];
+    }
+}
+
+// ------------------------------------------------------------------ host side
+template <typename Real>
+SchurArgs<Real> make_schur_args(const tlh_operator_t* op, const tlh_group_t& g, int sc, int sp, const void* xc,
+                                const void* bp, void* outp, const double* cinv) {
+  SchurArgs<Real> a{};
+  a.packed = static_cast<const Real*>(g.packed);
+  a.stride = g.stride;
+  a.tile_start = g.tile_start;
+  a.ntiles = g.ntiles;
+  a.tiles_per_wg = g.tiles_per_wg;
+  a.idx_c = g.slot[sc].index;
+  a.idx_p = g.slot[sp].index;
+  a.rank_c = g.slot[sc].rank;
+  a.seg_ptr_c = g.slot[sc].seg_ptr;
+  a.seg_start_c = g.slot[sc].seg_start;
+  a.seg_target_c = g.slot[sc].seg_target;
+  a.mode_c = g.slot[sc].mode;
+  a.dense_block_c = g.slot[sc].dense_block;
+  const tlh_type_t& tc = op->types[g.slot[sc].type];
+  const tlh_type_t& tp = op->types[g.slot[sp].type];
+  a.count_c = (int32_t)tc.count;
+  a.part_c = static_cast<Real*>(tc.partials);
+  a.seg_ptr_p = g.slot[sp].seg_ptr;
+  a.seg_start_p = g.slot[sp].seg_start;
+  a.xc = xc ? static_cast<const Real*>(xc) + tc.offset : nullptr;
+  a.bp = static_cast<const Real*>(bp);
+  a.outp = static_cast<Real*>(outp);
+  a.cinv = cinv;
+  (void)tp;
+  return a;
+}
+
+template <typename Real>
+constexpr int schur_items() {
+  return sizeof(Real) == 4 ? 4 : 2;
+}
+
+template <typename Real, int R, int DC, int DP, bool CFIRST, int MODE>
+int launch_schur_t(const tlh_operator_t* op, const tlh_group_t& g, int sc, int sp, const void* xc, const void* bp,
+                   void* outp, const double* cinv, hipStream_t s) {
+  constexpr int ITEMS = schur_items<Real>();
+  if (g.tile_cap > TLH_THREADS * ITEMS) return TLH_EINVAL;
+  constexpr int T = TLH_THREADS * ITEMS;
+  int64_t lds = (int64_t)T * ((DC > DP ? DC : DP) + DP) * sizeof(Real);
+  if (MODE != SCHUR_BACKSUB && g.slot[sc].mode == TLH_REDUCE_DENSE)
+    lds += op->types[g.slot[sc].type].count * DC * (int64_t)sizeof(Real);
+  auto kernel = schur_tile_kernel<Real, R, DC, DP, CFIRST, ITEMS, MODE>;
+  TLH_RETURN_IF_ERROR(allow_lds((const void*)kernel, lds));
+  const unsigned grid = MODE == SCHUR_BACKSUB ? (unsigned)g.ntiles : (unsigned)g.nwg;
+  SchurArgs<Real> args = make_schur_args<Real>(op, g, sc, sp, xc, bp, outp, cinv);
+  if (MODE == SCHUR_BACKSUB) args.tiles_per_wg = 1;
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(TLH_THREADS), (size_t)lds, s, args);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+template <typename Real, int MODE>
+int launch_schur(const tlh_operator_t* op, const tlh_group_t& g, int sc, int sp, const void* xc, const void* bp,
+                 void* outp, const double* cinv, hipStream_t s) {
+  const int R = g.rows, DC = g.slot[sc].dim, DP = g.slot[sp].dim;
+  const bool cfirst = sc < sp;
+  if (R == 2 && DC == 6 && DP == 3)
+    return cfirst ? launch_schur_t<Real, 2, 6, 3, true, MODE>(op, g, sc, sp, xc, bp, outp, cinv, s)
+                  : launch_schur_t<Real, 2, 6, 3, false, MODE>(op, g, sc, sp, xc, bp, outp, cinv, s);
+  if (R == 2 && DC == 7 && DP == 3)
+    return cfirst ? launch_schur_t<Real, 2, 7, 3, true, MODE>(op, g, sc, sp, xc, bp, outp, cinv, s)
+                  : launch_schur_t<Real, 2, 7, 3, false, MODE>(op, g, sc, sp, xc, bp, outp, cinv, s);
+  if (R == 3 && DC == 6 && DP == 3)
+    return cfirst ? launch_schur_t<Real, 3, 6, 3, true, MODE>(op, g, sc, sp, xc, bp, outp, cinv, s)
+                  : launch_schur_t<Real, 3, 6, 3, false, MODE>(op, g, sc, sp, xc, bp, outp, cinv, s);
+  return TLH_ENOTFOUND;
+}
+
+template <typename Real>
+int schur_dispatch(const tlh_operator_t* op, int wgroup, int eslot, int mode, const void* xc, const void* bp, void* outp,
+                   const double* cinv, hipStream_t s) {
+  const tlh_group_t& g = op->groups[wgroup];
+  // the eliminated slot is the group's primary slot: sorted, identity rank
+  if (g.nslots != 2 || eslot < 0 || eslot > 1 || g.slot[eslot].rank != nullptr) return TLH_EINVAL;
+  const int se = eslot, sc = 1 - eslot;
+  if (g.count == 0) return 0;
+  switch (mode) {
+    case SCHUR_MATVEC:
+      return launch_schur<Real, SCHUR_MATVEC>(op, g, sc, se, xc, bp, outp, cinv, s);
+    case SCHUR_RHS:
+      return launch_schur<Real, SCHUR_RHS>(op, g, sc, se, xc, bp, outp, cinv, s);
+    case SCHUR_BACKSUB:
+      return launch_schur<Real, SCHUR_BACKSUB>(op, g, sc, se, xc, bp, outp, cinv, s);
+  }
+  return TLH_EINVAL;
+}
+
+}  // namespace
+
+extern "C" int tlh_schur_apply(const tlh_operator_t* op, int wgroup, int eslot, int mode, const void* xc, const void* bp,
+                               void* outp, const double* cinv, tlh_stream_t stream) {
+  if (!op || wgroup < 0 || wgroup >= op->ngroups || !cinv) return TLH_EINVAL;
+  if (mode != SCHUR_RHS && !xc) return TLH_EINVAL;
+  if (mode != SCHUR_MATVEC && !bp) return TLH_EINVAL;
+  if (mode == SCHUR_BACKSUB && !outp) return TLH_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (op->dtype == TLH_F32) return schur_dispatch<float>(op, wgroup, eslot, mode, xc, bp, outp, cinv, s);
+  if (op->dtype == TLH_F64) return schur_dispatch<double>(op, wgroup, eslot, mode, xc, bp, outp, cinv, s);
+  return TLH_EINVAL;
+}
+
+extern "C" int tlh_schur_precond(const tlh_operator_t* op, int wgroup, int eslot, const double* cinv, const double* D,
+                                 double damping, double* M, int64_t lds_limit, tlh_stream_t stream) {
+  if (!op || wgroup < 0 || wgroup >= op->ngroups || !cinv || !D || !M) return TLH_EINVAL;
+  const tlh_group_t& g = op->groups[wgroup];
+  if (g.nslots != 2 || eslot < 0 || eslot > 1 || g.slot[eslot].rank != nullptr) return TLH_EINVAL;
+  const int se = eslot, sc = 1 - eslot;
+  const int R = g.rows, DC = g.slot[sc].dim, DP = g.slot[se].dim;
+  if (DC > 8 || DP > 8 || g.slot[sc].mode != TLH_REDUCE_SPARSE) return TLH_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const int W = DC * (DC + 1) / 2;
+  int chunk = (int)std::min<int64_t>(std::min(W, 8), lds_limit / ((int64_t)g.tile_cap * 8));
+  if (chunk < 1) return TLH_ELDS;
+  const int64_t lds = (int64_t)g.tile_cap * chunk * 8;
+  const tlh_type_t& tc = op->types[g.slot[sc].type];
+  if (g.count > 0) {
+    if (op->dtype == TLH_F32) {
+      SchurArgs<float> a = make_schur_args<float>(op, g, sc, se, nullptr, nullptr, nullptr, cinv);
+      TLH_RETURN_IF_ERROR(allow_lds((const void*)schur_precond_kernel<float>, lds));
+      hipLaunchKernelGGL(schur_precond_kernel<float>, dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds, s, a, R, DC,
+                         DP, (int)(sc < se), chunk);
+    } else {
+      SchurArgs<double> a = make_schur_args<double>(op, g, sc, se, nullptr, nullptr, nullptr, cinv);
+      TLH_RETURN_IF_ERROR(allow_lds((const void*)schur_precond_kernel<double>, lds));
+      hipLaunchKernelGGL(schur_precond_kernel<double>, dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds, s, a, R,
+                         DC, DP, (int)(sc < se), chunk);
+    }
+    TLH_RETURN_IF_LAUNCH_FAILED();
+  }
+  const int64_t total = tc.count * W;
+  if (total) {
+    hipLaunchKernelGGL(schur_precond_finish, dim3((unsigned)ceil_div(total, TLH_THREADS)), dim3(TLH_THREADS), 0, s,
+                       M, D, damping, static_cast<const double*>(tc.partials), tc.csr_ptr, tc.csr_idx, tc.count, DC);
+    TLH_RETURN_IF_LAUNCH_FAILED();
+  }
+  return 0;
+}
+
+extern "C" int tlh_schur_dense(const tlh_operator_t* op, int wgroup, int eslot, const double* cinv, const int32_t* pair_ptr,
+                               const int32_t* pair_i, const int32_t* pair_j, const int32_t* key_ci,
+                               const int32_t* key_cj, int64_t npairs, double* M, int64_t ld, tlh_stream_t stream) {
+  if (!op || wgroup < 0 || wgroup >= op->ngroups || !cinv || !M) return TLH_EINVAL;
+  const tlh_group_t& g = op->groups[wgroup];
+  if (g.nslots != 2 || eslot < 0 || eslot > 1 || g.slot[eslot].rank != nullptr) return TLH_EINVAL;
+  const int se = eslot, sc = 1 - eslot;
+  const int R = g.rows, DC = g.slot[sc].dim, DP = g.slot[se].dim;
+  if (DC > 8 || DP > 8) return TLH_EINVAL;
+  if (npairs == 0) return 0;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const dim3 grid((unsigned)ceil_div(npairs, TLH_THREADS)), block(TLH_THREADS);
+  if (op->dtype == TLH_F32) {
+    SchurArgs<float> a = make_schur_args<float>(op, g, sc, se, nullptr, nullptr, nullptr, cinv);
+    hipLaunchKernelGGL(schur_dense_kernel<float>, grid, block, 0, s, a, R, DC, DP, (int)(sc < se), pair_ptr, pair_i,
+                       pair_j, key_ci, key_cj, npairs, M, ld);
+  } else {
+    SchurArgs<double> a = make_schur_args<double>(op, g, sc, se, nullptr, nullptr, nullptr, cinv);
+    hipLaunchKernelGGL(schur_dense_kernel<double>, grid, block, 0, s, a, R, DC, DP, (int)(sc < se), pair_ptr,
+                       pair_i, pair_j, key_ci, key_cj, npairs, M, ld);
+  }
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}

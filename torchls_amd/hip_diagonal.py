@@ -1,0 +1,145 @@
+"""HIP diagonal blocks and block-Jacobi: drop-ins for warp_diagonal.py and block_jacobi.
+
+``HipDiagonalPlan(jacobian).diagonal_blocks(jacobian)`` mirrors
+``WarpDiagonalPlan`` (warp_diagonal.py:70-179): one full symmetric
+``[count, d, d]`` block per layout entry, in the Jacobian's dtype, including
+aliased-slot cross terms.  Internally the blocks are accumulated in float64
+(``diagonal_blocks_f64``), which is what the solver's float64 inversions use.
+
+``HipBlockJacobi`` holds float64 inverses of ``D_t + lambda I`` (closed-form
+Cholesky per block, tlh_block_inverse) and applies them as a callable, like the
+``apply`` closure returned by ``block_jacobi`` (solvers.py:259-285); the HIP PCG
+fuses its application into the step kernel.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .hip_normal import DeviceStructure, check_columns, plan_for
+
+
+class LinAlgError(RuntimeError):
+    """A damped diagonal block is not positive definite (torch.linalg.LinAlgError analogue)."""
+
+
+class HipDiagonalPlan:
+    def __init__(self, jacobian, *, shards=128, hot_threshold=512, dtype: torch.dtype | None = None):
+        proto = jacobian.groups[0].blocks[0]
+        if proto.device.type != "cuda" or proto.dtype not in (torch.float32, torch.float64):
+            raise ValueError("HIP diagonal blocks require GPU float32 or float64 blocks")
+        if shards < 1 or hot_threshold < 1:
+            raise ValueError("shards and hot_threshold must be positive")
+        self.device, self.block_dtype = proto.device, proto.dtype
+        self.dtype = proto.dtype if dtype is None else dtype
+        self.lib = _lib.load()
+        check_columns(jacobian)
+        self.layout = tuple((e.count, e.dim) for e in jacobian.layout)
+        self.structure = tuple(
+            (tuple(g.type_indices), tuple(g.variable_indices), tuple(b.shape for b in g.blocks))
+            for g in jacobian.groups
+        )
+        self.plan, self.dev = plan_for(jacobian, self.dtype, allow_dense=False)
+        self.partials = [
+            torch.empty(max(1, t.sparse_partials * t.dim * (t.dim + 1) // 2), dtype=torch.float64,
+                        device=self.device)
+            for t in self.plan.types
+        ]
+
+    def compatible_with(self, normal_plan) -> bool:
+        """True when a HipNormalPlan packs observations in this plan's order and tiles."""
+        a, b = self.plan.groups, normal_plan.plan.groups
+        return len(a) == len(b) and all(
+            x.count == y.count and np.array_equal(x.order, y.order) and np.array_equal(x.tile_start, y.tile_start)
+            for x, y in zip(a, b)
+        )
+
+    def _check(self, jacobian):
+        if tuple((e.count, e.dim) for e in jacobian.layout) != self.layout or len(jacobian.groups) != len(
+            self.structure
+        ):
+            raise ValueError("HIP diagonal plan requires unchanged incidence structure")
+        for group, (types, indices, shapes) in zip(jacobian.groups, self.structure, strict=True):
+            if (
+                tuple(group.type_indices) != types
+                or tuple(b.shape for b in group.blocks) != shapes
+                or any(b.device != self.device or b.dtype != self.block_dtype for b in group.blocks)
+                or len(group.variable_indices) != len(indices)
+                or any(a is not b and not torch.equal(a, b) for a, b in zip(group.variable_indices, indices, strict=True))
+            ):
+                raise ValueError("HIP diagonal plan requires unchanged incidence structure, shapes and dtype")
+
+    def _pack(self, jacobian) -> list:
+        packed = [
+            torch.empty((g.components, max(g.stride, 1)), dtype=self.dtype, device=self.device)
+            for g in self.plan.groups
+        ]
+        op = self.dev.operator(self.dtype, packed, [None] * len(self.plan.types))
+        stream = _lib.stream_handle(self.device)
+        for gi, group in enumerate(jacobian.groups):
+            blocks = [b.contiguous() for b in group.blocks]
+            ptrs = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in blocks] + [None] * (3 - len(blocks)))
+            _lib.check(self.lib.tlh_pack_blocks(ctypes.byref(op), gi, ptrs,
+                                                _lib.dtype_code(self.block_dtype), stream), "tlh_pack_blocks")
+        return packed
+
+    def diagonal_blocks_f64(self, packed: list) -> tuple:
+        """float64 diagonal blocks from packed blocks laid out in this plan's order."""
+        outs = tuple(torch.empty((c, d, d), dtype=torch.float64, device=self.device) for c, d in self.layout)
+        op = self.dev.operator(self.dtype, packed, self.partials)
+        ptrs = (ctypes.c_void_p * len(outs))(*[o.data_ptr() for o in outs])
+        lds = _lib.device_info(self.device).lds_per_block
+        _lib.check(self.lib.tlh_diagonal_blocks(ctypes.byref(op), ptrs, lds, _lib.stream_handle(self.device)),
+                   "tlh_diagonal_blocks")
+        return outs
+
+    def diagonal_blocks(self, jacobian) -> tuple:
+        self._check(jacobian)
+        outs = self.diagonal_blocks_f64(self._pack(jacobian))
+        return tuple(o.to(self.block_dtype) for o in outs)
+
+
+class HipBlockJacobi:
+    """Callable z = (D + lambda I)^-1 r per variable block, float64 inverses."""
+
+    def __init__(self, layout, diagonal_f64, damping: float):
+        device = diagonal_f64[0].device
+        self.lib = _lib.load()
+        self.layout = [(e.count, e.dim, e.offset) for e in layout]
+        self.size = sum(c * d for c, d, _ in self.layout)
+        status = torch.zeros(1, dtype=torch.int32, device=device)
+        stream = _lib.stream_handle(device)
+        self.inverses = []
+        desc = _lib.BlockDiag()
+        desc.ntypes = len(self.layout)
+        for t, ((c, d, off), block) in enumerate(zip(self.layout, diagonal_f64)):
+            block = block.to(torch.float64).contiguous()
+            inv = torch.empty(max(1, c * d * (d + 1) // 2), dtype=torch.float64, device=device)
+            _lib.check(self.lib.tlh_block_inverse(block.data_ptr(), c, d, float(damping), inv.data_ptr(),
+                                                  status.data_ptr(), stream), "tlh_block_inverse")
+            self.inverses.append(inv)
+            desc.dim[t], desc.count[t], desc.offset[t], desc.inv[t] = d, c, off, inv.data_ptr()
+        self.desc = desc
+        self.device = device
+        if int(status.item()) != 0:  # one host sync per factorisation, like torch.linalg.cholesky
+            raise LinAlgError("damped diagonal block is not positive definite")
+
+    def __call__(self, vector: torch.Tensor) -> torch.Tensor:
+        from .hip_pcg import PcgState
+
+        if vector.shape != (self.size,):
+            raise ValueError("preconditioner input has incorrect shape")
+        vector = vector.contiguous()
+        r = torch.empty_like(vector)
+        z = torch.empty_like(vector)
+        p = torch.empty_like(vector)
+        st = PcgState.scratch(vector.device)
+        _lib.check(self.lib.tlh_pcg_reset(ctypes.byref(self.desc), vector.data_ptr(), None, r.data_ptr(),
+                                          z.data_ptr(), p.data_ptr(), self.size, _lib.dtype_code(vector.dtype),
+                                          ctypes.byref(st.desc), _lib.stream_handle(vector.device)),
+                   "tlh_pcg_reset")
+        return z

@@ -1,0 +1,173 @@
+"""Host wrappers of the HIP float64 reductions and the BlockJacobian operators.
+
+``LinearizationKernels`` binds one incidence structure (built once per solve)
+and exposes, for any linearization with that structure:
+
+* ``gradient(J, residual)`` -- float64 ``J^T r`` (tlh_rmatvec_f64);
+* ``squared_matvec_norm(J, x)`` -- float64 ``||J x||^2``;
+* ``matvec(J, x)`` -- ``J x`` in original residual order;
+* ``trial(before, after, ...)`` -- float64 objective and trial reduction.
+
+``jacobian_operator(J)`` gives the ``BlockJacobian`` methods a cached
+``LinearizationKernels`` plus normal/diagonal plans for that Jacobian.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from .hip_diagonal import HipDiagonalPlan
+from .hip_normal import HipNormalPlan
+
+
+class Reduction:
+    """Partials/counter scratch for the fixed-order float64 grid reductions."""
+
+    def __init__(self, device, slots: int = 16):
+        self.partials = torch.zeros(2 * _lib.TLH_PCG_MAX_BLOCKS, dtype=torch.float64, device=device)
+        self.counter = torch.zeros(4, dtype=torch.int32, device=device)
+        self.out = torch.zeros((slots, 2), dtype=torch.float64, device=device)
+
+
+class LinearizationKernels:
+    def __init__(self, jacobian, diag_plan: HipDiagonalPlan | None = None):
+        self.lib = _lib.load()
+        self.device = jacobian.groups[0].blocks[0].device
+        self.layout = jacobian.layout
+        self.size = jacobian.shape[1]
+        self.diag_plan = diag_plan if diag_plan is not None else HipDiagonalPlan(jacobian)
+        self.red = Reduction(self.device, max(16, len(jacobian.groups) + 1))
+        self.grad_partials = [
+            torch.empty(max(1, t.sparse_partials * t.dim), dtype=torch.float64, device=self.device)
+            for t in self.diag_plan.plan.types
+        ]
+
+    def _raw(self, group):
+        dims = (ctypes.c_int32 * 3)(*[b.shape[2] for b in group.blocks] + [0] * (3 - len(group.blocks)))
+        offsets = (ctypes.c_int64 * 3)(
+            *[self.layout[t].offset for t in group.type_indices] + [0] * (3 - len(group.blocks))
+        )
+        index = [i.contiguous() for i in group.variable_indices]
+        blocks = [b.contiguous() for b in group.blocks]
+        iptr = (ctypes.c_void_p * 3)(*[i.data_ptr() for i in index] + [None] * (3 - len(index)))
+        bptr = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in blocks] + [None] * (3 - len(blocks)))
+        return dims, offsets, iptr, bptr, (index, blocks)
+
+    def gradient(self, jacobian, residual: torch.Tensor) -> torch.Tensor:
+        """float64 J^T residual (residual in original row order, any float dtype)."""
+        residual = residual.contiguous()
+        op = self.diag_plan.dev.operator(torch.float64, [None] * len(jacobian.groups), self.grad_partials)
+        stream = _lib.stream_handle(self.device)
+        lds = _lib.device_info(self.device).lds_per_block
+        offset = 0
+        keep = []
+        for gi, group in enumerate(jacobian.groups):
+            count, rows = group.blocks[0].shape[:2]
+            blocks = [b.contiguous() for b in group.blocks]
+            keep.append(blocks)
+            bptr = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in blocks] + [None] * (3 - len(blocks)))
+            v = residual[offset : offset + count * rows]
+            offset += count * rows
+            _lib.check(
+                self.lib.tlh_rmatvec_f64(ctypes.byref(op), gi, bptr, _lib.dtype_code(blocks[0].dtype), v.data_ptr(),
+                                         _lib.dtype_code(v.dtype), lds, stream),
+                "tlh_rmatvec_f64",
+            )
+        out = torch.empty(self.size, dtype=torch.float64, device=self.device)
+        _lib.check(self.lib.tlh_rmatvec_finish_f64(ctypes.byref(op), out.data_ptr(), stream), "tlh_rmatvec_finish_f64")
+        return out
+
+    def squared_matvec_norm(self, jacobian, x: torch.Tensor) -> torch.Tensor:
+        """float64 ||J x||^2 as a 0-d device tensor."""
+        x = x.contiguous()
+        stream = _lib.stream_handle(self.device)
+        for gi, group in enumerate(jacobian.groups):
+            count, rows = group.blocks[0].shape[:2]
+            dims, offsets, iptr, bptr, keep = self._raw(group)
+            _lib.check(
+                self.lib.tlh_squared_matvec_norm(
+                    count, rows, len(group.blocks), dims, offsets, iptr, bptr,
+                    _lib.dtype_code(group.blocks[0].dtype), x.data_ptr(), _lib.dtype_code(x.dtype),
+                    self.red.out[gi].data_ptr(), self.red.partials.data_ptr(), self.red.counter.data_ptr(), stream,
+                ),
+                "tlh_squared_matvec_norm",
+            )
+        return self.red.out[: len(jacobian.groups), 0].sum()
+
+    def matvec(self, jacobian, x: torch.Tensor) -> torch.Tensor:
+        x = x.contiguous()
+        stream = _lib.stream_handle(self.device)
+        dtype = jacobian.groups[0].blocks[0].dtype
+        out = torch.empty(jacobian.shape[0], dtype=dtype, device=self.device)
+        offset = 0
+        for group in jacobian.groups:
+            count, rows = group.blocks[0].shape[:2]
+            dims, offsets, iptr, bptr, keep = self._raw(group)
+            _lib.check(
+                self.lib.tlh_block_matvec(
+                    count, rows, len(group.blocks), dims, offsets, iptr, bptr, _lib.dtype_code(dtype),
+                    x.data_ptr(), _lib.dtype_code(x.dtype), out[offset:].data_ptr(), _lib.dtype_code(dtype), stream,
+                ),
+                "tlh_block_matvec",
+            )
+            offset += count * rows
+        return out
+
+
+def trial_objective(red: Reduction, slot: int, before, after, count: int, rows: int, *,
+                    huber_delta: float | None = None, weights: torch.Tensor | None = None) -> None:
+    """red.out[slot] = (sum w rho(||after||^2), sum w [rho(||before||^2) - rho(||after||^2)])."""
+    lib = _lib.load()
+    after = after.contiguous()
+    before = None if before is None else before.contiguous()
+    w = None if weights is None else weights.to(torch.float64).contiguous()
+    stride = 0 if (w is None or w.numel() == 1) else 1
+    _lib.check(
+        lib.tlh_trial_objective(
+            None if before is None else before.data_ptr(), _lib.dtype_code(before.dtype) if before is not None else 0,
+            after.data_ptr(), _lib.dtype_code(after.dtype), count, rows, 0 if huber_delta is None else 1,
+            float(huber_delta or 0.0), None if w is None else w.data_ptr(), stride, red.out[slot].data_ptr(),
+            red.partials.data_ptr(), red.counter.data_ptr(), _lib.stream_handle(after.device),
+        ),
+        "tlh_trial_objective",
+    )
+
+
+class JacobianOperator:
+    """Cached HIP plans behind the BlockJacobian methods."""
+
+    def __init__(self, jacobian):
+        self.kernels = LinearizationKernels(jacobian)
+        self.jacobian = jacobian
+        self._normal = None
+
+    def matvec(self, x):
+        return self.kernels.matvec(self.jacobian, x)
+
+    def rmatvec(self, v):
+        return self.kernels.gradient(self.jacobian, v).to(v.dtype)
+
+    def normal_matvec(self, x, damping=0.0):
+        if self._normal is None:
+            self._normal = HipNormalPlan(self.jacobian).bind(self.jacobian)
+        return self._normal.normal_matvec(x, damping)
+
+    def squared_matvec_norm(self, x):
+        return self.kernels.squared_matvec_norm(self.jacobian, x)
+
+    def diagonal_blocks(self):
+        return self.kernels.diag_plan.diagonal_blocks(self.jacobian)
+
+
+def jacobian_operator(jacobian) -> JacobianOperator:
+    proto = jacobian.groups[0].blocks[0]
+    if proto.device.type != "cuda":
+        raise ValueError("BlockJacobian operators run on the GPU (HIP); move the Jacobian to a cuda device")
+    op = jacobian.__dict__.get("_hip_operator")
+    if op is None:
+        op = JacobianOperator(jacobian)
+        object.__setattr__(jacobian, "_hip_operator", op)
+    return op

@@ -164,6 +164,8 @@ def build_structure(
     piece: int = 32,
     tile_cap: int | None = None,
     allow_dense: bool = True,
+    force_primary: dict | None = None,
+    split_primary: bool = True,
 ) -> StructurePlan:
     """Plan the deterministic reduction.
 
@@ -173,13 +175,15 @@ def build_structure(
     cap = tile_cap or THREADS * items_for(esize)
     types = [TypePlan(int(c), int(d), int(o)) for c, d, o in layout]
     gplans = []
-    for rows, slot_types, slot_idx in groups:
+    for gi, (rows, slot_types, slot_idx) in enumerate(groups):
         ns = len(slot_types)
         if not 1 <= ns <= 3:
             raise ValueError("the hip backend supports 1 to 3 variable slots per residual")
         count = int(slot_idx[0].size)
         sizes = [types[t].count for t in slot_types]
         primary = int(np.argmax(sizes))
+        if force_primary and gi in force_primary:
+            primary = int(force_primary[gi])
         keys = [np.arange(count)]
         keys += [slot_idx[s] for s in reversed(range(ns)) if s != primary]
         keys.append(slot_idx[primary])
@@ -197,9 +201,8 @@ def build_structure(
             dense = allow_dense and s != primary and ntiles > 1 and dense_bytes <= dense_budget
             if dense:
                 dense_budget -= dense_bytes
-            rank, identity, seg_start, seg_ptr, seg_var = _slot_segments(
-                sorted_idx[s], tile_start, None if dense else piece
-            )
+            split = None if dense or (s == primary and not split_primary) else piece
+            rank, identity, seg_start, seg_ptr, seg_var = _slot_segments(sorted_idx[s], tile_start, split)
             slots.append(
                 SlotPlan(
                     t,

@@ -49,3 +49,31 @@ def reprojection_batched(cameras: Tensor, points: Tensor, cam_idx: Tensor, pt_id
     DR = D @ R
     Jc = torch.cat((DR, -(DR @ skew(p))), dim=-1)
     return res, Jc, DR
+
+
+def _reprojection_rows(cams: Tensor, pts: Tensor, uv: Tensor):
+    R = cams[:, :3, :3]
+    local = (R @ pts[..., None])[..., 0] + cams[:, :3, 3]
+    x, y, z = local.unbind(-1)
+    return R, x, y, z, torch.stack((x / z, y / z), -1) - uv
+
+
+def _batched_residual(cams: Tensor, pts: Tensor, uv: Tensor) -> Tensor:
+    return _reprojection_rows(cams, pts, uv)[-1]
+
+
+def _batched_linearize(cams: Tensor, pts: Tensor, uv: Tensor):
+    R, x, y, z, res = _reprojection_rows(cams, pts, uv)
+    inv = 1 / z
+    D = torch.zeros(z.shape + (2, 3), dtype=z.dtype, device=z.device)
+    D[:, 0, 0] = inv
+    D[:, 0, 2] = -x * inv * inv
+    D[:, 1, 1] = inv
+    D[:, 1, 2] = -y * inv * inv
+    DR = D @ R
+    return res, (torch.cat((DR, -(DR @ skew(pts))), dim=-1), DR)
+
+
+# Batched evaluators used by Problem instead of vmap for this residual.
+reprojection.batched_residual = _batched_residual
+reprojection.batched_linearize = _batched_linearize

@@ -1,0 +1,320 @@
+"""Schur-complement linear solves on the HIP backend (drop-in for torchls.schur).
+
+``HipSchurPlan(problem, eliminate, device, dtype)`` is the symbolic analysis
+(SchurPlan, schur.py:17-161): the eliminated parameter group must be
+independent per residual (schur.py:44-49); the retained groups keep their
+layout order with re-based offsets (schur.py:27-37).  The HIP operator supports
+the bundle-adjustment structure: exactly one residual group ("W-group") that
+couples one retained slot with one eliminated slot, any number of groups on
+retained variables only (e.g. camera priors; each with one slot for the dense
+path), and any groups on eliminated variables only (e.g. point priors, which
+only enter C = D_e + lambda I).
+
+``plan.linearize(J, diagonal)`` packs one linearization (SchurLinearization,
+schur.py:192-257); ``.damped(lambda)`` gives the system with ``reduced_rhs``,
+``matvec``, ``preconditioner``, ``backsubstitute`` and ``dense_solve``
+(DampedSchurSystem, schur.py:263-397) -- all HIP kernels (csrc/schur.hip,
+csrc/cholesky.hip); W is never stored.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .hip_diagonal import LinAlgError
+from .hip_normal import DeviceStructure
+from .plan import build_structure
+
+
+class HipSchurPlan:
+    def __init__(self, problem, eliminate, device, *, dtype=torch.float64):
+        layout = problem.layout
+        matches = [i for i, e in enumerate(layout) if e.key == eliminate or e.name == eliminate]
+        if len(matches) != 1:
+            raise ValueError("eliminate must identify one parameter group by name")
+        self.e = matches[0]
+        self.retained = tuple(i for i in range(len(layout)) if i != self.e)
+        if not self.retained:
+            raise ValueError("Schur elimination needs a retained parameter group")
+        self.layout = layout
+        self.device, self.dtype = device, dtype
+        self.lib = _lib.load()
+        ent = layout[self.e]
+        self.ecount, self.edim, self.eoffset = ent.count, ent.dim, ent.offset
+        # retained layout (re-based) then the eliminated type last
+        self.local = {t: k for k, t in enumerate(self.retained)}
+        self.local[self.e] = len(self.retained)
+        rl, off = [], 0
+        for t in self.retained:
+            rl.append((layout[t].count, layout[t].dim, off))
+            off += layout[t].count * layout[t].dim
+        self.dim = off
+        rl.append((ent.count, ent.dim, off))
+        self.rlayout = rl
+        gidx = [torch.arange(layout[t].offset, layout[t].offset + layout[t].count * layout[t].dim)
+                for t in self.retained]
+        self.global_indices = torch.cat(gidx).to(device)
+        self.wgroup = None
+        self.groups = []  # (lin group index, [lin slot indices], [local types])
+        for gi, b in enumerate(problem.batches):
+            es = [s for s, t in enumerate(b.type_indices) if t == self.e]
+            rs = [s for s, t in enumerate(b.type_indices) if t != self.e]
+            if len(es) > 1 and any(not torch.equal(b.indices[s], b.indices[es[0]]) for s in es[1:]):
+                raise ValueError("Eliminated variables must be independent: a residual connects distinct eliminated indices")
+            if es and rs:
+                if self.wgroup is not None or len(es) != 1 or len(rs) != 1:
+                    raise NotImplementedError(
+                        "the HIP Schur operator supports one residual group coupling one retained slot "
+                        "with one eliminated slot")
+                self.wgroup = len(self.groups)
+                self.eslot = es[0]
+                self.groups.append((gi, [0, 1], [self.local[b.type_indices[0]], self.local[b.type_indices[1]]]))
+            elif rs:
+                self.groups.append((gi, list(range(len(rs))), [self.local[b.type_indices[s]] for s in rs]))
+        if self.wgroup is None:
+            raise NotImplementedError("no residual group couples the retained and eliminated groups")
+        self.retained_only_multi = any(len(sl) > 1 for k, (_, sl, _) in enumerate(self.groups) if k != self.wgroup)
+        specs = []
+        for gi, slots, ltypes in self.groups:
+            b = problem.batches[gi]
+            specs.append((None, ltypes, [b.indices[s].numpy() if s < len(b.indices) else None for s in slots], gi))
+        info = _lib.device_info(device)
+        esize = 4 if dtype == torch.float32 else 8
+        plan_groups = []
+        for _, ltypes, idx, gi in specs:
+            b = problem.batches[gi]
+            plan_groups.append((None, ltypes, [np.asarray(i, dtype=np.int64) for i in idx]))
+        self.rows = [None] * len(self.groups)
+        self._plan_groups = plan_groups
+        self._problem = problem
+        self._info = info
+        self._esize = esize
+        self.plan = None
+        self.sparse_plan = None
+        self.pairs = None
+
+    def _build(self, jacobian):
+        """Plans need residual row counts, known from the first linearization."""
+        groups = []
+        for k, (gi, slots, ltypes) in enumerate(self.groups):
+            rows = int(jacobian.groups[gi].blocks[0].shape[1])
+            self.rows[k] = rows
+            groups.append((rows, ltypes, self._plan_groups[k][2]))
+        eprim = {self.wgroup: self.eslot}
+        stage_extra = 256 * (4 if self._esize == 4 else 2) * self.edim * self._esize
+        self.plan = build_structure(self.rlayout, groups, esize=self._esize,
+                                    lds_limit=int(self._info.lds_per_block) - stage_extra,
+                                    cu_count=int(self._info.cu_count), force_primary=eprim, split_primary=False)
+        self.sparse_plan = build_structure(self.rlayout, groups, esize=self._esize,
+                                           lds_limit=int(self._info.lds_per_block), allow_dense=False,
+                                           force_primary=eprim, split_primary=False)
+        w = self.plan.groups[self.wgroup]
+        prim = w.slots[self.eslot].index
+        cuts = w.tile_start[1:-1]
+        if cuts.size and np.any(prim[cuts] == prim[cuts - 1]):
+            raise NotImplementedError("an eliminated variable has more observations than one tile holds")
+        self.dev = DeviceStructure(self.plan, self.device)
+        self.sdev = DeviceStructure(self.sparse_plan, self.device)
+        nt = len(self.retained)
+        self.partials = [torch.empty(max(1, t.partial_count * t.dim), dtype=self.dtype, device=self.device)
+                         for t in self.plan.types]
+        self.pre_partials = [
+            torch.empty(max(1, t.sparse_partials * t.dim * (t.dim + 1) // 2), dtype=torch.float64, device=self.device)
+            for t in self.sparse_plan.types
+        ]
+        self.ntypes_ret = nt
+
+    def _dense_pairs(self):
+        """Observation pairs of every point run with cam(i) <= cam(j), grouped by camera pair."""
+        if self.pairs is not None:
+            return self.pairs
+        w = self.plan.groups[self.wgroup]
+        cs = 1 - self.eslot
+        cam = w.slots[cs].index.astype(np.int64)
+        prim = w.slots[self.eslot].index.astype(np.int64)
+        n = prim.size
+        starts = np.concatenate(([0], np.flatnonzero(np.diff(prim)) + 1, [n]))
+        lens = np.diff(starts)
+        pi, pj = [], []
+        for L in np.unique(lens):
+            s0 = starts[:-1][lens == L]
+            a = np.arange(L)
+            ii = (s0[:, None, None] + a[None, :, None]).repeat(L, axis=2).reshape(-1)
+            jj = (s0[:, None, None] + a[None, None, :]).repeat(L, axis=1).reshape(-1)
+            keep = cam[ii] <= cam[jj]
+            pi.append(ii[keep])
+            pj.append(jj[keep])
+        pi = np.concatenate(pi)
+        pj = np.concatenate(pj)
+        nc = int(self.rlayout[self.plan.groups[self.wgroup].slots[cs].type][0])
+        key = cam[pi] * nc + cam[pj]
+        o = np.lexsort((pj, pi, key))
+        pi, pj, key = pi[o], pj[o], key[o]
+        ukey, first = np.unique(key, return_index=True)
+        ptr = np.concatenate((first, [key.size])).astype(np.int32)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a.astype(np.int32))).to(self.device)  # noqa: E731
+        self.pairs = dict(ptr=t(ptr), i=t(pi), j=t(pj), ci=t(ukey // nc), cj=t(ukey % nc), n=int(ukey.size))
+        return self.pairs
+
+    def linearize(self, jacobian, diagonal, *, dense: bool = False):
+        if self.plan is None:
+            self._build(jacobian)
+        if dense and self.retained_only_multi:
+            raise NotImplementedError("dense HIP Schur supports single-slot retained-only residual groups")
+        return HipSchurLinearization(self, jacobian, diagonal, dense)
+
+
+class HipSchurLinearization:
+    def __init__(self, plan: HipSchurPlan, jacobian, diagonal, dense):
+        self.plan = plan
+        self.dense = dense
+        self.diagonal = tuple(d.to(torch.float64).contiguous() for d in diagonal)
+        p = plan
+        self.packed = [
+            torch.empty((g.components, max(g.stride, 1)), dtype=p.dtype, device=p.device) for g in p.plan.groups
+        ]
+        self.op = p.dev.operator(p.dtype, self.packed, p.partials)
+        self.sop = p.sdev.operator(p.dtype, self.packed, p.pre_partials)
+        stream = _lib.stream_handle(p.device)
+        keep = []
+        for k, (gi, slots, _) in enumerate(p.groups):
+            grp = jacobian.groups[gi]
+            if k == p.wgroup:
+                blocks = [grp.blocks[0].contiguous(), grp.blocks[1].contiguous()]
+            else:
+                rs = [s for s, t in enumerate(grp.type_indices) if t != p.e]
+                blocks = [grp.blocks[s].contiguous() for s in rs]
+            keep.append(blocks)
+            ptrs = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in blocks] + [None] * (3 - len(blocks)))
+            _lib.check(p.lib.tlh_pack_blocks(ctypes.byref(self.op), k, ptrs, _lib.dtype_code(blocks[0].dtype), stream),
+                       "tlh_pack_blocks")
+        self._keep = keep
+
+    def damped(self, damping: float) -> "HipSchurSystem":
+        return HipSchurSystem(self, damping)
+
+
+class HipSchurSystem:
+    def __init__(self, lin: HipSchurLinearization, damping: float):
+        self.lin, self.plan, self.damping = lin, lin.plan, float(damping)
+        p = self.plan
+        dev = p.device
+        stream = _lib.stream_handle(dev)
+        status = torch.zeros(1, dtype=torch.int32, device=dev)
+        d = p.edim
+        self.cinv = torch.empty(max(1, p.ecount * d * (d + 1) // 2), dtype=torch.float64, device=dev)
+        _lib.check(p.lib.tlh_block_inverse(lin.diagonal[p.e].data_ptr(), p.ecount, d, self.damping,
+                                           self.cinv.data_ptr(), status.data_ptr(), stream), "tlh_block_inverse")
+        if int(status.item()) != 0:
+            raise LinAlgError("damped eliminated block is not positive definite")
+        self._bp = None
+
+    # -- helpers
+    def _apply(self, mode, xc, bp, outp):
+        p = self.plan
+        _lib.check(p.lib.tlh_schur_apply(ctypes.byref(self.lin.op), p.wgroup, p.eslot, mode,
+                                         None if xc is None else xc.data_ptr(), None if bp is None else bp.data_ptr(),
+                                         None if outp is None else outp.data_ptr(), self.cinv.data_ptr(),
+                                         _lib.stream_handle(p.device)), "tlh_schur_apply")
+
+    def _others(self, x):
+        p = self.plan
+        for k in range(len(p.groups)):
+            if k != p.wgroup:
+                _lib.check(p.lib.tlh_group_tiles(ctypes.byref(self.lin.op), k, x.data_ptr(),
+                                                 _lib.stream_handle(p.device)), "tlh_group_tiles")
+
+    def _finish(self, base, scale):
+        p = self.plan
+        y = torch.empty(p.dim, dtype=p.dtype, device=p.device)
+        _lib.check(p.lib.tlh_finish_types(ctypes.byref(self.lin.op), 0, p.ntypes_ret, base.data_ptr(), float(scale),
+                                          y.data_ptr(), _lib.stream_handle(p.device)), "tlh_finish_types")
+        return y
+
+    # -- DampedSchurSystem interface
+    def reduced_rhs(self, rhs: torch.Tensor) -> torch.Tensor:
+        p = self.plan
+        rhs = rhs.to(p.dtype)
+        self._bp = rhs[p.eoffset: p.eoffset + p.ecount * p.edim].contiguous()
+        b_ret = rhs[p.global_indices].contiguous()
+        self._apply(_lib_mode("RHS"), None, self._bp, None)
+        self._others(torch.zeros_like(b_ret))
+        return self._finish(b_ret, 1.0)
+
+    def matvec(self, vector: torch.Tensor) -> torch.Tensor:
+        p = self.plan
+        if vector.shape != (p.dim,) or vector.dtype != p.dtype:
+            raise ValueError("Schur matvec input must be the retained vector in the plan dtype")
+        x = vector.contiguous()
+        self._apply(_lib_mode("MATVEC"), x, None, None)
+        self._others(x)
+        return self._finish(x, self.damping)
+
+    def backsubstitute(self, retained: torch.Tensor) -> torch.Tensor:
+        p = self.plan
+        if self._bp is None:
+            raise ValueError("reduced_rhs must be called before backsubstitute")
+        xr = retained.to(p.dtype).contiguous()
+        xe = torch.empty(p.ecount * p.edim, dtype=p.dtype, device=p.device)
+        self._apply(_lib_mode("BACKSUB"), xr, self._bp, xe)
+        total = sum(e.count * e.dim for e in p.layout)
+        out = torch.zeros(total, dtype=torch.float64, device=p.device)
+        out[p.global_indices] = xr.to(torch.float64)
+        out[p.eoffset: p.eoffset + p.ecount * p.edim] = xe.to(torch.float64)
+        return out
+
+    def preconditioner_blocks(self):
+        """float64 retained blocks D + lambda I - sum W_e C^-1 W_e^T (schur.py:353-362)."""
+        p = self.plan
+        blocks = []
+        w = p.sparse_plan.groups[p.wgroup]
+        wtype = w.slots[1 - p.eslot].type
+        lds = _lib.device_info(p.device).lds_per_block
+        for k, t in enumerate(p.retained):
+            D = self.lin.diagonal[t]
+            if k == wtype:
+                M = torch.empty_like(D)
+                _lib.check(p.lib.tlh_schur_precond(ctypes.byref(self.lin.sop), p.wgroup, p.eslot, self.cinv.data_ptr(),
+                                                   D.data_ptr(), self.damping, M.data_ptr(), lds,
+                                                   _lib.stream_handle(p.device)), "tlh_schur_precond")
+            else:
+                M = D + self.damping * torch.eye(D.shape[-1], dtype=D.dtype, device=D.device)
+            blocks.append(M)
+        return blocks
+
+    def preconditioner(self, *, graph_safe=False):
+        from .hip_diagonal import HipBlockJacobi
+        from .sparse import VariableLayout
+
+        p = self.plan
+        lay = tuple(VariableLayout(p.layout[t].var_type, p.layout[t].ids, off)
+                    for t, (_, _, off) in zip(p.retained, p.rlayout[:-1]))
+        return HipBlockJacobi(lay, self.preconditioner_blocks(), 0.0)
+
+    def dense_matrix(self) -> torch.Tensor:
+        p = self.plan
+        pairs = p._dense_pairs()
+        n = p.dim
+        M = torch.zeros((n, n), dtype=torch.float64, device=p.device)
+        for t, (c, d, off) in zip(p.retained, p.rlayout[:-1]):
+            blk = self.lin.diagonal[t] + self.damping * torch.eye(d, dtype=torch.float64, device=p.device)
+            idx = off + torch.arange(c, device=p.device)[:, None] * d + torch.arange(d, device=p.device)
+            M[idx[:, :, None], idx[:, None, :]] = blk
+        _lib.check(p.lib.tlh_schur_dense(ctypes.byref(self.lin.op), p.wgroup, p.eslot, self.cinv.data_ptr(),
+                                         pairs["ptr"].data_ptr(), pairs["i"].data_ptr(), pairs["j"].data_ptr(),
+                                         pairs["ci"].data_ptr(), pairs["cj"].data_ptr(), pairs["n"], M.data_ptr(), n,
+                                         _lib.stream_handle(p.device)), "tlh_schur_dense")
+        return M
+
+    def dense_solve(self, rhs: torch.Tensor) -> torch.Tensor:
+        from .cholesky import cholesky_solve_f64
+
+        return cholesky_solve_f64(self.dense_matrix(), rhs.to(torch.float64)).to(self.plan.dtype)
+
+
+def _lib_mode(name: str) -> int:
+    return {"MATVEC": 0, "RHS": 1, "BACKSUB": 2}[name]

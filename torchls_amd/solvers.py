@@ -1,0 +1,453 @@
+"""Solver configuration, PCG dispatch, block-Jacobi and the LM / GN loop.
+
+Same interface and control flow as torchls.solvers (solvers.py:19-696) with the
+``"hip"`` backend: every operation of the block-sparse normal-equation path --
+diagonal blocks, damped block inversion, normal matvec, PCG, the Schur
+reduce/matvec/preconditioner/back-substitution, the dense reduced Cholesky, and
+the float64 gradient, objective and predicted-reduction reductions -- runs in
+the HIP kernels of this package.  ``matvec_dtype=torch.float32`` on float64
+parameters selects the mixed-precision mode (float32 blocks, vectors and
+matvec; float64 outer residual, gradient, objective, dots and inversions).
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from contextlib import nullcontext
+from dataclasses import dataclass, field
+from functools import partial
+
+import torch
+from torch import Tensor
+
+from .hip_diagonal import HipBlockJacobi, HipDiagonalPlan, LinAlgError
+from .hip_normal import HipNormalPlan
+from .hip_ops import LinearizationKernels
+
+
+@dataclass(frozen=True)
+class ConjugateGradientConfig:
+    rtol: float = 1e-6
+    atol: float = 1e-10
+    max_iterations: int = 200
+    preconditioner: str | None = "block_jacobi"
+    preconditioner_groups: tuple = ()
+    residual_check_interval: int = 50
+    restart_strategy: str = "adaptive"
+    residual_drift_tolerance: float = 0.1
+    backend: str = "hip"
+    normal_matvec: str = "hip"
+    normal_max_packed_bytes: int = 256 * 1024**2
+    graph_batch_iterations: int = 16
+    cache_retained_matrix: bool = False
+    # Storage/arithmetic dtype of the packed blocks and PCG vectors; None = the
+    # parameters' dtype.  torch.float32 with float64 parameters = mixed precision.
+    matvec_dtype: torch.dtype | None = None
+
+    def __post_init__(self):
+        if (isinstance(self.normal_max_packed_bytes, bool) or not isinstance(self.normal_max_packed_bytes, int)
+                or self.normal_max_packed_bytes < 0):
+            raise ValueError("normal_max_packed_bytes must be a nonnegative integer")
+        if (isinstance(self.graph_batch_iterations, bool) or not isinstance(self.graph_batch_iterations, int)
+                or self.graph_batch_iterations < 1):
+            raise ValueError("graph_batch_iterations must be a positive integer")
+        if self.normal_matvec != "hip":
+            raise ValueError("normal_matvec must be hip (this package implements the HIP backend)")
+        if self.backend != "hip":
+            raise ValueError("CG backend must be hip (this package implements the HIP backend)")
+        if self.rtol < 0 or self.atol < 0 or self.max_iterations < 1:
+            raise ValueError("Invalid CG tolerance or iteration limit")
+        if self.preconditioner not in ("block_jacobi", None):
+            raise ValueError("preconditioner must be block_jacobi or None")
+        if self.preconditioner_groups:
+            raise ValueError("joint preconditioner groups are not supported by the HIP backend")
+        if (isinstance(self.residual_check_interval, bool) or not isinstance(self.residual_check_interval, int)
+                or self.residual_check_interval < 0):
+            raise ValueError("residual_check_interval must be a nonnegative integer")
+        if self.restart_strategy not in ("adaptive", "periodic"):
+            raise ValueError("restart_strategy must be adaptive or periodic")
+        if not math.isfinite(self.residual_drift_tolerance) or self.residual_drift_tolerance <= 0:
+            raise ValueError("residual_drift_tolerance must be finite and positive")
+        if self.matvec_dtype not in (None, torch.float32, torch.float64):
+            raise ValueError("matvec_dtype must be None, torch.float32 or torch.float64")
+
+
+@dataclass(frozen=True)
+class TrustRegionConfig:
+    lambda_initial: float = 1e-3
+    lambda_min: float = 1e-12
+    lambda_max: float = 1e12
+    lambda_factor: float = 4.0
+    step_quality_min: float = 1e-4
+    max_trials: int = 20
+    strategy: str = "fixed_factor"
+    damping_mode: str = "identity"
+    diagonal_min: float = 1e-6
+    diagonal_max: float = 1e32
+
+    def __post_init__(self):
+        if not (0 < self.lambda_min <= self.lambda_initial <= self.lambda_max):
+            raise ValueError("Require 0 < lambda_min <= lambda_initial <= lambda_max")
+        if self.lambda_factor <= 1 or self.max_trials < 1 or not (0 <= self.step_quality_min < 1):
+            raise ValueError("Invalid trust region settings")
+        if self.strategy not in ("fixed_factor", "adaptive"):
+            raise ValueError("Trust region strategy must be fixed_factor or adaptive")
+        if self.damping_mode not in ("identity", "diagonal"):
+            raise ValueError("damping_mode must be identity or diagonal")
+        if not (0 < self.diagonal_min <= self.diagonal_max < math.inf):
+            raise ValueError("Require finite positive diagonal damping bounds")
+
+
+@dataclass(frozen=True)
+class TerminationConfig:
+    max_iterations: int = 50
+    cost_tolerance: float = 1e-10
+    gradient_tolerance: float = 1e-8
+    parameter_tolerance: float = 1e-10
+    target_cost: float | None = None
+    max_time_seconds: float | None = None
+
+    def __post_init__(self):
+        if self.max_iterations < 1 or min(self.cost_tolerance, self.gradient_tolerance, self.parameter_tolerance) < 0:
+            raise ValueError("Invalid termination settings")
+        if self.target_cost is not None and (not math.isfinite(self.target_cost) or self.target_cost < 0):
+            raise ValueError("target_cost must be finite and nonnegative")
+        if self.max_time_seconds is not None and (
+            not math.isfinite(self.max_time_seconds) or self.max_time_seconds <= 0
+        ):
+            raise ValueError("max_time_seconds must be finite and positive")
+
+
+_DEFAULT_CG = ConjugateGradientConfig()
+_DEFAULT_TRUST_REGION = TrustRegionConfig()
+_DEFAULT_TERMINATION = TerminationConfig()
+
+
+@dataclass(frozen=True)
+class PCGResult:
+    solution: Tensor
+    iterations: int
+    residual_norm: float
+    converged: bool
+    reason: str
+    restarts: int = 0
+    residual_checks: int = 0
+    rhs_norm: float = 0.0
+
+
+@dataclass
+class SolveSummary:
+    """Histories include the initial point and each accepted update (solvers.py:142-164)."""
+
+    initial_cost: float
+    final_cost: float
+    iterations: int = 0
+    accepted_steps: int = 0
+    rejected_steps: int = 0
+    linear_iterations: int = 0
+    termination_reason: str = "max_iterations"
+    gradient_norm: float = math.inf
+    cost_history: list = field(default_factory=list)
+    lambda_history: list = field(default_factory=list)
+    linear_iterations_history: list = field(default_factory=list)
+    linear_converged_history: list = field(default_factory=list)
+    trial_history: list = field(default_factory=list)
+    linear_solver: str = "conjugate_gradient"
+    jacobi_scaling: bool = False
+    reduced_dim: int | None = None
+    linear_restarts: int = 0
+    linear_residual_checks: int = 0
+    linear_relative_residual_history: list = field(default_factory=list)
+
+
+def pcg(matvec, rhs: Tensor, *, preconditioner=None, config: ConjugateGradientConfig = _DEFAULT_CG) -> PCGResult:
+    """Preconditioned CG with true-residual checks (solvers.py:167-256), on the HIP backend."""
+    if rhs.ndim != 1 or rhs.numel() == 0:
+        raise ValueError("PCG expects a nonempty vector")
+    from .hip_pcg import hip_pcg
+
+    return hip_pcg(matvec, rhs, preconditioner=preconditioner, config=config)
+
+
+def block_jacobi(jacobian, damping: float, *, diagonal=None, groups=(), graph_safe=False):
+    """float64 (D_t + damping I)^-1 per variable block (solvers.py:259-285).
+
+    Raises ``LinAlgError`` when a damped block is not positive definite."""
+    if groups:
+        raise ValueError("joint preconditioner groups are not supported by the HIP backend")
+    if diagonal is None:
+        diagonal = HipDiagonalPlan(jacobian).diagonal_blocks(jacobian)
+    return HipBlockJacobi(jacobian.layout, tuple(d.to(torch.float64) for d in diagonal), damping)
+
+
+def _scaled_jacobian(jacobian, scale: Tensor):
+    """J diag(scale) with the same symbolic structure (sparse.py:86-102)."""
+    from .sparse import BlockJacobian, JacobianGroup
+
+    groups = tuple(
+        JacobianGroup(
+            tuple((b * scale[c].to(b.dtype)[:, None, :]).contiguous() for b, c in zip(g.blocks, g.columns)),
+            g.columns, g.type_indices, g.variable_indices,
+        )
+        for g in jacobian.groups
+    )
+    return BlockJacobian(groups, jacobian.layout, jacobian.shape)
+
+
+@torch.no_grad()
+def solve(
+    problem,
+    initial_vals,
+    *,
+    linear_solver: str = "conjugate_gradient",
+    conjugate_gradient: ConjugateGradientConfig = _DEFAULT_CG,
+    trust_region: TrustRegionConfig | None = _DEFAULT_TRUST_REGION,
+    termination: TerminationConfig = _DEFAULT_TERMINATION,
+    return_summary: bool = False,
+    jacobi_scaling: bool = False,
+    eliminate=None,
+    profile: bool = False,
+    record_trials: bool = False,
+    bounds=None,
+    normal_blocks_backend: str = "auto",
+):
+    """Levenberg-Marquardt (Gauss-Newton when trust_region=None) on the HIP path.
+
+    ``problem`` is an ``Analysis`` (from ``Problem``); ``initial_vals`` maps
+    parameter names to tensors.  Returns the solved dict (and the summary).
+    """
+    if linear_solver not in ("conjugate_gradient", "dense_qr", "schur_dense", "schur_cg"):
+        raise ValueError("Unknown linear solver")
+    if normal_blocks_backend not in ("auto", "hip"):
+        raise ValueError("normal_blocks_backend must be auto or hip")
+    if bounds is not None:
+        raise ValueError("bounds are not supported by the HIP backend")
+    is_schur = linear_solver.startswith("schur_")
+    if eliminate is not None and not is_schur:
+        raise ValueError("eliminate requires a Schur linear solver")
+    if is_schur and eliminate is None:
+        raise ValueError("Schur solvers need eliminate=<parameter group name>")
+    start = time.perf_counter()
+
+    def region(name):
+        return torch.profiler.record_function("torchls/" + name) if profile else nullcontext()
+
+    def timed_out():
+        return termination.max_time_seconds is not None and time.perf_counter() - start >= termination.max_time_seconds
+
+    vals = {k: v.detach() for k, v in initial_vals.items()}
+    first = next(iter(vals.values()))
+    device, dtype = first.device, first.dtype
+    if device.type != "cuda":
+        raise ValueError("the HIP backend solves on a GPU device; move the parameters to cuda")
+    mv_dtype = conjugate_gradient.matvec_dtype or dtype
+    initial_cost = float(problem.cost(vals))
+    if not math.isfinite(initial_cost):
+        raise ValueError("Initial residuals are nonfinite")
+    summary = SolveSummary(initial_cost, initial_cost, cost_history=[initial_cost], linear_solver=linear_solver,
+                           jacobi_scaling=jacobi_scaling)
+    damping = trust_region.lambda_initial if trust_region else 0.0
+    rejection_factor = 2.0
+    schur_plan = None
+    if is_schur:
+        from .schur import HipSchurPlan
+
+        with region("analyze_schur"):
+            schur_plan = HipSchurPlan(problem, eliminate, device, dtype=mv_dtype)
+        if linear_solver == "schur_dense" and schur_plan.dim > 8192:
+            raise ValueError("Dense Schur is limited to 8192 retained dimensions; use schur_cg")
+        summary.reduced_dim = schur_plan.dim
+    normal_plan = diagonal_plan = kernels = None
+    column_scale = None
+    with region("linearize"):
+        lin = problem.linearize(vals)
+    for iteration in range(termination.max_iterations):
+        if timed_out():
+            summary.termination_reason = "time_limit"
+            break
+        summary.iterations = iteration + 1
+        residual, jacobian = lin.residual, lin.jacobian
+        finite = torch.stack([torch.isfinite(residual).all()]
+                             + [torch.isfinite(b).all() for g in jacobian.groups for b in g.blocks]).all()
+        if not bool(finite):
+            summary.termination_reason = "nonfinite_linearization"
+            break
+        if kernels is None:
+            diagonal_plan = HipDiagonalPlan(jacobian, dtype=mv_dtype)
+            kernels = LinearizationKernels(jacobian, diagonal_plan)
+        gradient = kernels.gradient(jacobian, residual)
+        summary.gradient_norm = float(gradient.abs().max())
+        lin_cost = float(lin.objective) if lin.objective is not None else None
+        if termination.target_cost is not None:
+            if lin_cost is None:
+                lin_cost = float(problem.cost(vals))
+            if lin_cost <= termination.target_cost:
+                summary.termination_reason = "target_cost"
+                break
+        if summary.gradient_norm <= termination.gradient_tolerance:
+            summary.termination_reason = "gradient_tolerance"
+            break
+        original_jacobian = jacobian
+        with region("prepare_linear_system"):
+            diagonal_damping = trust_region is not None and trust_region.damping_mode == "diagonal"
+            needs_diagonal = is_schur or (linear_solver == "conjugate_gradient"
+                                          and conjugate_gradient.preconditioner is not None)
+            diagonal = None
+            packed_unscaled = None
+            if needs_diagonal or diagonal_damping or (jacobi_scaling and column_scale is None):
+                packed_unscaled = diagonal_plan._pack(jacobian)
+                diagonal = diagonal_plan.diagonal_blocks_f64(packed_unscaled)
+            if jacobi_scaling and column_scale is None:
+                column_scale = torch.cat(
+                    [(1 / (1 + b.diagonal(dim1=-2, dim2=-1).clamp_min(0).sqrt())).reshape(-1) for b in diagonal]
+                )
+            solve_scale = column_scale
+            if diagonal_damping:
+                nd = torch.cat([b.diagonal(dim1=-2, dim2=-1).reshape(-1) for b in diagonal])
+                if column_scale is not None:
+                    nd = nd * column_scale.square()
+                inv_diag = nd.clamp(trust_region.diagonal_min, trust_region.diagonal_max).rsqrt()
+                solve_scale = inv_diag if column_scale is None else column_scale * inv_diag
+            if solve_scale is not None:
+                jacobian = _scaled_jacobian(jacobian, solve_scale)
+                linear_gradient = gradient * solve_scale
+                if diagonal is not None:
+                    diagonal = tuple(
+                        b * solve_scale[e.offset: e.offset + e.count * e.dim].reshape(e.count, e.dim, 1)
+                        * solve_scale[e.offset: e.offset + e.count * e.dim].reshape(e.count, 1, e.dim)
+                        for e, b in zip(problem.layout, diagonal)
+                    )
+            else:
+                linear_gradient = gradient
+            schur_linear = None
+            normal_operator = None
+            if is_schur:
+                schur_linear = schur_plan.linearize(jacobian, diagonal, dense=linear_solver == "schur_dense")
+            elif linear_solver == "conjugate_gradient":
+                if normal_plan is None:
+                    normal_plan = HipNormalPlan(jacobian, dtype=mv_dtype)
+                normal_operator = normal_plan.bind(jacobian)
+        old_cost = float(lin.objective) if lin.objective is not None else float(
+            0.5 * residual.to(torch.float64).square().sum())
+        accepted = False
+        trial_limit = trust_region.max_trials if trust_region else 1
+        for _ in range(trial_limit):
+            if timed_out():
+                summary.termination_reason = "time_limit"
+                break
+            summary.lambda_history.append(damping)
+            try:
+                if linear_solver == "dense_qr":
+                    dense = jacobian.to_dense()
+                    eye = torch.eye(problem.tangent_dim, dtype=dtype, device=device)
+                    aug = torch.cat((dense, math.sqrt(damping) * eye))
+                    rhs = torch.cat((-residual, gradient.new_zeros(problem.tangent_dim).to(residual.dtype)))
+                    delta = torch.linalg.lstsq(aug, rhs[:, None]).solution[:, 0]
+                    usable = bool(torch.isfinite(delta).all())
+                elif is_schur:
+                    with region("prepare_schur"):
+                        system = schur_linear.damped(damping)
+                        rhs = system.reduced_rhs(-linear_gradient)
+                    with region("linear_solve"):
+                        if linear_solver == "schur_dense":
+                            retained = system.dense_solve(rhs)
+                            iterations, converged = 1, True
+                            usable = bool(torch.isfinite(retained).all())
+                            summary.linear_relative_residual_history.append(None)
+                        else:
+                            result = pcg(system.matvec, rhs,
+                                         preconditioner=system.preconditioner() if conjugate_gradient.preconditioner
+                                         else None, config=conjugate_gradient)
+                            retained, iterations, converged = result.solution, result.iterations, result.converged
+                            usable = result.reason in ("residual_tolerance", "max_iterations") and bool(
+                                torch.isfinite(retained).all())
+                            summary.linear_restarts += result.restarts
+                            summary.linear_residual_checks += result.residual_checks
+                            summary.linear_relative_residual_history.append(
+                                result.residual_norm / result.rhs_norm if result.rhs_norm else result.residual_norm)
+                        delta = system.backsubstitute(retained)
+                    summary.linear_iterations += iterations
+                    summary.linear_iterations_history.append(iterations)
+                    summary.linear_converged_history.append(converged)
+                else:
+                    precondition = (
+                        HipBlockJacobi(problem.layout, diagonal, damping)
+                        if conjugate_gradient.preconditioner is not None else None
+                    )
+                    with region("linear_solve"):
+                        result = pcg(partial(normal_operator.normal_matvec, damping=damping),
+                                     (-linear_gradient).to(mv_dtype), preconditioner=precondition,
+                                     config=conjugate_gradient)
+                    summary.linear_iterations += result.iterations
+                    summary.linear_iterations_history.append(result.iterations)
+                    summary.linear_converged_history.append(result.converged)
+                    summary.linear_restarts += result.restarts
+                    summary.linear_residual_checks += result.residual_checks
+                    summary.linear_relative_residual_history.append(
+                        result.residual_norm / result.rhs_norm if result.rhs_norm else result.residual_norm)
+                    delta = result.solution
+                    usable = result.reason in ("residual_tolerance", "max_iterations") and bool(
+                        torch.isfinite(delta).all())
+            except (LinAlgError, torch.linalg.LinAlgError):
+                usable = False
+            if usable:
+                delta = delta.to(torch.float64)
+                if solve_scale is not None:
+                    delta = solve_scale.to(torch.float64) * delta
+                proposed = problem.retract(vals, delta)
+                with region("trial_residual"):
+                    objective, reduction = problem.evaluate_trial(proposed, lin)
+                sq = kernels.squared_matvec_norm(original_jacobian, delta)
+                pred_t = -torch.dot(gradient, delta) - 0.5 * sq
+                host = torch.stack((objective, reduction, pred_t)).cpu()
+                proposed_cost, actual, predicted = (float(v) for v in host)
+                quality = actual / predicted if predicted > 0 else -math.inf
+                accepted = math.isfinite(proposed_cost) and (
+                    trust_region is None or (actual > 0 and quality >= trust_region.step_quality_min))
+                if record_trials:
+                    summary.trial_history.append(dict(damping=damping, predicted_reduction=predicted,
+                                                      actual_reduction=actual, gain_ratio=quality,
+                                                      accepted=accepted))
+                if accepted:
+                    vals = proposed
+                    summary.accepted_steps += 1
+                    summary.final_cost = proposed_cost
+                    summary.cost_history.append(proposed_cost)
+                    if trust_region:
+                        if trust_region.strategy == "adaptive":
+                            mult = max(1 / 3, 1 - (2 * min(quality, 1.0) - 1) ** 3)
+                            damping = max(trust_region.lambda_min, min(trust_region.lambda_max, damping * mult))
+                            rejection_factor = 2.0
+                        elif quality > 0.75:
+                            damping = max(trust_region.lambda_min, damping / trust_region.lambda_factor)
+                    lin = residual = jacobian = original_jacobian = None
+                    normal_operator = schur_linear = system = diagonal = gradient = linear_gradient = None
+                    with region("linearize"):
+                        lin = problem.linearize(vals)
+                    final_gradient = kernels.gradient(lin.jacobian, lin.residual)
+                    summary.gradient_norm = float(final_gradient.abs().max())
+                    if termination.target_cost is not None and proposed_cost <= termination.target_cost:
+                        summary.termination_reason = "target_cost"
+                    elif summary.gradient_norm <= termination.gradient_tolerance:
+                        summary.termination_reason = "gradient_tolerance"
+                    elif abs(actual) <= termination.cost_tolerance * max(old_cost, torch.finfo(dtype).tiny):
+                        summary.termination_reason = "cost_tolerance"
+                    elif float(torch.linalg.vector_norm(delta)) <= termination.parameter_tolerance:
+                        summary.termination_reason = "parameter_tolerance"
+                    break
+            summary.rejected_steps += 1
+            if trust_region is None or damping >= trust_region.lambda_max:
+                break
+            if trust_region.strategy == "adaptive":
+                damping = min(trust_region.lambda_max, damping * rejection_factor)
+                rejection_factor = min(rejection_factor * 2, trust_region.lambda_max / trust_region.lambda_min)
+            else:
+                damping = min(trust_region.lambda_max, damping * trust_region.lambda_factor)
+        if not accepted:
+            if summary.termination_reason != "time_limit":
+                summary.termination_reason = "no_acceptable_step"
+            break
+        if summary.termination_reason != "max_iterations":
+            break
+    return (vals, summary) if return_summary else vals

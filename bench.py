@@ -1,7 +1,17 @@
 """Benchmark of the HIP block-sparse normal-equation path on one MI355X.
 
-Prints one JSON line (see --help).  The timed region holds only device work on
-inputs already resident in HBM; a bounded CPU-oracle sample is timed beside it.
+Workload (default): synthetic bundle adjustment with 200 cameras, 50,000 points,
+10 observations per point (500,000 reprojection residuals), Huber delta 1e-2,
+LM with Schur-PCG on the 1,200-dimensional reduced camera system.  A step is
+one LM iteration (linearization, diagonal blocks, Schur preconditioner,
+hipGraph-replayed PCG, trial evaluation).  The mixed-precision HIP solve
+(float32 blocks/matvec, float64 scalars, gradient and objective) is timed
+against the all-float64 HIP solve on the same seed, and its final cost is
+checked against the float64 solve.  The dominant kernel (the implicit Schur
+matvec) is timed separately with HIP events for the roofline figure, and a
+bounded float64 CPU-oracle sample is timed on the host cores.
+
+Prints one JSON line on stdout; progress goes to stderr.
 """
 
 from __future__ import annotations
@@ -22,82 +32,174 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
 def parse():
     p = argparse.ArgumentParser(description=__doc__)
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=50)
-    p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--workload", default="ba_large", choices=["ba_medium", "ba_large"])
+    p.add_argument("--steps", type=int, default=10, help="timed LM iterations")
+    p.add_argument("--warmup", type=int, default=2, help="LM iterations of the untimed warm-up solve")
+    p.add_argument("--workload", default="ba_medium", choices=["ba_small", "ba_medium", "ba_large"])
+    p.add_argument("--kernel-reps", type=int, default=50)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
     return p.parse_args()
 
 
-def cpu_baseline_matvec(prob, seconds=10.0):
-    """Oracle (numpy float64) normal matvec on the same workload, host cores."""
-    from oracle.sparse import NpGroup, NpJacobian
-    from oracle.ba import linearize_ba
+def build_problem(prob, device):
+    from torchls_amd import HuberLoss, Problem, SE3Var
+    from torchls_amd.residuals import reprojection
 
-    r, Jc, Jp, _, _ = linearize_ba(prob.cameras_init, prob.points_init, prob.cam_idx, prob.pt_idx,
-                                   prob.observations, prob.huber_delta)
-    J = NpJacobian([NpGroup([Jc, Jp], [0, 1], [prob.cam_idx, prob.pt_idx])],
-                   [(prob.n_cameras, 6, 0), (prob.n_points, 3, 6 * prob.n_cameras)])
-    x = np.random.default_rng(0).standard_normal(J.n)
-    J.normal_matvec(x, 1e-3)
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        J.normal_matvec(x, 1e-3)
-        reps += 1
-        if time.perf_counter() - t0 >= seconds or reps >= 50:
-            break
-    return (time.perf_counter() - t0) / reps
+    P = Problem({"cams": torch.tensor(prob.cameras_init, device=device),
+                 "points": torch.tensor(prob.points_init, device=device)}, manifolds={"cams": SE3Var})
+    P.add_residual(reprojection, ("cams", torch.tensor(prob.cam_idx)), ("points", torch.tensor(prob.pt_idx)),
+                   torch.tensor(prob.observations, device=device), loss=HuberLoss(prob.huber_delta))
+    return P
+
+
+def timed_solve(P, init, iters, mv_dtype):
+    from torchls_amd import ConjugateGradientConfig, TerminationConfig
+
+    P.parameters.update({k: v.clone() for k, v in init.items()})
+    cg = ConjugateGradientConfig(matvec_dtype=mv_dtype)
+    term = TerminationConfig(max_iterations=iters, cost_tolerance=0.0, gradient_tolerance=0.0,
+                             parameter_tolerance=0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, s = P.solve(return_summary=True, linear_solver="schur_cg", eliminate="points", conjugate_gradient=cg,
+                   termination=term)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, s
+
+
+def schur_matvec_timing(P, mv_dtype, reps):
+    """Average time of the implicit Schur matvec (HIP events, current stream) at lambda = 1e-3."""
+    from torchls_amd.schur import HipSchurPlan
+
+    an = P._analyze()
+    lin = an.linearize(P.parameters)
+    diag = tuple(d.double() for d in lin.jacobian.diagonal_blocks())
+    plan = HipSchurPlan(an, "points", lin.residual.device, dtype=mv_dtype)
+    system = plan.linearize(lin.jacobian, diag).damped(1e-3)
+    x = torch.randn(plan.dim, dtype=mv_dtype, device=lin.residual.device)
+    for _ in range(5):
+        system.matvec(x)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        system.matvec(x)
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / reps / 1e3
+
+
+def normal_matvec_timing(P, mv_dtype, reps):
+    from torchls_amd.hip_normal import HipNormalPlan
+
+    an = P._analyze()
+    lin = an.linearize(P.parameters)
+    op = HipNormalPlan(lin.jacobian, dtype=mv_dtype).bind(lin.jacobian)
+    x = torch.randn(lin.jacobian.shape[1], dtype=mv_dtype, device=lin.residual.device)
+    y = torch.empty_like(x)
+    for _ in range(5):
+        op.normal_matvec_into(x, y, 1e-3)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        op.normal_matvec_into(x, y, 1e-3)
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / reps / 1e3
+
+
+def cpu_baseline(prob, seconds):
+    """float64 numpy oracle LM (Schur-PCG) on the same workload, host cores, bounded time."""
+    from oracle.ba import BAProblem
+    from oracle.lm import Termination, solve
+
+    o = BAProblem(prob.cam_idx, prob.pt_idx, prob.observations, prob.n_cameras, prob.n_points,
+                  huber_delta=prob.huber_delta)
+    iters, t0 = 0, time.perf_counter()
+    vals = (prob.cameras_init, prob.points_init)
+    while time.perf_counter() - t0 < seconds:
+        vals, s = solve(o, vals, linear_solver="schur_cg", eliminate=1,
+                        term=Termination(max_iterations=1, cost_tolerance=0, gradient_tolerance=0,
+                                         parameter_tolerance=0))
+        iters += 1
+    return iters / (time.perf_counter() - t0), iters
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
+              "described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     from torchls_amd import generators
-    from torchls_amd.hip_normal import HipNormalPlan
-    from torchls_amd.workloads import ba_linearization, normal_matvec_bytes
+    from torchls_amd.workloads import normal_matvec_bytes
 
     device = torch.device("cuda:0")
+    torch.manual_seed(0)
     prob = getattr(generators, args.workload)()
-    print(f"workload {args.workload}: cams={prob.n_cameras} points={prob.n_points} obs={prob.n_obs}",
-          file=sys.stderr, flush=True)
-    res, J = ba_linearization(prob, device, torch.float64)
-    plan = HipNormalPlan(J, dtype=torch.float32)
-    op = plan.bind(J)
-    n = J.shape[1]
-    x = torch.randn(n, dtype=torch.float32, device=device)
-    y = torch.empty_like(x)
-    for _ in range(args.warmup):
-        op.normal_matvec_into(x, y, 1e-3)
-    torch.cuda.synchronize()
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record()
-    for _ in range(args.steps):
-        op.normal_matvec_into(x, y, 1e-3)
-    stop.record()
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(stop) / args.steps
-    bytes_mv = normal_matvec_bytes(prob.n_obs, 2, (6, 3), n, 4)
-    gbs = bytes_mv / (ms * 1e-3) / 1e9
+    log(f"workload {args.workload}: seed={prob.seed} cams={prob.n_cameras} points={prob.n_points} obs={prob.n_obs}")
+    P = build_problem(prob, device)
+    init = {k: v.clone() for k, v in P.parameters.items()}
+
+    # warm-up solves (plan construction, kernel loading, graph capture paths)
+    for dt in (torch.float32, None):
+        timed_solve(P, init, max(1, args.warmup), dt)
+    log("warm-up done")
+    t_mixed, s_mixed = timed_solve(P, init, args.steps, torch.float32)
+    log(f"mixed: {args.steps} LM iterations in {t_mixed:.3f}s final_cost={s_mixed.final_cost:.12e}")
+    t_f64, s_f64 = timed_solve(P, init, args.steps, None)
+    log(f"float64: {args.steps} LM iterations in {t_f64:.3f}s final_cost={s_f64.final_cost:.12e}")
+    rel = abs(s_mixed.final_cost - s_f64.final_cost) / abs(s_f64.final_cost)
+    iters_mixed = s_mixed.iterations
+    iters_f64 = s_f64.iterations
+    lm_mixed = iters_mixed / t_mixed
+    lm_f64 = iters_f64 / t_f64
+
+    P.parameters.update({k: v.clone() for k, v in init.items()})
+    t_schur = schur_matvec_timing(P, torch.float32, args.kernel_reps)
+    n_c, n_p, N = prob.n_cameras, prob.n_points, prob.n_obs
+    # implicit Schur matvec algorithmic bytes: blocks + two int32 ids per observation,
+    # C^-1 (float64 upper 3x3) per point, camera vector in/out
+    schur_bytes = N * (2 * 9 * 4 + 8) + n_p * 6 * 8 + 2 * 6 * n_c * 4
+    schur_gbs = schur_bytes / t_schur / 1e9
+    t_normal = normal_matvec_timing(P, torch.float32, args.kernel_reps)
+    n = 6 * n_c + 3 * n_p
+    normal_gbs = normal_matvec_bytes(N, 2, (6, 3), n, 4) / t_normal / 1e9
+    log(f"schur matvec {t_schur * 1e6:.2f} us ({schur_gbs:.1f} GB/s); normal matvec {t_normal * 1e6:.2f} us "
+        f"({normal_gbs:.1f} GB/s)")
+
     cpu = None
     if not args.no_cpu:
-        t = cpu_baseline_matvec(prob)
-        cpu = {"value": round(normal_matvec_bytes(prob.n_obs, 2, (6, 3), n, 8) / t / 1e9, 3),
-               "unit": "GB/s (float64 normal matvec)", "cores": 1, "kind": "port",
-               "sample": f"{args.workload} full size, numpy float64 oracle normal matvec"}
+        torch.set_num_threads(len(os.sched_getaffinity(0)))
+        rate, done = cpu_baseline(prob, args.cpu_seconds)
+        cpu = {"value": round(rate, 4), "unit": "LM iterations/s", "cores": 1, "kind": "port",
+               "sample": f"{done} LM iterations (Schur-PCG, float64 numpy oracle) of the same workload from "
+                         "the same initial point"}
     print(json.dumps({
-        "metric": "normal matvec HBM GB/s (float32 blocks, deterministic reduction)",
-        "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": args.workload, "cameras": prob.n_cameras, "points": prob.n_points,
-                   "observations": prob.n_obs},
-        "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+        "metric": "LM iterations/s (mixed-precision HIP Schur-PCG), with matvec HBM GB/s",
+        "value": round(lm_mixed, 3), "unit": "LM iterations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * t_mixed / max(1, iters_mixed), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 blocks/matvec, f64 scalars",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": args.workload, "cameras": n_c, "points": n_p, "observations": N, "seed": prob.seed,
+                   "linear_solver": "schur_cg", "huber_delta": prob.huber_delta},
+        "float64_lm_iters_per_s": round(lm_f64, 3), "mixed_over_float64": round(lm_mixed / lm_f64, 3),
+        "final_cost_mixed": s_mixed.final_cost, "final_cost_float64": s_f64.final_cost,
+        "final_cost_rel_err": rel, "pcg_iterations_mixed": s_mixed.linear_iterations,
+        "pcg_iterations_float64": s_f64.linear_iterations,
+        "normal_matvec_gbs": round(normal_gbs, 1), "normal_matvec_us": round(t_normal * 1e6, 2),
+        "roofline": {"bound": "hbm", "achieved": round(schur_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(schur_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "implicit Schur matvec (schur_tile_kernel + finish)",
+                     "us_per_launch": round(t_schur * 1e6, 2), "bytes_per_launch": schur_bytes},
         "cpu_baseline": cpu,
     }), flush=True)
 

@@ -57,6 +57,7 @@ typedef struct tlh_slot {
   const int32_t* seg_ptr;    /* [ntiles+1] segment range of each tile */
   const int32_t* seg_start;  /* [nseg] first position of the segment (tile-local) */
   const int32_t* seg_target; /* [nseg] SPARSE: partial id; DENSE: variable index */
+  int64_t pos_base;          /* wave kernel: positional partial ids start here */
 } tlh_slot_t;
 
 /* One residual group: N observations of r rows, 1..3 slots (JacobianGroup,
@@ -74,6 +75,13 @@ typedef struct tlh_group {
   const int32_t* tile_start; /* [ntiles+1] */
   const int64_t* order;      /* [count] original observation of each sorted position */
   void* packed;              /* [rows * sum(dim)][stride], component (slot, row, col) major */
+  int32_t kernel;            /* 0: segment-array tile kernel; 1: wave kernel (2 slots): the
+                                primary slot is reduced by a wavefront segmented scan in
+                                64-position pieces (piece end n -> partial pos_base + n, or a
+                                direct write of y when the type's `direct` flag is set), the
+                                other slot by on-the-fly segment heads (DENSE accumulator or
+                                partial pos_base + tile_start + head) */
+  int32_t primary;           /* slot the observations are sorted by */
   tlh_slot_t slot[TLH_MAX_SLOTS];
 } tlh_group_t;
 
@@ -88,6 +96,9 @@ typedef struct tlh_type {
   const int32_t* csr_ptr;    /* [count+1] sparse partials of each variable */
   const int32_t* csr_idx;    /* partial ids in fixed (group, tile, slot) order */
   void* partials;            /* workspace [(dense_blocks*count + sparse_partials) * dim] */
+  const uint8_t* direct;     /* [count] or NULL: 1 = written directly by a wave kernel */
+  const int32_t* finish_list;/* [finish_count] variables the finish writes; NULL = all */
+  int64_t finish_count;
 } tlh_type_t;
 
 typedef struct tlh_operator {
@@ -305,6 +316,18 @@ int tlh_cholesky_f64(double* A, int64_t n, int64_t ld, int32_t* status, tlh_stre
 /* b <- (L L^T)^-1 b with the factor from tlh_cholesky_f64 (float64, in place).
  * Replaces: torch.cholesky_solve (solvers.py:518). */
 int tlh_cholesky_solve_f64(const double* L, int64_t n, int64_t ld, double* b, tlh_stream_t stream);
+
+/* ---------------------------------------------------------------- implicit gradient */
+/* grad[i] = (rho'_i I + 2 rho''_i r_i r_i^T)(J_c,i w_c + J_p,i w_p) for every reprojection
+ * residual r_i = pi(T_c p) - obs_i (float64, Jacobian recomputed from cams [n_c,4,4] and
+ * pts [n_p,3]); w is the adjoint tangent vector, cam_offset/pt_offset the first tangent
+ * column of each type; huber_delta <= 0 means no loss.  This is dL/dobs for
+ * w = (J~^T J~ + lambda I)^-1 dL/dtheta.  No counterpart in the reference
+ * (solvers.py:288 solves under torch.no_grad). */
+int tlh_reprojection_obs_vjp(const double* cams, const double* pts, const double* obs,
+                             const int64_t* cam_idx, const int64_t* pt_idx, const double* w,
+                             int64_t cam_offset, int64_t pt_offset, double huber_delta, int64_t count,
+                             double* grad, tlh_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,7 @@ class Slot(ctypes.Structure):
         ("seg_ptr", c_void_p),
         ("seg_start", c_void_p),
         ("seg_target", c_void_p),
+        ("pos_base", i64),
     ]
 
 
@@ -62,6 +63,8 @@ class Group(ctypes.Structure):
         ("tile_start", c_void_p),
         ("order", c_void_p),
         ("packed", c_void_p),
+        ("kernel", i32),
+        ("primary", i32),
         ("slot", Slot * TLH_MAX_SLOTS),
     ]
 
@@ -76,6 +79,9 @@ class Type(ctypes.Structure):
         ("csr_ptr", c_void_p),
         ("csr_idx", c_void_p),
         ("partials", c_void_p),
+        ("direct", c_void_p),
+        ("finish_list", c_void_p),
+        ("finish_count", i64),
     ]
 
 
@@ -200,6 +206,11 @@ SIGNATURES = {
     "tlh_group_tiles": (ctypes.c_int, [ctypes.POINTER(Operator), ctypes.c_int, c_void_p, c_void_p]),
     "tlh_cholesky_f64": (ctypes.c_int, [c_void_p, i64, i64, c_void_p, c_void_p]),
     "tlh_cholesky_solve_f64": (ctypes.c_int, [c_void_p, i64, i64, c_void_p, c_void_p]),
+    "tlh_reprojection_obs_vjp": (
+        ctypes.c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, i64, i64, ctypes.c_double, i64, c_void_p,
+         c_void_p],
+    ),
 }
 
 _lib = None

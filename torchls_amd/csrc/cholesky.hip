@@ -8,7 +8,9 @@
 //      others solve one 64-row block of the panel below: A_ik <- A_ik L_kk^-T;
 //   2. trailing kernel: one workgroup per lower 64x64 block (i >= j) of the
 //      trailing matrix computes A_ij -= L_ik L_jk^T with
-//      v_mfma_f64_16x16x4_f64 (4 waves x 4 16x16 tiles, K = 64).
+//      v_mfma_f64_16x16x4_f64 (4 waves x 4 16x16 tiles, K = 64).  Operand
+//      layout: lane l supplies A[l % 16][k0 + l / 16] and B[k0 + l / 16][l % 16];
+//      accumulator element v of lane l is C[4 v + l / 16][l % 16].
 // Non-positive or non-finite pivots set *status (LinAlgError analogue).
 #include "common.hpp"
 
@@ -125,7 +127,7 @@ __global__ void __launch_bounds__(TLH_THREADS)
   for (int t = 0; t < 4; ++t) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int64_t row = ri + 16 * w + 4 * (lane / 16) + v;
+      const int64_t row = ri + 16 * w + 4 * v + lane / 16;
       const int64_t col = rj + 16 * t + (lane % 16);
       if (row < n && col < n && col <= row) A[row * ld + col] -= acc[t][v];
     }

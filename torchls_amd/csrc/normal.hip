@@ -207,6 +207,183 @@ __global__ void __launch_bounds__(TLH_THREADS) normal_tile_kernel(GroupArgs<Real
   flush_dense(g, acc);
 }
 
+// Wave kernel for two-slot groups (bundle adjustment, pose-graph edges).
+// Primary slot: the tile is sorted by its variable, so each wavefront holds 64
+// consecutive observations; an inclusive segmented scan over the lanes
+// (shuffles, fixed tree) sums every run of one variable inside the wavefront,
+// and the run's last lane writes y directly (variables observed only there) or
+// a positional partial.  Secondary slot: contributions and variable ids are
+// staged in LDS in the tile's order by that variable; each segment head sums
+// its run serially into the dense LDS accumulator or a positional partial.
+template <typename Real>
+struct WaveArgs {
+  const Real* packed;
+  int64_t stride;
+  const int32_t* tile_start;
+  int32_t ntiles;
+  int32_t tiles_per_wg;
+  const int32_t* idx_p;
+  const int32_t* idx_s;
+  const uint16_t* rank_s;
+  const Real* x_p;
+  const Real* x_s;
+  Real* y_p;
+  const uint8_t* direct_p;
+  Real* part_p;
+  int64_t pos_base_p;
+  Real* part_s;
+  int64_t pos_base_s;
+  int32_t mode_s;
+  int32_t dense_block_s;
+  int32_t count_s;
+  Real damping;
+};
+
+template <typename Real, int R, int D0, int D1, int P, int ITEMS>
+__global__ void __launch_bounds__(TLH_THREADS) normal_wave_kernel(WaveArgs<Real> a) {
+  constexpr int DP = P == 0 ? D0 : D1;
+  constexpr int DS = P == 0 ? D1 : D0;
+  constexpr int OP = P == 0 ? 0 : R * D0;
+  constexpr int OS = P == 0 ? R * D0 : 0;
+  constexpr int T = TLH_THREADS * ITEMS;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Real* cstage = reinterpret_cast<Real*>(smem_raw);
+  int32_t* cid = reinterpret_cast<int32_t*>(cstage + T * DS);
+  Real* acc = reinterpret_cast<Real*>(cid + T);
+  const bool dense = a.mode_s == TLH_REDUCE_DENSE;
+  if (dense)
+    for (int i = threadIdx.x; i < a.count_s * DS; i += TLH_THREADS) acc[i] = Real(0);
+  const int lane = threadIdx.x & 63;
+
+  const int t0 = blockIdx.x * a.tiles_per_wg;
+  const int t1 = min(t0 + a.tiles_per_wg, a.ntiles);
+  for (int t = t0; t < t1; ++t) {
+    const int start = a.tile_start[t];
+    const int len = a.tile_start[t + 1] - start;
+    Real cp[ITEMS][DP], cs[ITEMS][DS], xp[ITEMS][DP];
+    int ip[ITEMS], is[ITEMS], rk[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+      const int pos = (int)threadIdx.x + it * TLH_THREADS;
+      const int64_t n = start + min(pos, len - 1);
+      ip[it] = a.idx_p[n];
+      is[it] = a.idx_s[n];
+      rk[it] = a.rank_s ? (int)a.rank_s[n] : pos;
+      Real jp[R][DP], js[R][DS], xs[DS];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+#pragma unroll
+        for (int c = 0; c < DP; ++c) jp[r][c] = a.packed[(int64_t)(OP + r * DP + c) * a.stride + n];
+#pragma unroll
+        for (int c = 0; c < DS; ++c) js[r][c] = a.packed[(int64_t)(OS + r * DS + c) * a.stride + n];
+      }
+#pragma unroll
+      for (int c = 0; c < DP; ++c) xp[it][c] = a.x_p[(int64_t)ip[it] * DP + c];
+#pragma unroll
+      for (int c = 0; c < DS; ++c) xs[c] = a.x_s[(int64_t)is[it] * DS + c];
+      Real q[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        Real v = Real(0);
+#pragma unroll
+        for (int c = 0; c < DP; ++c) v += jp[r][c] * xp[it][c];
+#pragma unroll
+        for (int c = 0; c < DS; ++c) v += js[r][c] * xs[c];
+        q[r] = v;
+      }
+      const bool valid = pos < len;
+#pragma unroll
+      for (int c = 0; c < DP; ++c) {
+        Real v = Real(0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v += jp[r][c] * q[r];
+        cp[it][c] = valid ? v : Real(0);
+      }
+#pragma unroll
+      for (int c = 0; c < DS; ++c) {
+        Real v = Real(0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v += js[r][c] * q[r];
+        cs[it][c] = v;
+      }
+    }
+    // primary slot: wavefront segmented scan
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+      const int pos = (int)threadIdx.x + it * TLH_THREADS;
+      const int prev = __shfl_up(ip[it], 1);
+      const int next = __shfl_down(ip[it], 1);
+      int flag = (lane == 0 || prev != ip[it]) ? 1 : 0;
+      Real v[DP];
+#pragma unroll
+      for (int c = 0; c < DP; ++c) v[c] = cp[it][c];
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int fo = __shfl_up(flag, off);
+        Real vo[DP];
+#pragma unroll
+        for (int c = 0; c < DP; ++c) vo[c] = __shfl_up(v[c], off);
+        if (lane >= off) {
+          if (!flag) {
+#pragma unroll
+            for (int c = 0; c < DP; ++c) v[c] = vo[c] + v[c];
+          }
+          flag |= fo;
+        }
+      }
+      const bool tail = lane == 63 || next != ip[it] || pos == len - 1;
+      if (pos < len && tail) {
+        const int64_t vv = ip[it];
+        if (a.direct_p && a.direct_p[vv]) {
+#pragma unroll
+          for (int c = 0; c < DP; ++c) a.y_p[vv * DP + c] = v[c] + a.damping * xp[it][c];
+        } else {
+          Real* out = a.part_p + (a.pos_base_p + start + pos) * DP;
+#pragma unroll
+          for (int c = 0; c < DP; ++c) out[c] = v[c];
+        }
+      }
+    }
+    // secondary slot: LDS stage in variable order, on-the-fly segment heads
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+      const int pos = (int)threadIdx.x + it * TLH_THREADS;
+      if (pos < len) {
+#pragma unroll
+        for (int c = 0; c < DS; ++c) cstage[rk[it] * DS + c] = cs[it][c];
+        cid[rk[it]] = is[it];
+      }
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < len; p += TLH_THREADS) {
+      const int v = cid[p];
+      if (p > 0 && cid[p - 1] == v) continue;
+      Real sum[DS];
+#pragma unroll
+      for (int c = 0; c < DS; ++c) sum[c] = cstage[p * DS + c];
+      for (int q = p + 1; q < len && cid[q] == v; ++q) {
+#pragma unroll
+        for (int c = 0; c < DS; ++c) sum[c] += cstage[q * DS + c];
+      }
+      if (dense) {
+#pragma unroll
+        for (int c = 0; c < DS; ++c) acc[v * DS + c] += sum[c];
+      } else {
+        Real* out = a.part_s + (a.pos_base_s + start + p) * DS;
+#pragma unroll
+        for (int c = 0; c < DS; ++c) out[c] = sum[c];
+      }
+    }
+    __syncthreads();
+  }
+  if (dense) {
+    const int64_t nn = (int64_t)a.count_s * DS;
+    Real* out = a.part_s + (int64_t)(a.dense_block_s + blockIdx.x) * nn;
+    for (int64_t i = threadIdx.x; i < nn; i += TLH_THREADS) out[i] = acc[i];
+  }
+}
+
 // Generic kernel: any rows (<= 16) and slot dims (<= 16); re-reads J per slot.
 constexpr int GEN_RMAX = 16;
 constexpr int GEN_DSUM = 48;
@@ -263,11 +440,14 @@ template <typename Real>
 __global__ void __launch_bounds__(TLH_THREADS)
     finish_kernel(Real* __restrict__ y, const Real* __restrict__ x, Real damping,
                   const Real* __restrict__ partials, const int32_t* __restrict__ csr_ptr,
-                  const int32_t* __restrict__ csr_idx, int64_t count, int dim, int dense_blocks) {
-  const int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
-  if (i >= count * dim) return;
-  const int64_t v = i / dim;
-  const int c = (int)(i - v * dim);
+                  const int32_t* __restrict__ csr_idx, int64_t count, int dim, int dense_blocks,
+                  const int32_t* __restrict__ list, int64_t nlist) {
+  const int64_t e = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
+  if (e >= nlist * dim) return;
+  const int64_t le = e / dim;
+  const int c = (int)(e - le * dim);
+  const int64_t v = list ? (int64_t)list[le] : le;
+  const int64_t i = v * dim + c;
   Real sum = Real(0);
   for (int b = 0; b < dense_blocks; ++b) sum += partials[((int64_t)b * count + v) * dim + c];
   const int k1 = csr_ptr[v + 1];
@@ -285,8 +465,11 @@ template <typename Real>
 __global__ void __launch_bounds__(TLH_THREADS)
     finish_dense_kernel(Real* __restrict__ y, const Real* __restrict__ x, Real damping,
                         const Real* __restrict__ partials, const int32_t* __restrict__ csr_ptr,
-                        const int32_t* __restrict__ csr_idx, int64_t count, int dim, int dense_blocks) {
+                        const int32_t* __restrict__ csr_idx, int64_t count, int dim, int dense_blocks,
+                        const int32_t* __restrict__ list, int64_t nlist) {
   __shared__ Real lane_sum[FIN_LANES][FIN_COLS];
+  (void)list;
+  (void)nlist;
   const int col = threadIdx.x % FIN_COLS;
   const int lane = threadIdx.x / FIN_COLS;
   const int64_t total = count * dim;
@@ -412,10 +595,78 @@ int launch_generic(const tlh_operator_t* op, const tlh_group_t& g, const void* x
   return 0;
 }
 
+template <typename Real>
+int64_t wave_lds(const tlh_operator_t* op, const tlh_group_t& g) {
+  const int p = g.primary;
+  const int ds = g.slot[1 - p].dim;
+  int64_t b = (int64_t)g.tile_cap * (ds * (int64_t)sizeof(Real) + 4);
+  if (g.slot[1 - p].mode == TLH_REDUCE_DENSE) b += op->types[g.slot[1 - p].type].count * ds * (int64_t)sizeof(Real);
+  return b;
+}
+
+template <typename Real, int R, int D0, int D1, int P>
+int launch_wave_t(const tlh_operator_t* op, const tlh_group_t& g, const void* x, void* y, double damping,
+                  hipStream_t stream) {
+  constexpr int ITEMS = items_for<Real>();
+  if (g.tile_cap > TLH_THREADS * ITEMS) return TLH_EINVAL;
+  const tlh_slot_t& sp = g.slot[P];
+  const tlh_slot_t& ss = g.slot[1 - P];
+  const tlh_type_t& tp = op->types[sp.type];
+  const tlh_type_t& ts = op->types[ss.type];
+  WaveArgs<Real> a{};
+  a.packed = static_cast<const Real*>(g.packed);
+  a.stride = g.stride;
+  a.tile_start = g.tile_start;
+  a.ntiles = g.ntiles;
+  a.tiles_per_wg = g.tiles_per_wg;
+  a.idx_p = sp.index;
+  a.idx_s = ss.index;
+  a.rank_s = ss.rank;
+  a.x_p = static_cast<const Real*>(x) + tp.offset;
+  a.x_s = static_cast<const Real*>(x) + ts.offset;
+  a.y_p = static_cast<Real*>(y) + tp.offset;
+  a.direct_p = tp.direct;
+  a.part_p = static_cast<Real*>(tp.partials);
+  a.pos_base_p = sp.pos_base;
+  a.part_s = static_cast<Real*>(ts.partials);
+  a.pos_base_s = ss.pos_base;
+  a.mode_s = ss.mode;
+  a.dense_block_s = ss.dense_block;
+  a.count_s = (int32_t)ts.count;
+  a.damping = (Real)damping;
+  const int64_t lds = (int64_t)TLH_THREADS * ITEMS * ((1 - P == 0 ? D0 : D1) * (int64_t)sizeof(Real) + 4) +
+                      (ss.mode == TLH_REDUCE_DENSE ? ts.count * (1 - P == 0 ? D0 : D1) * (int64_t)sizeof(Real) : 0);
+  auto kernel = normal_wave_kernel<Real, R, D0, D1, P, ITEMS>;
+  TLH_RETURN_IF_ERROR(allow_lds((const void*)kernel, lds));
+  hipLaunchKernelGGL(kernel, dim3(g.nwg), dim3(TLH_THREADS), (size_t)lds, stream, a);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+template <typename Real>
+int launch_wave(const tlh_operator_t* op, const tlh_group_t& g, const void* x, void* y, double damping,
+                hipStream_t stream) {
+  if (g.nslots != 2 || !y) return TLH_EINVAL;
+  const int R = g.rows, d0 = g.slot[0].dim, d1 = g.slot[1].dim;
+  const int P = g.primary;
+  if (P < 0 || P > 1 || g.slot[P].rank != nullptr) return TLH_EINVAL;
+  if (R == 2 && d0 == 6 && d1 == 3)
+    return P == 0 ? launch_wave_t<Real, 2, 6, 3, 0>(op, g, x, y, damping, stream)
+                  : launch_wave_t<Real, 2, 6, 3, 1>(op, g, x, y, damping, stream);
+  if (R == 2 && d0 == 3 && d1 == 6)
+    return P == 0 ? launch_wave_t<Real, 2, 3, 6, 0>(op, g, x, y, damping, stream)
+                  : launch_wave_t<Real, 2, 3, 6, 1>(op, g, x, y, damping, stream);
+  if (R == 6 && d0 == 6 && d1 == 6)
+    return P == 0 ? launch_wave_t<Real, 6, 6, 6, 0>(op, g, x, y, damping, stream)
+                  : launch_wave_t<Real, 6, 6, 6, 1>(op, g, x, y, damping, stream);
+  return TLH_ENOTFOUND;
+}
+
 // Shapes with a specialised kernel; everything else takes the generic kernel.
 template <typename Real>
-int launch_group(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
+int launch_group(const tlh_operator_t* op, const tlh_group_t& g, const void* x, void* y, double damping,
                  hipStream_t stream) {
+  if (g.kernel == 1) return launch_wave<Real>(op, g, x, y, damping, stream);
   const int R = g.rows;
   const int d0 = g.slot[0].dim;
   const int d1 = g.nslots > 1 ? g.slot[1].dim : 0;
@@ -440,7 +691,7 @@ int normal_matvec_impl(const tlh_operator_t* op, const void* x, void* y, double 
   for (int gi = 0; gi < op->ngroups; ++gi) {
     const tlh_group_t& g = op->groups[gi];
     if (g.count == 0 || g.ntiles == 0) continue;
-    const int rc = launch_group<Real>(op, g, x, stream);
+    const int rc = launch_group<Real>(op, g, x, y, damping, stream);
     if (rc) return rc;
   }
   return finish_impl<Real>(op, 0, op->ntypes, x, damping, y, stream);
@@ -453,18 +704,20 @@ int finish_impl(const tlh_operator_t* op, int t0, int t1, const void* x, double 
     const tlh_type_t& t = op->types[ti];
     const int64_t total = t.count * t.dim;
     if (total == 0) continue;
-    if (t.dense_blocks >= 2 * FIN_LANES)
+    const int64_t nlist = t.finish_list ? t.finish_count : t.count;
+    if (nlist == 0) continue;
+    if (t.dense_blocks >= 2 * FIN_LANES && !t.finish_list)
       hipLaunchKernelGGL(finish_dense_kernel<Real>, dim3((unsigned)ceil_div(total, FIN_COLS)),
                          dim3(TLH_THREADS), 0, stream, static_cast<Real*>(y) + t.offset,
                          static_cast<const Real*>(x) + t.offset, (Real)damping,
                          static_cast<const Real*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim,
-                         t.dense_blocks);
+                         t.dense_blocks, t.finish_list, nlist);
     else
-      hipLaunchKernelGGL(finish_kernel<Real>, dim3((unsigned)ceil_div(total, TLH_THREADS)),
+      hipLaunchKernelGGL(finish_kernel<Real>, dim3((unsigned)ceil_div(nlist * t.dim, TLH_THREADS)),
                          dim3(TLH_THREADS), 0, stream, static_cast<Real*>(y) + t.offset,
                          static_cast<const Real*>(x) + t.offset, (Real)damping,
                          static_cast<const Real*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim,
-                         t.dense_blocks);
+                         t.dense_blocks, t.finish_list, nlist);
     TLH_RETURN_IF_LAUNCH_FAILED();
   }
   return 0;
@@ -518,6 +771,7 @@ extern "C" int tlh_device_info(int device, tlh_device_info_t* out) {
 extern "C" int64_t tlh_group_lds_bytes(const tlh_operator_t* op, int group) {
   if (check_op(op) || group < 0 || group >= op->ngroups) return -1;
   const tlh_group_t& g = op->groups[group];
+  if (g.kernel == 1) return op->dtype == TLH_F32 ? wave_lds<float>(op, g) : wave_lds<double>(op, g);
   int dmax = 0;
   for (int s = 0; s < g.nslots; ++s) dmax = std::max(dmax, (int)g.slot[s].dim);
   if (op->dtype == TLH_F32) {
@@ -566,6 +820,7 @@ extern "C" int tlh_group_tiles(const tlh_operator_t* op, int group, const void* 
   const tlh_group_t& g = op->groups[group];
   if (g.count == 0 || g.ntiles == 0) return 0;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  if (op->dtype == TLH_F32) return launch_group<float>(op, g, x, s);
-  return launch_group<double>(op, g, x, s);
+  if (g.kernel != 0) return TLH_EINVAL;
+  if (op->dtype == TLH_F32) return launch_group<float>(op, g, x, nullptr, 0.0, s);
+  return launch_group<double>(op, g, x, nullptr, 0.0, s);
 }

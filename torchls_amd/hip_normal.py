@@ -66,6 +66,8 @@ class DeviceStructure:
             d.tile_start = put(g.tile_start).data_ptr()
             d.order = put(g.order).data_ptr()
             d.packed = None
+            d.kernel = g.kernel
+            d.primary = g.primary
             for si, s in enumerate(g.slots):
                 ds = d.slot[si]
                 ds.type = s.type
@@ -77,6 +79,7 @@ class DeviceStructure:
                 ds.seg_ptr = put(s.seg_ptr).data_ptr()
                 ds.seg_start = put(s.seg_start).data_ptr()
                 ds.seg_target = put(s.seg_target).data_ptr()
+                ds.pos_base = s.pos_base
         self.types = (_lib.Type * len(plan.types))()
         for ti, t in enumerate(plan.types):
             d = self.types[ti]
@@ -88,6 +91,10 @@ class DeviceStructure:
             d.csr_ptr = put(t.csr_ptr).data_ptr()
             d.csr_idx = put(t.csr_idx if t.csr_idx.size else np.zeros(1, np.int32)).data_ptr()
             d.partials = None
+            d.direct = None if t.direct is None else put(t.direct).data_ptr()
+            d.finish_list = None if t.finish_list is None else put(
+                t.finish_list if t.finish_list.size else np.zeros(1, np.int32)).data_ptr()
+            d.finish_count = t.count if t.finish_list is None else int(t.finish_list.size)
         self.sorted_index = None
 
     def _keep(self, t: torch.Tensor) -> torch.Tensor:
@@ -116,6 +123,10 @@ class DeviceStructure:
         return op
 
 
+# (rows, dim slot 0, dim slot 1) with an instantiated wave kernel (csrc/normal.hip)
+WAVE_SHAPES = frozenset({(2, 6, 3), (2, 3, 6), (6, 6, 6)})
+
+
 def plan_for(jacobian, dtype: torch.dtype, *, allow_dense: bool = True, tile_cap: int | None = None):
     """Build the StructurePlan and its device copy for a BlockJacobian."""
     device = jacobian.groups[0].blocks[0].device
@@ -138,6 +149,7 @@ def plan_for(jacobian, dtype: torch.dtype, *, allow_dense: bool = True, tile_cap
         cu_count=int(info.cu_count),
         allow_dense=allow_dense,
         tile_cap=tile_cap,
+        wave_shapes=WAVE_SHAPES if allow_dense else frozenset(),
     )
     return plan, DeviceStructure(plan, device)
 

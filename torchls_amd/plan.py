@@ -48,8 +48,9 @@ class SlotPlan:
     seg_ptr: np.ndarray  # int32 [ntiles+1]
     seg_start: np.ndarray  # int32 [nseg] tile-local
     seg_var: np.ndarray  # int64 [nseg] variable of each segment
-    seg_target: np.ndarray | None = None  # int32 [nseg] partial id / variable
+    seg_target: np.ndarray | None = None  # int32 [nseg] partial id / variable (-1: direct write)
     dense_block: int = 0
+    pos_base: int = 0  # wave groups: first positional partial id of this slot
 
 
 @dataclass
@@ -63,6 +64,7 @@ class GroupPlan:
     nwg: int
     tiles_per_wg: int
     slots: list
+    kernel: int = 0  # 0: segment-array kernel, 1: wave kernel (2 slots, specialised shape)
 
     @property
     def ntiles(self) -> int:
@@ -86,6 +88,8 @@ class TypePlan:
     sparse_partials: int = 0
     csr_ptr: np.ndarray = field(default_factory=lambda: np.zeros(1, np.int32))
     csr_idx: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    direct: np.ndarray | None = None  # uint8 [count]: written by its only wave piece
+    finish_list: np.ndarray | None = None  # int32 variables the finish kernel writes (None: all)
 
     @property
     def partial_count(self) -> int:
@@ -153,6 +157,27 @@ def _slot_segments(index_sorted: np.ndarray, tile_start: np.ndarray, piece: int 
     return rank, identity, local[heads], seg_ptr, v[heads]
 
 
+WAVE = 64
+
+
+def _wave_pieces(index_sorted: np.ndarray, tile_start: np.ndarray):
+    """Primary-slot pieces of the wave kernel: runs of one variable, also cut at
+    every 64-position boundary of each tile (one wavefront per 64 positions)."""
+    n = index_sorted.size
+    ntiles = tile_start.size - 1
+    tile_of = np.repeat(np.arange(ntiles, dtype=np.int64), np.diff(tile_start))
+    pos = np.arange(n, dtype=np.int64)
+    local = pos - tile_start[tile_of]
+    head = np.ones(n, dtype=bool)
+    head[1:] = index_sorted[1:] != index_sorted[:-1]
+    head |= (local % WAVE) == 0
+    heads = np.flatnonzero(head)
+    ends = np.append(heads[1:], n) - 1
+    seg_ptr = np.zeros(ntiles + 1, dtype=np.int64)
+    np.cumsum(np.bincount(tile_of[heads], minlength=ntiles), out=seg_ptr[1:])
+    return local[heads], seg_ptr, index_sorted[heads], ends
+
+
 def build_structure(
     layout: list,
     groups: list,
@@ -166,6 +191,7 @@ def build_structure(
     allow_dense: bool = True,
     force_primary: dict | None = None,
     split_primary: bool = True,
+    wave_shapes: frozenset = frozenset(),
 ) -> StructurePlan:
     """Plan the deterministic reduction.
 
@@ -192,17 +218,24 @@ def build_structure(
         tile_start = _cut_tiles(sorted_idx[primary], cap)
         ntiles = tile_start.size - 1
         dmax = max(types[t].dim for t in slot_types)
-        stage = cap * dmax * esize
+        shape = (int(rows), *(types[t].dim for t in slot_types))
+        wave = ns == 2 and shape in wave_shapes and count > 0
+        stage = cap * dmax * esize + (cap * 4 if wave else 0)
         dense_budget = lds_limit - stage
         slots = []
+        ends = None
         for s in range(ns):
             t = slot_types[s]
             dense_bytes = types[t].count * types[t].dim * esize
             dense = allow_dense and s != primary and ntiles > 1 and dense_bytes <= dense_budget
             if dense:
                 dense_budget -= dense_bytes
-            split = None if dense or (s == primary and not split_primary) else piece
-            rank, identity, seg_start, seg_ptr, seg_var = _slot_segments(sorted_idx[s], tile_start, split)
+            split = None if dense or (s == primary and not split_primary) or wave else piece
+            if wave and s == primary:
+                seg_start, seg_ptr, seg_var, ends = _wave_pieces(sorted_idx[s], tile_start)
+                rank, identity = None, True
+            else:
+                rank, identity, seg_start, seg_ptr, seg_var = _slot_segments(sorted_idx[s], tile_start, split)
             slots.append(
                 SlotPlan(
                     t,
@@ -215,6 +248,8 @@ def build_structure(
                     seg_var.astype(np.int64),
                 )
             )
+            if wave and s == primary:
+                slots[-1].piece_end = ends
         has_dense = any(sl.mode == REDUCE_DENSE for sl in slots)
         if has_dense and ntiles:
             nwg = min(ntiles, cu_count * wgs_per_cu)
@@ -223,7 +258,8 @@ def build_structure(
         else:
             nwg, per = ntiles, 1
         gplans.append(
-            GroupPlan(count, int(rows), primary, order, tile_start.astype(np.int32), cap, nwg, per, slots)
+            GroupPlan(count, int(rows), primary, order, tile_start.astype(np.int32), cap, nwg, per, slots,
+                      kernel=1 if wave else 0)
         )
     # dense blocks first (ids b*count + v), then sparse partials
     for g in gplans:
@@ -236,11 +272,58 @@ def build_structure(
     next_id = [t.dense_blocks * t.count for t in types]
     for g in gplans:
         for sl in g.slots:
-            if sl.mode == REDUCE_SPARSE:
+            if sl.mode == REDUCE_SPARSE and g.kernel == 0:
                 ids = next_id[sl.type] + np.arange(sl.seg_var.size, dtype=np.int64)
                 next_id[sl.type] += sl.seg_var.size
                 sl.seg_target = ids.astype(np.int32)
                 pairs[sl.type].append((sl.seg_var, ids))
+    # wave groups: positional partial ids (one region of `count` ids per slot)
+    for g in gplans:
+        if g.kernel != 1:
+            continue
+        for si, sl in enumerate(g.slots):
+            if sl.mode == REDUCE_DENSE:
+                continue
+            sl.pos_base = next_id[sl.type]
+            next_id[sl.type] += g.count
+            if si == g.primary:
+                ids = sl.pos_base + sl.piece_end
+            else:
+                tile_of = np.repeat(np.arange(g.ntiles, dtype=np.int64), np.diff(sl.seg_ptr))
+                ids = sl.pos_base + g.tile_start[tile_of].astype(np.int64) + sl.seg_start
+            sl.seg_target = ids.astype(np.int64)
+            pairs[sl.type].append((sl.seg_var, ids))
+    # a variable whose only contribution is one primary wave piece is written directly
+    for t, ty in enumerate(types):
+        contrib = np.zeros(ty.count, dtype=np.int64)
+        if ty.dense_blocks:
+            contrib += 2
+        for var, _ in pairs[t]:
+            contrib += np.bincount(var, minlength=ty.count)
+        direct = np.zeros(ty.count, dtype=bool)
+        for g in gplans:
+            if g.kernel == 1:
+                sl = g.slots[g.primary]
+                if sl.type == t:
+                    cand = np.zeros(ty.count, dtype=bool)
+                    cand[sl.seg_var] = True
+                    direct |= cand & (contrib == 1)
+        if direct.any():
+            ty.direct = direct.astype(np.uint8)
+            ty.finish_list = np.flatnonzero(~direct).astype(np.int32)
+            keep = []
+            for var, ids in pairs[t]:
+                m = ~direct[var]
+                keep.append((var[m], ids[m]))
+            pairs[t] = keep
+            for g in gplans:
+                if g.kernel == 1 and g.slots[g.primary].type == t:
+                    sl = g.slots[g.primary]
+                    sl.seg_target = np.where(direct[sl.seg_var], -1, sl.seg_target)
+    for g in gplans:
+        for sl in g.slots:
+            if sl.seg_target is not None and sl.seg_target.dtype != np.int32:
+                sl.seg_target = sl.seg_target.astype(np.int32)
     for t, ty in enumerate(types):
         ty.sparse_partials = next_id[t] - ty.dense_blocks * ty.count
         if next_id[t] > 2**31 - 1:
@@ -266,6 +349,7 @@ def emulate_normal_matvec(plan: StructurePlan, packed: list, x: np.ndarray, damp
     """
     types = plan.types
     parts = [np.zeros((t.partial_count, t.dim)) for t in types]
+    direct_out = []
     for g, gp in zip(plan.groups, packed):
         offs = np.cumsum([0] + [s.dim for s in g.slots])
         R = g.rows
@@ -296,6 +380,9 @@ def emulate_normal_matvec(plan: StructurePlan, packed: list, x: np.ndarray, damp
                     total = stage[lo:hi].sum(axis=0)
                     if s.mode == REDUCE_DENSE:
                         dense[si][tile // g.tiles_per_wg, s.seg_target[k]] += total
+                    elif s.seg_target[k] < 0:
+                        v = s.seg_var[k]
+                        direct_out.append((t, v, total))
                     else:
                         parts[s.type][s.seg_target[k]] = total
             if s.mode == REDUCE_DENSE:
@@ -312,6 +399,9 @@ def emulate_normal_matvec(plan: StructurePlan, packed: list, x: np.ndarray, damp
                 acc[v] += parts[ti][t.csr_idx[k]]
         sl = slice(t.offset, t.offset + t.count * t.dim)
         y[sl] = acc.reshape(-1) + damping * x[sl]
+    for t, v, total in direct_out:
+        sl = slice(t.offset + v * t.dim, t.offset + (v + 1) * t.dim)
+        y[sl] = total + damping * x[sl]
     return y
 
 

@@ -75,3 +75,34 @@ def test_readme_triangulation(device):
     P.add_residual(reprojection_guarded, ("points", torch.tensor([0, 0])), cameras.to(device), obs.to(device))
     sol = P.solve()
     assert torch.allclose(sol["points"].cpu(), torch.tensor([[1.0, 0.5, 5.0]], dtype=torch.float64), atol=1e-8)
+
+
+GOLDEN_CASES = {
+    "cg": dict(),
+    "cg_adaptive": dict(trust_region="adaptive"),
+    "schur_cg": dict(linear_solver="schur_cg", eliminate="points"),
+    "schur_dense": dict(linear_solver="schur_dense", eliminate="points"),
+}
+
+
+@pytest.mark.parametrize("name", list(GOLDEN_CASES))
+def test_hip_lm_against_reference_golden(device, name):
+    """HIP LM (warp_pcg control flow) vs the reference's own torch-backend trajectory."""
+    import os
+
+    from torchls_amd import TrustRegionConfig
+
+    print(f"golden lm {name}", flush=True)
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "lm.npz"), allow_pickle=False)
+    P = Problem({"cams": torch.tensor(g["cams0"], device=device), "points": torch.tensor(g["pts0"], device=device)},
+                manifolds={"cams": SE3Var})
+    P.add_residual(reprojection, ("cams", torch.tensor(g["cam_idx"])), ("points", torch.tensor(g["pt_idx"])),
+                   torch.tensor(g["uv"], device=device), loss=HuberLoss(3e-3))
+    opts = dict(GOLDEN_CASES[name])
+    if opts.pop("trust_region", None) == "adaptive":
+        opts["trust_region"] = TrustRegionConfig(strategy="adaptive")
+    sol, s = P.solve(return_summary=True, termination=TerminationConfig(max_iterations=20), **opts)
+    assert s.termination_reason == str(g[name + "_reason"])
+    assert abs(s.iterations - int(g[name + "_counts"][0])) <= 1
+    assert math.isclose(s.final_cost, float(g[name + "_cost"][-1]), rel_tol=1e-8)
+    assert np.allclose(sol["points"].cpu().numpy(), g[name + "_pts"], rtol=0, atol=1e-6)

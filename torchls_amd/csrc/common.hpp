@@ -64,3 +64,23 @@ inline hipError_t allow_lds(const void* kernel, int64_t bytes) {
 }
 
 }  // namespace tlh
+
+namespace tlh {
+
+// Fixed-order sum over one variable's CSR list of partials (component c of W),
+// by one wavefront: lane l adds entries begin+l, begin+l+64, ..., then a fixed
+// xor-shuffle tree combines the lanes; lane 0's value is the same every run.
+template <typename T>
+__device__ __forceinline__ T csr_wave_sum(const T* __restrict__ partials, const int32_t* __restrict__ idx, int begin,
+                                         int end, int W, int c, int lane) {
+  T s = T(0);
+  for (int k = begin + lane; k < end; k += 64) s += partials[(int64_t)idx[k] * W + c];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  return s;
+}
+
+// True when a type's partial lists are long enough for one wavefront per variable.
+inline bool csr_heavy(int64_t sparse_partials, int64_t count) { return count > 0 && sparse_partials >= 8 * count; }
+
+}  // namespace tlh

@@ -116,6 +116,24 @@ __global__ void __launch_bounds__(TLH_THREADS)
   blk[j * dim + i] = sum;
 }
 
+__global__ void __launch_bounds__(TLH_THREADS)
+    diag_finish_wave(double* __restrict__ D, const double* __restrict__ partials, const int32_t* __restrict__ csr_ptr,
+                     const int32_t* __restrict__ csr_idx, int64_t count, int dim) {
+  const int W = dim * (dim + 1) / 2;
+  const int lane = threadIdx.x & 63;
+  const int64_t v = (int64_t)blockIdx.x * (TLH_THREADS / 64) + (threadIdx.x >> 6);
+  if (v >= count) return;
+  for (int k = 0; k < W; ++k) {
+    const double s = csr_wave_sum<double>(partials, csr_idx, csr_ptr[v], csr_ptr[v + 1], W, k, lane);
+    if (lane == 0) {
+      int i, j;
+      upper_ij(k, dim, i, j);
+      D[v * dim * dim + i * dim + j] = s;
+      D[v * dim * dim + j * dim + i] = s;
+    }
+  }
+}
+
 template <typename Real>
 int diag_impl(const tlh_operator_t* op, double* const* out, hipStream_t stream, int64_t lds_limit) {
   for (int gi = 0; gi < op->ngroups; ++gi) {
@@ -160,9 +178,14 @@ int diag_impl(const tlh_operator_t* op, double* const* out, hipStream_t stream, 
     const int64_t total = t.count * (t.dim * (t.dim + 1) / 2);
     if (total == 0) continue;
     if (t.dense_blocks != 0) return TLH_EINVAL;
-    hipLaunchKernelGGL(diag_finish_kernel, dim3((unsigned)ceil_div(total, TLH_THREADS)), dim3(TLH_THREADS), 0,
-                       stream, out[ti], static_cast<const double*>(t.partials), t.csr_ptr, t.csr_idx,
-                       t.count, t.dim);
+    if (csr_heavy(t.sparse_partials, t.count))
+      hipLaunchKernelGGL(diag_finish_wave, dim3((unsigned)ceil_div(t.count, TLH_THREADS / 64)), dim3(TLH_THREADS), 0,
+                         stream, out[ti], static_cast<const double*>(t.partials), t.csr_ptr, t.csr_idx, t.count,
+                         t.dim);
+    else
+      hipLaunchKernelGGL(diag_finish_kernel, dim3((unsigned)ceil_div(total, TLH_THREADS)), dim3(TLH_THREADS), 0,
+                         stream, out[ti], static_cast<const double*>(t.partials), t.csr_ptr, t.csr_idx,
+                         t.count, t.dim);
     TLH_RETURN_IF_LAUNCH_FAILED();
   }
   return 0;

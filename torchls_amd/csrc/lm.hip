@@ -148,6 +148,19 @@ __global__ void __launch_bounds__(TLH_THREADS)
   out[i] = sum;
 }
 
+__global__ void __launch_bounds__(TLH_THREADS)
+    rmatvec_finish_wave(double* __restrict__ out, const double* __restrict__ partials,
+                        const int32_t* __restrict__ csr_ptr, const int32_t* __restrict__ csr_idx, int64_t count,
+                        int dim) {
+  const int lane = threadIdx.x & 63;
+  const int64_t v = (int64_t)blockIdx.x * (TLH_THREADS / 64) + (threadIdx.x >> 6);
+  if (v >= count) return;
+  for (int c = 0; c < dim; ++c) {
+    const double s = csr_wave_sum<double>(partials, csr_idx, csr_ptr[v], csr_ptr[v + 1], dim, c, lane);
+    if (lane == 0) out[v * dim + c] = s;
+  }
+}
+
 // ------------------------------------------------------- ||J x||^2 (float64)
 struct RawGroup {
   int64_t count;
@@ -309,9 +322,14 @@ extern "C" int tlh_rmatvec_finish_f64(const tlh_operator_t* op, double* out, tlh
     const tlh_type_t& t = op->types[ti];
     const int64_t total = t.count * t.dim;
     if (total == 0) continue;
-    hipLaunchKernelGGL(rmatvec_finish_kernel, dim3((unsigned)ceil_div(total, TLH_THREADS)), dim3(TLH_THREADS), 0,
-                       reinterpret_cast<hipStream_t>(stream), out + t.offset,
-                       static_cast<const double*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim);
+    if (csr_heavy(t.sparse_partials, t.count))
+      hipLaunchKernelGGL(rmatvec_finish_wave, dim3((unsigned)ceil_div(t.count, TLH_THREADS / 64)), dim3(TLH_THREADS),
+                         0, reinterpret_cast<hipStream_t>(stream), out + t.offset,
+                         static_cast<const double*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim);
+    else
+      hipLaunchKernelGGL(rmatvec_finish_kernel, dim3((unsigned)ceil_div(total, TLH_THREADS)), dim3(TLH_THREADS), 0,
+                         reinterpret_cast<hipStream_t>(stream), out + t.offset,
+                         static_cast<const double*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim);
     TLH_RETURN_IF_LAUNCH_FAILED();
   }
   return 0;

@@ -458,7 +458,7 @@ __global__ void __launch_bounds__(TLH_THREADS)
 // Finish for types with many dense blocks: a workgroup owns 32 consecutive
 // columns; its 8 column lanes each sum every 8th dense block (fixed stride),
 // then the 8 lane sums are added in lane order.  Same result for every run.
-constexpr int FIN_COLS = 32;
+constexpr int FIN_COLS = 8;
 constexpr int FIN_LANES = TLH_THREADS / FIN_COLS;
 
 template <typename Real>
@@ -489,6 +489,28 @@ __global__ void __launch_bounds__(TLH_THREADS)
     const int k1 = csr_ptr[v + 1];
     for (int k = csr_ptr[v]; k < k1; ++k) acc += partials[(int64_t)csr_idx[k] * dim + c];
     y[i] = acc + damping * x[i];
+  }
+}
+
+// One wavefront per variable (4 per workgroup) for types with long partial lists.
+template <typename Real>
+__global__ void __launch_bounds__(TLH_THREADS)
+    finish_wave_kernel(Real* __restrict__ y, const Real* __restrict__ x, Real damping,
+                       const Real* __restrict__ partials, const int32_t* __restrict__ csr_ptr,
+                       const int32_t* __restrict__ csr_idx, int64_t count, int dim, int dense_blocks,
+                       const int32_t* __restrict__ list, int64_t nlist) {
+  const int lane = threadIdx.x & 63;
+  const int64_t le = (int64_t)blockIdx.x * (TLH_THREADS / 64) + (threadIdx.x >> 6);
+  if (le >= nlist) return;
+  const int64_t v = list ? (int64_t)list[le] : le;
+  const int k0 = csr_ptr[v], k1 = csr_ptr[v + 1];
+  for (int c = 0; c < dim; ++c) {
+    Real sum = csr_wave_sum<Real>(partials, csr_idx, k0, k1, dim, c, lane);
+    if (lane == 0) {
+      Real d = Real(0);
+      for (int b = 0; b < dense_blocks; ++b) d += partials[((int64_t)b * count + v) * dim + c];
+      y[v * dim + c] = (d + sum) + damping * x[v * dim + c];
+    }
   }
 }
 
@@ -708,6 +730,12 @@ int finish_impl(const tlh_operator_t* op, int t0, int t1, const void* x, double 
     if (nlist == 0) continue;
     if (t.dense_blocks >= 2 * FIN_LANES && !t.finish_list)
       hipLaunchKernelGGL(finish_dense_kernel<Real>, dim3((unsigned)ceil_div(total, FIN_COLS)),
+                         dim3(TLH_THREADS), 0, stream, static_cast<Real*>(y) + t.offset,
+                         static_cast<const Real*>(x) + t.offset, (Real)damping,
+                         static_cast<const Real*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim,
+                         t.dense_blocks, t.finish_list, nlist);
+    else if (csr_heavy(t.sparse_partials, t.count))
+      hipLaunchKernelGGL(finish_wave_kernel<Real>, dim3((unsigned)ceil_div(nlist, TLH_THREADS / 64)),
                          dim3(TLH_THREADS), 0, stream, static_cast<Real*>(y) + t.offset,
                          static_cast<const Real*>(x) + t.offset, (Real)damping,
                          static_cast<const Real*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim,

@@ -333,6 +333,29 @@ __global__ void __launch_bounds__(TLH_THREADS)
   M[v * dim * dim + j * dim + i] = val;
 }
 
+__global__ void __launch_bounds__(TLH_THREADS)
+    schur_precond_finish_wave(double* __restrict__ M, const double* __restrict__ D, double damping,
+                              const double* __restrict__ partials, const int32_t* __restrict__ csr_ptr,
+                              const int32_t* __restrict__ csr_idx, int64_t count, int dim) {
+  const int W = dim * (dim + 1) / 2;
+  const int lane = threadIdx.x & 63;
+  const int64_t v = (int64_t)blockIdx.x * (TLH_THREADS / 64) + (threadIdx.x >> 6);
+  if (v >= count) return;
+  int i = 0, j = 0;
+  for (int k = 0; k < W; ++k) {
+    const double s = csr_wave_sum<double>(partials, csr_idx, csr_ptr[v], csr_ptr[v + 1], W, k, lane);
+    if (lane == 0) {
+      const double val = D[v * dim * dim + i * dim + j] + (i == j ? damping : 0.0) - s;
+      M[v * dim * dim + i * dim + j] = val;
+      M[v * dim * dim + j * dim + i] = val;
+    }
+    if (++j == dim) {
+      ++i;
+      j = i;
+    }
+  }
+}
+
 // ------------------------------------------------------- dense reduced matrix
 // One thread per unique camera pair (ci <= cj): sum over the pair's
 // observation pairs (i in edge(ci,p), j in edge(cj,p)) of
@@ -528,8 +551,13 @@ extern "C" int tlh_schur_precond(const tlh_operator_t* op, int wgroup, int eslot
   }
   const int64_t total = tc.count * W;
   if (total) {
-    hipLaunchKernelGGL(schur_precond_finish, dim3((unsigned)ceil_div(total, TLH_THREADS)), dim3(TLH_THREADS), 0, s,
-                       M, D, damping, static_cast<const double*>(tc.partials), tc.csr_ptr, tc.csr_idx, tc.count, DC);
+    if (csr_heavy(tc.sparse_partials, tc.count))
+      hipLaunchKernelGGL(schur_precond_finish_wave, dim3((unsigned)ceil_div(tc.count, TLH_THREADS / 64)),
+                         dim3(TLH_THREADS), 0, s, M, D, damping, static_cast<const double*>(tc.partials), tc.csr_ptr,
+                         tc.csr_idx, tc.count, DC);
+    else
+      hipLaunchKernelGGL(schur_precond_finish, dim3((unsigned)ceil_div(total, TLH_THREADS)), dim3(TLH_THREADS), 0, s,
+                         M, D, damping, static_cast<const double*>(tc.partials), tc.csr_ptr, tc.csr_idx, tc.count, DC);
     TLH_RETURN_IF_LAUNCH_FAILED();
   }
   return 0;

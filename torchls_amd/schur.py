@@ -109,9 +109,11 @@ class HipSchurPlan:
         self.plan = build_structure(self.rlayout, groups, esize=self._esize,
                                     lds_limit=int(self._info.lds_per_block) - stage_extra,
                                     cu_count=int(self._info.cu_count), force_primary=eprim, split_primary=False)
-        self.sparse_plan = build_structure(self.rlayout, groups, esize=self._esize,
+        # the preconditioner reduces W-group edge blocks only: its plan holds that group alone,
+        # so every camera partial it lists is written by tlh_schur_precond
+        self.sparse_plan = build_structure(self.rlayout, [groups[self.wgroup]], esize=self._esize,
                                            lds_limit=int(self._info.lds_per_block), allow_dense=False,
-                                           force_primary=eprim, split_primary=False)
+                                           force_primary={0: self.eslot}, split_primary=False)
         w = self.plan.groups[self.wgroup]
         prim = w.slots[self.eslot].index
         cuts = w.tile_start[1:-1]
@@ -178,7 +180,7 @@ class HipSchurLinearization:
             torch.empty((g.components, max(g.stride, 1)), dtype=p.dtype, device=p.device) for g in p.plan.groups
         ]
         self.op = p.dev.operator(p.dtype, self.packed, p.partials)
-        self.sop = p.sdev.operator(p.dtype, self.packed, p.pre_partials)
+        self.sop = p.sdev.operator(p.dtype, [self.packed[p.wgroup]], p.pre_partials)
         stream = _lib.stream_handle(p.device)
         keep = []
         for k, (gi, slots, _) in enumerate(p.groups):
@@ -271,14 +273,14 @@ class HipSchurSystem:
         """float64 retained blocks D + lambda I - sum W_e C^-1 W_e^T (schur.py:353-362)."""
         p = self.plan
         blocks = []
-        w = p.sparse_plan.groups[p.wgroup]
+        w = p.sparse_plan.groups[0]
         wtype = w.slots[1 - p.eslot].type
         lds = _lib.device_info(p.device).lds_per_block
         for k, t in enumerate(p.retained):
             D = self.lin.diagonal[t]
             if k == wtype:
                 M = torch.empty_like(D)
-                _lib.check(p.lib.tlh_schur_precond(ctypes.byref(self.lin.sop), p.wgroup, p.eslot, self.cinv.data_ptr(),
+                _lib.check(p.lib.tlh_schur_precond(ctypes.byref(self.lin.sop), 0, p.eslot, self.cinv.data_ptr(),
                                                    D.data_ptr(), self.damping, M.data_ptr(), lds,
                                                    _lib.stream_handle(p.device)), "tlh_schur_precond")
             else:

@@ -317,6 +317,47 @@ int tlh_cholesky_f64(double* A, int64_t n, int64_t ld, int32_t* status, tlh_stre
  * Replaces: torch.cholesky_solve (solvers.py:518). */
 int tlh_cholesky_solve_f64(const double* L, int64_t n, int64_t ld, double* b, tlh_stream_t stream);
 
+/* ---------------------------------------------------------------- reprojection residual */
+/* One reprojection residual batch r_i = pi(R_c p + t_c) - obs_i (pi(x,y,z) = (x/z, y/z);
+ * poses world-to-camera, rows [R | t], cam_stride 12 (3x4) or 16 (4x4) elements per
+ * pose; tangent [v; w] applied on the right) linearized in `dtype` (TLH_F32/TLH_F64;
+ * cams, pts, obs and all outputs in that dtype):
+ *   raw[i]  = r_i (may be NULL),  model[i] = sqrt(w_i rho'_i) r_i,
+ *   jc[i]   = sqrt(w_i rho'_i) [D R, -D R [p]x]  ([count,2,6]),
+ *   jp[i]   = sqrt(w_i rho'_i) D R               ([count,2,3]),
+ *   out2[0] = sum_i w_i rho(||r_i||^2) in float64 (fixed order), out2[1] = the number
+ *   of observations whose residual or blocks are not finite.
+ * rho is the block Huber loss for huber_delta > 0, else the squared norm; weights
+ * (float64 [count]) may be NULL.  partials: 2*TLH_PCG_MAX_BLOCKS doubles, counter: one
+ * zeroed uint32 (left zeroed).
+ * Replaces, for this residual: AnalyzedLeastSquaresProblem.linearize's vmapped
+ * jacrev/jacfwd and loss rescaling (linearization.py:62-128, losses.py:56-90). */
+int tlh_reprojection_linearize(int dtype, const void* cams, int64_t cam_stride, const void* pts,
+                               const void* obs, const int64_t* cam_idx, const int64_t* pt_idx,
+                               int64_t count, double huber_delta, const double* weights, void* raw,
+                               void* model, void* jc, void* jp, double* out2, double* partials,
+                               uint32_t* counter, tlh_stream_t stream);
+
+/* Trial objective of the same batch at new poses/points without materializing the
+ * residual: out2 = { sum w rho(||r_new||^2), sum w [rho(||before||^2) - rho(||r_new||^2)] }
+ * (divided-difference Huber reduction), before = the linearization's raw residual
+ * ([count*2], before_dtype, may be NULL).
+ * Replaces: AnalyzedLeastSquaresProblem._evaluate_trial for this residual
+ * (problem.py:315-348). */
+int tlh_reprojection_trial(int dtype, const void* cams, int64_t cam_stride, const void* pts,
+                           const void* obs, const int64_t* cam_idx, const int64_t* pt_idx, int64_t count,
+                           double huber_delta, const double* weights, const void* before,
+                           int before_dtype, double* out2, double* partials, uint32_t* counter,
+                           tlh_stream_t stream);
+
+/* out[i] = poses[i] Exp(delta[i]) for `count` SE(3) poses (row-major, rows 3 or 4 of 4
+ * columns, `dtype` storage; delta float64 [count, delta_stride], first 6 entries =
+ * [translation, rotation]); arithmetic in `dtype` like the torch retraction.
+ * Replaces: the vmapped SE3Var retraction of the LM update (geometry.py:40-57,
+ * 130-137). */
+int tlh_se3_retract(int dtype, const void* poses, int rows, int64_t count, const double* delta,
+                    int64_t delta_stride, void* out, tlh_stream_t stream);
+
 /* ---------------------------------------------------------------- implicit gradient */
 /* grad[i] = (rho'_i I + 2 rho''_i r_i r_i^T)(J_c,i w_c + J_p,i w_p) for every reprojection
  * residual r_i = pi(T_c p) - obs_i (float64, Jacobian recomputed from cams [n_c,4,4] and

@@ -60,7 +60,7 @@ def polish(P, steps=4):
 
 # refine = 0 is the single Gauss-Newton adjoint solve; its error is the Gauss-Newton
 # approximation of the Hessian at nonzero residuals (about 2% on this problem).
-@pytest.mark.parametrize("delta,refine,tol", [(None, 8, 1e-5), (3e-3, 8, 1e-5), (None, 0, 1e-1)])
+@pytest.mark.parametrize("delta,refine,tol", [(None, 30, 1e-5), (3e-3, 30, 1e-5), (None, 0, 1e-1)])
 def test_adjoint_gradient_matches_finite_differences(device, delta, refine, tol):
     print(f"implicit delta={delta} refine={refine}", flush=True)
     prob, P, obs, truth = build(device, delta)

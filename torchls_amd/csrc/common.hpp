@@ -80,7 +80,20 @@ __device__ __forceinline__ T csr_wave_sum(const T* __restrict__ partials, const 
   return s;
 }
 
-// True when a type's partial lists are long enough for one wavefront per variable.
+// (i, j) of the k-th entry of a row-major upper triangle of a d x d matrix.
+__device__ __forceinline__ void upper_index(int k, int d, int& i, int& j) {
+  i = 0;
+  int row = d;
+  while (k >= row) {
+    k -= row;
+    ++i;
+    --row;
+  }
+  j = i + k;
+}
+
+// True when a type's partial lists are long enough for one wavefront per
+// (variable, component); such kernels take count * W wavefronts.
 inline bool csr_heavy(int64_t sparse_partials, int64_t count) { return count > 0 && sparse_partials >= 8 * count; }
 
 }  // namespace tlh

@@ -121,16 +121,16 @@ __global__ void __launch_bounds__(TLH_THREADS)
                      const int32_t* __restrict__ csr_idx, int64_t count, int dim) {
   const int W = dim * (dim + 1) / 2;
   const int lane = threadIdx.x & 63;
-  const int64_t v = (int64_t)blockIdx.x * (TLH_THREADS / 64) + (threadIdx.x >> 6);
-  if (v >= count) return;
-  for (int k = 0; k < W; ++k) {
-    const double s = csr_wave_sum<double>(partials, csr_idx, csr_ptr[v], csr_ptr[v + 1], W, k, lane);
-    if (lane == 0) {
-      int i, j;
-      upper_ij(k, dim, i, j);
-      D[v * dim * dim + i * dim + j] = s;
-      D[v * dim * dim + j * dim + i] = s;
-    }
+  const int64_t e = (int64_t)blockIdx.x * (TLH_THREADS / 64) + (threadIdx.x >> 6);
+  if (e >= count * W) return;
+  const int64_t v = e / W;
+  const int k = (int)(e - v * W);
+  const double s = csr_wave_sum<double>(partials, csr_idx, csr_ptr[v], csr_ptr[v + 1], W, k, lane);
+  if (lane == 0) {
+    int i, j;
+    upper_index(k, dim, i, j);
+    D[v * dim * dim + i * dim + j] = s;
+    D[v * dim * dim + j * dim + i] = s;
   }
 }
 
@@ -179,7 +179,7 @@ int diag_impl(const tlh_operator_t* op, double* const* out, hipStream_t stream, 
     if (total == 0) continue;
     if (t.dense_blocks != 0) return TLH_EINVAL;
     if (csr_heavy(t.sparse_partials, t.count))
-      hipLaunchKernelGGL(diag_finish_wave, dim3((unsigned)ceil_div(t.count, TLH_THREADS / 64)), dim3(TLH_THREADS), 0,
+      hipLaunchKernelGGL(diag_finish_wave, dim3((unsigned)ceil_div(total, TLH_THREADS / 64)), dim3(TLH_THREADS), 0,
                          stream, out[ti], static_cast<const double*>(t.partials), t.csr_ptr, t.csr_idx, t.count,
                          t.dim);
     else

@@ -15,69 +15,11 @@
 #include <math.h>
 
 #include "common.hpp"
+#include "reduce.hpp"
 
 using namespace tlh;
 
 namespace {
-
-constexpr int RED_BLOCKS = TLH_PCG_MAX_BLOCKS;
-
-__device__ __forceinline__ void block_sum2(double (&v)[2], double* smem) {
-  smem[threadIdx.x] = v[0];
-  smem[TLH_THREADS + threadIdx.x] = v[1];
-  __syncthreads();
-  for (int w = TLH_THREADS / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      smem[threadIdx.x] += smem[threadIdx.x + w];
-      smem[TLH_THREADS + threadIdx.x] += smem[TLH_THREADS + threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  v[0] = smem[0];
-  v[1] = smem[TLH_THREADS];
-  __syncthreads();
-}
-
-// Fixed-order grid reduction of two sums; the last block writes out[0..1].
-__device__ __forceinline__ void grid_sum2(double (&v)[2], double* partials, uint32_t* counter, double* out,
-                                          double* smem) {
-  __shared__ int last;
-  block_sum2(v, smem);
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = v[0];
-    partials[RED_BLOCKS + blockIdx.x] = v[1];
-    const uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = t == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  double a[2] = {0.0, 0.0};
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += TLH_THREADS) {
-    a[0] += __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    a[1] += __hip_atomic_load(&partials[RED_BLOCKS + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  block_sum2(a, smem);
-  if (threadIdx.x == 0) {
-    out[0] = a[0];
-    out[1] = a[1];
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
-inline int red_grid(int64_t work) {
-  int64_t g = ceil_div(work, TLH_THREADS);
-  return (int)(g < 1 ? 1 : (g > RED_BLOCKS ? RED_BLOCKS : g));
-}
-
-template <typename T>
-__device__ __forceinline__ double ld(const void* p, int64_t i) {
-  return (double)static_cast<const T*>(p)[i];
-}
-
-__device__ __forceinline__ double load_any(const void* p, int64_t i, int dtype) {
-  return dtype == TLH_F64 ? ld<double>(p, i) : ld<float>(p, i);
-}
 
 // ------------------------------------------------------------ rmatvec (f64)
 struct RmatvecArgs {
@@ -153,12 +95,12 @@ __global__ void __launch_bounds__(TLH_THREADS)
                         const int32_t* __restrict__ csr_ptr, const int32_t* __restrict__ csr_idx, int64_t count,
                         int dim) {
   const int lane = threadIdx.x & 63;
-  const int64_t v = (int64_t)blockIdx.x * (TLH_THREADS / 64) + (threadIdx.x >> 6);
-  if (v >= count) return;
-  for (int c = 0; c < dim; ++c) {
-    const double s = csr_wave_sum<double>(partials, csr_idx, csr_ptr[v], csr_ptr[v + 1], dim, c, lane);
-    if (lane == 0) out[v * dim + c] = s;
-  }
+  const int64_t e = (int64_t)blockIdx.x * (TLH_THREADS / 64) + (threadIdx.x >> 6);
+  if (e >= count * dim) return;
+  const int64_t v = e / dim;
+  const int c = (int)(e - v * dim);
+  const double s = csr_wave_sum<double>(partials, csr_idx, csr_ptr[v], csr_ptr[v + 1], dim, c, lane);
+  if (lane == 0) out[e] = s;
 }
 
 // ------------------------------------------------------- ||J x||^2 (float64)
@@ -217,29 +159,6 @@ __global__ void __launch_bounds__(TLH_THREADS)
     else
       static_cast<float*>(y)[o * R + r] = (float)q[r];
   }
-}
-
-// ---------------------------------------------------- trial objective (f64)
-__device__ __forceinline__ double huber_value(double s, double delta) {
-  const double norm = sqrt(fmax(s, 2.2250738585072014e-308));
-  return s > delta * delta ? delta * (2.0 * norm - delta) : s;
-}
-
-// Divided difference of the Huber loss between two squared norms (losses.py:95-107).
-__device__ __forceinline__ double huber_reduction(double before, double after, double diff, double delta) {
-  const double tiny = 2.2250738585072014e-308;
-  const double a = sqrt(before), b = sqrt(after);
-  const double low = fmin(a, b), high = fmax(a, b);
-  const double shortfall = fmax(delta - low, 0.0);
-  const double crossing = (2.0 * delta - shortfall * (shortfall / fmax(high - low, tiny))) / fmax(a + b, tiny);
-  double coef;
-  if (high <= delta)
-    coef = 1.0;
-  else if (low >= delta)
-    coef = 2.0 * delta / fmax(a + b, tiny);
-  else
-    coef = crossing;
-  return diff * coef;
 }
 
 __global__ void __launch_bounds__(TLH_THREADS)
@@ -323,7 +242,7 @@ extern "C" int tlh_rmatvec_finish_f64(const tlh_operator_t* op, double* out, tlh
     const int64_t total = t.count * t.dim;
     if (total == 0) continue;
     if (csr_heavy(t.sparse_partials, t.count))
-      hipLaunchKernelGGL(rmatvec_finish_wave, dim3((unsigned)ceil_div(t.count, TLH_THREADS / 64)), dim3(TLH_THREADS),
+      hipLaunchKernelGGL(rmatvec_finish_wave, dim3((unsigned)ceil_div(total, TLH_THREADS / 64)), dim3(TLH_THREADS),
                          0, reinterpret_cast<hipStream_t>(stream), out + t.offset,
                          static_cast<const double*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim);
     else

@@ -492,7 +492,7 @@ __global__ void __launch_bounds__(TLH_THREADS)
   }
 }
 
-// One wavefront per variable (4 per workgroup) for types with long partial lists.
+// One wavefront per (variable, component) for types with long partial lists.
 template <typename Real>
 __global__ void __launch_bounds__(TLH_THREADS)
     finish_wave_kernel(Real* __restrict__ y, const Real* __restrict__ x, Real damping,
@@ -500,17 +500,16 @@ __global__ void __launch_bounds__(TLH_THREADS)
                        const int32_t* __restrict__ csr_idx, int64_t count, int dim, int dense_blocks,
                        const int32_t* __restrict__ list, int64_t nlist) {
   const int lane = threadIdx.x & 63;
-  const int64_t le = (int64_t)blockIdx.x * (TLH_THREADS / 64) + (threadIdx.x >> 6);
-  if (le >= nlist) return;
+  const int64_t e = (int64_t)blockIdx.x * (TLH_THREADS / 64) + (threadIdx.x >> 6);
+  if (e >= nlist * dim) return;
+  const int64_t le = e / dim;
+  const int c = (int)(e - le * dim);
   const int64_t v = list ? (int64_t)list[le] : le;
-  const int k0 = csr_ptr[v], k1 = csr_ptr[v + 1];
-  for (int c = 0; c < dim; ++c) {
-    Real sum = csr_wave_sum<Real>(partials, csr_idx, k0, k1, dim, c, lane);
-    if (lane == 0) {
-      Real d = Real(0);
-      for (int b = 0; b < dense_blocks; ++b) d += partials[((int64_t)b * count + v) * dim + c];
-      y[v * dim + c] = (d + sum) + damping * x[v * dim + c];
-    }
+  const Real sum = csr_wave_sum<Real>(partials, csr_idx, csr_ptr[v], csr_ptr[v + 1], dim, c, lane);
+  if (lane == 0) {
+    Real d = Real(0);
+    for (int b = 0; b < dense_blocks; ++b) d += partials[((int64_t)b * count + v) * dim + c];
+    y[v * dim + c] = (d + sum) + damping * x[v * dim + c];
   }
 }
 
@@ -735,7 +734,7 @@ int finish_impl(const tlh_operator_t* op, int t0, int t1, const void* x, double 
                          static_cast<const Real*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim,
                          t.dense_blocks, t.finish_list, nlist);
     else if (csr_heavy(t.sparse_partials, t.count))
-      hipLaunchKernelGGL(finish_wave_kernel<Real>, dim3((unsigned)ceil_div(nlist, TLH_THREADS / 64)),
+      hipLaunchKernelGGL(finish_wave_kernel<Real>, dim3((unsigned)ceil_div(nlist * t.dim, TLH_THREADS / 64)),
                          dim3(TLH_THREADS), 0, stream, static_cast<Real*>(y) + t.offset,
                          static_cast<const Real*>(x) + t.offset, (Real)damping,
                          static_cast<const Real*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim,

@@ -309,6 +309,103 @@ __global__ void __launch_bounds__(TLH_THREADS)
   }
 }
 
+// Same result as schur_precond_kernel for fixed block shapes: every edge value
+// lives in registers (no local arrays), all W = DC(DC+1)/2 components are
+// computed once per pass, and the LDS staging takes ceil(W / chunk) passes.
+template <typename Real, int R, int DC, int DP, bool CFIRST>
+__global__ void __launch_bounds__(TLH_THREADS)
+    schur_precond_fixed(SchurArgs<Real> a, int chunk) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* stage = reinterpret_cast<double*>(smem_raw);
+  constexpr int OC = CFIRST ? 0 : R * DP;
+  constexpr int OP = CFIRST ? R * DC : 0;
+  constexpr int W = DC * (DC + 1) / 2;
+  constexpr int WP = DP * (DP + 1) / 2;
+  double* part = reinterpret_cast<double*>(a.part_c);
+  const int t = blockIdx.x;
+  if (t >= a.ntiles) return;
+  const int start = a.tile_start[t];
+  const int len = a.tile_start[t + 1] - start;
+  for (int c0 = 0; c0 < W; c0 += chunk) {
+    const int cw = min(chunk, W - c0);
+    __syncthreads();
+    for (int pos = threadIdx.x; pos < len; pos += TLH_THREADS) {
+      const int64_t n = start + pos;
+      const int rk = a.rank_c ? (int)a.rank_c[n] : pos;
+      const int32_t ic = a.idx_c[n], ip = a.idx_p[n];
+      const bool head = pos == 0 || ic != a.idx_c[n - 1] || ip != a.idx_p[n - 1];
+      double vals[W];
+#pragma unroll
+      for (int k = 0; k < W; ++k) vals[k] = 0.0;
+      if (head) {
+        double w[DC][DP];
+#pragma unroll
+        for (int i = 0; i < DC; ++i)
+#pragma unroll
+          for (int j = 0; j < DP; ++j) w[i][j] = 0.0;
+        for (int m = pos; m < len; ++m) {
+          const int64_t nm = start + m;
+          if (m > pos && (a.idx_c[nm] != ic || a.idx_p[nm] != ip)) break;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            double jp[DP];
+#pragma unroll
+            for (int j = 0; j < DP; ++j) jp[j] = (double)a.packed[(int64_t)(OP + r * DP + j) * a.stride + nm];
+#pragma unroll
+            for (int i = 0; i < DC; ++i) {
+              const double ci = (double)a.packed[(int64_t)(OC + r * DC + i) * a.stride + nm];
+#pragma unroll
+              for (int j = 0; j < DP; ++j) w[i][j] += ci * jp[j];
+            }
+          }
+        }
+        double cm[WP];
+        const double* mp = a.cinv + (int64_t)ip * WP;
+#pragma unroll
+        for (int k = 0; k < WP; ++k) cm[k] = mp[k];
+        double wm[DC][DP];
+#pragma unroll
+        for (int i = 0; i < DC; ++i)
+#pragma unroll
+          for (int j = 0; j < DP; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < DP; ++k) {
+              const int lo = k < j ? k : j, hi = k < j ? j : k;
+              acc += w[i][k] * cm[lo * DP - lo * (lo - 1) / 2 + (hi - lo)];
+            }
+            wm[i][j] = acc;
+          }
+        int e = 0;
+#pragma unroll
+        for (int i = 0; i < DC; ++i)
+#pragma unroll
+          for (int j = i; j < DC; ++j, ++e) {
+            double acc = 0.0;
+#pragma unroll
+            for (int l = 0; l < DP; ++l) acc += wm[i][l] * w[j][l];
+            vals[e] = acc;
+          }
+      }
+#pragma unroll
+      for (int k = 0; k < W; ++k)
+        if (k >= c0 && k < c0 + cw) stage[rk * cw + (k - c0)] = vals[k];
+    }
+    __syncthreads();
+    const int sb = a.seg_ptr_c[t], se = a.seg_ptr_c[t + 1];
+    for (int k = sb + (int)threadIdx.x; k < se; k += TLH_THREADS) {
+      const int lo = a.seg_start_c[k];
+      const int hi = (k + 1 < se) ? a.seg_start_c[k + 1] : len;
+      double* dst = part + (int64_t)a.seg_target_c[k] * W + c0;
+      for (int c = 0; c < cw; ++c) {
+        double acc = stage[lo * cw + c];
+        for (int i = lo + 1; i < hi; ++i) acc += stage[i * cw + c];
+        dst[c] = acc;
+      }
+    }
+  }
+}
+
 // M_c = D_c + lambda I - sum of the camera's edge partials (full symmetric).
 __global__ void __launch_bounds__(TLH_THREADS)
     schur_precond_finish(double* __restrict__ M, const double* __restrict__ D, double damping,
@@ -339,20 +436,17 @@ __global__ void __launch_bounds__(TLH_THREADS)
                               const int32_t* __restrict__ csr_idx, int64_t count, int dim) {
   const int W = dim * (dim + 1) / 2;
   const int lane = threadIdx.x & 63;
-  const int64_t v = (int64_t)blockIdx.x * (TLH_THREADS / 64) + (threadIdx.x >> 6);
-  if (v >= count) return;
-  int i = 0, j = 0;
-  for (int k = 0; k < W; ++k) {
-    const double s = csr_wave_sum<double>(partials, csr_idx, csr_ptr[v], csr_ptr[v + 1], W, k, lane);
-    if (lane == 0) {
-      const double val = D[v * dim * dim + i * dim + j] + (i == j ? damping : 0.0) - s;
-      M[v * dim * dim + i * dim + j] = val;
-      M[v * dim * dim + j * dim + i] = val;
-    }
-    if (++j == dim) {
-      ++i;
-      j = i;
-    }
+  const int64_t e = (int64_t)blockIdx.x * (TLH_THREADS / 64) + (threadIdx.x >> 6);
+  if (e >= count * W) return;
+  const int64_t v = e / W;
+  const int k = (int)(e - v * W);
+  const double s = csr_wave_sum<double>(partials, csr_idx, csr_ptr[v], csr_ptr[v + 1], W, k, lane);
+  if (lane == 0) {
+    int i, j;
+    upper_index(k, dim, i, j);
+    const double val = D[v * dim * dim + i * dim + j] + (i == j ? damping : 0.0) - s;
+    M[v * dim * dim + i * dim + j] = val;
+    M[v * dim * dim + j * dim + i] = val;
   }
 }
 
@@ -531,11 +625,40 @@ extern "C" int tlh_schur_precond(const tlh_operator_t* op, int wgroup, int eslot
   if (DC > 8 || DP > 8 || g.slot[sc].mode != TLH_REDUCE_SPARSE) return TLH_EINVAL;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const int W = DC * (DC + 1) / 2;
-  int chunk = (int)std::min<int64_t>(std::min(W, 8), lds_limit / ((int64_t)g.tile_cap * 8));
+  const bool fixed = R == 2 && DC == 6 && DP == 3;
+  const int64_t fit = lds_limit / ((int64_t)g.tile_cap * 8);
+  int chunk = (int)std::min<int64_t>(std::min(W, fixed ? W : 8), fit);
   if (chunk < 1) return TLH_ELDS;
+  if (fixed) chunk = (int)ceil_div(W, ceil_div(W, chunk));  // equal passes
   const int64_t lds = (int64_t)g.tile_cap * chunk * 8;
   const tlh_type_t& tc = op->types[g.slot[sc].type];
-  if (g.count > 0) {
+  if (g.count > 0 && fixed) {
+    const bool cfirst = sc < se;
+    if (op->dtype == TLH_F32) {
+      SchurArgs<float> a = make_schur_args<float>(op, g, sc, se, nullptr, nullptr, nullptr, cinv);
+      const void* k = cfirst ? (const void*)schur_precond_fixed<float, 2, 6, 3, true>
+                             : (const void*)schur_precond_fixed<float, 2, 6, 3, false>;
+      TLH_RETURN_IF_ERROR(allow_lds(k, lds));
+      if (cfirst)
+        hipLaunchKernelGGL((schur_precond_fixed<float, 2, 6, 3, true>), dim3(g.ntiles), dim3(TLH_THREADS),
+                           (size_t)lds, s, a, chunk);
+      else
+        hipLaunchKernelGGL((schur_precond_fixed<float, 2, 6, 3, false>), dim3(g.ntiles), dim3(TLH_THREADS),
+                           (size_t)lds, s, a, chunk);
+    } else {
+      SchurArgs<double> a = make_schur_args<double>(op, g, sc, se, nullptr, nullptr, nullptr, cinv);
+      const void* k = cfirst ? (const void*)schur_precond_fixed<double, 2, 6, 3, true>
+                             : (const void*)schur_precond_fixed<double, 2, 6, 3, false>;
+      TLH_RETURN_IF_ERROR(allow_lds(k, lds));
+      if (cfirst)
+        hipLaunchKernelGGL((schur_precond_fixed<double, 2, 6, 3, true>), dim3(g.ntiles), dim3(TLH_THREADS),
+                           (size_t)lds, s, a, chunk);
+      else
+        hipLaunchKernelGGL((schur_precond_fixed<double, 2, 6, 3, false>), dim3(g.ntiles), dim3(TLH_THREADS),
+                           (size_t)lds, s, a, chunk);
+    }
+    TLH_RETURN_IF_LAUNCH_FAILED();
+  } else if (g.count > 0) {
     if (op->dtype == TLH_F32) {
       SchurArgs<float> a = make_schur_args<float>(op, g, sc, se, nullptr, nullptr, nullptr, cinv);
       TLH_RETURN_IF_ERROR(allow_lds((const void*)schur_precond_kernel<float>, lds));
@@ -552,7 +675,7 @@ extern "C" int tlh_schur_precond(const tlh_operator_t* op, int wgroup, int eslot
   const int64_t total = tc.count * W;
   if (total) {
     if (csr_heavy(tc.sparse_partials, tc.count))
-      hipLaunchKernelGGL(schur_precond_finish_wave, dim3((unsigned)ceil_div(tc.count, TLH_THREADS / 64)),
+      hipLaunchKernelGGL(schur_precond_finish_wave, dim3((unsigned)ceil_div(total, TLH_THREADS / 64)),
                          dim3(TLH_THREADS), 0, s, M, D, damping, static_cast<const double*>(tc.partials), tc.csr_ptr,
                          tc.csr_idx, tc.count, DC);
     else

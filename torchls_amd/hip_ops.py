@@ -171,3 +171,45 @@ def jacobian_operator(jacobian) -> JacobianOperator:
         op = JacobianOperator(jacobian)
         object.__setattr__(jacobian, "_hip_operator", op)
     return op
+
+
+def _reprojection_common(operands, idx, weights):
+    cams, pts, obs, delta = operands
+    n = idx[0].numel()
+    wptr = None if weights is None else weights.data_ptr()
+    return cams, pts, obs, delta, n, wptr
+
+
+def reprojection_linearize(red: Reduction, slot: int, operands, idx, weights, want_raw: bool):
+    """HIP linearization of one reprojection batch (tlh_reprojection_linearize).
+
+    Returns (raw residual or None, model residual, (J_c, J_p), objective), the
+    objective being the float64 sum w rho(||r||^2) as a 0-d view of
+    ``red.out[slot, 0]``."""
+    cams, pts, obs, delta, n, wptr = _reprojection_common(operands, idx, weights)
+    dt, dev = cams.dtype, cams.device
+    raw = torch.empty((n, 2), dtype=dt, device=dev) if want_raw else None
+    model = torch.empty((n, 2), dtype=dt, device=dev)
+    jc = torch.empty((n, 2, 6), dtype=dt, device=dev)
+    jp = torch.empty((n, 2, 3), dtype=dt, device=dev)
+    lib = _lib.load()
+    _lib.check(lib.tlh_reprojection_linearize(
+        _lib.dtype_code(dt), cams.data_ptr(), cams[0].numel(), pts.data_ptr(), obs.data_ptr(),
+        idx[0].data_ptr(), idx[1].data_ptr(), n, delta, wptr, None if raw is None else raw.data_ptr(),
+        model.data_ptr(), jc.data_ptr(), jp.data_ptr(), red.out[slot].data_ptr(), red.partials.data_ptr(),
+        red.counter.data_ptr(), _lib.stream_handle(dev)), "tlh_reprojection_linearize")
+    return raw, model, (jc, jp), red.out[slot, 0]
+
+
+def reprojection_trial(red: Reduction, slot: int, operands, idx, weights, before) -> None:
+    """red.out[slot] = (sum w rho(||r_new||^2), trial reduction) (tlh_reprojection_trial)."""
+    cams, pts, obs, delta, n, wptr = _reprojection_common(operands, idx, weights)
+    if before is not None:
+        before = before.contiguous()
+    lib = _lib.load()
+    _lib.check(lib.tlh_reprojection_trial(
+        _lib.dtype_code(cams.dtype), cams.data_ptr(), cams[0].numel(), pts.data_ptr(), obs.data_ptr(),
+        idx[0].data_ptr(), idx[1].data_ptr(), n, delta, wptr,
+        None if before is None else before.data_ptr(), _lib.dtype_code(before.dtype) if before is not None else 0,
+        red.out[slot].data_ptr(), red.partials.data_ptr(), red.counter.data_ptr(), _lib.stream_handle(cams.device)),
+        "tlh_reprojection_trial")

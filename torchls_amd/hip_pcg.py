@@ -143,14 +143,19 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph):
             dot_into(r, r, _lib.PCG_NORM2)
         _lib.check(lib.tlh_pcg_direction(p.data_ptr(), z.data_ptr(), n, code, sd, sh), "tlh_pcg_direction")
 
-    # Warm up once (as warp_pcg.py:162-164), capture one iteration, reset.
+    # Warm up once (as warp_pcg.py:162-164), capture one iteration, reset.  The
+    # capture is begun on the work stream directly: torch.cuda.graph's context
+    # manager would add a device synchronize, a garbage collection and an
+    # allocator cache flush to every solve.
     iteration()
-    stream.synchronize()
     graph = None
     if use_graph:
         graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(graph, stream=stream):
+        graph.capture_begin()
+        try:
             iteration()
+        finally:
+            graph.capture_end()
     reset(rhs, True)
     st.control.zero_()
 

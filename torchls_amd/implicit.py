@@ -10,9 +10,11 @@ Backward (no reference counterpart; the reference solve is
    through the retraction at delta = 0;
 2. one adjoint PCG solve ``(J~^T J~ + lambda I) w = dL/dtheta`` on the solver's
    own HIP operator at the optimum (HIP normal matvec, block Jacobi, HIP PCG);
-   ``refine > 0`` adds iterative-refinement steps against the exact Hessian
-   (``torch.func`` Hessian-vector products of the float64 objective), which
-   removes the Gauss-Newton approximation error;
+   ``refine > 0`` continues with up to ``refine`` conjugate-gradient steps on
+   the exact Hessian (``torch.func`` Hessian-vector products of the float64
+   objective), preconditioned by that Gauss-Newton solve, which removes the
+   Gauss-Newton approximation error (large for Huber outliers, whose exact
+   curvature along their residual vanishes);
 3. the HIP backward kernel ``tlh_reprojection_obs_vjp`` recomputes each
    observation's Jacobian blocks and applies the residual-vs-observation
    Jacobian transpose: ``dL/dobs_i = (rho' I + 2 rho'' r r^T)(J_i w)``.
@@ -77,6 +79,28 @@ def _objective(analysis, values, delta):
     return total
 
 
+def _exact_adjoint(hvp, precondition, b, w, iterations, rtol=1e-12):
+    """Conjugate gradients on H w = b from ``w`` (H = exact Hessian through ``hvp``),
+    preconditioned by the Gauss-Newton solve ``precondition``."""
+    r = b - hvp(w)
+    z = precondition(r)
+    p = z.clone()
+    rz = torch.dot(r, z)
+    bound = rtol * float(torch.linalg.vector_norm(b))
+    for _ in range(iterations):
+        if float(torch.linalg.vector_norm(r)) <= bound:
+            break
+        q = hvp(p)
+        alpha = rz / torch.dot(p, q)
+        w = w + alpha * p
+        r = r - alpha * q
+        z = precondition(r)
+        rz_next = torch.dot(r, z)
+        p = z + (rz_next / rz) * p
+        rz = rz_next
+    return w
+
+
 class _ImplicitSolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, observations, state):
@@ -106,13 +130,17 @@ class _ImplicitSolve(torch.autograd.Function):
         pre = HipBlockJacobi(J.layout, tuple(d.double() for d in diag), lam)
         cfg = st["adjoint_config"]
         matvec = lambda v: op.normal_matvec(v, lam)  # noqa: E731
-        w = hip_pcg(matvec, b, preconditioner=pre, config=cfg).solution
-        for _ in range(st["refine"]):
+        gn_solve = lambda v: hip_pcg(matvec, v, preconditioner=pre, config=cfg).solution  # noqa: E731
+        w = gn_solve(b)
+        if st["refine"] > 0:
             zero = torch.zeros_like(w)
-            with torch.enable_grad():
-                hv = torch.func.jvp(torch.func.grad(lambda d: _objective(analysis, values, d)), (zero,), (w,))[1]
-            corr = hip_pcg(matvec, b - hv - lam * w, preconditioner=pre, config=cfg).solution
-            w = w + corr
+
+            def hvp(v):
+                with torch.enable_grad():
+                    hv = torch.func.jvp(torch.func.grad(lambda d: _objective(analysis, values, d)), (zero,), (v,))[1]
+                return hv + lam * v
+
+            w = _exact_adjoint(hvp, gn_solve, b, w, st["refine"])
         k, pos, opts = st["batch"], st["arg"], st["opts"]
         batch = analysis.batches[k]
         cams = values[analysis.layout[batch.type_indices[0]].name].to(torch.float64).contiguous()

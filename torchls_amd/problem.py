@@ -13,7 +13,9 @@ JVP chart (or a user ``jacobian_fn``), then the loss rescaling and the optional
 weights.  Residual functions may also provide batched evaluators as attributes
 (``fn.batched_residual(*args)`` and ``fn.batched_linearize(*args)``, see
 ``torchls_amd.residuals``), which skip vmap for the benchmark residuals.
-The float64 objectives run in HIP (tlh_trial_objective).
+The float64 objectives run in HIP (tlh_trial_objective); the reprojection
+residual with SE(3) poses is linearized and trial-evaluated entirely in HIP
+(tlh_reprojection_linearize / tlh_reprojection_trial, see ``hip_operands``).
 """
 
 from __future__ import annotations
@@ -50,6 +52,8 @@ class Linearization:
     jacobian: BlockJacobian
     raw_residual: Tensor | None = None
     objective: Tensor | None = None
+    # 0-d device count of observations with a nonfinite residual or block entry
+    nonfinite: Tensor | None = None
 
 
 class _Batch:
@@ -129,6 +133,37 @@ class _Batch:
                 out.append(t.to(device=device, dtype=dtype if t.is_floating_point() else t.dtype))
         return out
 
+    def hip_operands(self, values, layout, device, dtype):
+        """(cams, pts, obs, huber_delta) when the HIP reprojection kernels apply, else None.
+
+        They apply to the reprojection residual (``residuals.reprojection``)
+        called as fn(("poses", idx), ("points", idx), observations[N, 2]) with
+        SE(3) poses ([n, 4, 4] or [n, 3, 4]), Euclidean 3-vector points, no
+        custom Jacobian, and no loss or the Huber loss."""
+        if getattr(self.fn, "hip_residual", None) != "reprojection" or self.jacobian_fn is not None:
+            return None
+        if self.slots != (0, 1) or len(self.args) != 3 or not self.batched:
+            return None
+        if self.in_dims is not None and tuple(self.in_dims) != (0, 0, 0):
+            return None
+        if self.loss is not None and type(self.loss) is not HuberLoss:
+            return None
+        from .geometry import SE3Var
+
+        tc, tp = self.type_indices
+        if layout[tc].var_type is not SE3Var or not layout[tp].var_type.is_euclidean or layout[tp].dim != 3:
+            return None
+        cams, pts = values[self.args[0].name], values[self.args[1].name]
+        if cams.shape[1:] not in ((4, 4), (3, 4)) or pts.shape[1:] != (3,):
+            return None
+        obs = self.args[2]
+        if not isinstance(obs, Tensor) or obs.shape != (self.count, 2):
+            return None
+        if obs.device != device or obs.dtype != dtype or not obs.is_contiguous():
+            obs = obs.to(device=device, dtype=dtype).contiguous()
+        delta = self.loss.delta if self.loss is not None else -1.0
+        return cams.contiguous(), pts.contiguous(), obs, float(delta)
+
     def residuals(self, values, layout, device, dtype) -> Tensor:
         batched = getattr(self.fn, "batched_residual", None)
         chunk = self.batch_size if isinstance(self.batch_size, int) else None
@@ -203,6 +238,7 @@ class Analysis:
         self.tangent_dim = tangent_dim
         self.has_loss = any(b.loss is not None or b.weights is not None for b in batches)
         self._red = {}
+        self.plans = {}  # symbolic HIP plans, reused by every solve of this structure
 
     def _reduction(self, device):
         from .hip_ops import Reduction
@@ -219,12 +255,18 @@ class Analysis:
         """Per batch: HIP float64 (objective, reduction) sums; before_rows per batch or None."""
         from .hip_ops import trial_objective
 
+        from .hip_ops import reprojection_trial
+
         dev, dt = _values_device_dtype(values)
         red = self._reduction(dev)
         for k, b in enumerate(self.batches):
-            after = b.residuals(values, self.layout, dev, dt)
             before = None if before_rows is None else before_rows[k]
-            w = b.device_state(dev, self.layout)[2]
+            idx, _, w = b.device_state(dev, self.layout)
+            hip = b.hip_operands(values, self.layout, dev, dt)
+            if hip is not None:
+                reprojection_trial(red, k, hip, idx, w, before)
+                continue
+            after = b.residuals(values, self.layout, dev, dt)
             if b.loss is None or isinstance(b.loss, HuberLoss):
                 trial_objective(red, k, before, after, b.count, after.shape[1],
                                 huber_delta=None if b.loss is None else b.loss.delta, weights=w)
@@ -258,10 +300,23 @@ class Analysis:
         return terms[0], terms[1]
 
     def linearize(self, values) -> Linearization:
+        from .hip_ops import reprojection_linearize
+
         dev, dt = _values_device_dtype(values)
-        groups, residuals, raws, objectives = [], [], [], []
-        for b in self.batches:
+        groups, residuals, raws, objectives, bad = [], [], [], [], []
+        for k, b in enumerate(self.batches):
             idx, cols, w = b.device_state(dev, self.layout)
+            hip = b.hip_operands(values, self.layout, dev, dt)
+            if hip is not None:
+                red = self._reduction(dev)
+                raw, r, blocks, obj = reprojection_linearize(red, k, hip, idx, w, self.has_loss)
+                if self.has_loss:
+                    raws.append(raw.reshape(-1))
+                    objectives.append(obj)
+                bad.append(red.out[k, 1])
+                residuals.append(r.reshape(-1))
+                groups.append(JacobianGroup(blocks, cols, b.type_indices, idx))
+                continue
             r, blocks = b.linearize(values, self.layout, dev, dt)
             if self.has_loss:
                 raws.append(r.reshape(-1))
@@ -279,20 +334,43 @@ class Analysis:
                     obj = obj * w
                 objectives.append(obj.sum())
             residuals.append(r.reshape(-1))
-            groups.append(JacobianGroup(tuple(x.contiguous() for x in blocks), cols, b.type_indices, idx))
+            blocks = tuple(x.contiguous() for x in blocks)
+            bad.append(sum((~torch.isfinite(x)).sum() for x in (r, *blocks)).to(torch.float64))
+            groups.append(JacobianGroup(blocks, cols, b.type_indices, idx))
         residual = torch.cat(residuals)
         J = BlockJacobian(tuple(groups), self.layout, (residual.numel(), self.tangent_dim))
         return Linearization(
             residual, J,
             torch.cat(raws) if self.has_loss else None,
             0.5 * torch.stack(objectives).sum() if self.has_loss else None,
+            torch.stack(bad).sum(),
         )
 
-    def retract(self, values, delta: Tensor):
+    def retract(self, values, delta: Tensor, *, kernel: bool = False):
+        """Retract every parameter group by its slice of ``delta``.  With
+        ``kernel=True`` (the solver's non-differentiable update) SE(3) poses go
+        through tlh_se3_retract and Euclidean groups through one add."""
+        from .geometry import SE3Var
+
         out = {}
         for e in self.layout:
-            d = delta[e.offset : e.offset + e.count * e.dim].reshape(e.count, e.dim).to(values[e.name].dtype)
-            out[e.name] = torch.func.vmap(e.var_type.retract_fn)(values[e.name], d)
+            x = values[e.name]
+            flat = delta[e.offset : e.offset + e.count * e.dim]
+            if kernel and e.var_type is SE3Var and x.shape[1:] in ((4, 4), (3, 4)) and delta.dtype == torch.float64:
+                from . import _lib
+
+                src = x.contiguous()
+                moved = torch.empty_like(src)
+                _lib.check(_lib.load().tlh_se3_retract(
+                    _lib.dtype_code(src.dtype), src.data_ptr(), src.shape[1], e.count, flat.contiguous().data_ptr(),
+                    e.dim, moved.data_ptr(), _lib.stream_handle(src.device)), "tlh_se3_retract")
+                out[e.name] = moved
+                continue
+            d = flat.reshape(e.count, e.dim).to(x.dtype)
+            if kernel and e.var_type.is_euclidean:
+                out[e.name] = x + d.reshape(x.shape)
+                continue
+            out[e.name] = torch.func.vmap(e.var_type.retract_fn)(x, d)
         return out
 
 

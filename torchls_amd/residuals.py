@@ -74,6 +74,9 @@ def _batched_linearize(cams: Tensor, pts: Tensor, uv: Tensor):
     return res, (torch.cat((DR, -(DR @ skew(pts))), dim=-1), DR)
 
 
-# Batched evaluators used by Problem instead of vmap for this residual.
+# Batched evaluators used by Problem instead of vmap for this residual; with
+# SE(3) poses and Euclidean points Problem runs the HIP kernels
+# tlh_reprojection_linearize / tlh_reprojection_trial instead (hip_ops).
 reprojection.batched_residual = _batched_residual
 reprojection.batched_linearize = _batched_linearize
+reprojection.hip_residual = "reprojection"

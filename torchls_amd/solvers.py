@@ -249,42 +249,64 @@ def solve(
                            jacobi_scaling=jacobi_scaling)
     damping = trust_region.lambda_initial if trust_region else 0.0
     rejection_factor = 2.0
+    # Symbolic plans depend only on the residual structure, which an Analysis
+    # fixes: they are built on the first solve and reused by later solves.
+    plan_cache = getattr(problem, "plans", None)
+
+    def cached(key, make):
+        if plan_cache is None:
+            return make()
+        key = key + (str(device), dtype, mv_dtype)
+        if key not in plan_cache:
+            plan_cache[key] = make()
+        return plan_cache[key]
+
     schur_plan = None
     if is_schur:
         from .schur import HipSchurPlan
 
         with region("analyze_schur"):
-            schur_plan = HipSchurPlan(problem, eliminate, device, dtype=mv_dtype)
+            schur_plan = cached(("schur", eliminate),
+                                lambda: HipSchurPlan(problem, eliminate, device, dtype=mv_dtype))
         if linear_solver == "schur_dense" and schur_plan.dim > 8192:
             raise ValueError("Dense Schur is limited to 8192 retained dimensions; use schur_cg")
         summary.reduced_dim = schur_plan.dim
     normal_plan = diagonal_plan = kernels = None
     column_scale = None
+
+    def examine(lin):
+        """float64 gradient of a linearization plus one host read of
+        (finite, max |g|, 0.5 sum rho) -- the only sync per linearization."""
+        nonlocal diagonal_plan, kernels
+        jac = lin.jacobian
+        if kernels is None:
+            diagonal_plan = cached(("diagonal",), lambda: HipDiagonalPlan(jac, dtype=mv_dtype))
+            kernels = cached(("kernels",), lambda: LinearizationKernels(jac, diagonal_plan))
+        g = kernels.gradient(jac, lin.residual)
+        cost = lin.objective if lin.objective is not None else 0.5 * lin.residual.to(torch.float64).square().sum()
+        bad = lin.nonfinite if lin.nonfinite is not None else (~torch.isfinite(lin.residual)).sum().double()
+        host = torch.stack((bad.to(torch.float64), g.abs().max(), cost.to(torch.float64))).cpu()
+        return g, bool(host[0] == 0), float(host[1]), float(host[2])
+
     with region("linearize"):
         lin = problem.linearize(vals)
+    head = None
     for iteration in range(termination.max_iterations):
         if timed_out():
             summary.termination_reason = "time_limit"
             break
         summary.iterations = iteration + 1
         residual, jacobian = lin.residual, lin.jacobian
-        finite = torch.stack([torch.isfinite(residual).all()]
-                             + [torch.isfinite(b).all() for g in jacobian.groups for b in g.blocks]).all()
-        if not bool(finite):
+        if head is None:
+            head = examine(lin)
+        gradient, finite, summary.gradient_norm, old_cost = head
+        head = None
+        if not finite:
             summary.termination_reason = "nonfinite_linearization"
             break
-        if kernels is None:
-            diagonal_plan = HipDiagonalPlan(jacobian, dtype=mv_dtype)
-            kernels = LinearizationKernels(jacobian, diagonal_plan)
-        gradient = kernels.gradient(jacobian, residual)
-        summary.gradient_norm = float(gradient.abs().max())
-        lin_cost = float(lin.objective) if lin.objective is not None else None
-        if termination.target_cost is not None:
-            if lin_cost is None:
-                lin_cost = float(problem.cost(vals))
-            if lin_cost <= termination.target_cost:
-                summary.termination_reason = "target_cost"
-                break
+        if termination.target_cost is not None and old_cost <= termination.target_cost:
+            summary.termination_reason = "target_cost"
+            break
         if summary.gradient_norm <= termination.gradient_tolerance:
             summary.termination_reason = "gradient_tolerance"
             break
@@ -326,10 +348,8 @@ def solve(
                 schur_linear = schur_plan.linearize(jacobian, diagonal, dense=linear_solver == "schur_dense")
             elif linear_solver == "conjugate_gradient":
                 if normal_plan is None:
-                    normal_plan = HipNormalPlan(jacobian, dtype=mv_dtype)
+                    normal_plan = cached(("normal",), lambda: HipNormalPlan(jacobian, dtype=mv_dtype))
                 normal_operator = normal_plan.bind(jacobian)
-        old_cost = float(lin.objective) if lin.objective is not None else float(
-            0.5 * residual.to(torch.float64).square().sum())
         accepted = False
         trial_limit = trust_region.max_trials if trust_region else 1
         for _ in range(trial_limit):
@@ -395,13 +415,13 @@ def solve(
                 delta = delta.to(torch.float64)
                 if solve_scale is not None:
                     delta = solve_scale.to(torch.float64) * delta
-                proposed = problem.retract(vals, delta)
+                proposed = problem.retract(vals, delta, kernel=True)
                 with region("trial_residual"):
                     objective, reduction = problem.evaluate_trial(proposed, lin)
                 sq = kernels.squared_matvec_norm(original_jacobian, delta)
                 pred_t = -torch.dot(gradient, delta) - 0.5 * sq
-                host = torch.stack((objective, reduction, pred_t)).cpu()
-                proposed_cost, actual, predicted = (float(v) for v in host)
+                host = torch.stack((objective, reduction, pred_t, torch.linalg.vector_norm(delta))).cpu()
+                proposed_cost, actual, predicted, step_norm = (float(v) for v in host)
                 quality = actual / predicted if predicted > 0 else -math.inf
                 accepted = math.isfinite(proposed_cost) and (
                     trust_region is None or (actual > 0 and quality >= trust_region.step_quality_min))
@@ -425,15 +445,15 @@ def solve(
                     normal_operator = schur_linear = system = diagonal = gradient = linear_gradient = None
                     with region("linearize"):
                         lin = problem.linearize(vals)
-                    final_gradient = kernels.gradient(lin.jacobian, lin.residual)
-                    summary.gradient_norm = float(final_gradient.abs().max())
+                    head = examine(lin)
+                    summary.gradient_norm = head[2]
                     if termination.target_cost is not None and proposed_cost <= termination.target_cost:
                         summary.termination_reason = "target_cost"
                     elif summary.gradient_norm <= termination.gradient_tolerance:
                         summary.termination_reason = "gradient_tolerance"
                     elif abs(actual) <= termination.cost_tolerance * max(old_cost, torch.finfo(dtype).tiny):
                         summary.termination_reason = "cost_tolerance"
-                    elif float(torch.linalg.vector_norm(delta)) <= termination.parameter_tolerance:
+                    elif step_norm <= termination.parameter_tolerance:
                         summary.termination_reason = "parameter_tolerance"
                     break
             summary.rejected_steps += 1

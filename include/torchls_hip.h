@@ -99,6 +99,7 @@ typedef struct tlh_type {
   const uint8_t* direct;     /* [count] or NULL: 1 = written directly by a wave kernel */
   const int32_t* finish_list;/* [finish_count] variables the finish writes; NULL = all */
   int64_t finish_count;
+  int64_t csr_entries;       /* csr_ptr[count]: sparse partials the finish reads */
 } tlh_type_t;
 
 typedef struct tlh_operator {

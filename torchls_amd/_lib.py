@@ -82,6 +82,7 @@ class Type(ctypes.Structure):
         ("direct", c_void_p),
         ("finish_list", c_void_p),
         ("finish_count", i64),
+        ("csr_entries", i64),
     ]
 
 

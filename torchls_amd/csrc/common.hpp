@@ -94,6 +94,8 @@ __device__ __forceinline__ void upper_index(int k, int d, int& i, int& j) {
 
 // True when a type's partial lists are long enough for one wavefront per
 // (variable, component); such kernels take count * W wavefronts.
-inline bool csr_heavy(int64_t sparse_partials, int64_t count) { return count > 0 && sparse_partials >= 8 * count; }
+inline bool csr_heavy(int64_t csr_entries, int64_t variables) {
+  return variables > 0 && csr_entries >= 8 * variables;
+}
 
 }  // namespace tlh

@@ -178,7 +178,7 @@ int diag_impl(const tlh_operator_t* op, double* const* out, hipStream_t stream, 
     const int64_t total = t.count * (t.dim * (t.dim + 1) / 2);
     if (total == 0) continue;
     if (t.dense_blocks != 0) return TLH_EINVAL;
-    if (csr_heavy(t.sparse_partials, t.count))
+    if (csr_heavy(t.csr_entries, t.count))
       hipLaunchKernelGGL(diag_finish_wave, dim3((unsigned)ceil_div(total, TLH_THREADS / 64)), dim3(TLH_THREADS), 0,
                          stream, out[ti], static_cast<const double*>(t.partials), t.csr_ptr, t.csr_idx, t.count,
                          t.dim);

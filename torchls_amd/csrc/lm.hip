@@ -241,7 +241,7 @@ extern "C" int tlh_rmatvec_finish_f64(const tlh_operator_t* op, double* out, tlh
     const tlh_type_t& t = op->types[ti];
     const int64_t total = t.count * t.dim;
     if (total == 0) continue;
-    if (csr_heavy(t.sparse_partials, t.count))
+    if (csr_heavy(t.csr_entries, t.count))
       hipLaunchKernelGGL(rmatvec_finish_wave, dim3((unsigned)ceil_div(total, TLH_THREADS / 64)), dim3(TLH_THREADS),
                          0, reinterpret_cast<hipStream_t>(stream), out + t.offset,
                          static_cast<const double*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim);

@@ -262,6 +262,8 @@ __global__ void __launch_bounds__(TLH_THREADS) normal_wave_kernel(WaveArgs<Real>
     const int len = a.tile_start[t + 1] - start;
     Real cp[ITEMS][DP], cs[ITEMS][DS], xp[ITEMS][DP];
     int ip[ITEMS], is[ITEMS], rk[ITEMS];
+    // Loads in three phases (ids, blocks, then the id-dependent x gathers) so
+    // that one memory latency covers every item instead of one per item.
 #pragma unroll
     for (int it = 0; it < ITEMS; ++it) {
       const int pos = (int)threadIdx.x + it * TLH_THREADS;
@@ -269,26 +271,49 @@ __global__ void __launch_bounds__(TLH_THREADS) normal_wave_kernel(WaveArgs<Real>
       ip[it] = a.idx_p[n];
       is[it] = a.idx_s[n];
       rk[it] = a.rank_s ? (int)a.rank_s[n] : pos;
-      Real jp[R][DP], js[R][DS], xs[DS];
+    }
+    Real jp[ITEMS][R][DP], js[ITEMS][R][DS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+      const int64_t n = start + min((int)threadIdx.x + it * TLH_THREADS, len - 1);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
 #pragma unroll
-        for (int c = 0; c < DP; ++c) jp[r][c] = a.packed[(int64_t)(OP + r * DP + c) * a.stride + n];
+        for (int c = 0; c < DP; ++c) jp[it][r][c] = a.packed[(int64_t)(OP + r * DP + c) * a.stride + n];
 #pragma unroll
-        for (int c = 0; c < DS; ++c) js[r][c] = a.packed[(int64_t)(OS + r * DS + c) * a.stride + n];
+        for (int c = 0; c < DS; ++c) js[it][r][c] = a.packed[(int64_t)(OS + r * DS + c) * a.stride + n];
       }
+    }
+    Real xsv[ITEMS][DS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
 #pragma unroll
       for (int c = 0; c < DP; ++c) xp[it][c] = a.x_p[(int64_t)ip[it] * DP + c];
 #pragma unroll
-      for (int c = 0; c < DS; ++c) xs[c] = a.x_s[(int64_t)is[it] * DS + c];
+      for (int c = 0; c < DS; ++c) xsv[it][c] = a.x_s[(int64_t)is[it] * DS + c];
+    }
+    int direct[ITEMS];
+    if (a.direct_p) {
+#pragma unroll
+      for (int it = 0; it < ITEMS; ++it) direct[it] = a.direct_p[ip[it]];
+    } else {
+#pragma unroll
+      for (int it = 0; it < ITEMS; ++it) direct[it] = 0;
+    }
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+      const int pos = (int)threadIdx.x + it * TLH_THREADS;
+      const Real (&jpi)[R][DP] = jp[it];
+      const Real (&jsi)[R][DS] = js[it];
+      const Real (&xs)[DS] = xsv[it];
       Real q[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         Real v = Real(0);
 #pragma unroll
-        for (int c = 0; c < DP; ++c) v += jp[r][c] * xp[it][c];
+        for (int c = 0; c < DP; ++c) v += jpi[r][c] * xp[it][c];
 #pragma unroll
-        for (int c = 0; c < DS; ++c) v += js[r][c] * xs[c];
+        for (int c = 0; c < DS; ++c) v += jsi[r][c] * xs[c];
         q[r] = v;
       }
       const bool valid = pos < len;
@@ -296,14 +321,14 @@ __global__ void __launch_bounds__(TLH_THREADS) normal_wave_kernel(WaveArgs<Real>
       for (int c = 0; c < DP; ++c) {
         Real v = Real(0);
 #pragma unroll
-        for (int r = 0; r < R; ++r) v += jp[r][c] * q[r];
+        for (int r = 0; r < R; ++r) v += jpi[r][c] * q[r];
         cp[it][c] = valid ? v : Real(0);
       }
 #pragma unroll
       for (int c = 0; c < DS; ++c) {
         Real v = Real(0);
 #pragma unroll
-        for (int r = 0; r < R; ++r) v += js[r][c] * q[r];
+        for (int r = 0; r < R; ++r) v += jsi[r][c] * q[r];
         cs[it][c] = v;
       }
     }
@@ -334,7 +359,7 @@ __global__ void __launch_bounds__(TLH_THREADS) normal_wave_kernel(WaveArgs<Real>
       const bool tail = lane == 63 || next != ip[it] || pos == len - 1;
       if (pos < len && tail) {
         const int64_t vv = ip[it];
-        if (a.direct_p && a.direct_p[vv]) {
+        if (direct[it]) {
 #pragma unroll
           for (int c = 0; c < DP; ++c) a.y_p[vv * DP + c] = v[c] + a.damping * xp[it][c];
         } else {
@@ -733,7 +758,7 @@ int finish_impl(const tlh_operator_t* op, int t0, int t1, const void* x, double 
                          static_cast<const Real*>(x) + t.offset, (Real)damping,
                          static_cast<const Real*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim,
                          t.dense_blocks, t.finish_list, nlist);
-    else if (csr_heavy(t.sparse_partials, t.count))
+    else if (csr_heavy(t.csr_entries, nlist))
       hipLaunchKernelGGL(finish_wave_kernel<Real>, dim3((unsigned)ceil_div(nlist * t.dim, TLH_THREADS / 64)),
                          dim3(TLH_THREADS), 0, stream, static_cast<Real*>(y) + t.offset,
                          static_cast<const Real*>(x) + t.offset, (Real)damping,

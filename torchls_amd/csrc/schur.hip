@@ -674,7 +674,7 @@ extern "C" int tlh_schur_precond(const tlh_operator_t* op, int wgroup, int eslot
   }
   const int64_t total = tc.count * W;
   if (total) {
-    if (csr_heavy(tc.sparse_partials, tc.count))
+    if (csr_heavy(tc.csr_entries, tc.count))
       hipLaunchKernelGGL(schur_precond_finish_wave, dim3((unsigned)ceil_div(total, TLH_THREADS / 64)),
                          dim3(TLH_THREADS), 0, s, M, D, damping, static_cast<const double*>(tc.partials), tc.csr_ptr,
                          tc.csr_idx, tc.count, DC);

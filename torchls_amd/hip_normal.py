@@ -95,6 +95,7 @@ class DeviceStructure:
             d.finish_list = None if t.finish_list is None else put(
                 t.finish_list if t.finish_list.size else np.zeros(1, np.int32)).data_ptr()
             d.finish_count = t.count if t.finish_list is None else int(t.finish_list.size)
+            d.csr_entries = int(t.csr_idx.size)
         self.sorted_index = None
 
     def _keep(self, t: torch.Tensor) -> torch.Tensor:

@@ -37,6 +37,11 @@ typedef struct ihipStream_t* tlh_stream_t; /* == hipStream_t */
 
 #define TLH_MAX_SLOTS 3
 #define TLH_TILE_THREADS 256
+/* Wave-kernel groups (tlh_group_t.kernel == 1): workgroups of TLH_WAVE_THREADS
+ * threads over aligned tiles of TLH_WAVE_TILE (float32) or TLH_WAVE_TILE / 2
+ * (float64) sorted positions; tile t starts at t * tile_cap. */
+#define TLH_WAVE_THREADS 512
+#define TLH_WAVE_TILE 1024
 
 #define TLH_REDUCE_SPARSE 0 /* one partial per (tile, variable) segment */
 #define TLH_REDUCE_DENSE 1  /* per-workgroup dense LDS accumulator of the whole type */

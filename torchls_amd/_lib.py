@@ -18,6 +18,8 @@ TLH_F32 = 0
 TLH_F64 = 1
 TLH_MAX_SLOTS = 3
 TLH_TILE_THREADS = 256
+TLH_WAVE_THREADS = 512
+TLH_WAVE_TILE = 1024
 TLH_REDUCE_SPARSE = 0
 TLH_REDUCE_DENSE = 1
 TLH_EINVAL = 1000

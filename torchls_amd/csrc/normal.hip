@@ -208,18 +208,29 @@ __global__ void __launch_bounds__(TLH_THREADS) normal_tile_kernel(GroupArgs<Real
 }
 
 // Wave kernel for two-slot groups (bundle adjustment, pose-graph edges).
-// Primary slot: the tile is sorted by its variable, so each wavefront holds 64
-// consecutive observations; an inclusive segmented scan over the lanes
-// (shuffles, fixed tree) sums every run of one variable inside the wavefront,
-// and the run's last lane writes y directly (variables observed only there) or
-// a positional partial.  Secondary slot: contributions and variable ids are
-// staged in LDS in the tile's order by that variable; each segment head sums
-// its run serially into the dense LDS accumulator or a positional partial.
+// Tiles are aligned: tile t covers sorted positions [t*T, t*T + T), T = 256 *
+// ITEMS, and thread i of the workgroup owns the ITEMS consecutive positions
+// ITEMS*i .. ITEMS*i + ITEMS-1, so every block component and id array is read
+// with one 16-byte (8 for ids of float64 tiles) load per thread.
+// Primary slot (the tile is sorted by its variable): each thread first sums
+// runs of one variable over its own items, then an inclusive segmented scan
+// over the 64 lanes (shuffles, fixed tree) carries every run across lanes; a
+// run is cut at the end of each wavefront's 64*ITEMS positions, and its last
+// item writes y directly (variables observed only there) or a positional
+// partial.  Secondary slot: contributions and variable ids are staged in LDS
+// in the tile's order by that variable; each segment head sums its run
+// serially into the dense LDS accumulator or a positional partial.  The last
+// tile of a group may be short; it is read with clamped scalar loads.
+template <typename T, int N>
+struct VecT {
+  typedef T type __attribute__((ext_vector_type(N)));
+};
+
 template <typename Real>
 struct WaveArgs {
   const Real* packed;
   int64_t stride;
-  const int32_t* tile_start;
+  int64_t count;
   int32_t ntiles;
   int32_t tiles_per_wg;
   const int32_t* idx_p;
@@ -239,161 +250,221 @@ struct WaveArgs {
   Real damping;
 };
 
-template <typename Real, int R, int D0, int D1, int P, int ITEMS>
-__global__ void __launch_bounds__(TLH_THREADS) normal_wave_kernel(WaveArgs<Real> a) {
+// out[k] = base[start + b + k]; a short tile (len < T) clamps to its last position.
+template <typename T, int ITEMS>
+__device__ __forceinline__ void load_items(const T* __restrict__ base, int64_t start, int b, int len, bool full,
+                                           T (&out)[ITEMS]) {
+  if constexpr (ITEMS == 1) {
+    out[0] = base[start + min(b, len - 1)];
+  } else if (full) {
+    const typename VecT<T, ITEMS>::type v =
+        *reinterpret_cast<const typename VecT<T, ITEMS>::type*>(base + start + b);
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) out[k] = v[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) out[k] = base[start + min(b + k, len - 1)];
+  }
+}
+
+// Raw per-thread data of one tile: ids, ranks and block components.
+template <typename Real, int R, int DP, int DS, int ITEMS>
+struct WaveTile {
+  int ip[ITEMS], is[ITEMS], rk[ITEMS];
+  Real jp[R][DP][ITEMS], js[R][DS][ITEMS];
+};
+
+template <typename Real, int R, int DP, int DS, int OP, int OS, int ITEMS, int NT>
+__device__ __forceinline__ void load_wave_tile(const WaveArgs<Real>& a, int t, int b,
+                                               WaveTile<Real, R, DP, DS, ITEMS>& d) {
+  constexpr int T = NT * ITEMS;
+  const int64_t start = (int64_t)t * T;
+  const int len = (int)min((int64_t)T, a.count - start);
+  const bool full = len == T;
+  load_items<int32_t, ITEMS>(a.idx_p, start, b, len, full, d.ip);
+  load_items<int32_t, ITEMS>(a.idx_s, start, b, len, full, d.is);
+  if (a.rank_s) {
+    uint16_t r16[ITEMS];
+    load_items<uint16_t, ITEMS>(a.rank_s, start, b, len, full, r16);
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) d.rk[k] = r16[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) d.rk[k] = b + k;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+#pragma unroll
+    for (int c = 0; c < DP; ++c)
+      load_items<Real, ITEMS>(a.packed + (int64_t)(OP + r * DP + c) * a.stride, start, b, len, full, d.jp[r][c]);
+#pragma unroll
+    for (int c = 0; c < DS; ++c)
+      load_items<Real, ITEMS>(a.packed + (int64_t)(OS + r * DS + c) * a.stride, start, b, len, full, d.js[r][c]);
+  }
+}
+
+template <typename Real, int R, int D0, int D1, int P, int ITEMS, int NT>
+__global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
   constexpr int DP = P == 0 ? D0 : D1;
   constexpr int DS = P == 0 ? D1 : D0;
   constexpr int OP = P == 0 ? 0 : R * D0;
   constexpr int OS = P == 0 ? R * D0 : 0;
-  constexpr int T = TLH_THREADS * ITEMS;
+  constexpr int T = NT * ITEMS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Real* cstage = reinterpret_cast<Real*>(smem_raw);
   int32_t* cid = reinterpret_cast<int32_t*>(cstage + T * DS);
   Real* acc = reinterpret_cast<Real*>(cid + T);
   const bool dense = a.mode_s == TLH_REDUCE_DENSE;
   if (dense)
-    for (int i = threadIdx.x; i < a.count_s * DS; i += TLH_THREADS) acc[i] = Real(0);
+    for (int i = threadIdx.x; i < a.count_s * DS; i += NT) acc[i] = Real(0);
   const int lane = threadIdx.x & 63;
+  const int b = ITEMS * (int)threadIdx.x;  // first tile-local position of this thread
 
   const int t0 = blockIdx.x * a.tiles_per_wg;
   const int t1 = min(t0 + a.tiles_per_wg, a.ntiles);
   for (int t = t0; t < t1; ++t) {
-    const int start = a.tile_start[t];
-    const int len = a.tile_start[t + 1] - start;
-    Real cp[ITEMS][DP], cs[ITEMS][DS], xp[ITEMS][DP];
+    WaveTile<Real, R, DP, DS, ITEMS> cur;
+    load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT>(a, t, b, cur);
+    const int64_t start = (int64_t)t * T;
+    const int len = (int)min((int64_t)T, a.count - start);
+    const int left = len - b;  // items of this thread inside the tile (may be <= 0)
     int ip[ITEMS], is[ITEMS], rk[ITEMS];
-    // Loads in three phases (ids, blocks, then the id-dependent x gathers) so
-    // that one memory latency covers every item instead of one per item.
 #pragma unroll
-    for (int it = 0; it < ITEMS; ++it) {
-      const int pos = (int)threadIdx.x + it * TLH_THREADS;
-      const int64_t n = start + min(pos, len - 1);
-      ip[it] = a.idx_p[n];
-      is[it] = a.idx_s[n];
-      rk[it] = a.rank_s ? (int)a.rank_s[n] : pos;
+    for (int k = 0; k < ITEMS; ++k) {
+      ip[k] = cur.ip[k];
+      is[k] = cur.is[k];
+      rk[k] = cur.rk[k];
     }
-    Real jp[ITEMS][R][DP], js[ITEMS][R][DS];
+    Real xp[ITEMS][DP], xs[ITEMS][DS];
 #pragma unroll
-    for (int it = 0; it < ITEMS; ++it) {
-      const int64_t n = start + min((int)threadIdx.x + it * TLH_THREADS, len - 1);
+    for (int k = 0; k < ITEMS; ++k) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
+      for (int c = 0; c < DP; ++c) xp[k][c] = a.x_p[(int64_t)ip[k] * DP + c];
 #pragma unroll
-        for (int c = 0; c < DP; ++c) jp[it][r][c] = a.packed[(int64_t)(OP + r * DP + c) * a.stride + n];
-#pragma unroll
-        for (int c = 0; c < DS; ++c) js[it][r][c] = a.packed[(int64_t)(OS + r * DS + c) * a.stride + n];
-      }
-    }
-    Real xsv[ITEMS][DS];
-#pragma unroll
-    for (int it = 0; it < ITEMS; ++it) {
-#pragma unroll
-      for (int c = 0; c < DP; ++c) xp[it][c] = a.x_p[(int64_t)ip[it] * DP + c];
-#pragma unroll
-      for (int c = 0; c < DS; ++c) xsv[it][c] = a.x_s[(int64_t)is[it] * DS + c];
+      for (int c = 0; c < DS; ++c) xs[k][c] = a.x_s[(int64_t)is[k] * DS + c];
     }
     int direct[ITEMS];
     if (a.direct_p) {
 #pragma unroll
-      for (int it = 0; it < ITEMS; ++it) direct[it] = a.direct_p[ip[it]];
+      for (int k = 0; k < ITEMS; ++k) direct[k] = a.direct_p[ip[k]];
     } else {
 #pragma unroll
-      for (int it = 0; it < ITEMS; ++it) direct[it] = 0;
+      for (int k = 0; k < ITEMS; ++k) direct[k] = 0;
     }
+    // contributions J_p^T q and J_s^T q with q = J_p x_p + J_s x_s
+    Real cp[ITEMS][DP], cs[ITEMS][DS];
+    int key[ITEMS];
 #pragma unroll
-    for (int it = 0; it < ITEMS; ++it) {
-      const int pos = (int)threadIdx.x + it * TLH_THREADS;
-      const Real (&jpi)[R][DP] = jp[it];
-      const Real (&jsi)[R][DS] = js[it];
-      const Real (&xs)[DS] = xsv[it];
+    for (int k = 0; k < ITEMS; ++k) {
       Real q[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         Real v = Real(0);
 #pragma unroll
-        for (int c = 0; c < DP; ++c) v += jpi[r][c] * xp[it][c];
+        for (int c = 0; c < DP; ++c) v += cur.jp[r][c][k] * xp[k][c];
 #pragma unroll
-        for (int c = 0; c < DS; ++c) v += jsi[r][c] * xs[c];
+        for (int c = 0; c < DS; ++c) v += cur.js[r][c][k] * xs[k][c];
         q[r] = v;
       }
-      const bool valid = pos < len;
+      const bool valid = k < left;
+      key[k] = valid ? ip[k] : -1;
 #pragma unroll
       for (int c = 0; c < DP; ++c) {
         Real v = Real(0);
 #pragma unroll
-        for (int r = 0; r < R; ++r) v += jpi[r][c] * q[r];
-        cp[it][c] = valid ? v : Real(0);
+        for (int r = 0; r < R; ++r) v += cur.jp[r][c][k] * q[r];
+        cp[k][c] = valid ? v : Real(0);
       }
 #pragma unroll
       for (int c = 0; c < DS; ++c) {
         Real v = Real(0);
 #pragma unroll
-        for (int r = 0; r < R; ++r) v += jsi[r][c] * q[r];
-        cs[it][c] = v;
+        for (int r = 0; r < R; ++r) v += cur.js[r][c][k] * q[r];
+        cs[k][c] = v;
       }
     }
-    // primary slot: wavefront segmented scan
+    // primary slot: runs inside the thread, then a segmented scan over lanes
 #pragma unroll
-    for (int it = 0; it < ITEMS; ++it) {
-      const int pos = (int)threadIdx.x + it * TLH_THREADS;
-      const int prev = __shfl_up(ip[it], 1);
-      const int next = __shfl_down(ip[it], 1);
-      int flag = (lane == 0 || prev != ip[it]) ? 1 : 0;
-      Real v[DP];
+    for (int k = 1; k < ITEMS; ++k)
+      if (key[k] == key[k - 1]) {
 #pragma unroll
-      for (int c = 0; c < DP; ++c) v[c] = cp[it][c];
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int fo = __shfl_up(flag, off);
-        Real vo[DP];
-#pragma unroll
-        for (int c = 0; c < DP; ++c) vo[c] = __shfl_up(v[c], off);
-        if (lane >= off) {
-          if (!flag) {
-#pragma unroll
-            for (int c = 0; c < DP; ++c) v[c] = vo[c] + v[c];
-          }
-          flag |= fo;
-        }
+        for (int c = 0; c < DP; ++c) cp[k][c] += cp[k - 1][c];
       }
-      const bool tail = lane == 63 || next != ip[it] || pos == len - 1;
-      if (pos < len && tail) {
-        const int64_t vv = ip[it];
-        if (direct[it]) {
+    bool same = true;
 #pragma unroll
-          for (int c = 0; c < DP; ++c) a.y_p[vv * DP + c] = v[c] + a.damping * xp[it][c];
+    for (int k = 1; k < ITEMS; ++k) same = same && key[k] == key[0];
+    const int prev_last = __shfl_up(key[ITEMS - 1], 1);
+    const int next_first = __shfl_down(key[0], 1);
+    const bool joins = lane > 0 && key[0] == prev_last;  // first run continues the previous lane's
+    int flag = (joins && same) ? 0 : 1;
+    Real v[DP];
+#pragma unroll
+    for (int c = 0; c < DP; ++c) v[c] = cp[ITEMS - 1][c];
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int fo = __shfl_up(flag, off);
+      Real vo[DP];
+#pragma unroll
+      for (int c = 0; c < DP; ++c) vo[c] = __shfl_up(v[c], off);
+      if (lane >= off) {
+        if (!flag) {
+#pragma unroll
+          for (int c = 0; c < DP; ++c) v[c] = vo[c] + v[c];
+        }
+        flag |= fo;
+      }
+    }
+    // carry into this lane's first run: the previous lane's scanned trailing run
+    Real carry[DP];
+#pragma unroll
+    for (int c = 0; c < DP; ++c) {
+      const Real up = __shfl_up(v[c], 1);
+      carry[c] = joins ? up : Real(0);
+    }
+    bool first_run = true;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      if (k > 0 && key[k] != key[k - 1]) first_run = false;
+      const bool tail = k < left && (k + 1 < ITEMS ? key[k + 1] != key[k] : (lane == 63 || next_first != key[k]));
+      if (tail) {
+        Real s[DP];
+#pragma unroll
+        for (int c = 0; c < DP; ++c) s[c] = first_run ? cp[k][c] + carry[c] : cp[k][c];
+        const int64_t vv = ip[k];
+        if (direct[k]) {
+#pragma unroll
+          for (int c = 0; c < DP; ++c) a.y_p[vv * DP + c] = s[c] + a.damping * xp[k][c];
         } else {
-          Real* out = a.part_p + (a.pos_base_p + start + pos) * DP;
+          Real* out = a.part_p + (a.pos_base_p + start + b + k) * DP;
 #pragma unroll
-          for (int c = 0; c < DP; ++c) out[c] = v[c];
+          for (int c = 0; c < DP; ++c) out[c] = s[c];
         }
       }
     }
     // secondary slot: LDS stage in variable order, on-the-fly segment heads
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < ITEMS; ++it) {
-      const int pos = (int)threadIdx.x + it * TLH_THREADS;
-      if (pos < len) {
+    for (int k = 0; k < ITEMS; ++k) {
+      if (k < left) {
 #pragma unroll
-        for (int c = 0; c < DS; ++c) cstage[rk[it] * DS + c] = cs[it][c];
-        cid[rk[it]] = is[it];
+        for (int c = 0; c < DS; ++c) cstage[rk[k] * DS + c] = cs[k][c];
+        cid[rk[k]] = is[k];
       }
     }
     __syncthreads();
-    for (int p = threadIdx.x; p < len; p += TLH_THREADS) {
-      const int v = cid[p];
-      if (p > 0 && cid[p - 1] == v) continue;
+    for (int p = threadIdx.x; p < len; p += NT) {
+      const int vsec = cid[p];
+      if (p > 0 && cid[p - 1] == vsec) continue;
       Real sum[DS];
 #pragma unroll
       for (int c = 0; c < DS; ++c) sum[c] = cstage[p * DS + c];
-      for (int q = p + 1; q < len && cid[q] == v; ++q) {
+      for (int q = p + 1; q < len && cid[q] == vsec; ++q) {
 #pragma unroll
         for (int c = 0; c < DS; ++c) sum[c] += cstage[q * DS + c];
       }
       if (dense) {
 #pragma unroll
-        for (int c = 0; c < DS; ++c) acc[v * DS + c] += sum[c];
+        for (int c = 0; c < DS; ++c) acc[vsec * DS + c] += sum[c];
       } else {
         Real* out = a.part_s + (a.pos_base_s + start + p) * DS;
 #pragma unroll
@@ -405,7 +476,7 @@ __global__ void __launch_bounds__(TLH_THREADS) normal_wave_kernel(WaveArgs<Real>
   if (dense) {
     const int64_t nn = (int64_t)a.count_s * DS;
     Real* out = a.part_s + (int64_t)(a.dense_block_s + blockIdx.x) * nn;
-    for (int64_t i = threadIdx.x; i < nn; i += TLH_THREADS) out[i] = acc[i];
+    for (int64_t i = threadIdx.x; i < nn; i += NT) out[i] = acc[i];
   }
 }
 
@@ -653,16 +724,19 @@ int64_t wave_lds(const tlh_operator_t* op, const tlh_group_t& g) {
 template <typename Real, int R, int D0, int D1, int P>
 int launch_wave_t(const tlh_operator_t* op, const tlh_group_t& g, const void* x, void* y, double damping,
                   hipStream_t stream) {
-  constexpr int ITEMS = items_for<Real>();
-  if (g.tile_cap > TLH_THREADS * ITEMS) return TLH_EINVAL;
+  constexpr int NT = TLH_WAVE_THREADS;
+  constexpr int ITEMS = TLH_WAVE_TILE / NT / (int)(sizeof(Real) / 4);
   const tlh_slot_t& sp = g.slot[P];
   const tlh_slot_t& ss = g.slot[1 - P];
   const tlh_type_t& tp = op->types[sp.type];
   const tlh_type_t& ts = op->types[ss.type];
   WaveArgs<Real> a{};
+  // aligned tiles of exactly NT * ITEMS sorted positions
+  if (g.tile_cap != NT * ITEMS || g.ntiles != ceil_div(g.count, NT * ITEMS) || g.stride % ITEMS != 0)
+    return TLH_EINVAL;
   a.packed = static_cast<const Real*>(g.packed);
   a.stride = g.stride;
-  a.tile_start = g.tile_start;
+  a.count = g.count;
   a.ntiles = g.ntiles;
   a.tiles_per_wg = g.tiles_per_wg;
   a.idx_p = sp.index;
@@ -680,11 +754,11 @@ int launch_wave_t(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
   a.dense_block_s = ss.dense_block;
   a.count_s = (int32_t)ts.count;
   a.damping = (Real)damping;
-  const int64_t lds = (int64_t)TLH_THREADS * ITEMS * ((1 - P == 0 ? D0 : D1) * (int64_t)sizeof(Real) + 4) +
+  const int64_t lds = (int64_t)NT * ITEMS * ((1 - P == 0 ? D0 : D1) * (int64_t)sizeof(Real) + 4) +
                       (ss.mode == TLH_REDUCE_DENSE ? ts.count * (1 - P == 0 ? D0 : D1) * (int64_t)sizeof(Real) : 0);
-  auto kernel = normal_wave_kernel<Real, R, D0, D1, P, ITEMS>;
+  auto kernel = normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT>;
   TLH_RETURN_IF_ERROR(allow_lds((const void*)kernel, lds));
-  hipLaunchKernelGGL(kernel, dim3(g.nwg), dim3(TLH_THREADS), (size_t)lds, stream, a);
+  hipLaunchKernelGGL(kernel, dim3(g.nwg), dim3(NT), (size_t)lds, stream, a);
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }
@@ -702,9 +776,6 @@ int launch_wave(const tlh_operator_t* op, const tlh_group_t& g, const void* x, v
   if (R == 2 && d0 == 3 && d1 == 6)
     return P == 0 ? launch_wave_t<Real, 2, 3, 6, 0>(op, g, x, y, damping, stream)
                   : launch_wave_t<Real, 2, 3, 6, 1>(op, g, x, y, damping, stream);
-  if (R == 6 && d0 == 6 && d1 == 6)
-    return P == 0 ? launch_wave_t<Real, 6, 6, 6, 0>(op, g, x, y, damping, stream)
-                  : launch_wave_t<Real, 6, 6, 6, 1>(op, g, x, y, damping, stream);
   return TLH_ENOTFOUND;
 }
 

@@ -125,7 +125,9 @@ class DeviceStructure:
 
 
 # (rows, dim slot 0, dim slot 1) with an instantiated wave kernel (csrc/normal.hip)
-WAVE_SHAPES = frozenset({(2, 6, 3), (2, 3, 6), (6, 6, 6)})
+# (6x6 pose-graph blocks run on the segment kernel: their wave kernel would
+# hold 2 x 72 block values per item in registers and spill)
+WAVE_SHAPES = frozenset({(2, 6, 3), (2, 3, 6)})
 
 
 def plan_for(jacobian, dtype: torch.dtype, *, allow_dense: bool = True, tile_cap: int | None = None):

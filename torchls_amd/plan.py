@@ -38,6 +38,15 @@ def items_for(esize: int) -> int:
     return 4 if esize == 4 else 2
 
 
+WAVE_THREADS = 512  # TLH_WAVE_THREADS
+WAVE_TILE = 1024  # TLH_WAVE_TILE (float32; half for float64)
+
+
+def wave_items(esize: int) -> int:
+    """Consecutive positions per thread of the wave kernel (csrc/normal.hip)."""
+    return WAVE_TILE // WAVE_THREADS // (esize // 4)
+
+
 @dataclass
 class SlotPlan:
     type: int
@@ -160,9 +169,15 @@ def _slot_segments(index_sorted: np.ndarray, tile_start: np.ndarray, piece: int 
 WAVE = 64
 
 
-def _wave_pieces(index_sorted: np.ndarray, tile_start: np.ndarray):
+def _aligned_tiles(n: int, cap: int) -> np.ndarray:
+    """Tiles of exactly ``cap`` sorted positions (the last one may be short)."""
+    return np.append(np.arange(0, n, cap, dtype=np.int64), n) if n else np.zeros(1, dtype=np.int64)
+
+
+def _wave_pieces(index_sorted: np.ndarray, tile_start: np.ndarray, span: int):
     """Primary-slot pieces of the wave kernel: runs of one variable, also cut at
-    every 64-position boundary of each tile (one wavefront per 64 positions)."""
+    every ``span``-position boundary of each tile (one wavefront covers 64
+    threads x ITEMS consecutive positions)."""
     n = index_sorted.size
     ntiles = tile_start.size - 1
     tile_of = np.repeat(np.arange(ntiles, dtype=np.int64), np.diff(tile_start))
@@ -170,7 +185,7 @@ def _wave_pieces(index_sorted: np.ndarray, tile_start: np.ndarray):
     local = pos - tile_start[tile_of]
     head = np.ones(n, dtype=bool)
     head[1:] = index_sorted[1:] != index_sorted[:-1]
-    head |= (local % WAVE) == 0
+    head |= (local % span) == 0
     heads = np.flatnonzero(head)
     ends = np.append(heads[1:], n) - 1
     seg_ptr = np.zeros(ntiles + 1, dtype=np.int64)
@@ -215,12 +230,13 @@ def build_structure(
         keys.append(slot_idx[primary])
         order = np.lexsort(keys).astype(np.int64) if count else np.zeros(0, np.int64)
         sorted_idx = [np.asarray(slot_idx[s])[order] for s in range(ns)]
-        tile_start = _cut_tiles(sorted_idx[primary], cap)
-        ntiles = tile_start.size - 1
         dmax = max(types[t].dim for t in slot_types)
         shape = (int(rows), *(types[t].dim for t in slot_types))
         wave = ns == 2 and shape in wave_shapes and count > 0
-        stage = cap * dmax * esize + (cap * 4 if wave else 0)
+        gcap = WAVE_THREADS * wave_items(esize) if wave else cap
+        tile_start = _aligned_tiles(count, gcap) if wave else _cut_tiles(sorted_idx[primary], cap)
+        ntiles = tile_start.size - 1
+        stage = gcap * dmax * esize + (gcap * 4 if wave else 0)
         dense_budget = lds_limit - stage
         slots = []
         ends = None
@@ -232,7 +248,8 @@ def build_structure(
                 dense_budget -= dense_bytes
             split = None if dense or (s == primary and not split_primary) or wave else piece
             if wave and s == primary:
-                seg_start, seg_ptr, seg_var, ends = _wave_pieces(sorted_idx[s], tile_start)
+                seg_start, seg_ptr, seg_var, ends = _wave_pieces(sorted_idx[s], tile_start,
+                                                                 WAVE * wave_items(esize))
                 rank, identity = None, True
             else:
                 rank, identity, seg_start, seg_ptr, seg_var = _slot_segments(sorted_idx[s], tile_start, split)
@@ -258,7 +275,7 @@ def build_structure(
         else:
             nwg, per = ntiles, 1
         gplans.append(
-            GroupPlan(count, int(rows), primary, order, tile_start.astype(np.int32), cap, nwg, per, slots,
+            GroupPlan(count, int(rows), primary, order, tile_start.astype(np.int32), gcap, nwg, per, slots,
                       kernel=1 if wave else 0)
         )
     # dense blocks first (ids b*count + v), then sparse partials

@@ -44,7 +44,10 @@ def parse():
     p.add_argument("--workload", default="ba_medium", choices=["ba_small", "ba_medium", "ba_large"])
     p.add_argument("--kernel-reps", type=int, default=50)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--no-large", action="store_true", help="skip the 5M-observation normal-matvec measurement")
+    p.add_argument("--repeats", type=int, default=3, help="timed solves per precision (median reported)")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="only run a 1 GiB device copy 20 times (HBM counter calibration) and exit")
     return p.parse_args()
 
 
@@ -75,7 +78,7 @@ def timed_solve(P, init, iters, mv_dtype):
 
 
 def schur_matvec_timing(P, mv_dtype, reps):
-    """Average time of the implicit Schur matvec (HIP events, current stream) at lambda = 1e-3."""
+    """Average time of the implicit Schur matvec (HIP events, launch stream) at lambda = 1e-3."""
     from torchls_amd.schur import HipSchurPlan
 
     an = P._analyze()
@@ -116,25 +119,76 @@ def normal_matvec_timing(P, mv_dtype, reps):
     return start.elapsed_time(stop) / reps / 1e3
 
 
-def cpu_baseline(prob, seconds):
-    """float64 numpy oracle LM (Schur-PCG) on the same workload, host cores, bounded time."""
+def cpu_baseline(prob, iters):
+    """float64 numpy oracle LM (Schur-PCG) on the same workload and initial point, host cores:
+    the same number of LM iterations as the timed GPU solves, so its final cost is the
+    float64 reference for the mixed-precision result."""
     from oracle.ba import BAProblem
     from oracle.lm import Termination, solve
 
+    from threadpoolctl import threadpool_limits
+
     o = BAProblem(prob.cam_idx, prob.pt_idx, prob.observations, prob.n_cameras, prob.n_points,
                   huber_delta=prob.huber_delta)
-    iters, t0 = 0, time.perf_counter()
-    vals = (prob.cameras_init, prob.points_init)
-    while time.perf_counter() - t0 < seconds:
-        vals, s = solve(o, vals, linear_solver="schur_cg", eliminate=1,
-                        term=Termination(max_iterations=1, cost_tolerance=0, gradient_tolerance=0,
-                                         parameter_tolerance=0))
-        iters += 1
-    return iters / (time.perf_counter() - t0), iters
+    with threadpool_limits(limits=1):  # one host core: the reported core count
+        t0 = time.perf_counter()
+        _, s = solve(o, (prob.cameras_init, prob.points_init), linear_solver="schur_cg", eliminate=1,
+                     term=Termination(max_iterations=iters, cost_tolerance=0, gradient_tolerance=0,
+                                      parameter_tolerance=0))
+        dt = time.perf_counter() - t0
+    return s.iterations / dt, s
+
+
+def large_normal_matvec(device, reps):
+    """Normal matvec (float32 blocks) on the 2,000-camera / 500,000-point / 5M-observation
+    workload: HIP-event time per launch (matvec kernel + finish kernels, launch stream)."""
+    from torchls_amd import generators
+    from torchls_amd.hip_normal import HipNormalPlan
+    from torchls_amd.workloads import normal_matvec_bytes
+
+    prob = generators.ba_large()
+    P = build_problem(prob, device)
+    lin = P._analyze().linearize(P.parameters)
+    op = HipNormalPlan(lin.jacobian, dtype=torch.float32).bind(lin.jacobian)
+    n = lin.jacobian.shape[1]
+    x = torch.randn(n, dtype=torch.float32, device=device)
+    y = torch.empty_like(x)
+    t = event_time(lambda: op.normal_matvec_into(x, y, 1e-3), reps)
+    nbytes = normal_matvec_bytes(prob.n_obs, 2, (6, 3), n, 4)
+    return {"workload": "ba_large", "observations": prob.n_obs, "cameras": prob.n_cameras,
+            "points": prob.n_points, "seed": prob.seed, "us_per_launch": round(t * 1e6, 2),
+            "bytes_per_launch": nbytes, "achieved": round(nbytes / t / 1e9, 1), "unit": "GB/s",
+            "peak": HBM_PEAK_GBS, "frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def event_time(fn, reps):
+    """Average seconds per call: HIP events on the current stream around ``reps`` calls."""
+    for _ in range(5):
+        fn()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / reps / 1e3
+
+
+def calibrate_copy():
+    """A plain device copy of known size, for scaling the HBM traffic counters."""
+    n = 256 * 1024 * 1024
+    src = torch.randn(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    t = event_time(lambda: dst.copy_(src), 20)
+    log(f"copy of {4 * n} bytes: {t * 1e6:.1f} us, {2 * 4 * n / t / 1e9:.0f} GB/s (read + write)")
 
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
               "described in DESIGN.md)", file=sys.stderr)
@@ -158,10 +212,15 @@ def main():
     plan_s = cold[torch.float32] - warm
     log(f"warm-up done: first mixed solve {cold[torch.float32]:.3f}s, warm {warm:.3f}s "
         f"(symbolic analysis and first-call cost {plan_s:.3f}s)")
-    t_mixed, s_mixed = timed_solve(P, init, args.steps, torch.float32)
-    log(f"mixed: {args.steps} LM iterations in {t_mixed:.3f}s final_cost={s_mixed.final_cost:.12e}")
-    t_f64, s_f64 = timed_solve(P, init, args.steps, None)
-    log(f"float64: {args.steps} LM iterations in {t_f64:.3f}s final_cost={s_f64.final_cost:.12e}")
+    runs = {torch.float32: [], None: []}
+    for _ in range(max(1, args.repeats)):
+        for dt in (torch.float32, None):
+            runs[dt].append(timed_solve(P, init, args.steps, dt))
+    t_mixed, s_mixed = sorted(runs[torch.float32], key=lambda r: r[0])[len(runs[torch.float32]) // 2]
+    t_f64, s_f64 = sorted(runs[None], key=lambda r: r[0])[len(runs[None]) // 2]
+    log(f"mixed: {args.steps} LM iterations in {t_mixed:.4f}s (median of {len(runs[None])}) "
+        f"final_cost={s_mixed.final_cost:.12e}")
+    log(f"float64: {args.steps} LM iterations in {t_f64:.4f}s final_cost={s_f64.final_cost:.12e}")
     rel = abs(s_mixed.final_cost - s_f64.final_cost) / abs(s_f64.final_cost)
     iters_mixed = s_mixed.iterations
     iters_f64 = s_f64.iterations
@@ -181,13 +240,22 @@ def main():
     log(f"schur matvec {t_schur * 1e6:.2f} us ({schur_gbs:.1f} GB/s); normal matvec {t_normal * 1e6:.2f} us "
         f"({normal_gbs:.1f} GB/s)")
 
+    large = None
+    if not args.no_large:
+        large = large_normal_matvec(device, args.kernel_reps)
+        log(f"normal matvec at 5M observations: {large['us_per_launch']} us ({large['achieved']} GB/s, "
+            f"{100 * large['frac']:.1f}% of {HBM_PEAK_GBS:.0f} GB/s)")
+
     cpu = None
+    oracle_rel = None
     if not args.no_cpu:
-        torch.set_num_threads(len(os.sched_getaffinity(0)))
-        rate, done = cpu_baseline(prob, args.cpu_seconds)
+        rate, ref = cpu_baseline(prob, args.steps)
+        oracle_rel = abs(s_mixed.final_cost - ref.final_cost) / abs(ref.final_cost)
+        log(f"float64 CPU oracle: {ref.iterations} LM iterations at {rate:.4f} it/s, final_cost="
+            f"{ref.final_cost:.12e}; mixed HIP rel err {oracle_rel:.2e}")
         cpu = {"value": round(rate, 4), "unit": "LM iterations/s", "cores": 1, "kind": "port",
-               "sample": f"{done} LM iterations (Schur-PCG, float64 numpy oracle) of the same workload from "
-                         "the same initial point"}
+               "sample": f"{ref.iterations} LM iterations (Schur-PCG, float64 numpy oracle, BLAS limited to one "
+                         "thread) of the same 500,000-observation workload from the same initial point"}
     print(json.dumps({
         "metric": "LM iterations/s (mixed-precision HIP Schur-PCG), with matvec HBM GB/s",
         "value": round(lm_mixed, 3), "unit": "LM iterations/s", "n_gpus": 1, "steps": args.steps,
@@ -198,7 +266,8 @@ def main():
                    "linear_solver": "schur_cg", "huber_delta": prob.huber_delta},
         "float64_lm_iters_per_s": round(lm_f64, 3), "mixed_over_float64": round(lm_mixed / lm_f64, 3),
         "final_cost_mixed": s_mixed.final_cost, "final_cost_float64": s_f64.final_cost,
-        "final_cost_rel_err": rel, "pcg_iterations_mixed": s_mixed.linear_iterations,
+        "final_cost_rel_err": rel, "final_cost_rel_err_vs_cpu_oracle": oracle_rel,
+        "pcg_iterations_mixed": s_mixed.linear_iterations,
         "pcg_iterations_float64": s_f64.linear_iterations,
         "normal_matvec_gbs": round(normal_gbs, 1), "normal_matvec_us": round(t_normal * 1e6, 2),
         "first_solve_extra_s": round(plan_s, 4),
@@ -206,6 +275,7 @@ def main():
                      "frac": round(schur_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "implicit Schur matvec (schur_tile_kernel + finish)",
                      "us_per_launch": round(t_schur * 1e6, 2), "bytes_per_launch": schur_bytes},
+        "normal_matvec_large": large,
         "cpu_baseline": cpu,
     }), flush=True)
 

@@ -219,6 +219,17 @@ int tlh_pcg_step(const tlh_block_diag_t* pc, int mode, void* x, void* r, void* z
 int tlh_pcg_direction(void* p, const void* z, int64_t n, int dtype, const tlh_pcg_state_t* st,
                       tlh_stream_t stream);
 
+/* One whole iteration after ap = A p for n <= 16384 (one workgroup): p.Ap into
+ * scalars[CURVATURE], the guarded step, mode 0: z = M^-1 r with pc / 1: z = r,
+ * scalars[NEW_RZ] and [NORM2], the guarded direction and the status update --
+ * the work of tlh_dot(p, ap, CURVATURE) + tlh_pcg_step + tlh_pcg_direction in
+ * one launch.  Returns TLH_EINVAL for larger n.
+ * Replaces warp_pcg._step/_direction/_finish (33-92) and the dots of
+ * warp_pcg.py:139, 147-148 for small (e.g. reduced Schur) systems. */
+int tlh_pcg_iteration_small(const tlh_block_diag_t* pc, int mode, void* x, void* r, void* z, void* p,
+                            const void* ap, int64_t n, int dtype, const tlh_pcg_state_t* st,
+                            tlh_stream_t stream);
+
 /* true_r = rhs - ax; scalars[TRUE2] = ||true_r||^2, scalars[DRIFT2] = ||true_r - r||^2.
  * Replaces the true-residual check of warp_pcg.py:189-204. */
 int tlh_pcg_true_residual(const void* rhs, const void* ax, const void* r, void* true_r, int64_t n,

@@ -20,6 +20,7 @@ TLH_MAX_SLOTS = 3
 TLH_TILE_THREADS = 256
 TLH_WAVE_THREADS = 512
 TLH_WAVE_TILE = 1024
+PCG_SMALL_MAX_N = 16384
 TLH_REDUCE_SPARSE = 0
 TLH_REDUCE_DENSE = 1
 TLH_EINVAL = 1000
@@ -154,6 +155,11 @@ SIGNATURES = {
          ctypes.POINTER(PcgStateDesc), c_void_p],
     ),
     "tlh_pcg_step": (
+        ctypes.c_int,
+        [ctypes.POINTER(BlockDiag), ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, i64,
+         ctypes.c_int, ctypes.POINTER(PcgStateDesc), c_void_p],
+    ),
+    "tlh_pcg_iteration_small": (
         ctypes.c_int,
         [ctypes.POINTER(BlockDiag), ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, i64,
          ctypes.c_int, ctypes.POINTER(PcgStateDesc), c_void_p],

@@ -96,6 +96,71 @@ __global__ void __launch_bounds__(TLH_THREADS)
   }
 }
 
+// Fixed-shape version for two slots of different types (no alias terms): each
+// observation's block rows are loaded once into registers and all upper-
+// triangle products of both slots are formed there; they are staged to LDS in
+// ceil(W / chunk) passes for the same segment sums as diag_tile_kernel.
+template <typename Real, int R, int D0, int D1>
+__global__ void __launch_bounds__(TLH_THREADS) diag_pair_kernel(GroupArgs<Real> g, int chunk) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* stage = reinterpret_cast<double*>(smem_raw);
+  constexpr int W0 = D0 * (D0 + 1) / 2, W1 = D1 * (D1 + 1) / 2;
+  const int t = blockIdx.x;
+  if (t >= g.ntiles) return;
+  const int start = g.tile_start[t];
+  const int len = g.tile_start[t + 1] - start;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const SlotArgs<Real>& sl = g.slot[s];
+    const int D = s == 0 ? D0 : D1;
+    const int W = s == 0 ? W0 : W1;
+    const int off = s == 0 ? 0 : R * D0;
+    double* out = reinterpret_cast<double*>(sl.partials);
+    for (int c0 = 0; c0 < W; c0 += chunk) {
+      const int cw = min(chunk, W - c0);
+      __syncthreads();
+      for (int pos = threadIdx.x; pos < len; pos += TLH_THREADS) {
+        const int64_t n = start + pos;
+        const int rk = sl.rank ? (int)sl.rank[n] : pos;
+        double j[R][D0 > D1 ? D0 : D1];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+          for (int c = 0; c < (D0 > D1 ? D0 : D1); ++c)
+            j[r][c] = c < D ? (double)g.packed[(int64_t)(off + r * D + c) * g.stride + n] : 0.0;
+        int e = 0;
+#pragma unroll
+        for (int a = 0; a < (D0 > D1 ? D0 : D1); ++a)
+#pragma unroll
+          for (int b = a; b < (D0 > D1 ? D0 : D1); ++b) {
+            if (a < D && b < D) {
+              if (e >= c0 && e < c0 + cw) {
+                double v = 0.0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) v += j[r][a] * j[r][b];
+                stage[rk * cw + (e - c0)] = v;
+              }
+              ++e;
+            }
+          }
+      }
+      __syncthreads();
+      const int sb = sl.seg_ptr[t];
+      const int se = sl.seg_ptr[t + 1];
+      for (int k = sb + (int)threadIdx.x; k < se; k += TLH_THREADS) {
+        const int a = sl.seg_start[k];
+        const int b = (k + 1 < se) ? sl.seg_start[k + 1] : len;
+        double* dst = out + (int64_t)sl.seg_target[k] * W + c0;
+        for (int c = 0; c < cw; ++c) {
+          double sum = stage[a * cw + c];
+          for (int p = a + 1; p < b; ++p) sum += stage[p * cw + c];
+          dst[c] = sum;
+        }
+      }
+    }
+  }
+}
+
 // D[v] = sum of the variable's partials (csr order), mirrored to full d x d.
 __global__ void __launch_bounds__(TLH_THREADS)
     diag_finish_kernel(double* __restrict__ D, const double* __restrict__ partials,
@@ -168,6 +233,20 @@ int diag_impl(const tlh_operator_t* op, double* const* out, hipStream_t stream, 
     int chunk = std::min<int64_t>(wmax, lds_limit / ((int64_t)g.tile_cap * 8));
     if (chunk < 1) return TLH_ELDS;
     const int64_t lds = (int64_t)g.tile_cap * chunk * 8;
+    const bool pair = g.nslots == 2 && g.rows == 2 && alias[1] == 0;
+    if (pair && ((g.slot[0].dim == 6 && g.slot[1].dim == 3) || (g.slot[0].dim == 3 && g.slot[1].dim == 6))) {
+      const void* k = g.slot[0].dim == 6 ? (const void*)diag_pair_kernel<Real, 2, 6, 3>
+                                         : (const void*)diag_pair_kernel<Real, 2, 3, 6>;
+      TLH_RETURN_IF_ERROR(allow_lds(k, lds));
+      if (g.slot[0].dim == 6)
+        hipLaunchKernelGGL((diag_pair_kernel<Real, 2, 6, 3>), dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds, stream,
+                           a, chunk);
+      else
+        hipLaunchKernelGGL((diag_pair_kernel<Real, 2, 3, 6>), dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds, stream,
+                           a, chunk);
+      TLH_RETURN_IF_LAUNCH_FAILED();
+      continue;
+    }
     TLH_RETURN_IF_ERROR(allow_lds((const void*)diag_tile_kernel<Real>, lds));
     hipLaunchKernelGGL(diag_tile_kernel<Real>, dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds, stream,
                        a, chunk, alias[0], alias[1], alias[2]);

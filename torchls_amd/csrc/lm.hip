@@ -77,6 +77,93 @@ __global__ void __launch_bounds__(TLH_THREADS) rmatvec_tile_kernel(RmatvecArgs a
   }
 }
 
+// Fixed-shape rmatvec (float64 blocks and v, two slots, R rows): each
+// observation's v row and block rows are read as 16-byte vectors in one burst;
+// both slots' J^T v are formed once, then staged slot by slot for the same
+// segment sums as rmatvec_tile_kernel.
+template <int R, int D0, int D1>
+__global__ void __launch_bounds__(TLH_THREADS) rmatvec_pair_kernel(RmatvecArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* stage = reinterpret_cast<double*>(smem_raw);
+  typedef double v2d __attribute__((ext_vector_type(2)));
+  constexpr int K0 = R * D0, K1 = R * D1, DM = D0 > D1 ? D0 : D1;
+  static_assert(K0 % 2 == 0 && K1 % 2 == 0 && R % 2 == 0, "16-byte rows");
+  const int t = blockIdx.x;
+  const int start = a.tile_start[t];
+  const int len = a.tile_start[t + 1] - start;
+  constexpr int PER = 4;  // tile positions per thread (tile_cap <= 1024)
+  double g0[PER][D0], g1[PER][D1];
+#pragma unroll
+  for (int it = 0; it < PER; ++it) {
+    const int pos = (int)threadIdx.x + it * TLH_THREADS;
+    if (pos >= len) continue;
+    const int64_t o = a.order[start + pos];
+    double vr[R], j0[K0], j1[K1];
+    const v2d* pv = reinterpret_cast<const v2d*>(static_cast<const double*>(a.v) + o * R);
+    const v2d* p0 = reinterpret_cast<const v2d*>(static_cast<const double*>(a.blocks[0]) + o * K0);
+    const v2d* p1 = reinterpret_cast<const v2d*>(static_cast<const double*>(a.blocks[1]) + o * K1);
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) {
+      const v2d v = pv[k];
+      vr[2 * k] = v[0];
+      vr[2 * k + 1] = v[1];
+    }
+#pragma unroll
+    for (int k = 0; k < K0 / 2; ++k) {
+      const v2d v = p0[k];
+      j0[2 * k] = v[0];
+      j0[2 * k + 1] = v[1];
+    }
+#pragma unroll
+    for (int k = 0; k < K1 / 2; ++k) {
+      const v2d v = p1[k];
+      j1[2 * k] = v[0];
+      j1[2 * k + 1] = v[1];
+    }
+#pragma unroll
+    for (int c = 0; c < D0; ++c) {
+      double acc = 0.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc += j0[r * D0 + c] * vr[r];
+      g0[it][c] = acc;
+    }
+#pragma unroll
+    for (int c = 0; c < D1; ++c) {
+      double acc = 0.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc += j1[r * D1 + c] * vr[r];
+      g1[it][c] = acc;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int D = s == 0 ? D0 : D1;
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < PER; ++it) {
+      const int pos = (int)threadIdx.x + it * TLH_THREADS;
+      if (pos >= len) continue;
+      const int64_t n = start + pos;
+      const int rk = a.rank[s] ? (int)a.rank[s][n] : pos;
+#pragma unroll
+      for (int c = 0; c < DM; ++c)
+        if (c < D) stage[rk * D + c] = s == 0 ? g0[it][c < D0 ? c : 0] : g1[it][c < D1 ? c : 0];
+    }
+    __syncthreads();
+    const int sb = a.seg_ptr[s][t], se = a.seg_ptr[s][t + 1];
+    for (int k = sb + (int)threadIdx.x; k < se; k += TLH_THREADS) {
+      const int lo = a.seg_start[s][k];
+      const int hi = (k + 1 < se) ? a.seg_start[s][k + 1] : len;
+      double* dst = a.partials[s] + (int64_t)a.seg_target[s][k] * D;
+      for (int c = 0; c < D; ++c) {
+        double sum = stage[lo * D + c];
+        for (int i = lo + 1; i < hi; ++i) sum += stage[i * D + c];
+        dst[c] = sum;
+      }
+    }
+  }
+}
+
 __global__ void __launch_bounds__(TLH_THREADS)
     rmatvec_finish_kernel(double* __restrict__ out, const double* __restrict__ partials,
                           const int32_t* __restrict__ csr_ptr, const int32_t* __restrict__ csr_idx,
@@ -228,6 +315,23 @@ extern "C" int tlh_rmatvec_f64(const tlh_operator_t* op, int group, const void* 
   a.v_dtype = v_dtype;
   const int64_t lds = (int64_t)g.tile_cap * dmax * 8;
   if (lds > lds_limit) return TLH_ELDS;
+  const bool pair = g.nslots == 2 && g.rows == 2 && block_dtype == TLH_F64 && v_dtype == TLH_F64 &&
+                    g.tile_cap <= 4 * TLH_THREADS && ((uintptr_t)v & 15) == 0 &&
+                    ((uintptr_t)blocks[0] & 15) == 0 && ((uintptr_t)blocks[1] & 15) == 0;
+  if (pair && a.dims[0] == 6 && a.dims[1] == 3) {
+    TLH_RETURN_IF_ERROR(allow_lds((const void*)rmatvec_pair_kernel<2, 6, 3>, lds));
+    hipLaunchKernelGGL((rmatvec_pair_kernel<2, 6, 3>), dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds,
+                       reinterpret_cast<hipStream_t>(stream), a);
+    TLH_RETURN_IF_LAUNCH_FAILED();
+    return 0;
+  }
+  if (pair && a.dims[0] == 3 && a.dims[1] == 6) {
+    TLH_RETURN_IF_ERROR(allow_lds((const void*)rmatvec_pair_kernel<2, 3, 6>, lds));
+    hipLaunchKernelGGL((rmatvec_pair_kernel<2, 3, 6>), dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds,
+                       reinterpret_cast<hipStream_t>(stream), a);
+    TLH_RETURN_IF_LAUNCH_FAILED();
+    return 0;
+  }
   TLH_RETURN_IF_ERROR(allow_lds((const void*)rmatvec_tile_kernel, lds));
   hipLaunchKernelGGL(rmatvec_tile_kernel, dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds,
                      reinterpret_cast<hipStream_t>(stream), a);

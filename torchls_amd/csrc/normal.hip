@@ -628,6 +628,40 @@ __global__ void __launch_bounds__(TLH_THREADS)
   }
 }
 
+// Two-slot pack with each observation's block rows read as 8/16-byte vectors
+// in one burst (observation order is usually a random permutation, so a
+// row-at-a-time gather touches every cache line once instead of once per
+// element).  K0, K1 = rows * dims of the two slots (even).
+template <typename In, typename Out, int K0, int K1>
+__global__ void __launch_bounds__(TLH_THREADS)
+    pack_pair_kernel(Out* __restrict__ packed, int64_t stride, const int64_t* __restrict__ order, int64_t count,
+                     const In* __restrict__ b0, const In* __restrict__ b1) {
+  static_assert(K0 % 2 == 0 && K1 % 2 == 0, "rows of even length");
+  typedef typename VecT<In, 2>::type V2;
+  const int64_t n = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
+  if (n >= count) return;
+  const int64_t o = order[n];
+  In r0[K0], r1[K1];
+  const V2* p0 = reinterpret_cast<const V2*>(b0 + o * K0);
+  const V2* p1 = reinterpret_cast<const V2*>(b1 + o * K1);
+#pragma unroll
+  for (int k = 0; k < K0 / 2; ++k) {
+    const V2 v = p0[k];
+    r0[2 * k] = v[0];
+    r0[2 * k + 1] = v[1];
+  }
+#pragma unroll
+  for (int k = 0; k < K1 / 2; ++k) {
+    const V2 v = p1[k];
+    r1[2 * k] = v[0];
+    r1[2 * k + 1] = v[1];
+  }
+#pragma unroll
+  for (int e = 0; e < K0; ++e) packed[(int64_t)e * stride + n] = (Out)r0[e];
+#pragma unroll
+  for (int e = 0; e < K1; ++e) packed[(int64_t)(K0 + e) * stride + n] = (Out)r1[e];
+}
+
 // ------------------------------------------------------------------- host side
 template <typename Real>
 GroupArgs<Real> make_args(const tlh_operator_t* op, const tlh_group_t& g, const void* x) {
@@ -854,9 +888,20 @@ int pack_impl(const tlh_group_t& g, const void* const* blocks, hipStream_t strea
     b[s] = static_cast<const In*>(blocks[s]);
     d[s] = g.slot[s].dim;
   }
-  hipLaunchKernelGGL((pack_kernel<In, Out>), dim3((unsigned)ceil_div(g.count, TLH_THREADS)),
-                     dim3(TLH_THREADS), 0, stream, static_cast<Out*>(g.packed), g.stride, g.order,
-                     g.count, g.rows, g.nslots, b[0], b[1], b[2], d[0], d[1], d[2]);
+  const dim3 grid((unsigned)ceil_div(g.count, TLH_THREADS)), block(TLH_THREADS);
+  Out* out = static_cast<Out*>(g.packed);
+  const int k0 = g.rows * d[0], k1 = g.rows * d[1];
+  const uintptr_t vec = 2 * sizeof(In) - 1;  // the pair kernel reads 2-element vectors
+  const bool aligned = g.nslots == 2 && ((uintptr_t)b[0] & vec) == 0 && ((uintptr_t)b[1] & vec) == 0;
+  if (aligned && k0 == 12 && k1 == 6)
+    hipLaunchKernelGGL((pack_pair_kernel<In, Out, 12, 6>), grid, block, 0, stream, out, g.stride, g.order, g.count,
+                       b[0], b[1]);
+  else if (aligned && k0 == 6 && k1 == 12)
+    hipLaunchKernelGGL((pack_pair_kernel<In, Out, 6, 12>), grid, block, 0, stream, out, g.stride, g.order, g.count,
+                       b[0], b[1]);
+  else
+    hipLaunchKernelGGL((pack_kernel<In, Out>), grid, block, 0, stream, out, g.stride, g.order, g.count, g.rows,
+                       g.nslots, b[0], b[1], b[2], d[0], d[1], d[2]);
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }

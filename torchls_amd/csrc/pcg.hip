@@ -151,6 +151,41 @@ __device__ __forceinline__ void apply_block(const double* inv, int d, const Real
   }
 }
 
+template <typename Real, int D>
+__device__ __forceinline__ void apply_block_fixed(const double* inv, const Real* r, Real* z, double& rz,
+                                                  double& rr) {
+  constexpr int W = D * (D + 1) / 2;
+  double m[W], rv[D];
+#pragma unroll
+  for (int k = 0; k < W; ++k) m[k] = inv[k];
+#pragma unroll
+  for (int i = 0; i < D; ++i) rv[i] = (double)r[i];
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int a = i < j ? i : j, b = i < j ? j : i;
+      t += m[a * D - a * (a - 1) / 2 + (b - a)] * rv[j];
+    }
+    const Real zi = (Real)t;
+    z[i] = zi;
+    rz += rv[i] * (double)zi;
+    rr += rv[i] * rv[i];
+  }
+}
+
+// Block-Jacobi apply with the BA block sizes unrolled (same arithmetic order).
+template <typename Real>
+__device__ __forceinline__ void apply_any(const double* inv, int d, const Real* r, Real* z, double& rz, double& rr) {
+  if (d == 6)
+    apply_block_fixed<Real, 6>(inv, r, z, rz, rr);
+  else if (d == 3)
+    apply_block_fixed<Real, 3>(inv, r, z, rz, rr);
+  else
+    apply_block<Real>(inv, d, r, z, rz, rr);
+}
+
 __device__ __forceinline__ int find_type(const tlh_block_diag_t& pc, int64_t b, int64_t& local) {
   int t = 0;
   int64_t start = 0;
@@ -203,7 +238,7 @@ __global__ void __launch_bounds__(TLH_THREADS)
           r[e0 + k] = (Real)((double)r[e0 + k] - alpha * (double)ap[e0 + k]);
         }
       }
-      apply_block<Real>(pc.inv[t] + local * (d * (d + 1) / 2), d, r + e0, z + e0, v[0], v[1]);
+      apply_any<Real>(pc.inv[t] + local * (d * (d + 1) / 2), d, r + e0, z + e0, v[0], v[1]);
     }
   } else {
     for (int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; i < n; i += stride) {
@@ -259,6 +294,108 @@ __global__ void __launch_bounds__(TLH_THREADS)
   }
 }
 
+// One whole PCG iteration after Ap, for systems that fit one workgroup: the
+// curvature p.Ap, the step, z = M^-1 r with r.z and r.r, the direction and the
+// status machine -- the work of dot_kernel + step_kernel + direction_kernel in
+// one launch, with the same guards and status codes (warp_pcg.py:33-92) and
+// workgroup-wide fixed-order sums.  Each thread updates p for exactly the
+// entries whose z it wrote, so no cross-thread global-memory handoff is needed.
+constexpr int SMALL_THREADS = 1024;
+constexpr int64_t SMALL_MAX_N = 16384;
+
+template <int NV>
+__device__ __forceinline__ void small_sum(double (&v)[NV], double* smem) {
+  for (int k = 0; k < NV; ++k) smem[k * SMALL_THREADS + threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = SMALL_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < NV; ++k) smem[k * SMALL_THREADS + threadIdx.x] += smem[k * SMALL_THREADS + threadIdx.x + w];
+    __syncthreads();
+  }
+  for (int k = 0; k < NV; ++k) v[k] = smem[k * SMALL_THREADS];
+  __syncthreads();
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(SMALL_THREADS)
+    small_iteration_kernel(tlh_block_diag_t pc, int identity, Real* __restrict__ x, Real* __restrict__ r,
+                           Real* __restrict__ z, Real* __restrict__ p, const Real* __restrict__ ap, int64_t n,
+                           int32_t* control, double* scalars) {
+  __shared__ double smem[2 * SMALL_THREADS];
+  double cv[1] = {0.0};
+  for (int64_t i = threadIdx.x; i < n; i += SMALL_THREADS) cv[0] += (double)p[i] * (double)ap[i];
+  small_sum<1>(cv, smem);
+  const double c = cv[0];
+  const double rz = scalars[TLH_PCG_RZ];
+  const bool stopped = frozen(control);
+  const bool move = !stopped && finite(c) && finite(rz) && c > 0.0 && rz > 0.0;
+  const double alpha = move ? rz / c : 0.0;
+  double v[2] = {0.0, 0.0};
+  int64_t blocks = 0;
+  if (!identity)
+    for (int t = 0; t < pc.ntypes; ++t) blocks += pc.count[t];
+  if (!identity) {
+    for (int64_t bidx = threadIdx.x; bidx < blocks; bidx += SMALL_THREADS) {
+      int64_t local;
+      const int t = find_type(pc, bidx, local);
+      const int d = pc.dim[t];
+      const int64_t e0 = pc.offset[t] + local * d;
+      if (move) {
+        for (int k = 0; k < d; ++k) {
+          x[e0 + k] = (Real)((double)x[e0 + k] + alpha * (double)p[e0 + k]);
+          r[e0 + k] = (Real)((double)r[e0 + k] - alpha * (double)ap[e0 + k]);
+        }
+      }
+      apply_any<Real>(pc.inv[t] + local * (d * (d + 1) / 2), d, r + e0, z + e0, v[0], v[1]);
+    }
+  } else {
+    for (int64_t i = threadIdx.x; i < n; i += SMALL_THREADS) {
+      if (move) {
+        x[i] = (Real)((double)x[i] + alpha * (double)p[i]);
+        r[i] = (Real)((double)r[i] - alpha * (double)ap[i]);
+      }
+      const double ri = (double)r[i];
+      z[i] = r[i];
+      v[0] += ri * ri;
+      v[1] += ri * ri;
+    }
+  }
+  small_sum<2>(v, smem);
+  const double nrz = v[0], norm2 = v[1];
+  const bool turn = !stopped && finite(c) && finite(rz) && finite(nrz) && c > 0.0 && rz > 0.0;
+  const double beta = turn ? nrz / rz : 0.0;
+  if (turn) {
+    if (!identity) {
+      for (int64_t bidx = threadIdx.x; bidx < blocks; bidx += SMALL_THREADS) {
+        int64_t local;
+        const int t = find_type(pc, bidx, local);
+        const int d = pc.dim[t];
+        const int64_t e0 = pc.offset[t] + local * d;
+        for (int k = 0; k < d; ++k) p[e0 + k] = (Real)((double)z[e0 + k] + beta * (double)p[e0 + k]);
+      }
+    } else {
+      for (int64_t i = threadIdx.x; i < n; i += SMALL_THREADS) p[i] = (Real)((double)z[i] + beta * (double)p[i]);
+    }
+  }
+  if (threadIdx.x == 0) {
+    scalars[TLH_PCG_CURVATURE] = c;
+    scalars[TLH_PCG_NEW_RZ] = nrz;
+    scalars[TLH_PCG_NORM2] = norm2;
+    if (!stopped) {
+      control[0] += 1;
+      if (!finite(c) || !finite(rz) || !finite(norm2))
+        control[1] = 2;
+      else if (c <= 0.0 || rz <= 0.0)
+        control[1] = 3;
+      else if (norm2 <= scalars[TLH_PCG_THR2])
+        control[1] = 1;
+      else if (!finite(nrz))
+        control[1] = 2;
+      scalars[TLH_PCG_RZ] = nrz;
+    }
+  }
+}
+
 // r = src; optionally x = 0; z = M^-1 r; p = z; rz = r.z (warp_pcg.py:130-132, 169-172, 210-212).
 template <typename Real>
 __global__ void __launch_bounds__(TLH_THREADS)
@@ -280,7 +417,7 @@ __global__ void __launch_bounds__(TLH_THREADS)
         r[e0 + k] = src[e0 + k];
         if (x) x[e0 + k] = Real(0);
       }
-      apply_block<Real>(pc.inv[t] + local * (d * (d + 1) / 2), d, r + e0, z + e0, v[0], v[1]);
+      apply_any<Real>(pc.inv[t] + local * (d * (d + 1) / 2), d, r + e0, z + e0, v[0], v[1]);
       for (int k = 0; k < d; ++k) p[e0 + k] = z[e0 + k];
     }
   } else {
@@ -363,6 +500,28 @@ bool precond_ok(const tlh_block_diag_t* pc, int64_t n) {
 }
 
 }  // namespace
+
+extern "C" int tlh_pcg_iteration_small(const tlh_block_diag_t* pc, int mode, void* x, void* r, void* z, void* p,
+                                       const void* ap, int64_t n, int dtype, const tlh_pcg_state_t* st,
+                                       tlh_stream_t stream) {
+  if (!x || !r || !z || !p || !ap || n < 1 || n > SMALL_MAX_N || !state_ok(st)) return TLH_EINVAL;
+  if (mode != 0 && mode != 1) return TLH_EINVAL;
+  if (mode == 0 && (!pc || !precond_ok(pc, n))) return TLH_EINVAL;
+  tlh_block_diag_t empty{};
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (dtype == TLH_F32)
+    hipLaunchKernelGGL(small_iteration_kernel<float>, dim3(1), dim3(SMALL_THREADS), 0, s, mode == 0 ? *pc : empty,
+                       mode, (float*)x, (float*)r, (float*)z, (float*)p, (const float*)ap, n, st->control,
+                       st->scalars);
+  else if (dtype == TLH_F64)
+    hipLaunchKernelGGL(small_iteration_kernel<double>, dim3(1), dim3(SMALL_THREADS), 0, s, mode == 0 ? *pc : empty,
+                       mode, (double*)x, (double*)r, (double*)z, (double*)p, (const double*)ap, n, st->control,
+                       st->scalars);
+  else
+    return TLH_EINVAL;
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
 
 extern "C" int tlh_block_inverse(const double* blocks, int64_t count, int dim, double damping, double* inverse,
                                  int32_t* status, tlh_stream_t stream) {

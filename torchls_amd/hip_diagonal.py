@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .hip_normal import DeviceStructure, check_columns, plan_for
+from .hip_normal import DeviceStructure, check_columns, packed_blocks, plan_for
 
 
 class LinAlgError(RuntimeError):
@@ -74,17 +74,20 @@ class HipDiagonalPlan:
                 raise ValueError("HIP diagonal plan requires unchanged incidence structure, shapes and dtype")
 
     def _pack(self, jacobian) -> list:
-        packed = [
-            torch.empty((g.components, max(g.stride, 1)), dtype=self.dtype, device=self.device)
-            for g in self.plan.groups
-        ]
-        op = self.dev.operator(self.dtype, packed, [None] * len(self.plan.types))
         stream = _lib.stream_handle(self.device)
-        for gi, group in enumerate(jacobian.groups):
-            blocks = [b.contiguous() for b in group.blocks]
-            ptrs = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in blocks] + [None] * (3 - len(blocks)))
-            _lib.check(self.lib.tlh_pack_blocks(ctypes.byref(op), gi, ptrs,
-                                                _lib.dtype_code(self.block_dtype), stream), "tlh_pack_blocks")
+        packed = []
+        for gi, (g, group) in enumerate(zip(self.plan.groups, jacobian.groups)):
+            def pack(out, gi=gi, group=group):
+                bufs = [None] * len(self.plan.groups)
+                bufs[gi] = out
+                op = self.dev.operator(self.dtype, bufs, [None] * len(self.plan.types))
+                blocks = [b.contiguous() for b in group.blocks]
+                ptrs = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in blocks] + [None] * (3 - len(blocks)))
+                _lib.check(self.lib.tlh_pack_blocks(ctypes.byref(op), gi, ptrs,
+                                                    _lib.dtype_code(self.block_dtype), stream), "tlh_pack_blocks")
+
+            packed.append(packed_blocks(jacobian, gi, self.dev.order_keys[gi], g.components, g.stride, self.dtype,
+                                        pack))
         return packed
 
     def diagonal_blocks_f64(self, packed: list) -> tuple:

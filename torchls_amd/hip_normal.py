@@ -15,6 +15,7 @@ caller keeping float64 outer quantities.
 from __future__ import annotations
 
 import ctypes
+import hashlib
 
 import numpy as np
 import torch
@@ -44,6 +45,22 @@ def check_columns(jacobian) -> None:
                 raise ValueError("HIP operators require layout-ordered columns index*dim+offset+arange(dim)")
 
 
+def packed_blocks(jacobian, group: int, order_key, components: int, stride: int, dtype, pack) -> torch.Tensor:
+    """Packed [components, stride] blocks of one residual group, shared by every plan
+    whose sorted order, stride and dtype match (``order_key``); ``pack(out)`` fills a
+    new buffer.  Cached on the BlockJacobian, so each linearization packs once."""
+    cache = jacobian.__dict__.setdefault("_hip_packed", {})
+    # in-place edits of the blocks bump their version counters and force a re-pack
+    blocks = jacobian.groups[group].blocks
+    key = (group, order_key, components, stride, dtype, tuple((b.data_ptr(), b._version) for b in blocks))
+    out = cache.get(key)
+    if out is None:
+        out = torch.empty((components, max(stride, 1)), dtype=dtype, device=jacobian.groups[group].blocks[0].device)
+        pack(out)
+        cache[key] = out
+    return out
+
+
 class DeviceStructure:
     """Device copies of a StructurePlan plus the ctypes descriptors of the C ABI."""
 
@@ -51,6 +68,9 @@ class DeviceStructure:
         self.plan = plan
         self.device = device
         self.keep = []
+        # identity of each group's sorted order: plans with equal keys share packed blocks
+        self.order_keys = [hashlib.blake2b(np.ascontiguousarray(g.order).tobytes(), digest_size=16).hexdigest()
+                           for g in plan.groups]
         put = lambda a: self._keep(_np_to_device(a, device))  # noqa: E731
         self.groups = (_lib.Group * max(1, len(plan.groups)))()
         for gi, g in enumerate(plan.groups):
@@ -222,22 +242,24 @@ class HipNormalLinearization:
     def __init__(self, plan: HipNormalPlan, jacobian):
         self.plan = plan
         device, dtype = plan.device, plan.dtype
-        self.packed = []
-        for g in plan.plan.groups:
-            self.packed.append(torch.empty((g.components, max(g.stride, 1)), dtype=dtype, device=device))
         self.partials = [
             torch.empty(max(1, t.partial_count * t.dim), dtype=dtype, device=device) for t in plan.plan.types
         ]
-        self.op = plan.dev.operator(dtype, self.packed, self.partials)
         stream = _lib.stream_handle(device)
         in_code = _lib.dtype_code(plan.block_dtype)
-        for gi, group in enumerate(jacobian.groups):
-            blocks = [b.contiguous() for b in group.blocks]
-            ptrs = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in blocks] + [None] * (3 - len(blocks)))
-            _lib.check(
-                plan.lib.tlh_pack_blocks(ctypes.byref(self.op), gi, ptrs, in_code, stream),
-                "tlh_pack_blocks",
-            )
+        self.packed = []
+        for gi, (g, group) in enumerate(zip(plan.plan.groups, jacobian.groups)):
+            def pack(out, gi=gi, group=group):
+                bufs = [None] * len(plan.plan.groups)
+                bufs[gi] = out
+                op = plan.dev.operator(dtype, bufs, self.partials)
+                blocks = [b.contiguous() for b in group.blocks]
+                ptrs = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in blocks] + [None] * (3 - len(blocks)))
+                _lib.check(plan.lib.tlh_pack_blocks(ctypes.byref(op), gi, ptrs, in_code, stream), "tlh_pack_blocks")
+
+            self.packed.append(packed_blocks(jacobian, gi, plan.dev.order_keys[gi], g.components, g.stride, dtype,
+                                             pack))
+        self.op = plan.dev.operator(dtype, self.packed, self.partials)
 
     def _check_vector(self, vector: torch.Tensor) -> None:
         p = self.plan

@@ -126,8 +126,15 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph):
     st.scalars[_lib.PCG_THR2] = threshold * threshold
     st.control.copy_(torch.tensor([0, 0, config.max_iterations, 0], dtype=torch.int32))
 
+    small = (fused or apply is None) and n <= _lib.PCG_SMALL_MAX_N
+
     def iteration():
         ap = matvec(p)
+        if small:  # whole iteration in one workgroup (curvature, step, M^-1, direction, status)
+            _lib.check(lib.tlh_pcg_iteration_small(pc, 0 if fused else 1, x.data_ptr(), r.data_ptr(), z.data_ptr(),
+                                                   p.data_ptr(), ap.data_ptr(), n, code, sd, sh),
+                       "tlh_pcg_iteration_small")
+            return
         dot_into(p, ap, _lib.PCG_CURVATURE)
         if fused:
             _lib.check(lib.tlh_pcg_step(pc, 0, x.data_ptr(), r.data_ptr(), z.data_ptr(), p.data_ptr(),

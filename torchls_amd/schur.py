@@ -26,7 +26,7 @@ import torch
 
 from . import _lib
 from .hip_diagonal import LinAlgError
-from .hip_normal import DeviceStructure
+from .hip_normal import DeviceStructure, packed_blocks
 from .plan import build_structure
 
 
@@ -176,25 +176,37 @@ class HipSchurLinearization:
         self.dense = dense
         self.diagonal = tuple(d.to(torch.float64).contiguous() for d in diagonal)
         p = plan
-        self.packed = [
-            torch.empty((g.components, max(g.stride, 1)), dtype=p.dtype, device=p.device) for g in p.plan.groups
-        ]
-        self.op = p.dev.operator(p.dtype, self.packed, p.partials)
-        self.sop = p.sdev.operator(p.dtype, [self.packed[p.wgroup]], p.pre_partials)
         stream = _lib.stream_handle(p.device)
+        self.packed = []
         keep = []
         for k, (gi, slots, _) in enumerate(p.groups):
             grp = jacobian.groups[gi]
+            g = p.plan.groups[k]
             if k == p.wgroup:
                 blocks = [grp.blocks[0].contiguous(), grp.blocks[1].contiguous()]
             else:
                 rs = [s for s, t in enumerate(grp.type_indices) if t != p.e]
                 blocks = [grp.blocks[s].contiguous() for s in rs]
             keep.append(blocks)
-            ptrs = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in blocks] + [None] * (3 - len(blocks)))
-            _lib.check(p.lib.tlh_pack_blocks(ctypes.byref(self.op), k, ptrs, _lib.dtype_code(blocks[0].dtype), stream),
-                       "tlh_pack_blocks")
+
+            def pack(out, k=k, blocks=blocks):
+                bufs = [None] * len(p.plan.groups)
+                bufs[k] = out
+                op = p.dev.operator(p.dtype, bufs, p.partials)
+                ptrs = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in blocks] + [None] * (3 - len(blocks)))
+                _lib.check(p.lib.tlh_pack_blocks(ctypes.byref(op), k, ptrs, _lib.dtype_code(blocks[0].dtype),
+                                                 stream), "tlh_pack_blocks")
+
+            if k == p.wgroup:  # every slot of the jacobian group: shareable with other plans
+                self.packed.append(packed_blocks(jacobian, gi, p.dev.order_keys[k], g.components, g.stride, p.dtype,
+                                                 pack))
+            else:
+                out = torch.empty((g.components, max(g.stride, 1)), dtype=p.dtype, device=p.device)
+                pack(out)
+                self.packed.append(out)
         self._keep = keep
+        self.op = p.dev.operator(p.dtype, self.packed, p.partials)
+        self.sop = p.sdev.operator(p.dtype, [self.packed[p.wgroup]], p.pre_partials)
 
     def damped(self, damping: float) -> "HipSchurSystem":
         return HipSchurSystem(self, damping)

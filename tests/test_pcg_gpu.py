@@ -1,6 +1,7 @@
 """GPU parity of the HIP PCG (warp_pcg control flow) against the float64 oracle."""
 
 import math
+from functools import partial
 
 import numpy as np
 import pytest
@@ -8,8 +9,8 @@ import torch
 
 from oracle.pcg import Config, block_jacobi as np_block_jacobi, pcg_batched
 from tests.helpers import ba_jacobian, to_block_jacobian
-from torchls_amd import generators
-from torchls_amd.hip_diagonal import HipBlockJacobi, HipDiagonalPlan
+from torchls_amd import _lib, generators
+from torchls_amd.hip_diagonal import HipBlockJacobi, HipDiagonalPlan, LinAlgError
 from torchls_amd.hip_normal import HipNormalPlan
 from torchls_amd.hip_pcg import hip_pcg
 from torchls_amd.solvers import ConjugateGradientConfig
@@ -17,8 +18,8 @@ from torchls_amd.solvers import ConjugateGradientConfig
 pytestmark = pytest.mark.gpu
 
 
-def _system(device, dtype=torch.float64, seed=5, lam=1e-2):
-    prob = generators.bundle_adjustment(16, 500, 8, seed=seed)
+def _system(device, dtype=torch.float64, seed=5, lam=1e-2, size=(16, 500, 8)):
+    prob = generators.bundle_adjustment(*size, seed=seed)
     J, r = ba_jacobian(prob, delta=1e-2)
     BJ = to_block_jacobian(J, device, torch.float64)
     op = HipNormalPlan(BJ, dtype=dtype).bind(BJ)
@@ -53,6 +54,48 @@ def test_matches_oracle_control_flow(device, cfg):
     x = res.solution.cpu().numpy()
     assert np.linalg.norm(x - ref.solution) <= 1e-9 * np.linalg.norm(ref.solution)
     assert math.isclose(res.residual_norm, ref.residual_norm, rel_tol=1e-6, abs_tol=1e-14)
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 5])
+def test_matches_oracle_control_flow_multi_workgroup(device, cfg):
+    """n = 18,240 > 16,384: the multi-workgroup dot/step/direction kernels."""
+    print(f"pcg control flow (large) cfg={CONFIGS[cfg]}", flush=True)
+    J, BJ, op, diag, g, lam = _system(device, size=(40, 6000, 6))
+    assert J.n > _lib.PCG_SMALL_MAX_N
+    config = ConjugateGradientConfig(**CONFIGS[cfg])
+    pre = HipBlockJacobi(BJ.layout, diag, lam)
+    res = hip_pcg(partial(op.normal_matvec, damping=lam), torch.tensor(-g, device=device), preconditioner=pre,
+                  config=config)
+    ocfg = Config(**{k: getattr(config, k) for k in Config.__dataclass_fields__})
+    ref = pcg_batched(lambda v: J.normal_matvec(v, lam), -g,
+                      np_block_jacobi([d.cpu().numpy() for d in diag], J.layout, lam), ocfg)
+    assert (res.iterations, res.restarts, res.residual_checks, res.reason, res.converged) == (
+        ref.iterations, ref.restarts, ref.residual_checks, ref.reason, ref.converged)
+    x = res.solution.cpu().numpy()
+    assert np.linalg.norm(x - ref.solution) <= 1e-9 * np.linalg.norm(ref.solution)
+
+
+def test_graph_without_warmup_matches_eager(device):
+    """A bound HIP operator is captured without the warm-up iteration."""
+    J, BJ, op, diag, g, lam = _system(device)
+    pre = HipBlockJacobi(BJ.layout, diag, lam)
+    rhs = torch.tensor(-g, device=device)
+    cfg = ConjugateGradientConfig(residual_check_interval=10, graph_batch_iterations=4)
+    mv = partial(op.normal_matvec, damping=lam)
+    a = hip_pcg(mv, rhs, preconditioner=pre, config=cfg, use_graph=True)
+    b = hip_pcg(mv, rhs, preconditioner=pre, config=cfg, use_graph=False)
+    assert torch.equal(a.solution, b.solution) and a.iterations == b.iterations
+
+
+def test_deferred_positive_definiteness_check(device):
+    J, BJ, op, diag, g, lam = _system(device)
+    bad = tuple(-d.double() for d in diag)
+    pre = HipBlockJacobi(BJ.layout, bad, 0.0, defer_check=True)  # no host sync, no raise yet
+    with pytest.raises(LinAlgError):
+        hip_pcg(partial(op.normal_matvec, damping=lam), torch.tensor(-g, device=device), preconditioner=pre,
+                config=ConjugateGradientConfig())
+    with pytest.raises(LinAlgError):
+        HipBlockJacobi(BJ.layout, bad, 0.0)
 
 
 def test_graph_replay_matches_eager(device):

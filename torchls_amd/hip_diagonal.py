@@ -109,12 +109,18 @@ class HipDiagonalPlan:
 class HipBlockJacobi:
     """Callable z = (D + lambda I)^-1 r per variable block, float64 inverses."""
 
-    def __init__(self, layout, diagonal_f64, damping: float):
+    def __init__(self, layout, diagonal_f64, damping: float, *, status: torch.Tensor | None = None,
+                 defer_check: bool = False):
         device = diagonal_f64[0].device
         self.lib = _lib.load()
         self.layout = [(e.count, e.dim, e.offset) for e in layout]
         self.size = sum(c * d for c, d, _ in self.layout)
-        status = torch.zeros(1, dtype=torch.int32, device=device)
+        if status is None:
+            status = torch.zeros(1, dtype=torch.int32, device=device)
+        # defer_check: the positive-definiteness flag stays on the device and the
+        # HIP PCG reads it together with its first scalar read (one sync less)
+        self.status = status
+        self.deferred = bool(defer_check)
         stream = _lib.stream_handle(device)
         self.inverses = []
         desc = _lib.BlockDiag()
@@ -128,7 +134,12 @@ class HipBlockJacobi:
             desc.dim[t], desc.count[t], desc.offset[t], desc.inv[t] = d, c, off, inv.data_ptr()
         self.desc = desc
         self.device = device
-        if int(status.item()) != 0:  # one host sync per factorisation, like torch.linalg.cholesky
+        if not self.deferred:
+            self.check()
+
+    def check(self) -> None:
+        """Raise LinAlgError if a damped block was not positive definite (one host sync)."""
+        if int(self.status.item()) != 0:
             raise LinAlgError("damped diagonal block is not positive definite")
 
     def __call__(self, vector: torch.Tensor) -> torch.Tensor:

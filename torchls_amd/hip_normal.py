@@ -239,6 +239,8 @@ class HipNormalPlan:
 class HipNormalLinearization:
     """One linearization packed into the plan layout (WarpNormalLinearization)."""
 
+    hip_graph_safe = True  # normal_matvec launches only HIP kernels on the current stream
+
     def __init__(self, plan: HipNormalPlan, jacobian):
         self.plan = plan
         device, dtype = plan.device, plan.dtype

@@ -100,7 +100,16 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph):
     x = torch.zeros_like(rhs)
     r = rhs.clone()
     dot_into(rhs, rhs, _lib.PCG_AUX)
-    (rhs_norm2,) = read_scalars(_lib.PCG_AUX)
+    status = getattr(preconditioner, "status", None) if getattr(preconditioner, "deferred", False) else None
+    if status is None:
+        (rhs_norm2,) = read_scalars(_lib.PCG_AUX)
+    else:  # the preconditioner's deferred positive-definiteness flag rides on the same read
+        host = torch.stack((st.scalars[_lib.PCG_AUX], status[0].to(torch.float64))).cpu()
+        rhs_norm2, bad = float(host[0]), float(host[1])
+        if bad != 0:
+            from .hip_diagonal import LinAlgError
+
+            raise LinAlgError("damped block is not positive definite")
     rhs_norm = math.sqrt(rhs_norm2) if rhs_norm2 >= 0 else math.nan
     threshold = max(config.atol, config.rtol * rhs_norm)
     if not math.isfinite(rhs_norm) or rhs_norm <= threshold:
@@ -153,8 +162,14 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph):
     # Warm up once (as warp_pcg.py:162-164), capture one iteration, reset.  The
     # capture is begun on the work stream directly: torch.cuda.graph's context
     # manager would add a device synchronize, a garbage collection and an
-    # allocator cache flush to every solve.
-    iteration()
+    # allocator cache flush to every solve.  Operators made only of this
+    # library's kernels (hip_graph_safe) need no warm-up: nothing is initialised
+    # lazily, and the capture records without executing, so the state set by
+    # the reset above is still intact.
+    owner = getattr(getattr(matvec, "func", matvec), "__self__", None)
+    warm = not (use_graph and getattr(owner, "hip_graph_safe", False) and (fused or apply is None))
+    if warm:
+        iteration()
     graph = None
     if use_graph:
         graph = torch.cuda.CUDAGraph()
@@ -163,8 +178,9 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph):
             iteration()
         finally:
             graph.capture_end()
-    reset(rhs, True)
-    st.control.zero_()
+    if warm:
+        reset(rhs, True)
+        st.control.zero_()
 
     host_control = torch.zeros(4, dtype=torch.int32).pin_memory()
     iterations = checks = restarts = 0

@@ -208,24 +208,38 @@ class HipSchurLinearization:
         self.op = p.dev.operator(p.dtype, self.packed, p.partials)
         self.sop = p.sdev.operator(p.dtype, [self.packed[p.wgroup]], p.pre_partials)
 
-    def damped(self, damping: float) -> "HipSchurSystem":
-        return HipSchurSystem(self, damping)
+    def damped(self, damping: float, *, defer_check: bool = False) -> "HipSchurSystem":
+        return HipSchurSystem(self, damping, defer_check=defer_check)
 
 
 class HipSchurSystem:
-    def __init__(self, lin: HipSchurLinearization, damping: float):
+    """DampedSchurSystem (schur.py:259-397) on the HIP kernels.
+
+    With ``defer_check`` the positive-definiteness flag of the eliminated-block
+    inverses stays on the device; the preconditioner shares it and the HIP PCG
+    reads it with its first scalar read, raising LinAlgError before any
+    iteration (the reference raises from torch.linalg.cholesky at this point)."""
+
+    hip_graph_safe = True  # matvec launches only HIP kernels on the current stream
+
+    def __init__(self, lin: HipSchurLinearization, damping: float, *, defer_check: bool = False):
         self.lin, self.plan, self.damping = lin, lin.plan, float(damping)
         p = self.plan
         dev = p.device
         stream = _lib.stream_handle(dev)
-        status = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.status = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.deferred = bool(defer_check)
         d = p.edim
         self.cinv = torch.empty(max(1, p.ecount * d * (d + 1) // 2), dtype=torch.float64, device=dev)
         _lib.check(p.lib.tlh_block_inverse(lin.diagonal[p.e].data_ptr(), p.ecount, d, self.damping,
-                                           self.cinv.data_ptr(), status.data_ptr(), stream), "tlh_block_inverse")
-        if int(status.item()) != 0:
-            raise LinAlgError("damped eliminated block is not positive definite")
+                                           self.cinv.data_ptr(), self.status.data_ptr(), stream), "tlh_block_inverse")
+        if not self.deferred:
+            self.check()
         self._bp = None
+
+    def check(self) -> None:
+        if int(self.status.item()) != 0:
+            raise LinAlgError("damped eliminated block is not positive definite")
 
     # -- helpers
     def _apply(self, mode, xc, bp, outp):
@@ -307,7 +321,7 @@ class HipSchurSystem:
         p = self.plan
         lay = tuple(VariableLayout(p.layout[t].var_type, p.layout[t].ids, off)
                     for t, (_, _, off) in zip(p.retained, p.rlayout[:-1]))
-        return HipBlockJacobi(lay, self.preconditioner_blocks(), 0.0)
+        return HipBlockJacobi(lay, self.preconditioner_blocks(), 0.0, status=self.status, defer_check=self.deferred)
 
     def dense_matrix(self) -> torch.Tensor:
         p = self.plan
@@ -327,6 +341,8 @@ class HipSchurSystem:
     def dense_solve(self, rhs: torch.Tensor) -> torch.Tensor:
         from .cholesky import cholesky_solve_f64
 
+        if self.deferred:
+            self.check()
         return cholesky_solve_f64(self.dense_matrix(), rhs.to(torch.float64)).to(self.plan.dtype)
 
 

@@ -367,7 +367,10 @@ def solve(
                     usable = bool(torch.isfinite(delta).all())
                 elif is_schur:
                     with region("prepare_schur"):
-                        system = schur_linear.damped(damping)
+                        # with a preconditioner, the PCG's first read also checks the damped
+                        # blocks' positive definiteness (LinAlgError before any iteration)
+                        system = schur_linear.damped(damping, defer_check=linear_solver == "schur_cg" and
+                                                     conjugate_gradient.preconditioner is not None)
                         rhs = system.reduced_rhs(-linear_gradient)
                     with region("linear_solve"):
                         if linear_solver == "schur_dense":
@@ -392,7 +395,7 @@ def solve(
                     summary.linear_converged_history.append(converged)
                 else:
                     precondition = (
-                        HipBlockJacobi(problem.layout, diagonal, damping)
+                        HipBlockJacobi(problem.layout, diagonal, damping, defer_check=True)
                         if conjugate_gradient.preconditioner is not None else None
                     )
                     with region("linear_solve"):

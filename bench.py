@@ -48,6 +48,8 @@ def parse():
     p.add_argument("--repeats", type=int, default=3, help="timed solves per precision (median reported)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a 1 GiB device copy 20 times (HBM counter calibration) and exit")
+    p.add_argument("--dense-only", action="store_true",
+                   help="only time the dense Schur MFMA Cholesky (for counter profiles) and exit")
     return p.parse_args()
 
 
@@ -161,6 +163,40 @@ def large_normal_matvec(device, reps):
             "peak": HBM_PEAK_GBS, "frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+def dense_schur_cholesky(device, reps):
+    """MFMA float64 Cholesky of the 1,536 x 1,536 reduced camera system of a 256-camera
+    problem (dense Schur path): HIP-event time per factorisation and n^3/3 flop rate."""
+    from torchls_amd import _lib, generators
+    from torchls_amd.schur import HipSchurPlan
+
+    prob = generators.ba_dense_schur()
+    P = build_problem(prob, device)
+    an = P._analyze()
+    lin = an.linearize(P.parameters)
+    diag = tuple(d.double() for d in lin.jacobian.diagonal_blocks())
+    plan = HipSchurPlan(an, "points", device, dtype=torch.float64)
+    M = plan.linearize(lin.jacobian, diag, dense=True).damped(1e-3).dense_matrix()
+    n = M.shape[0]
+    work = M.clone()
+    status = torch.zeros(1, dtype=torch.int32, device=device)
+    lib = _lib.load()
+
+    def factor():
+        work.copy_(M)
+        _lib.check(lib.tlh_cholesky_f64(work.data_ptr(), n, n, status.data_ptr(), _lib.stream_handle(device)),
+                   "tlh_cholesky_f64")
+
+    t_all = event_time(factor, reps)
+    t_copy = event_time(lambda: work.copy_(M), reps)
+    t = max(t_all - t_copy, 1e-9)
+    if int(status.item()) != 0:
+        raise RuntimeError("dense Schur matrix is not positive definite")
+    flops = n ** 3 / 3
+    return {"workload": "ba_dense_schur", "cameras": prob.n_cameras, "n": n, "us_per_factorisation": round(t * 1e6, 1),
+            "tflops": round(flops / t / 1e12, 3), "peak_tflops_fp64_mfma": 78.6,
+            "frac": round(flops / t / 1e12 / 78.6, 4)}
+
+
 def event_time(fn, reps):
     """Average seconds per call: HIP events on the current stream around ``reps`` calls."""
     for _ in range(5):
@@ -188,6 +224,9 @@ def main():
     args = parse()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.dense_only:
+        print(json.dumps(dense_schur_cholesky(torch.device("cuda:0"), 20)), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
@@ -218,6 +257,8 @@ def main():
             runs[dt].append(timed_solve(P, init, args.steps, dt))
     t_mixed, s_mixed = sorted(runs[torch.float32], key=lambda r: r[0])[len(runs[torch.float32]) // 2]
     t_f64, s_f64 = sorted(runs[None], key=lambda r: r[0])[len(runs[None]) // 2]
+    log("timed solves (s): mixed " + " ".join(f"{r[0]:.4f}" for r in runs[torch.float32]) + "; float64 " +
+        " ".join(f"{r[0]:.4f}" for r in runs[None]))
     log(f"mixed: {args.steps} LM iterations in {t_mixed:.4f}s (median of {len(runs[None])}) "
         f"final_cost={s_mixed.final_cost:.12e}")
     log(f"float64: {args.steps} LM iterations in {t_f64:.4f}s final_cost={s_f64.final_cost:.12e}")
@@ -245,6 +286,9 @@ def main():
         large = large_normal_matvec(device, args.kernel_reps)
         log(f"normal matvec at 5M observations: {large['us_per_launch']} us ({large['achieved']} GB/s, "
             f"{100 * large['frac']:.1f}% of {HBM_PEAK_GBS:.0f} GB/s)")
+
+    chol = dense_schur_cholesky(device, 20)
+    log(f"dense Schur Cholesky n={chol['n']}: {chol['us_per_factorisation']} us, {chol['tflops']} TFLOP/s")
 
     cpu = None
     oracle_rel = None
@@ -276,6 +320,7 @@ def main():
                      "kernel": "implicit Schur matvec (schur_tile_kernel + finish)",
                      "us_per_launch": round(t_schur * 1e6, 2), "bytes_per_launch": schur_bytes},
         "normal_matvec_large": large,
+        "dense_schur_cholesky": chol,
         "cpu_baseline": cpu,
     }), flush=True)
 

@@ -1,0 +1,204 @@
+"""Pins of the float64 oracle itself, by closed forms, brute force and textbook routines.
+
+Each check uses code written here, independent of the oracle function it pins, so
+that a plausible mistake in the oracle (a dropped alias term, a transposed block,
+a wrong sign or a missing damping term) fails at least one assertion.
+"""
+
+from fractions import Fraction
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle.ba import huber_reduction, huber_value, huber_weights, linearize_ba, reprojection
+from oracle.pcg import Config, block_jacobi, pcg_batched, pcg_reference
+from oracle.schur import SchurSystem
+from oracle.sparse import NpGroup, NpJacobian, random_jacobian
+from torchls_amd import generators
+
+LAYOUT = [(6, 6), (40, 3), (3, 1)]
+GROUPS = [(120, 2, [0, 1]), (15, 3, [1]), (30, 6, [0, 0]), (10, 2, [2, 1, 2])]
+
+
+def brute_dense(J):
+    """Dense J by explicit loops over observations, slots, rows and columns."""
+    A = np.zeros((J.m, J.n))
+    row0 = 0
+    for g in J.groups:
+        for o in range(g.count):
+            for s, t in enumerate(g.types):
+                count, dim, off = J.layout[t]
+                for r in range(g.rows):
+                    for c in range(dim):
+                        A[row0 + o * g.rows + r, off + g.indices[s][o] * dim + c] += g.blocks[s][o, r, c]
+        row0 += g.count * g.rows
+    return A
+
+
+@pytest.fixture(scope="module")
+def jac():
+    return random_jacobian(np.random.default_rng(11), LAYOUT, GROUPS)
+
+
+def test_sparse_operators_against_brute_force(jac):
+    A = brute_dense(jac)
+    rng = np.random.default_rng(2)
+    x, y = rng.standard_normal(jac.n), rng.standard_normal(jac.m)
+    assert np.allclose(jac.to_dense(), A, rtol=0, atol=0)
+    assert np.allclose(jac.matvec(x), A @ x, rtol=1e-13, atol=1e-13)
+    assert np.allclose(jac.rmatvec(y), A.T @ y, rtol=1e-13, atol=1e-13)
+    assert np.allclose(jac.normal_matvec(x, 0.3), A.T @ (A @ x) + 0.3 * x, rtol=1e-12, atol=1e-12)
+    assert np.isclose(jac.squared_matvec_norm(x), np.sum((A @ x) ** 2), rtol=1e-12)
+    s = rng.uniform(0.5, 2.0, jac.n)
+    assert np.allclose(jac.scale_columns(s).to_dense(), A * s[None, :], rtol=1e-14, atol=0)
+
+
+def test_diagonal_blocks_include_alias_terms(jac):
+    A = brute_dense(jac)
+    N = A.T @ A
+    blocks = jac.diagonal_blocks()
+    for (count, dim, off), D in zip(jac.layout, blocks):
+        for v in range(count):
+            sl = slice(off + v * dim, off + (v + 1) * dim)
+            assert np.allclose(D[v], N[sl, sl], rtol=1e-12, atol=1e-12)
+
+
+def _se3_twist(xi):
+    v, w = xi[:3], xi[3:]
+    m = np.zeros((4, 4))
+    m[:3, :3] = [[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]]
+    m[:3, 3] = v
+    return m
+
+
+def test_reprojection_blocks_are_the_derivatives():
+    """Central differences of the residual through T expm(hat(delta)) and p + delta."""
+    prob = generators.bundle_adjustment(4, 6, 3, seed=9)
+    cams, pts = prob.cameras_init, prob.points_init
+    ci, pi, uv = prob.cam_idx, prob.pt_idx, prob.observations
+    _, Jc, Jp = reprojection(cams, pts, ci, pi, uv)
+    h = 1e-6
+    for o in range(len(ci)):
+        T, p = cams[ci[o]], pts[pi[o]]
+
+        def res(Tm, pm):
+            q = Tm[:3, :3] @ pm + Tm[:3, 3]
+            return q[:2] / q[2] - uv[o]
+
+        for k in range(6):
+            e = np.zeros(6)
+            e[k] = h
+            fd = (res(T @ scipy.linalg.expm(_se3_twist(e)), p) - res(T @ scipy.linalg.expm(_se3_twist(-e)), p)) / (2 * h)
+            assert np.allclose(Jc[o][:, k], fd, rtol=1e-6, atol=1e-8)
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = h
+            fd = (res(T, p + e) - res(T, p - e)) / (2 * h)
+            assert np.allclose(Jp[o][:, k], fd, rtol=1e-6, atol=1e-8)
+
+
+def test_huber_weighting_closed_form():
+    prob = generators.bundle_adjustment(4, 30, 3, seed=3)
+    delta = 3e-3
+    model, Jc, Jp, raw, half = linearize_ba(prob.cameras_init, prob.points_init, prob.cam_idx, prob.pt_idx,
+                                            prob.observations, delta)
+    s = np.sum(raw ** 2, -1)
+    w = np.where(np.sqrt(s) > delta, np.sqrt(delta / np.sqrt(s)), 1.0)
+    assert np.allclose(model, raw * w[:, None], rtol=1e-15)
+    rho = np.where(np.sqrt(s) > delta, 2 * delta * np.sqrt(s) - delta ** 2, s)
+    assert np.isclose(half, 0.5 * rho.sum(), rtol=1e-14)
+    sw, v = huber_weights(raw, delta)
+    assert np.allclose(sw, w, rtol=1e-15) and np.allclose(v, rho, rtol=1e-15)
+
+
+@pytest.mark.parametrize("a,b", [(0.5625, 0.25), (4.0, 9.0), (0.25, 16.0), (16.0, 0.25), (2.25, 2.25)])
+def test_huber_reduction_is_the_exact_difference(a, b):
+    """rho(a) - rho(b) in exact rational arithmetic (delta = 1, perfect squares)."""
+    def rho(s):
+        r = Fraction(s).limit_denominator()
+        root = Fraction(int(round(float(r) ** 0.5 * 4)), 4)
+        assert root * root == r
+        return r if root <= 1 else 2 * root - 1
+
+    exact = float(rho(a) - rho(b))
+    diff = a - b  # one-row residuals: (x - y)(x + y) = a - b
+    got = float(huber_reduction(np.array([a]), np.array([b]), np.array([diff]), 1.0)[0])
+    assert abs(got - exact) <= 1e-15 * max(1.0, abs(exact))
+    assert float(huber_value(np.array([a]), 1.0)[0]) == float(rho(a))
+
+
+def _ba_system(damping):
+    prob = generators.bundle_adjustment(5, 25, 4, seed=21)
+    _, Jc, Jp, _, _ = linearize_ba(prob.cameras_init, prob.points_init, prob.cam_idx, prob.pt_idx,
+                                   prob.observations, None)
+    layout = [(5, 6, 0), (25, 3, 30)]
+    return NpJacobian([NpGroup([Jc, Jp], [0, 1], [prob.cam_idx, prob.pt_idx])], layout)
+
+
+def test_schur_against_dense_closed_form():
+    lam = 0.02
+    J = _ba_system(lam)
+    A = brute_dense(J)
+    H = A.T @ A + lam * np.eye(J.n)
+    c, e = slice(0, 30), slice(30, J.n)
+    S = H[c, c] - H[c, e] @ np.linalg.solve(H[e, e], H[e, c])
+    sysm = SchurSystem(J, J.diagonal_blocks(), 1, lam)
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(30)
+    assert np.allclose(sysm.matvec(x), S @ x, rtol=1e-10, atol=1e-10)
+    assert np.allclose(sysm.dense_matrix(), S, rtol=1e-10, atol=1e-10)
+    for v, M in enumerate(sysm.preconditioner_blocks()[0]):
+        assert np.allclose(M, S[6 * v: 6 * v + 6, 6 * v: 6 * v + 6], rtol=1e-10, atol=1e-10)
+    rhs = rng.standard_normal(J.n)
+    xr = np.linalg.solve(S, sysm.reduced_rhs(rhs))
+    assert np.allclose(sysm.backsubstitute(xr), np.linalg.solve(H, rhs), rtol=1e-9, atol=1e-10)
+
+
+def _spd(n, seed):
+    rng = np.random.default_rng(seed)
+    B = rng.standard_normal((n, n))
+    return B @ B.T + n * np.eye(n), rng.standard_normal(n)
+
+
+def test_pcg_reference_is_textbook_pcg():
+    M, b = _spd(30, 1)
+    d = np.diag(M).copy()
+    res = pcg_reference(lambda v: M @ v, b, lambda v: v / d, Config(rtol=1e-12, atol=0, max_iterations=200))
+    assert res.converged and np.allclose(res.solution, np.linalg.solve(M, b), rtol=1e-9)
+    # first three iterates of the Hestenes-Stiefel recurrence (Saad, Alg. 9.1)
+    x = np.zeros(30)
+    r = b.copy()
+    z = r / d
+    p = z.copy()
+    for _ in range(3):
+        a = (r @ z) / (p @ (M @ p))
+        x = x + a * p
+        r_new = r - a * (M @ p)
+        z_new = r_new / d
+        p = z_new + (r_new @ z_new) / (r @ z) * p
+        r, z = r_new, z_new
+    three = pcg_reference(lambda v: M @ v, b, lambda v: v / d, Config(rtol=0, atol=0, max_iterations=3))
+    assert np.allclose(three.solution, x, rtol=1e-12, atol=1e-14) and three.iterations == 3
+
+
+def test_pcg_batched_matches_reference_without_checks():
+    M, b = _spd(40, 2)
+    cfg = Config(rtol=0, atol=0, max_iterations=12, residual_check_interval=0)
+    a = pcg_reference(lambda v: M @ v, b, None, cfg)
+    c = pcg_batched(lambda v: M @ v, b, None, cfg)
+    assert a.iterations == c.iterations == 12
+    assert np.allclose(a.solution, c.solution, rtol=1e-12, atol=1e-14)
+    full = pcg_batched(lambda v: M @ v, b, None, Config(rtol=1e-12, atol=0, max_iterations=400))
+    assert full.converged and np.allclose(full.solution, np.linalg.solve(M, b), rtol=1e-9)
+
+
+def test_block_jacobi_is_the_blockwise_solve(jac):
+    blocks = jac.diagonal_blocks()
+    apply = block_jacobi(blocks, jac.layout, 0.7)
+    v = np.random.default_rng(8).standard_normal(jac.n)
+    out = apply(v)
+    for (count, dim, off), D in zip(jac.layout, blocks):
+        for k in range(count):
+            sl = slice(off + k * dim, off + (k + 1) * dim)
+            assert np.allclose(out[sl], np.linalg.solve(D[k] + 0.7 * np.eye(dim), v[sl]), rtol=1e-10, atol=1e-12)

@@ -251,14 +251,17 @@ struct WaveArgs {
 };
 
 // out[k] = base[start + b + k]; a short tile (len < T) clamps to its last position.
-template <typename T, int ITEMS>
+// STREAM marks data read once per launch from a working set larger than the
+// last-level cache: the loads carry the non-temporal hint.
+template <typename T, int ITEMS, bool STREAM = false>
 __device__ __forceinline__ void load_items(const T* __restrict__ base, int64_t start, int b, int len, bool full,
                                            T (&out)[ITEMS]) {
   if constexpr (ITEMS == 1) {
     out[0] = base[start + min(b, len - 1)];
   } else if (full) {
-    const typename VecT<T, ITEMS>::type v =
-        *reinterpret_cast<const typename VecT<T, ITEMS>::type*>(base + start + b);
+    typedef typename VecT<T, ITEMS>::type V;
+    const V v = STREAM ? __builtin_nontemporal_load(reinterpret_cast<const V*>(base + start + b))
+                       : *reinterpret_cast<const V*>(base + start + b);
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) out[k] = v[k];
   } else {
@@ -274,7 +277,7 @@ struct WaveTile {
   Real jp[R][DP][ITEMS], js[R][DS][ITEMS];
 };
 
-template <typename Real, int R, int DP, int DS, int OP, int OS, int ITEMS, int NT>
+template <typename Real, int R, int DP, int DS, int OP, int OS, int ITEMS, int NT, bool STREAM>
 __device__ __forceinline__ void load_wave_tile(const WaveArgs<Real>& a, int t, int b,
                                                WaveTile<Real, R, DP, DS, ITEMS>& d) {
   constexpr int T = NT * ITEMS;
@@ -296,14 +299,16 @@ __device__ __forceinline__ void load_wave_tile(const WaveArgs<Real>& a, int t, i
   for (int r = 0; r < R; ++r) {
 #pragma unroll
     for (int c = 0; c < DP; ++c)
-      load_items<Real, ITEMS>(a.packed + (int64_t)(OP + r * DP + c) * a.stride, start, b, len, full, d.jp[r][c]);
+      load_items<Real, ITEMS, STREAM>(a.packed + (int64_t)(OP + r * DP + c) * a.stride, start, b, len, full,
+                                      d.jp[r][c]);
 #pragma unroll
     for (int c = 0; c < DS; ++c)
-      load_items<Real, ITEMS>(a.packed + (int64_t)(OS + r * DS + c) * a.stride, start, b, len, full, d.js[r][c]);
+      load_items<Real, ITEMS, STREAM>(a.packed + (int64_t)(OS + r * DS + c) * a.stride, start, b, len, full,
+                                      d.js[r][c]);
   }
 }
 
-template <typename Real, int R, int D0, int D1, int P, int ITEMS, int NT>
+template <typename Real, int R, int D0, int D1, int P, int ITEMS, int NT, bool STREAM>
 __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
   constexpr int DP = P == 0 ? D0 : D1;
   constexpr int DS = P == 0 ? D1 : D0;
@@ -324,7 +329,7 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
   const int t1 = min(t0 + a.tiles_per_wg, a.ntiles);
   for (int t = t0; t < t1; ++t) {
     WaveTile<Real, R, DP, DS, ITEMS> cur;
-    load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT>(a, t, b, cur);
+    load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM>(a, t, b, cur);
     const int64_t start = (int64_t)t * T;
     const int len = (int)min((int64_t)T, a.count - start);
     const int left = len - b;  // items of this thread inside the tile (may be <= 0)
@@ -554,7 +559,7 @@ __global__ void __launch_bounds__(TLH_THREADS)
 // Finish for types with many dense blocks: a workgroup owns 32 consecutive
 // columns; its 8 column lanes each sum every 8th dense block (fixed stride),
 // then the 8 lane sums are added in lane order.  Same result for every run.
-constexpr int FIN_COLS = 8;
+constexpr int FIN_COLS = 32;
 constexpr int FIN_LANES = TLH_THREADS / FIN_COLS;
 
 template <typename Real>
@@ -572,6 +577,7 @@ __global__ void __launch_bounds__(TLH_THREADS)
   const int64_t i = (int64_t)blockIdx.x * FIN_COLS + col;
   Real sum = Real(0);
   if (i < total) {
+#pragma unroll 4
     for (int b = lane; b < dense_blocks; b += FIN_LANES) sum += partials[(int64_t)b * total + i];
   }
   lane_sum[lane][col] = sum;
@@ -790,7 +796,10 @@ int launch_wave_t(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
   a.damping = (Real)damping;
   const int64_t lds = (int64_t)NT * ITEMS * ((1 - P == 0 ? D0 : D1) * (int64_t)sizeof(Real) + 4) +
                       (ss.mode == TLH_REDUCE_DENSE ? ts.count * (1 - P == 0 ? D0 : D1) * (int64_t)sizeof(Real) : 0);
-  auto kernel = normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT>;
+  // blocks that cannot stay in the 256 MB last-level cache between matvecs are streamed
+  const bool stream_blocks = (int64_t)g.stride * g.rows * (D0 + D1) * (int64_t)sizeof(Real) > (192ll << 20);
+  auto kernel = stream_blocks ? normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, true>
+                              : normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, false>;
   TLH_RETURN_IF_ERROR(allow_lds((const void*)kernel, lds));
   hipLaunchKernelGGL(kernel, dim3(g.nwg), dim3(NT), (size_t)lds, stream, a);
   TLH_RETURN_IF_LAUNCH_FAILED();

@@ -39,7 +39,9 @@ typedef struct ihipStream_t* tlh_stream_t; /* == hipStream_t */
 #define TLH_TILE_THREADS 256
 /* Wave-kernel groups (tlh_group_t.kernel == 1): workgroups of TLH_WAVE_THREADS
  * threads over aligned tiles of TLH_WAVE_TILE (float32) or TLH_WAVE_TILE / 2
- * (float64) sorted positions; tile t starts at t * tile_cap. */
+ * (float64) sorted positions; tile t starts at t * tile_cap.  The packed stride
+ * and every slot's index (and rank) array must cover ntiles * tile_cap positions,
+ * the padding indices naming valid variables. */
 #define TLH_WAVE_THREADS 512
 #define TLH_WAVE_TILE 1024
 

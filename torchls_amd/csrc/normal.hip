@@ -270,40 +270,57 @@ __device__ __forceinline__ void load_items(const T* __restrict__ base, int64_t s
   }
 }
 
-// Raw per-thread data of one tile: ids, ranks and block components.
+// Raw per-thread data of one tile: ids, ranks and block components.  Ranks stay
+// packed as loaded (two 16-bit ranks per word for even ITEMS) and are unpacked
+// only where they are used, after the block loads are in flight.
 template <typename Real, int R, int DP, int DS, int ITEMS>
 struct WaveTile {
-  int ip[ITEMS], is[ITEMS], rk[ITEMS];
+  static constexpr int RW = (ITEMS + 1) / 2;
+  int ip[ITEMS], is[ITEMS];
+  uint32_t rk[RW];
   Real jp[R][DP][ITEMS], js[R][DS][ITEMS];
+  __device__ __forceinline__ int rank(int k) const {
+    return ITEMS == 1 ? (int)rk[0] : (int)((k & 1) ? (rk[k >> 1] >> 16) : (rk[k >> 1] & 0xffffu));
+  }
 };
 
+// Whole-tile loads: the packed stride and the id/rank arrays of a wave group are
+// padded to ntiles * T positions (ids with a valid variable), so no load needs a
+// bounds branch; positions past the group's count are masked by the caller.
+// Ranks are read as single 16-bit loads (zero-extended, no unpacking) so that
+// nothing forces a wait between the id loads and the block loads.
 template <typename Real, int R, int DP, int DS, int OP, int OS, int ITEMS, int NT, bool STREAM>
 __device__ __forceinline__ void load_wave_tile(const WaveArgs<Real>& a, int t, int b,
                                                WaveTile<Real, R, DP, DS, ITEMS>& d) {
   constexpr int T = NT * ITEMS;
   const int64_t start = (int64_t)t * T;
-  const int len = (int)min((int64_t)T, a.count - start);
-  const bool full = len == T;
-  load_items<int32_t, ITEMS>(a.idx_p, start, b, len, full, d.ip);
-  load_items<int32_t, ITEMS>(a.idx_s, start, b, len, full, d.is);
+  load_items<int32_t, ITEMS>(a.idx_p, start, b, T, true, d.ip);
+  load_items<int32_t, ITEMS>(a.idx_s, start, b, T, true, d.is);
   if (a.rank_s) {
-    uint16_t r16[ITEMS];
-    load_items<uint16_t, ITEMS>(a.rank_s, start, b, len, full, r16);
+    if constexpr (ITEMS == 1) {
+      d.rk[0] = a.rank_s[start + b];
+    } else {
+      const uint32_t* r32 = reinterpret_cast<const uint32_t*>(a.rank_s + start + b);
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) d.rk[k] = r16[k];
+      for (int w = 0; w < ITEMS / 2; ++w) d.rk[w] = r32[w];
+    }
   } else {
+    if constexpr (ITEMS == 1) {
+      d.rk[0] = (uint32_t)b;
+    } else {
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) d.rk[k] = b + k;
+      for (int w = 0; w < ITEMS / 2; ++w) d.rk[w] = (uint32_t)(b + 2 * w) | ((uint32_t)(b + 2 * w + 1) << 16);
+    }
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
 #pragma unroll
     for (int c = 0; c < DP; ++c)
-      load_items<Real, ITEMS, STREAM>(a.packed + (int64_t)(OP + r * DP + c) * a.stride, start, b, len, full,
+      load_items<Real, ITEMS, STREAM>(a.packed + (int64_t)(OP + r * DP + c) * a.stride, start, b, T, true,
                                       d.jp[r][c]);
 #pragma unroll
     for (int c = 0; c < DS; ++c)
-      load_items<Real, ITEMS, STREAM>(a.packed + (int64_t)(OS + r * DS + c) * a.stride, start, b, len, full,
+      load_items<Real, ITEMS, STREAM>(a.packed + (int64_t)(OS + r * DS + c) * a.stride, start, b, T, true,
                                       d.js[r][c]);
   }
 }
@@ -333,12 +350,11 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
     const int64_t start = (int64_t)t * T;
     const int len = (int)min((int64_t)T, a.count - start);
     const int left = len - b;  // items of this thread inside the tile (may be <= 0)
-    int ip[ITEMS], is[ITEMS], rk[ITEMS];
+    int ip[ITEMS], is[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       ip[k] = cur.ip[k];
       is[k] = cur.is[k];
-      rk[k] = cur.rk[k];
     }
     Real xp[ITEMS][DP], xs[ITEMS][DS];
 #pragma unroll
@@ -451,9 +467,10 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       if (k < left) {
+        const int rk = cur.rank(k);
 #pragma unroll
-        for (int c = 0; c < DS; ++c) cstage[rk[k] * DS + c] = cs[k][c];
-        cid[rk[k]] = is[k];
+        for (int c = 0; c < DS; ++c) cstage[rk * DS + c] = cs[k][c];
+        cid[rk] = is[k];
       }
     }
     __syncthreads();
@@ -771,8 +788,10 @@ int launch_wave_t(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
   const tlh_type_t& tp = op->types[sp.type];
   const tlh_type_t& ts = op->types[ss.type];
   WaveArgs<Real> a{};
-  // aligned tiles of exactly NT * ITEMS sorted positions
-  if (g.tile_cap != NT * ITEMS || g.ntiles != ceil_div(g.count, NT * ITEMS) || g.stride % ITEMS != 0)
+  // aligned tiles of exactly NT * ITEMS sorted positions; whole-tile loads need the
+  // packed stride (and the caller's id/rank arrays) to cover ntiles * tile_cap positions
+  if (g.tile_cap != NT * ITEMS || g.ntiles != ceil_div(g.count, NT * ITEMS) || g.stride % ITEMS != 0 ||
+      g.stride < (int64_t)g.ntiles * g.tile_cap)
     return TLH_EINVAL;
   a.packed = static_cast<const Real*>(g.packed);
   a.stride = g.stride;

@@ -94,8 +94,13 @@ class DeviceStructure:
                 ds.dim = s.dim
                 ds.mode = s.mode
                 ds.dense_block = s.dense_block
-                ds.index = put(s.index).data_ptr()
-                ds.rank = None if s.rank is None else put(s.rank).data_ptr()
+                index, rank = s.index, s.rank
+                if g.kernel == 1:  # wave groups read ids and ranks of whole tiles: pad to ntiles * tile_cap
+                    pad = g.ntiles * g.tile_cap - g.count
+                    index = np.concatenate((index, np.full(pad, index[-1], dtype=index.dtype)))
+                    rank = None if rank is None else np.concatenate((rank, np.zeros(pad, dtype=rank.dtype)))
+                ds.index = put(index).data_ptr()
+                ds.rank = None if rank is None else put(rank).data_ptr()
                 ds.seg_ptr = put(s.seg_ptr).data_ptr()
                 ds.seg_start = put(s.seg_start).data_ptr()
                 ds.seg_target = put(s.seg_target).data_ptr()

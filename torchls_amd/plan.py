@@ -81,7 +81,8 @@ class GroupPlan:
 
     @property
     def stride(self) -> int:
-        return int(-(-self.count // 64) * 64) if self.count else 0
+        # rounded up to a whole wave-kernel tile: its loads never need bounds branches
+        return int(-(-self.count // WAVE_TILE) * WAVE_TILE) if self.count else 0
 
     @property
     def components(self) -> int:

@@ -65,6 +65,11 @@ typedef struct tlh_slot {
   const int32_t* seg_start;  /* [nseg] first position of the segment (tile-local) */
   const int32_t* seg_target; /* [nseg] SPARSE: partial id; DENSE: variable index */
   int64_t pos_base;          /* wave kernel: positional partial ids start here */
+  const int32_t* rounds;     /* wave kernel, DENSE secondary slot: [ntiles] or NULL.  When set,
+                                rank holds each position's occurrence index (earlier positions
+                                of its tile with the same variable) and tile t adds its
+                                contributions to the accumulator in rounds[t] rounds, round k
+                                adding the k-th occurrences; NULL: staged segment sums */
 } tlh_slot_t;
 
 /* One residual group: N observations of r rows, 1..3 slots (JacobianGroup,

@@ -23,3 +23,18 @@ def test_library_exports_every_declared_symbol():
     for name in declared_symbols():
         assert hasattr(lib, name), name
         assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+
+
+def test_python_constants_match_the_header():
+    """The host-side plan mirrors the kernels' tile geometry; a mismatch silently
+    mis-assigns primary runs, so every #define the Python side restates must agree."""
+    from torchls_amd import plan
+
+    defines = {k: int(v) for k, v in re.findall(r"^#define\s+(TLH_\w+)\s+(-?\d+)\b", HEADER.read_text(), re.M)}
+    restated = [name for name in dir(_lib) if name.startswith("TLH_") and name in defines]
+    assert {"TLH_WAVE_THREADS", "TLH_WAVE_TILE", "TLH_TILE_THREADS", "TLH_F32", "TLH_F64"} <= set(restated)
+    for name in restated:
+        assert getattr(_lib, name) == defines[name], name
+    assert plan.WAVE_THREADS == defines["TLH_WAVE_THREADS"]
+    assert plan.WAVE_TILE == defines["TLH_WAVE_TILE"]
+    assert plan.THREADS == defines["TLH_TILE_THREADS"]

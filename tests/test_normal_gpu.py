@@ -38,7 +38,9 @@ def test_random_structures(device, shape, dtype, tol):
     assert rel_err(y.cpu().numpy(), ref) <= tol
 
 
-@pytest.mark.parametrize("n_cams,n_pts,k", [(16, 500, 8), (200, 50_000, 10)])
+# (2000, 6000, 4): many cameras per tile, so the camera slot is reduced in
+# occurrence rounds; the others stage camera segments in LDS
+@pytest.mark.parametrize("n_cams,n_pts,k", [(16, 500, 8), (200, 50_000, 10), (2000, 6000, 4)])
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 1e-5)])
 def test_bundle_adjustment(device, n_cams, n_pts, k, dtype, tol):
     print(f"normal matvec BA cams={n_cams} pts={n_pts} dtype={dtype}", flush=True)
@@ -51,7 +53,11 @@ def test_bundle_adjustment(device, n_cams, n_pts, k, dtype, tol):
     x = rng.standard_normal(J.n)
     ref = J.normal_matvec(x, 1e-3)
     BJ = to_block_jacobian(J, device, dtype)
-    op = HipNormalPlan(BJ).bind(BJ)
+    plan = HipNormalPlan(BJ)
+    g = plan.plan.groups[0]
+    assert g.kernel == 1
+    assert (g.slots[0].rounds is not None) == (n_cams == 2000)
+    op = plan.bind(BJ)
     xt = torch.tensor(x, dtype=dtype, device=device)
     y1 = op.normal_matvec(xt, 1e-3)
     y2 = op.normal_matvec(xt, 1e-3)

@@ -50,6 +50,7 @@ class Slot(ctypes.Structure):
         ("seg_start", c_void_p),
         ("seg_target", c_void_p),
         ("pos_base", i64),
+        ("rounds", c_void_p),
     ]
 
 

@@ -207,20 +207,66 @@ __global__ void __launch_bounds__(TLH_THREADS) normal_tile_kernel(GroupArgs<Real
   flush_dense(g, acc);
 }
 
-// Wave kernel for two-slot groups (bundle adjustment, pose-graph edges).
-// Tiles are aligned: tile t covers sorted positions [t*T, t*T + T), T = 256 *
-// ITEMS, and thread i of the workgroup owns the ITEMS consecutive positions
-// ITEMS*i .. ITEMS*i + ITEMS-1, so every block component and id array is read
-// with one 16-byte (8 for ids of float64 tiles) load per thread.
-// Primary slot (the tile is sorted by its variable): each thread first sums
-// runs of one variable over its own items, then an inclusive segmented scan
-// over the 64 lanes (shuffles, fixed tree) carries every run across lanes; a
-// run is cut at the end of each wavefront's 64*ITEMS positions, and its last
-// item writes y directly (variables observed only there) or a positional
-// partial.  Secondary slot: contributions and variable ids are staged in LDS
-// in the tile's order by that variable; each segment head sums its run
-// serially into the dense LDS accumulator or a positional partial.  The last
-// tile of a group may be short; it is read with clamped scalar loads.
+// Cross-lane moves by DPP (data-parallel primitives: register reads of another
+// lane, no LDS traffic).  A lane whose source is outside the pattern (or whose
+// row is masked off) receives 0.
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, DPP_ROW_SHR8 = 0x118;
+constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_int(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_real(float x) {
+  return __int_as_float(dpp_int<CTRL, ROWS>(__float_as_int(x)));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_real(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_int<CTRL, ROWS>((int)(b & 0xffffffffll)), hi = dpp_int<CTRL, ROWS>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// One step of the inclusive segmented scan: (f, v) of the source lane is
+// combined into this lane's unless this lane starts a segment (flag set).
+template <int CTRL, int ROWS, typename Real, int D>
+__device__ __forceinline__ void segmented_step(int& flag, Real (&v)[D]) {
+  const int fo = dpp_int<CTRL, ROWS>(flag);
+  Real vo[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) vo[c] = dpp_real<CTRL, ROWS>(v[c]);
+  if (!flag) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) v[c] = vo[c] + v[c];
+  }
+  flag |= fo;
+}
+
+// Inclusive segmented scan over the 64 lanes of a wavefront, in a fixed order:
+// Hillis-Steele within each row of 16 lanes (shifts 1, 2, 4, 8), then row 0's
+// total into row 1 and row 2's into row 3 (broadcast of lane 15 of the row
+// before), then lanes 0-31's total into rows 2 and 3 (broadcast of lane 31).
+template <typename Real, int D>
+__device__ __forceinline__ void wave_segmented_scan(int& flag, Real (&v)[D]) {
+  segmented_step<DPP_ROW_SHR1, 0xf>(flag, v);
+  segmented_step<DPP_ROW_SHR2, 0xf>(flag, v);
+  segmented_step<DPP_ROW_SHR4, 0xf>(flag, v);
+  segmented_step<DPP_ROW_SHR8, 0xf>(flag, v);
+  segmented_step<DPP_ROW_BCAST15, 0xa>(flag, v);
+  segmented_step<DPP_ROW_BCAST31, 0xc>(flag, v);
+}
+
+// Empty asm with the values as register inputs and a memory clobber: the values
+// are computed before it and no memory access moves across it (it emits nothing).
+template <typename T, int A, int B>
+__device__ __forceinline__ void pin_before_loads(const T (&v)[A][B]) {
+#pragma unroll
+  for (int i = 0; i < A; ++i)
+#pragma unroll
+    for (int j = 0; j < B; ++j) asm volatile("" ::"v"(v[i][j]) : "memory");
+}
+
 template <typename T, int N>
 struct VecT {
   typedef T type __attribute__((ext_vector_type(N)));
@@ -248,6 +294,7 @@ struct WaveArgs {
   int32_t dense_block_s;
   int32_t count_s;
   Real damping;
+  const int32_t* rounds;  // occurrence rounds per tile (dense secondary) or NULL
 };
 
 // out[k] = base[start + b + k]; a short tile (len < T) clamps to its last position.
@@ -325,7 +372,31 @@ __device__ __forceinline__ void load_wave_tile(const WaveArgs<Real>& a, int t, i
   }
 }
 
-template <typename Real, int R, int D0, int D1, int P, int ITEMS, int NT, bool STREAM>
+// Wave kernel for two-slot groups (bundle adjustment, pose-graph edges).
+// Tiles are aligned: tile t covers sorted positions [t*T, t*T + T), T = NT *
+// ITEMS, and thread i of the workgroup owns the ITEMS consecutive positions
+// ITEMS*i .. ITEMS*i + ITEMS-1, so every block component and id array is read
+// with one vector load per thread.  Two register buffers alternate so that the
+// next tile's loads are in flight while the current tile is reduced.
+// Primary slot (the tile is sorted by its variable): each thread first sums
+// runs of one variable over its own items, then an inclusive segmented scan
+// over the 64 lanes (DPP moves, fixed order) carries every run across lanes; a
+// run is cut at the end of each wavefront's 64*ITEMS positions, and its last
+// item writes y directly (variables observed only there) or a positional
+// partial.
+// Secondary slot, dense accumulator with occurrence rounds (when no tile needs
+// more than 8): round r of a tile adds the contributions of the positions
+// holding the r-th occurrence of their variable in the tile; rounds
+// are separated by barriers, so every accumulator entry has one writer per
+// round and a fixed summation order.  Otherwise contributions and variable ids
+// are staged in LDS in the tile's order by that variable and each segment head
+// sums its run serially into the dense LDS accumulator or a positional partial.
+// Positions past the group's count (the last tile may be short) are masked.
+// XL: a dense secondary type whose x slice also fits in LDS (2,000 cameras in
+// float32 do) has that slice copied into LDS once per workgroup, so the
+// per-observation secondary gathers are LDS reads rather than scattered L2
+// requests.
+template <typename Real, int R, int D0, int D1, int P, int ITEMS, int NT, bool STREAM, bool XL>
 __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
   constexpr int DP = P == 0 ? D0 : D1;
   constexpr int DS = P == 0 ? D1 : D0;
@@ -333,20 +404,25 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
   constexpr int OS = P == 0 ? R * D0 : 0;
   constexpr int T = NT * ITEMS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  // LDS: [stage: T contributions + T ids (staged reduction only)] [dense accumulator] [XL: x slice]
   Real* cstage = reinterpret_cast<Real*>(smem_raw);
   int32_t* cid = reinterpret_cast<int32_t*>(cstage + T * DS);
-  Real* acc = reinterpret_cast<Real*>(cid + T);
+  Real* acc = a.rounds ? reinterpret_cast<Real*>(smem_raw) : reinterpret_cast<Real*>(cid + T);
+  Real* xl = acc + (int64_t)a.count_s * DS;
   const bool dense = a.mode_s == TLH_REDUCE_DENSE;
-  if (dense)
+  if (dense) {
     for (int i = threadIdx.x; i < a.count_s * DS; i += NT) acc[i] = Real(0);
+    __syncthreads();  // the first occurrence round may add to any entry
+  }
   const int lane = threadIdx.x & 63;
   const int b = ITEMS * (int)threadIdx.x;  // first tile-local position of this thread
-
   const int t0 = blockIdx.x * a.tiles_per_wg;
   const int t1 = min(t0 + a.tiles_per_wg, a.ntiles);
-  for (int t = t0; t < t1; ++t) {
-    WaveTile<Real, R, DP, DS, ITEMS> cur;
-    load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM>(a, t, b, cur);
+
+  // One tile: gathers and contributions from `cur`, then `prefetch` (which issues
+  // the following tile's loads into the other register buffer), then the
+  // primary scan and the secondary LDS reduction.
+  auto process = [&](const WaveTile<Real, R, DP, DS, ITEMS>& cur, int t, auto&& prefetch) {
     const int64_t start = (int64_t)t * T;
     const int len = (int)min((int64_t)T, a.count - start);
     const int left = len - b;  // items of this thread inside the tile (may be <= 0)
@@ -362,7 +438,7 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
 #pragma unroll
       for (int c = 0; c < DP; ++c) xp[k][c] = a.x_p[(int64_t)ip[k] * DP + c];
 #pragma unroll
-      for (int c = 0; c < DS; ++c) xs[k][c] = a.x_s[(int64_t)is[k] * DS + c];
+      for (int c = 0; c < DS; ++c) xs[k][c] = XL ? xl[is[k] * DS + c] : a.x_s[(int64_t)is[k] * DS + c];
     }
     int direct[ITEMS];
     if (a.direct_p) {
@@ -404,6 +480,12 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
         cs[k][c] = v;
       }
     }
+    // The next tile's loads overlap this tile's reductions.  They must be issued
+    // after every use of this tile's blocks: vmcnt waits are in issue order, so a
+    // block use sunk below them would wait for the whole prefetch.
+    pin_before_loads(cp);
+    pin_before_loads(cs);
+    prefetch();
     // primary slot: runs inside the thread, then a segmented scan over lanes
 #pragma unroll
     for (int k = 1; k < ITEMS; ++k)
@@ -414,32 +496,19 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
     bool same = true;
 #pragma unroll
     for (int k = 1; k < ITEMS; ++k) same = same && key[k] == key[0];
-    const int prev_last = __shfl_up(key[ITEMS - 1], 1);
-    const int next_first = __shfl_down(key[0], 1);
+    const int prev_last = dpp_int<DPP_WAVE_SHR1, 0xf>(key[ITEMS - 1]);
+    const int next_first = dpp_int<DPP_WAVE_SHL1, 0xf>(key[0]);
     const bool joins = lane > 0 && key[0] == prev_last;  // first run continues the previous lane's
     int flag = (joins && same) ? 0 : 1;
     Real v[DP];
 #pragma unroll
     for (int c = 0; c < DP; ++c) v[c] = cp[ITEMS - 1][c];
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int fo = __shfl_up(flag, off);
-      Real vo[DP];
-#pragma unroll
-      for (int c = 0; c < DP; ++c) vo[c] = __shfl_up(v[c], off);
-      if (lane >= off) {
-        if (!flag) {
-#pragma unroll
-          for (int c = 0; c < DP; ++c) v[c] = vo[c] + v[c];
-        }
-        flag |= fo;
-      }
-    }
+    wave_segmented_scan<Real, DP>(flag, v);
     // carry into this lane's first run: the previous lane's scanned trailing run
     Real carry[DP];
 #pragma unroll
     for (int c = 0; c < DP; ++c) {
-      const Real up = __shfl_up(v[c], 1);
+      const Real up = dpp_real<DPP_WAVE_SHR1, 0xf>(v[c]);
       carry[c] = joins ? up : Real(0);
     }
     bool first_run = true;
@@ -461,6 +530,24 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
           for (int c = 0; c < DP; ++c) out[c] = s[c];
         }
       }
+    }
+    if (a.rounds) {
+      // secondary slot, occurrence rounds: round r adds every r-th occurrence of a
+      // variable in this tile, so each accumulator entry has one writer per round
+      // (a plain read-modify-write; LDS float atomics measured twice as slow) and
+      // receives the tile's contributions in position order
+      const int nr = a.rounds[t];
+      for (int r = 0; r < nr; ++r) {
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+          if (k < left && cur.rank(k) == r) {
+#pragma unroll
+            for (int c = 0; c < DS; ++c) acc[is[k] * DS + c] += cs[k][c];
+          }
+        }
+        __syncthreads();
+      }
+      return;
     }
     // secondary slot: LDS stage in variable order, on-the-fly segment heads
     __syncthreads();
@@ -494,6 +581,23 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
       }
     }
     __syncthreads();
+  };
+  // Two register buffers used alternately (no copies between them), so the loads
+  // of tile t+1 are in flight while tile t is reduced through LDS.
+  WaveTile<Real, R, DP, DS, ITEMS> ta, tb;
+  if (t0 < t1) load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM>(a, t0, b, ta);
+  if constexpr (XL) {
+    for (int i = threadIdx.x; i < a.count_s * DS; i += NT) xl[i] = a.x_s[i];
+    __syncthreads();
+  }
+  for (int t = t0; t < t1; t += 2) {
+    process(ta, t, [&] {
+      if (t + 1 < t1) load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM>(a, t + 1, b, tb);
+    });
+    if (t + 1 < t1)
+      process(tb, t + 1, [&] {
+        if (t + 2 < t1) load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM>(a, t + 2, b, ta);
+      });
   }
   if (dense) {
     const int64_t nn = (int64_t)a.count_s * DS;
@@ -769,13 +873,36 @@ int launch_generic(const tlh_operator_t* op, const tlh_group_t& g, const void* x
   return 0;
 }
 
+// LDS bytes of a wave group: stage (contribution + id per position), the dense
+// secondary accumulator and, when all of that fits, the secondary x slice.
 template <typename Real>
-int64_t wave_lds(const tlh_operator_t* op, const tlh_group_t& g) {
-  const int p = g.primary;
-  const int ds = g.slot[1 - p].dim;
-  int64_t b = (int64_t)g.tile_cap * (ds * (int64_t)sizeof(Real) + 4);
-  if (g.slot[1 - p].mode == TLH_REDUCE_DENSE) b += op->types[g.slot[1 - p].type].count * ds * (int64_t)sizeof(Real);
+int64_t wave_lds(const tlh_group_t& g, const tlh_type_t& ts, int64_t lds_max, bool& xl) {
+  const int ds = g.slot[1 - g.primary].dim;
+  const tlh_slot_t& ss = g.slot[1 - g.primary];
+  const bool rounds = ss.mode == TLH_REDUCE_DENSE && ss.rounds;
+  int64_t b = rounds ? 0 : (int64_t)g.tile_cap * (ds * (int64_t)sizeof(Real) + 4);
+  xl = false;
+  if (g.slot[1 - g.primary].mode == TLH_REDUCE_DENSE) {
+    const int64_t slice = ts.count * ds * (int64_t)sizeof(Real);
+    b += slice;
+    xl = b + slice <= lds_max;
+    if (xl) b += slice;
+  }
   return b;
+}
+
+// Largest dynamic LDS allocation of one workgroup on the current device.
+inline int64_t device_lds_max() {
+  static int64_t cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
+  if (!cached[dev]) {
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lds <= 0)
+      return 65536;
+    cached[dev] = lds;
+  }
+  return cached[dev];
 }
 
 template <typename Real, int R, int D0, int D1, int P>
@@ -791,7 +918,7 @@ int launch_wave_t(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
   // aligned tiles of exactly NT * ITEMS sorted positions; whole-tile loads need the
   // packed stride (and the caller's id/rank arrays) to cover ntiles * tile_cap positions
   if (g.tile_cap != NT * ITEMS || g.ntiles != ceil_div(g.count, NT * ITEMS) || g.stride % ITEMS != 0 ||
-      g.stride < (int64_t)g.ntiles * g.tile_cap)
+      g.stride < (int64_t)g.ntiles * g.tile_cap || g.primary != P)
     return TLH_EINVAL;
   a.packed = static_cast<const Real*>(g.packed);
   a.stride = g.stride;
@@ -813,12 +940,15 @@ int launch_wave_t(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
   a.dense_block_s = ss.dense_block;
   a.count_s = (int32_t)ts.count;
   a.damping = (Real)damping;
-  const int64_t lds = (int64_t)NT * ITEMS * ((1 - P == 0 ? D0 : D1) * (int64_t)sizeof(Real) + 4) +
-                      (ss.mode == TLH_REDUCE_DENSE ? ts.count * (1 - P == 0 ? D0 : D1) * (int64_t)sizeof(Real) : 0);
+  a.rounds = ss.mode == TLH_REDUCE_DENSE ? ss.rounds : nullptr;
+  bool xl = false;
+  const int64_t lds = wave_lds<Real>(g, ts, device_lds_max(), xl);
   // blocks that cannot stay in the 256 MB last-level cache between matvecs are streamed
   const bool stream_blocks = (int64_t)g.stride * g.rows * (D0 + D1) * (int64_t)sizeof(Real) > (192ll << 20);
-  auto kernel = stream_blocks ? normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, true>
-                              : normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, false>;
+  auto kernel = stream_blocks ? (xl ? normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, true, true>
+                                    : normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, true, false>)
+                              : (xl ? normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, false, true>
+                                    : normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, false, false>);
   TLH_RETURN_IF_ERROR(allow_lds((const void*)kernel, lds));
   hipLaunchKernelGGL(kernel, dim3(g.nwg), dim3(NT), (size_t)lds, stream, a);
   TLH_RETURN_IF_LAUNCH_FAILED();
@@ -967,7 +1097,12 @@ extern "C" int tlh_device_info(int device, tlh_device_info_t* out) {
 extern "C" int64_t tlh_group_lds_bytes(const tlh_operator_t* op, int group) {
   if (check_op(op) || group < 0 || group >= op->ngroups) return -1;
   const tlh_group_t& g = op->groups[group];
-  if (g.kernel == 1) return op->dtype == TLH_F32 ? wave_lds<float>(op, g) : wave_lds<double>(op, g);
+  if (g.kernel == 1) {
+    bool xl = false;
+    const tlh_type_t& ts = op->types[g.slot[1 - g.primary].type];
+    return op->dtype == TLH_F32 ? wave_lds<float>(g, ts, device_lds_max(), xl)
+                                : wave_lds<double>(g, ts, device_lds_max(), xl);
+  }
   int dmax = 0;
   for (int s = 0; s < g.nslots; ++s) dmax = std::max(dmax, (int)g.slot[s].dim);
   if (op->dtype == TLH_F32) {

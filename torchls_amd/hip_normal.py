@@ -105,6 +105,7 @@ class DeviceStructure:
                 ds.seg_start = put(s.seg_start).data_ptr()
                 ds.seg_target = put(s.seg_target).data_ptr()
                 ds.pos_base = s.pos_base
+                ds.rounds = None if s.rounds is None else put(s.rounds.astype(np.int32)).data_ptr()
         self.types = (_lib.Type * len(plan.types))()
         for ti, t in enumerate(plan.types):
             d = self.types[ti]

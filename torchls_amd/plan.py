@@ -60,6 +60,10 @@ class SlotPlan:
     seg_target: np.ndarray | None = None  # int32 [nseg] partial id / variable (-1: direct write)
     dense_block: int = 0
     pos_base: int = 0  # wave groups: first positional partial id of this slot
+    # wave groups, DENSE secondary slot reduced in occurrence rounds: rank then holds
+    # each position's occurrence index (earlier positions of the tile with the same
+    # variable) and rounds[t] = 1 + the largest occurrence index of tile t
+    rounds: np.ndarray | None = None  # int32 [ntiles]
 
 
 @dataclass
@@ -175,6 +179,32 @@ def _aligned_tiles(n: int, cap: int) -> np.ndarray:
     return np.append(np.arange(0, n, cap, dtype=np.int64), n) if n else np.zeros(1, dtype=np.int64)
 
 
+# Most rounds a tile may take in the occurrence-round reduction; groups with a
+# more repetitive secondary slot (few variables) use the staged reduction.
+MAX_ROUNDS = 8
+
+
+def _occurrences(index_sorted: np.ndarray, tile_start: np.ndarray):
+    """Occurrence index of each position's variable inside its tile (0 for the
+    first position of the tile with that variable, in position order) and the
+    number of rounds (1 + largest occurrence index) of each tile."""
+    n = index_sorted.size
+    ntiles = tile_start.size - 1
+    if n == 0:
+        return np.zeros(0, np.int64), np.zeros(ntiles, np.int32)
+    tile = np.repeat(np.arange(ntiles, dtype=np.int64), np.diff(tile_start))
+    key = tile * (int(index_sorted.max()) + 1) + index_sorted.astype(np.int64)
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    first = np.r_[True, ks[1:] != ks[:-1]]
+    start = np.maximum.accumulate(np.where(first, np.arange(n), 0))
+    occ = np.empty(n, np.int64)
+    occ[order] = np.arange(n) - start
+    rounds = np.zeros(ntiles, np.int64)
+    np.maximum.at(rounds, tile, occ + 1)
+    return occ, rounds.astype(np.int32)
+
+
 def _wave_pieces(index_sorted: np.ndarray, tile_start: np.ndarray, span: int):
     """Primary-slot pieces of the wave kernel: runs of one variable, also cut at
     every ``span``-position boundary of each tile (one wavefront covers 64
@@ -268,9 +298,15 @@ def build_structure(
             )
             if wave and s == primary:
                 slots[-1].piece_end = ends
+            elif wave and dense:
+                occ, rounds = _occurrences(sorted_idx[s], tile_start)
+                if rounds.size and int(rounds.max()) <= MAX_ROUNDS:
+                    slots[-1].rank = occ.astype(np.uint16)
+                    slots[-1].rounds = rounds
         has_dense = any(sl.mode == REDUCE_DENSE for sl in slots)
         if has_dense and ntiles:
-            nwg = min(ntiles, cu_count * wgs_per_cu)
+            # a wave-kernel workgroup (1024 threads) fills its compute unit's wave slots
+            nwg = min(ntiles, cu_count * (1 if wave else wgs_per_cu))
             per = -(-ntiles // nwg)
             nwg = -(-ntiles // per)
         else:

@@ -194,7 +194,8 @@ int tlh_block_inverse(const double* blocks, int64_t count, int dim, double dampi
 
 /* Device-resident PCG state (warp_pcg.py:131-132): control = {iterations,
  * status, stop_at, spare}; status 0 active, 1 recurrence tolerance, 2
- * nonfinite, 3 nonpositive curvature (warp_pcg.py:78-79).  partials holds
+ * nonfinite, 3 nonpositive curvature (warp_pcg.py:78-79), 4 the damped
+ * preconditioner blocks are not positive definite (tlh_pcg_begin).  partials holds
  * 2 * TLH_PCG_MAX_BLOCKS doubles; counter must start at zero (kernels reset it). */
 typedef struct tlh_pcg_state {
   int32_t* control;
@@ -207,6 +208,15 @@ typedef struct tlh_pcg_state {
  * Replaces: torch.dot in warp_pcg.py:139,147-148 and solvers.py:198-246. */
 int tlh_dot(const void* a, const void* b, int64_t n, int dtype, const tlh_pcg_state_t* st,
             int slot, tlh_stream_t stream);
+
+/* Starts a solve on the device, with no host read: from scalars[AUX] = rhs.rhs,
+ * threshold = max(atol, rtol * ||rhs||) (scalars[THR2] = threshold^2) and
+ * control = {0, status, stop_at, 0}, status 4 if *pd_status != 0 (pd_status may be
+ * NULL), else 2 if ||rhs|| is not finite, else 1 if ||rhs|| <= threshold (the early
+ * exits of warp_pcg.py:119-128, which the host reports after its first read), else 0.
+ * Replaces: the threshold computation and first host read of warp_pcg.py:117-129. */
+int tlh_pcg_begin(const tlh_pcg_state_t* st, double atol, double rtol, int32_t stop_at,
+                  const int32_t* pd_status, tlh_stream_t stream);
 
 /* r = src, x = 0 (when x != NULL), z = M^-1 r, p = z, scalars[RZ] = r.z.
  * pc == NULL means no preconditioner.  Replaces warp_pcg.py:130-132, 169-172, 210-212. */

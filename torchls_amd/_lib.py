@@ -150,6 +150,10 @@ SIGNATURES = {
         ctypes.c_int,
         [c_void_p, c_void_p, i64, ctypes.c_int, ctypes.POINTER(PcgStateDesc), ctypes.c_int, c_void_p],
     ),
+    "tlh_pcg_begin": (
+        ctypes.c_int,
+        [ctypes.POINTER(PcgStateDesc), ctypes.c_double, ctypes.c_double, i32, c_void_p, c_void_p],
+    ),
     "tlh_pcg_reset": (
         ctypes.c_int,
         [ctypes.POINTER(BlockDiag), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, i64, ctypes.c_int,

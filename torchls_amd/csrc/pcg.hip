@@ -567,6 +567,40 @@ extern "C" int tlh_dot(const void* a, const void* b, int64_t n, int dtype, const
   return 0;
 }
 
+namespace {
+
+__global__ void begin_kernel(double* scalars, int32_t* control, double atol, double rtol, int32_t stop_at,
+                             const int32_t* pd_status) {
+  if (threadIdx.x != 0) return;
+  const double rhs2 = scalars[TLH_PCG_AUX];
+  const double norm = rhs2 >= 0.0 ? sqrt(rhs2) : (double)NAN;
+  const double scaled = rtol * norm;
+  const double thr = scaled > atol ? scaled : atol;  // max(atol, rtol * norm); a NaN norm gives atol
+  scalars[TLH_PCG_THR2] = thr * thr;
+  int32_t status = 0;
+  if (pd_status && *pd_status != 0)
+    status = 4;
+  else if (!isfinite(norm))
+    status = 2;
+  else if (norm <= thr)
+    status = 1;
+  control[0] = 0;
+  control[1] = status;
+  control[2] = stop_at;
+  control[3] = 0;
+}
+
+}  // namespace
+
+extern "C" int tlh_pcg_begin(const tlh_pcg_state_t* st, double atol, double rtol, int32_t stop_at,
+                             const int32_t* pd_status, tlh_stream_t stream) {
+  if (!state_ok(st) || stop_at < 0) return TLH_EINVAL;
+  hipLaunchKernelGGL(begin_kernel, dim3(1), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), st->scalars,
+                     st->control, atol, rtol, stop_at, pd_status);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
 extern "C" int tlh_pcg_reset(const tlh_block_diag_t* pc, const void* src, void* x, void* r, void* z, void* p,
                              int64_t n, int dtype, const tlh_pcg_state_t* st, tlh_stream_t stream) {
   if (!state_ok(st) || !precond_ok(pc, n) || !src || !r || !z || !p) return TLH_EINVAL;

@@ -60,21 +60,32 @@ class PcgState:
         return cls._scratch[key]
 
 
-def hip_pcg(matvec, rhs, *, preconditioner, config, use_graph: bool = True):
-    """Solve A x = rhs by PCG on the GPU; returns a PCGResult like warp_pcg."""
+# Replays per host read after the first batch: iterations past convergence are
+# frozen by the control word but still run the operator, so later batches are short.
+SMALL_BATCH = 4
+
+
+def hip_pcg(matvec, rhs, *, preconditioner, config, use_graph: bool = True, first_batch: int | None = None):
+    """Solve A x = rhs by PCG on the GPU; returns a PCGResult like warp_pcg.
+
+    ``first_batch`` (optional) is the number of iterations replayed before the
+    first host read, e.g. the previous solve's count; it changes how often the
+    host reads the control word, never the iterates.
+    """
     if rhs.device.type != "cuda" or rhs.dtype not in (torch.float32, torch.float64):
         raise ValueError("HIP PCG requires GPU float32 or float64 tensors")
     caller = torch.cuda.current_stream(rhs.device)
     stream = _work_stream(rhs.device)
     stream.wait_stream(caller)
     with torch.cuda.stream(stream):
-        result = _run(matvec, rhs, preconditioner=preconditioner, config=config, use_graph=use_graph)
+        result = _run(matvec, rhs, preconditioner=preconditioner, config=config, use_graph=use_graph,
+                      first_batch=first_batch)
     caller.wait_stream(stream)
     result.solution.record_stream(caller)
     return result
 
 
-def _run(matvec, rhs, *, preconditioner, config, use_graph):
+def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None):
     from .solvers import PCGResult
 
     lib = _lib.load()
@@ -89,33 +100,24 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph):
     fused = isinstance(preconditioner, HipBlockJacobi)
     apply = preconditioner
     pc = ctypes.byref(preconditioner.desc) if fused else None
+    interval = config.residual_check_interval
+
+    def window_end(it):
+        """Iteration at which the batch stops for a true-residual check (warp_pcg.py:177-185)."""
+        return min(config.max_iterations, ((it // interval + 1) * interval) if interval else config.max_iterations)
 
     def dot_into(a, b, slot):
         _lib.check(lib.tlh_dot(a.data_ptr(), b.data_ptr(), n, code, sd, slot, sh), "tlh_dot")
 
-    def read_scalars(*slots):
-        st.host[: len(slots)].copy_(st.scalars[list(slots)], non_blocking=False)
-        return [float(v) for v in st.host[: len(slots)].tolist()]
+    # a deferred positive-definiteness flag of the preconditioner is read with the first batch
+    pd_flag = getattr(preconditioner, "status", None) if getattr(preconditioner, "deferred", False) else None
+
+    def begin():
+        _lib.check(lib.tlh_pcg_begin(sd, float(config.atol), float(config.rtol), window_end(0),
+                                     None if pd_flag is None else pd_flag.data_ptr(), sh), "tlh_pcg_begin")
 
     x = torch.zeros_like(rhs)
-    r = rhs.clone()
-    dot_into(rhs, rhs, _lib.PCG_AUX)
-    status = getattr(preconditioner, "status", None) if getattr(preconditioner, "deferred", False) else None
-    if status is None:
-        (rhs_norm2,) = read_scalars(_lib.PCG_AUX)
-    else:  # the preconditioner's deferred positive-definiteness flag rides on the same read
-        host = torch.stack((st.scalars[_lib.PCG_AUX], status[0].to(torch.float64))).cpu()
-        rhs_norm2, bad = float(host[0]), float(host[1])
-        if bad != 0:
-            from .hip_diagonal import LinAlgError
-
-            raise LinAlgError("damped block is not positive definite")
-    rhs_norm = math.sqrt(rhs_norm2) if rhs_norm2 >= 0 else math.nan
-    threshold = max(config.atol, config.rtol * rhs_norm)
-    if not math.isfinite(rhs_norm) or rhs_norm <= threshold:
-        finite = math.isfinite(rhs_norm)
-        return PCGResult(x, 0, rhs_norm, finite, "residual_tolerance" if finite else "nonfinite", rhs_norm=rhs_norm)
-
+    r = torch.empty_like(rhs)
     z = torch.empty_like(rhs)
     p = torch.empty_like(rhs)
     true_r = torch.empty_like(rhs)
@@ -131,9 +133,11 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph):
             p.copy_(apply(r))
             dot_into(r, p, _lib.PCG_RZ)
 
+    # threshold, early exits and the preconditioner check are decided on the device:
+    # nothing is read back before the first batch of iterations
+    dot_into(rhs, rhs, _lib.PCG_AUX)
+    begin()
     reset(rhs, True)
-    st.scalars[_lib.PCG_THR2] = threshold * threshold
-    st.control.copy_(torch.tensor([0, 0, config.max_iterations, 0], dtype=torch.int32))
 
     small = (fused or apply is None) and n <= _lib.PCG_SMALL_MAX_N
 
@@ -179,33 +183,50 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph):
         finally:
             graph.capture_end()
     if warm:
+        begin()
         reset(rhs, True)
-        st.control.zero_()
+
+    def check_and_read():
+        """True residual of the current iterate, then one host read of (rhs.rhs,
+        ||rhs - Ax||^2, drift^2, iterations, status)."""
+        ax = matvec(x)
+        _lib.check(lib.tlh_pcg_true_residual(rhs.data_ptr(), ax.data_ptr(), r.data_ptr(), true_r.data_ptr(), n,
+                                             code, sd, sh), "tlh_pcg_true_residual")
+        host = torch.cat((st.scalars[_lib.PCG_TRUE2: _lib.PCG_AUX + 1], st.control[:2].to(torch.float64))).cpu()
+        true2, drift2, rhs2, it, status = host.tolist()
+        return rhs2, true2, drift2, int(it), int(status)
 
     host_control = torch.zeros(4, dtype=torch.int32).pin_memory()
     iterations = checks = restarts = 0
+    stop_at = window_end(0)
+    rhs_norm = threshold = None
     reason = "max_iterations"
     true_norm = math.nan
-    while iterations < config.max_iterations:
-        interval = config.residual_check_interval
-        stop_at = min(
-            config.max_iterations,
-            ((iterations // interval + 1) * interval) if interval else config.max_iterations,
-        )
-        host_control.copy_(torch.tensor([iterations, 0, stop_at, 0], dtype=torch.int32))
-        st.control.copy_(host_control, non_blocking=True)
-        for _ in range(min(config.graph_batch_iterations, stop_at - iterations)):
+    batch = max(1, int(first_batch)) if first_batch else config.graph_batch_iterations
+    while True:
+        for _ in range(max(0, min(batch, stop_at - iterations))):
             if graph is not None:
                 graph.replay()
             else:
                 iteration()
-        iterations, status, _, _ = st.control.tolist()
+        batch = min(SMALL_BATCH, config.graph_batch_iterations)
+        rhs2, true2, drift2, it, status = check_and_read()
+        if rhs_norm is None:
+            rhs_norm = math.sqrt(rhs2) if rhs2 >= 0 else math.nan
+            threshold = max(config.atol, config.rtol * rhs_norm)
+            if status == 4:
+                from .hip_diagonal import LinAlgError
+
+                raise LinAlgError("damped block is not positive definite")
+            if not math.isfinite(rhs_norm) or rhs_norm <= threshold:
+                finite = math.isfinite(rhs_norm)
+                return PCGResult(x, 0, rhs_norm, finite, "residual_tolerance" if finite else "nonfinite",
+                                 rhs_norm=rhs_norm)
+            if config.max_iterations <= 0:
+                return PCGResult(x, 0, math.nan, False, "max_iterations", 0, 0, rhs_norm)
+        iterations = it
         if status == 0 and iterations < stop_at:
             continue
-        ax = matvec(x)
-        _lib.check(lib.tlh_pcg_true_residual(rhs.data_ptr(), ax.data_ptr(), r.data_ptr(), true_r.data_ptr(), n,
-                                             code, sd, sh), "tlh_pcg_true_residual")
-        true2, drift2 = read_scalars(_lib.PCG_TRUE2, _lib.PCG_DRIFT2)
         true_norm = math.sqrt(true2) if true2 >= 0 else math.nan
         checks += 1
         if not math.isfinite(true_norm):
@@ -226,4 +247,7 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph):
         ):
             reset(true_r, False)
             restarts += 1
+        stop_at = window_end(iterations)
+        host_control.copy_(torch.tensor([iterations, 0, stop_at, 0], dtype=torch.int32))
+        st.control.copy_(host_control, non_blocking=True)
     return PCGResult(x, iterations, true_norm, False, reason, restarts, checks, rhs_norm)

@@ -161,13 +161,18 @@ class SolveSummary:
     linear_relative_residual_history: list = field(default_factory=list)
 
 
-def pcg(matvec, rhs: Tensor, *, preconditioner=None, config: ConjugateGradientConfig = _DEFAULT_CG) -> PCGResult:
-    """Preconditioned CG with true-residual checks (solvers.py:167-256), on the HIP backend."""
+def pcg(matvec, rhs: Tensor, *, preconditioner=None, config: ConjugateGradientConfig = _DEFAULT_CG,
+        first_batch: int | None = None) -> PCGResult:
+    """Preconditioned CG with true-residual checks (solvers.py:167-256), on the HIP backend.
+
+    ``first_batch``: iterations run before the first host read (default
+    ``config.graph_batch_iterations``); the LM loop passes the previous solve's count.
+    """
     if rhs.ndim != 1 or rhs.numel() == 0:
         raise ValueError("PCG expects a nonempty vector")
     from .hip_pcg import hip_pcg
 
-    return hip_pcg(matvec, rhs, preconditioner=preconditioner, config=config)
+    return hip_pcg(matvec, rhs, preconditioner=preconditioner, config=config, first_batch=first_batch)
 
 
 def block_jacobi(jacobian, damping: float, *, diagonal=None, groups=(), graph_safe=False):
@@ -291,6 +296,7 @@ def solve(
     with region("linearize"):
         lin = problem.linearize(vals)
     head = None
+    pcg_hint = None  # iterations of the previous PCG solve: the next one's first batch
     for iteration in range(termination.max_iterations):
         if timed_out():
             summary.termination_reason = "time_limit"
@@ -357,6 +363,7 @@ def solve(
                 summary.termination_reason = "time_limit"
                 break
             summary.lambda_history.append(damping)
+            finite_step = None  # device flag of the iterative solves, read with the trial
             try:
                 if linear_solver == "dense_qr":
                     dense = jacobian.to_dense()
@@ -381,10 +388,12 @@ def solve(
                         else:
                             result = pcg(system.matvec, rhs,
                                          preconditioner=system.preconditioner() if conjugate_gradient.preconditioner
-                                         else None, config=conjugate_gradient)
+                                         else None, config=conjugate_gradient, first_batch=pcg_hint)
+                            pcg_hint = max(1, result.iterations)
                             retained, iterations, converged = result.solution, result.iterations, result.converged
-                            usable = result.reason in ("residual_tolerance", "max_iterations") and bool(
-                                torch.isfinite(retained).all())
+                            # the step's finiteness is read with the trial (one host read per trial)
+                            usable = result.reason in ("residual_tolerance", "max_iterations")
+                            finite_step = torch.isfinite(retained).all()
                             summary.linear_restarts += result.restarts
                             summary.linear_residual_checks += result.residual_checks
                             summary.linear_relative_residual_history.append(
@@ -401,7 +410,8 @@ def solve(
                     with region("linear_solve"):
                         result = pcg(partial(normal_operator.normal_matvec, damping=damping),
                                      (-linear_gradient).to(mv_dtype), preconditioner=precondition,
-                                     config=conjugate_gradient)
+                                     config=conjugate_gradient, first_batch=pcg_hint)
+                        pcg_hint = max(1, result.iterations)
                     summary.linear_iterations += result.iterations
                     summary.linear_iterations_history.append(result.iterations)
                     summary.linear_converged_history.append(result.converged)
@@ -410,8 +420,8 @@ def solve(
                     summary.linear_relative_residual_history.append(
                         result.residual_norm / result.rhs_norm if result.rhs_norm else result.residual_norm)
                     delta = result.solution
-                    usable = result.reason in ("residual_tolerance", "max_iterations") and bool(
-                        torch.isfinite(delta).all())
+                    usable = result.reason in ("residual_tolerance", "max_iterations")
+                    finite_step = torch.isfinite(delta).all()
             except (LinAlgError, torch.linalg.LinAlgError):
                 usable = False
             if usable:
@@ -423,8 +433,14 @@ def solve(
                     objective, reduction = problem.evaluate_trial(proposed, lin)
                 sq = kernels.squared_matvec_norm(original_jacobian, delta)
                 pred_t = -torch.dot(gradient, delta) - 0.5 * sq
-                host = torch.stack((objective, reduction, pred_t, torch.linalg.vector_norm(delta))).cpu()
-                proposed_cost, actual, predicted, step_norm = (float(v) for v in host)
+                parts = [objective, reduction, pred_t, torch.linalg.vector_norm(delta)]
+                if finite_step is not None:
+                    parts.append(finite_step.to(torch.float64))
+                host = torch.stack(parts).cpu().tolist()
+                proposed_cost, actual, predicted, step_norm = host[:4]
+                # a nonfinite step is unusable: rejected without a trial record
+                usable = finite_step is None or host[4] != 0
+            if usable:
                 quality = actual / predicted if predicted > 0 else -math.inf
                 accepted = math.isfinite(proposed_cost) and (
                     trust_region is None or (actual > 0 and quality >= trust_region.step_quality_min))

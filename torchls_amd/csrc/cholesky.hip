@@ -3,9 +3,10 @@
 // Replaces torch.linalg.cholesky / cholesky_solve on the dense Schur matrix
 // (solvers.py:516-518).  Blocked right-looking algorithm on the lower
 // triangle, panel width 64:
-//   1. panel kernel: every workgroup factors the 64x64 diagonal block in LDS
-//      (redundantly, no inter-workgroup sync); workgroup 0 writes L_kk, the
-//      others solve one 64-row block of the panel below: A_ik <- A_ik L_kk^-T;
+//   1. panel kernel (one wavefront per workgroup, a row per lane): every
+//      workgroup factors the 64x64 diagonal block (redundantly, no
+//      inter-workgroup sync); workgroup 0 writes L_kk, the others solve one
+//      64-row block of the panel below: A_ik <- A_ik L_kk^-T;
 //   2. trailing kernel: one workgroup per lower 64x64 block (i >= j) of the
 //      trailing matrix computes A_ij -= L_ik L_jk^T with
 //      v_mfma_f64_16x16x4_f64 (4 waves x 4 16x16 tiles, K = 64).  Operand
@@ -21,73 +22,119 @@ namespace {
 constexpr int NB = 64;
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-__device__ void factor_diag(double (*D)[NB + 1], int kb, bool& ok) {
-  __shared__ double pivot;
-  for (int j = 0; j < kb; ++j) {
-    if (threadIdx.x == 0) {
-      const double d = D[j][j];
-      if (!(d > 0.0) || !isfinite(d)) ok = false;
-      pivot = sqrt(d > 0.0 ? d : 1.0);
-      D[j][j] = pivot;
+// Panel kernel: one wavefront per workgroup, lane r owning row r of a 64-wide
+// block kept in LDS; columns are processed in sub-panels of 16 held in
+// registers (v[16]), so every register index is a constant.  Inside a
+// sub-panel the pivot and the column entries a lane needs from other rows are
+// read with lane reads (no LDS round trip, no barrier); the update of the
+// later sub-panels reads the finished sub-panel rows from LDS.
+constexpr int PT = 64;
+constexpr int SUB = 16;
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// sum_l v[l] w[l] over a sub-panel, as four independent partial sums (the
+// chains overlap instead of waiting on each other's FMA latency).
+__device__ __forceinline__ double dot_sub(const double (&v)[SUB], const double* w) {
+  double p[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int l = 0; l < SUB; ++l) p[l & 3] += v[l] * w[l];
+  return (p[0] + p[1]) + (p[2] + p[3]);
+}
+
+// Cholesky of the 64x64 block in W (row r = lane; lower triangle; padded rows
+// hold the identity).  Right-looking by sub-panels of 16 columns.
+__device__ __forceinline__ void factor_block(double (*W)[NB + 1], int lane, bool& ok) {
+#pragma unroll
+  for (int sp = 0; sp < NB / SUB; ++sp) {
+    const int c0 = sp * SUB;
+    double v[SUB];
+#pragma unroll
+    for (int c = 0; c < SUB; ++c) v[c] = W[lane][c0 + c];
+#pragma unroll
+    for (int j = 0; j < SUB; ++j) {
+      const int pr = c0 + j;
+      const double dj = readlane_f64(v[j], pr);
+      const bool good = dj > 0.0 && isfinite(dj);
+      if (!good) ok = false;
+      const double p = sqrt(good ? dj : 1.0);
+      const double inv = 1.0 / p;
+      if (lane == pr) v[j] = p;
+      else if (lane > pr) v[j] *= inv;
+#pragma unroll
+      for (int c = j + 1; c < SUB; ++c) {
+        const double lc = readlane_f64(v[j], c0 + c);  // L[c0 + c][pr]
+        if (lane >= c0 + c) v[c] -= v[j] * lc;
+      }
     }
+#pragma unroll
+    for (int c = 0; c < SUB; ++c) W[lane][c0 + c] = v[c];
     __syncthreads();
-    const double p = pivot;
-    for (int i = j + 1 + threadIdx.x; i < kb; i += TLH_THREADS) D[i][j] /= p;
-    __syncthreads();
-    const int m = kb - j - 1;
-    for (int e = threadIdx.x; e < m * m; e += TLH_THREADS) {
-      const int ii = j + 1 + e / m, ll = j + 1 + e % m;
-      if (ll <= ii) D[ii][ll] -= D[ii][j] * D[ll][j];
+    for (int c = c0 + SUB; c < NB; ++c) {
+      if (lane >= c) W[lane][c] -= dot_sub(v, &W[c][c0]);
     }
     __syncthreads();
   }
 }
 
-__global__ void __launch_bounds__(TLH_THREADS)
+// Panel step k: every workgroup factors the diagonal block (redundantly, no
+// inter-workgroup sync); workgroup 0 writes L_kk, workgroup w >= 1 solves
+// X L_kk^T = A_ik for the w-th 64-row block below, lane r forming row r of X
+// independently (sub-panel by sub-panel, L_kk read from LDS by every lane at
+// the same address).
+__global__ void __launch_bounds__(PT)
     chol_panel_kernel(double* __restrict__ A, int64_t n, int64_t ld, int64_t k, int32_t* status) {
-  __shared__ double D[NB][NB + 1];
-  __shared__ double P[NB][NB + 1];
-  __shared__ int okflag;
+  __shared__ double W[NB][NB + 1];
+  __shared__ double X[NB][NB + 1];
+  __shared__ double inv_diag[NB];
+  const int lane = threadIdx.x;
   const int kb = (int)min((int64_t)NB, n - k);
-  for (int e = threadIdx.x; e < NB * NB; e += TLH_THREADS) {
-    const int i = e / NB, j = e % NB;
-    D[i][j] = (i < kb && j <= i) ? A[(k + i) * ld + k + j] : 0.0;
-  }
-  if (threadIdx.x == 0) okflag = 1;
+#pragma unroll 16
+  for (int r = 0; r < NB; ++r)  // coalesced: lane = column; unrolled so the row loads overlap
+    W[r][lane] = (r < kb && lane <= r) ? A[(k + r) * ld + k + lane] : (r == lane ? 1.0 : 0.0);
   __syncthreads();
   bool ok = true;
-  factor_diag(D, kb, ok);
-  if (threadIdx.x == 0 && !ok) okflag = 0;
-  __syncthreads();
+  factor_block(W, lane, ok);
+  const bool all_ok = __builtin_amdgcn_readfirstlane((int)__any(!ok)) == 0;
   if (blockIdx.x == 0) {
-    for (int e = threadIdx.x; e < kb * kb; e += TLH_THREADS) {
-      const int i = e / kb, j = e % kb;
-      if (j <= i) A[(k + i) * ld + k + j] = D[i][j];
-    }
-    if (threadIdx.x == 0 && !okflag) __hip_atomic_fetch_or(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll 16
+    for (int r = 0; r < kb; ++r)
+      if (lane <= r) A[(k + r) * ld + k + lane] = W[r][lane];
+    if (lane == 0 && !all_ok) __hip_atomic_fetch_or(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
   const int64_t r0 = k + kb + (int64_t)(blockIdx.x - 1) * NB;
   const int rows = (int)min((int64_t)NB, n - r0);
   if (rows <= 0) return;
-  for (int e = threadIdx.x; e < rows * kb; e += TLH_THREADS) {
-    const int i = e / kb, j = e % kb;
-    P[i][j] = A[(r0 + i) * ld + k + j];
-  }
+  inv_diag[lane] = 1.0 / W[lane][lane];
+#pragma unroll 16
+  for (int r = 0; r < NB; ++r) X[r][lane] = (r < rows && lane < kb) ? A[(r0 + r) * ld + k + lane] : 0.0;
   __syncthreads();
-  if ((int)threadIdx.x < rows) {
-    const int t = threadIdx.x;
-    for (int j = 0; j < kb; ++j) {
-      double s = P[t][j];
-      for (int l = 0; l < j; ++l) s -= P[t][l] * D[j][l];
-      P[t][j] = s / D[j][j];
+#pragma unroll
+  for (int sp = 0; sp < NB / SUB; ++sp) {
+    const int c0 = sp * SUB;
+    double v[SUB];
+#pragma unroll
+    for (int c = 0; c < SUB; ++c) v[c] = X[lane][c0 + c];
+#pragma unroll
+    for (int j = 0; j < SUB; ++j) {
+      v[j] *= inv_diag[c0 + j];
+#pragma unroll
+      for (int c = j + 1; c < SUB; ++c) v[c] -= v[j] * W[c0 + c][c0 + j];
     }
+#pragma unroll
+    for (int c = 0; c < SUB; ++c) X[lane][c0 + c] = v[c];
+    for (int c = c0 + SUB; c < NB; ++c) X[lane][c] -= dot_sub(v, &W[c][c0]);
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < rows * kb; e += TLH_THREADS) {
-    const int i = e / kb, j = e % kb;
-    A[(r0 + i) * ld + k + j] = P[i][j];
-  }
+#pragma unroll 16
+  for (int r = 0; r < rows; ++r)
+    if (lane < kb) A[(r0 + r) * ld + k + lane] = X[r][lane];
 }
 
 // Trailing update of the lower blocks (bi >= bj) right of panel k.
@@ -104,6 +151,7 @@ __global__ void __launch_bounds__(TLH_THREADS)
   const int64_t base = k + NB;
   const int64_t ri = base + bi * NB, rj = base + bj * NB;
   const int kb = (int)min((int64_t)NB, n - k);
+#pragma unroll 16
   for (int e = threadIdx.x; e < NB * NB; e += TLH_THREADS) {
     const int i = e / NB, j = e % NB;
     Pi[i][j] = (ri + i < n && j < kb) ? A[(ri + i) * ld + k + j] : 0.0;
@@ -134,32 +182,103 @@ __global__ void __launch_bounds__(TLH_THREADS)
   }
 }
 
-// Forward (L y = b) then backward (L^T x = y) substitution, one workgroup.
-__global__ void __launch_bounds__(1024) chol_solve_kernel(const double* __restrict__ L, int64_t n, int64_t ld,
-                                                          double* __restrict__ b) {
-  __shared__ double xs;
-  for (int64_t j = 0; j < n; ++j) {
-    if (threadIdx.x == 0) {
-      const double v = b[j] / L[j * ld + j];
-      b[j] = v;
-      xs = v;
+// Forward (L y = b) then backward (L^T x = y) substitution, one workgroup of
+// 1024 threads, x in LDS.  Blocked by 64: the off-diagonal part of a block row
+// (column, going backward) is a coalesced GEMV reduced in a fixed order; the
+// 64x64 diagonal triangle is staged in LDS and solved by one wavefront, lane r
+// holding row r, the solved value broadcast by a lane read.
+constexpr int SOLVE_THREADS = 1024;
+
+__global__ void __launch_bounds__(SOLVE_THREADS)
+    chol_solve_kernel(const double* __restrict__ L, int64_t n, int64_t ld, double* __restrict__ b) {
+  extern __shared__ double xs[];  // [n]
+  __shared__ double Ld[NB][NB + 1];
+  __shared__ double inv_d[NB];
+  __shared__ double part[SOLVE_THREADS / NB][NB];
+  const int t = threadIdx.x;
+  for (int64_t i = t; i < n; i += SOLVE_THREADS) xs[i] = b[i];
+  __syncthreads();
+  // forward: block rows in order
+  for (int64_t i0 = 0; i0 < n; i0 += NB) {
+    const int m = (int)min((int64_t)NB, n - i0);
+    {
+      const int row = t >> 4, g = t & 15;  // 16 consecutive lanes per row
+      double p4[4] = {0.0, 0.0, 0.0, 0.0};
+      if (row < m) {
+        const double* lr = L + (i0 + row) * ld;
+        int64_t j = g;
+        for (; j + 112 < i0; j += 128) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) p4[u & 3] += lr[j + 16 * u] * xs[j + 16 * u];
+        }
+        for (; j < i0; j += 16) p4[0] += lr[j] * xs[j];
+      }
+      double sacc = (p4[0] + p4[1]) + (p4[2] + p4[3]);
+#pragma unroll
+      for (int off = 8; off > 0; off >>= 1) sacc += __shfl_xor(sacc, off, 16);
+      for (int e = t; e < NB * NB; e += SOLVE_THREADS) {
+        const int r = e / NB, c = e % NB;
+        Ld[r][c] = (r < m && c <= r) ? L[(i0 + r) * ld + i0 + c] : 0.0;
+      }
+      if (t < NB) inv_d[t] = t < m ? 1.0 / L[(i0 + t) * ld + i0 + t] : 1.0;
+      __syncthreads();
+      if (g == 0 && row < m) xs[i0 + row] -= sacc;
+      __syncthreads();
+    }
+    if (t < 64) {
+      double v = t < m ? xs[i0 + t] : 0.0;
+      for (int j = 0; j < m; ++j) {
+        const double xj = readlane_f64(v, j) * inv_d[j];
+        if (t == j) v = xj;
+        if (t > j) v -= Ld[t][j] * xj;
+      }
+      if (t < m) xs[i0 + t] = v;
     }
     __syncthreads();
-    const double v = xs;
-    for (int64_t i = j + 1 + threadIdx.x; i < n; i += blockDim.x) b[i] -= L[i * ld + j] * v;
-    __syncthreads();
   }
-  for (int64_t j = n - 1; j >= 0; --j) {
-    if (threadIdx.x == 0) {
-      const double v = b[j] / L[j * ld + j];
-      b[j] = v;
-      xs = v;
+  // backward: block columns in reverse order
+  const int64_t nblocks = (n + NB - 1) / NB;
+  for (int64_t bi = nblocks - 1; bi >= 0; --bi) {
+    const int64_t i0 = bi * NB;
+    const int m = (int)min((int64_t)NB, n - i0);
+    {
+      const int c = t & (NB - 1), g = t / NB;  // 64 consecutive lanes read one row of L
+      constexpr int G = SOLVE_THREADS / NB;
+      double p4[4] = {0.0, 0.0, 0.0, 0.0};
+      if (c < m) {
+        int64_t j = i0 + m + g;
+        for (; j + 7 * G < n; j += 8 * G) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) p4[u & 3] += L[(j + u * G) * ld + i0 + c] * xs[j + u * G];
+        }
+        for (; j < n; j += G) p4[0] += L[j * ld + i0 + c] * xs[j];
+      }
+      part[g][c] = (p4[0] + p4[1]) + (p4[2] + p4[3]);
+      for (int e = t; e < NB * NB; e += SOLVE_THREADS) {
+        const int r = e / NB, cc = e % NB;
+        Ld[r][cc] = (r < m && cc <= r) ? L[(i0 + r) * ld + i0 + cc] : 0.0;
+      }
+      if (t < NB) inv_d[t] = t < m ? 1.0 / L[(i0 + t) * ld + i0 + t] : 1.0;
+      __syncthreads();
+      if (t < m) {
+        double tot = 0.0;
+        for (int gg = 0; gg < SOLVE_THREADS / NB; ++gg) tot += part[gg][t];
+        xs[i0 + t] -= tot;
+      }
+      __syncthreads();
+    }
+    if (t < 64) {
+      double v = t < m ? xs[i0 + t] : 0.0;
+      for (int j = m - 1; j >= 0; --j) {
+        const double xj = readlane_f64(v, j) * inv_d[j];
+        if (t == j) v = xj;
+        if (t < j) v -= Ld[j][t] * xj;
+      }
+      if (t < m) xs[i0 + t] = v;
     }
     __syncthreads();
-    const double v = xs;
-    for (int64_t i = threadIdx.x; i < j; i += blockDim.x) b[i] -= L[j * ld + i] * v;
-    __syncthreads();
   }
+  for (int64_t i = t; i < n; i += SOLVE_THREADS) b[i] = xs[i];
 }
 
 }  // namespace
@@ -169,7 +288,7 @@ extern "C" int tlh_cholesky_f64(double* A, int64_t n, int64_t ld, int32_t* statu
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   for (int64_t k = 0; k < n; k += NB) {
     const int64_t below = n - k - NB > 0 ? ceil_div(n - k - NB, NB) : 0;
-    hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)(1 + below)), dim3(TLH_THREADS), 0, s, A, n, ld, k, status);
+    hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)(1 + below)), dim3(PT), 0, s, A, n, ld, k, status);
     TLH_RETURN_IF_LAUNCH_FAILED();
     if (below > 0) {
       const int64_t blocks = below * (below + 1) / 2;
@@ -181,9 +300,13 @@ extern "C" int tlh_cholesky_f64(double* A, int64_t n, int64_t ld, int32_t* statu
 }
 
 extern "C" int tlh_cholesky_solve_f64(const double* L, int64_t n, int64_t ld, double* b, tlh_stream_t stream) {
-  if (!L || !b || n < 0 || ld < n) return TLH_EINVAL;
+  if (!L || !b || n < 0 || ld < n || n > 8192) return TLH_EINVAL;
   if (n == 0) return 0;
-  hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(1024), 0, reinterpret_cast<hipStream_t>(stream), L, n, ld, b);
+  const int64_t lds = n * (int64_t)sizeof(double);  // x; the diagonal block and GEMV partials are static
+  TLH_RETURN_IF_ERROR(hipFuncSetAttribute((const void*)chol_solve_kernel,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(SOLVE_THREADS), (size_t)lds, reinterpret_cast<hipStream_t>(stream),
+                     L, n, ld, b);
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }

@@ -455,26 +455,38 @@ __global__ void __launch_bounds__(TLH_THREADS)
 // observation pairs (i in edge(ci,p), j in edge(cj,p)) of
 // (J_c^T J_p)_i C_p^-1 (J_c^T J_p)_j^T, subtract from block (ci, cj) of M
 // (and its transpose).  M must hold blockdiag(D_c + lambda I) beforehand.
-template <typename Real>
+// FR/FDC/FDP > 0 fix the block shape at compile time (bundle adjustment: 2, 6,
+// 3), so the per-thread blocks live in registers; 0 reads it from the
+// arguments (the arrays then live in private memory).
+template <typename Real, int FR, int FDC, int FDP>
 __global__ void __launch_bounds__(TLH_THREADS)
-    schur_dense_kernel(SchurArgs<Real> a, int R, int DC, int DP, int cfirst, const int32_t* __restrict__ pair_ptr,
+    schur_dense_kernel(SchurArgs<Real> a, int R_, int DC_, int DP_, int cfirst, const int32_t* __restrict__ pair_ptr,
                        const int32_t* __restrict__ pair_i, const int32_t* __restrict__ pair_j,
                        const int32_t* __restrict__ pair_key_ci, const int32_t* __restrict__ pair_key_cj,
                        int64_t npairs, double* __restrict__ M, int64_t ld) {
+  constexpr int MC = FDC ? FDC : 8, MP = FDP ? FDP : 8;
+  const int R = FR ? FR : R_, DC = FDC ? FDC : DC_, DP = FDP ? FDP : DP_;
   const int64_t key = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
   if (key >= npairs) return;
   const int OC = cfirst ? 0 : R * DP;
   const int OP = cfirst ? R * DC : 0;
-  double acc[8][8];
-  for (int i = 0; i < DC; ++i)
-    for (int j = 0; j < DC; ++j) acc[i][j] = 0.0;
+  double acc[MC][MC];
+#pragma unroll
+  for (int i = 0; i < MC; ++i)
+#pragma unroll
+    for (int j = 0; j < MC; ++j) acc[i][j] = 0.0;
   for (int e = pair_ptr[key]; e < pair_ptr[key + 1]; ++e) {
     const int64_t ni = pair_i[e], nj = pair_j[e];
-    double wi[8][8], wj[8][8];
-    for (int c = 0; c < DC; ++c)
-      for (int p = 0; p < DP; ++p) {
+    double wi[MC][MP], wj[MC][MP];
+#pragma unroll
+    for (int c = 0; c < MC; ++c)
+#pragma unroll
+      for (int p = 0; p < MP; ++p) {
+        if (c >= DC || p >= DP) continue;
         double si = 0.0, sj = 0.0;
-        for (int r = 0; r < R; ++r) {
+#pragma unroll
+        for (int r = 0; r < (FR ? FR : 16); ++r) {
+          if (r >= R) break;
           si += (double)a.packed[(int64_t)(OC + r * DC + c) * a.stride + ni] *
                 (double)a.packed[(int64_t)(OP + r * DP + p) * a.stride + ni];
           sj += (double)a.packed[(int64_t)(OC + r * DC + c) * a.stride + nj] *
@@ -484,26 +496,39 @@ __global__ void __launch_bounds__(TLH_THREADS)
         wj[c][p] = sj;
       }
     const double* m = a.cinv + (int64_t)a.idx_p[ni] * (DP * (DP + 1) / 2);
-    for (int c = 0; c < DC; ++c) {
-      double t[8];
-      for (int p = 0; p < DP; ++p) {
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+      if (c >= DC) continue;
+      double t[MP];
+#pragma unroll
+      for (int p = 0; p < MP; ++p) {
+        if (p >= DP) continue;
         double s = 0.0;
-        for (int q = 0; q < DP; ++q) {
+#pragma unroll
+        for (int q = 0; q < MP; ++q) {
+          if (q >= DP) continue;
           const int lo = q < p ? q : p, hi = q < p ? p : q;
           s += wi[c][q] * m[lo * DP - lo * (lo - 1) / 2 + (hi - lo)];
         }
         t[p] = s;
       }
-      for (int d = 0; d < DC; ++d) {
+#pragma unroll
+      for (int d = 0; d < MC; ++d) {
+        if (d >= DC) continue;
         double s = 0.0;
-        for (int p = 0; p < DP; ++p) s += t[p] * wj[d][p];
+#pragma unroll
+        for (int p = 0; p < MP; ++p)
+          if (p < DP) s += t[p] * wj[d][p];
         acc[c][d] += s;
       }
     }
   }
   const int64_t ci = pair_key_ci[key], cj = pair_key_cj[key];
-  for (int c = 0; c < DC; ++c)
-    for (int d = 0; d < DC; ++d) {
+#pragma unroll
+  for (int c = 0; c < MC; ++c)
+#pragma unroll
+    for (int d = 0; d < MC; ++d) {
+      if (c >= DC || d >= DC) continue;
       M[(ci * DC + c) * ld + cj * DC + d] -= acc[c][d];
       if (ci != cj) M[(cj * DC + d) * ld + ci * DC + c] -= acc[c][d];
     }
@@ -698,14 +723,17 @@ extern "C" int tlh_schur_dense(const tlh_operator_t* op, int wgroup, int eslot, 
   if (npairs == 0) return 0;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const dim3 grid((unsigned)ceil_div(npairs, TLH_THREADS)), block(TLH_THREADS);
+  const bool ba = R == 2 && DC == 6 && DP == 3;
   if (op->dtype == TLH_F32) {
     SchurArgs<float> a = make_schur_args<float>(op, g, sc, se, nullptr, nullptr, nullptr, cinv);
-    hipLaunchKernelGGL(schur_dense_kernel<float>, grid, block, 0, s, a, R, DC, DP, (int)(sc < se), pair_ptr, pair_i,
-                       pair_j, key_ci, key_cj, npairs, M, ld);
+    auto kernel = ba ? schur_dense_kernel<float, 2, 6, 3> : schur_dense_kernel<float, 0, 0, 0>;
+    hipLaunchKernelGGL(kernel, grid, block, 0, s, a, R, DC, DP, (int)(sc < se), pair_ptr, pair_i, pair_j, key_ci,
+                       key_cj, npairs, M, ld);
   } else {
     SchurArgs<double> a = make_schur_args<double>(op, g, sc, se, nullptr, nullptr, nullptr, cinv);
-    hipLaunchKernelGGL(schur_dense_kernel<double>, grid, block, 0, s, a, R, DC, DP, (int)(sc < se), pair_ptr,
-                       pair_i, pair_j, key_ci, key_cj, npairs, M, ld);
+    auto kernel = ba ? schur_dense_kernel<double, 2, 6, 3> : schur_dense_kernel<double, 0, 0, 0>;
+    hipLaunchKernelGGL(kernel, grid, block, 0, s, a, R, DC, DP, (int)(sc < se), pair_ptr, pair_i, pair_j, key_ci,
+                       key_cj, npairs, M, ld);
   }
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;

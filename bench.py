@@ -165,6 +165,69 @@ def large_normal_matvec(device, reps):
             "peak": HBM_PEAK_GBS, "frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+def lm_large(device, steps):
+    """LM iterations/s of the mixed-precision and float64 HIP Schur-PCG at 5M observations
+    (2,000 cameras, 500,000 points, 5 % outliers), where the blocks no longer fit the last-level
+    cache: the same fixed-iteration protocol as the headline line, one warm-up solve each."""
+    from torchls_amd import generators
+
+    prob = generators.ba_large()
+    P = build_problem(prob, device)
+    init = {k: v.clone() for k, v in P.parameters.items()}
+    out = {"workload": "ba_large", "cameras": prob.n_cameras, "points": prob.n_points, "observations": prob.n_obs,
+           "seed": prob.seed, "steps": steps}
+    res = {}
+    for dt in (torch.float32, None):
+        timed_solve(P, init, 1, dt)
+        res[dt] = timed_solve(P, init, steps, dt)
+    (t32, s32), (t64, s64) = res[torch.float32], res[None]
+    out.update({"mixed_lm_iters_per_s": round(s32.iterations / t32, 3),
+                "float64_lm_iters_per_s": round(s64.iterations / t64, 3),
+                "mixed_over_float64": round((s32.iterations / t32) / (s64.iterations / t64), 3),
+                "final_cost_mixed": s32.final_cost, "final_cost_float64": s64.final_cost,
+                "final_cost_rel_err": abs(s32.final_cost - s64.final_cost) / abs(s64.final_cost),
+                "pcg_iterations_mixed": s32.linear_iterations, "pcg_iterations_float64": s64.linear_iterations,
+                "accepted_rejected_mixed": [s32.accepted_steps, s32.rejected_steps],
+                "accepted_rejected_float64": [s64.accepted_steps, s64.rejected_steps]})
+    return out
+
+
+def pose_graph_lm(device, steps):
+    """LM iterations/s on the 10,000-pose SE(3) pose graph (odometry + 3 % loop closures,
+    6x6 aliased blocks): full-system PCG with block-Jacobi, mixed precision vs float64."""
+    from torchls_amd import ConjugateGradientConfig, TerminationConfig, generators
+    from torchls_amd.workloads import pose_graph_problem
+
+    pg = generators.pose_graph()
+    P = pose_graph_problem(pg, device)
+    init = {k: v.clone() for k, v in P.parameters.items()}
+
+    def run(dt, iters):
+        P.parameters.update({k: v.clone() for k, v in init.items()})
+        term = TerminationConfig(max_iterations=iters, cost_tolerance=0.0, gradient_tolerance=0.0,
+                                 parameter_tolerance=0.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, s = P.solve(return_summary=True, linear_solver="conjugate_gradient",
+                       conjugate_gradient=ConjugateGradientConfig(matvec_dtype=dt), termination=term)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, s
+
+    res = {}
+    for dt in (torch.float32, None):
+        run(dt, 1)
+        res[dt] = run(dt, steps)
+    (t32, s32), (t64, s64) = res[torch.float32], res[None]
+    return {"workload": "pose_graph", "poses": pg.n_poses, "factors": pg.n_factors, "loop_closures": pg.n_loop,
+            "seed": pg.seed, "steps": steps, "linear_solver": "conjugate_gradient (block-Jacobi)",
+            "mixed_lm_iters_per_s": round(s32.iterations / t32, 3),
+            "float64_lm_iters_per_s": round(s64.iterations / t64, 3),
+            "mixed_over_float64": round((s32.iterations / t32) / (s64.iterations / t64), 3),
+            "initial_cost": s64.initial_cost, "final_cost_mixed": s32.final_cost, "final_cost_float64": s64.final_cost,
+            "final_cost_rel_err": abs(s32.final_cost - s64.final_cost) / abs(s64.final_cost),
+            "pcg_iterations_mixed": s32.linear_iterations, "pcg_iterations_float64": s64.linear_iterations}
+
+
 def dense_schur_cholesky(device, reps):
     """MFMA float64 Cholesky of the 1,536 x 1,536 reduced camera system of a 256-camera
     problem (dense Schur path): HIP-event time per factorisation and n^3/3 flop rate."""
@@ -301,6 +364,17 @@ def main():
         log(f"normal matvec at 5M observations: {large['us_per_launch']} us ({large['achieved']} GB/s, "
             f"{100 * large['frac']:.1f}% of {HBM_PEAK_GBS:.0f} GB/s)")
 
+    lm5m = None
+    if not args.no_large:
+        lm5m = lm_large(device, 3)
+        log(f"LM at 5M observations: mixed {lm5m['mixed_lm_iters_per_s']} it/s, float64 "
+            f"{lm5m['float64_lm_iters_per_s']} it/s (x{lm5m['mixed_over_float64']}), final-cost rel err "
+            f"{lm5m['final_cost_rel_err']:.2e}")
+
+    pgraph = pose_graph_lm(device, 5)
+    log(f"pose graph ({pgraph['poses']} poses, {pgraph['factors']} factors): mixed {pgraph['mixed_lm_iters_per_s']} "
+        f"it/s, float64 {pgraph['float64_lm_iters_per_s']} it/s, final-cost rel err {pgraph['final_cost_rel_err']:.2e}")
+
     chol = dense_schur_cholesky(device, 20)
     log(f"dense Schur Cholesky n={chol['n']}: {chol['us_per_factorisation']} us, {chol['tflops']} TFLOP/s")
 
@@ -335,7 +409,7 @@ def main():
                      "frac": round(schur_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "implicit Schur matvec (schur_tile_kernel + finish)",
                      "us_per_launch": round(t_schur * 1e6, 2), "bytes_per_launch": schur_bytes},
-        "normal_matvec_large": large,
+        "normal_matvec_large": large, "lm_large": lm5m, "pose_graph": pgraph,
         "dense_schur_cholesky": chol,
         "cpu_baseline": cpu,
     }), flush=True)

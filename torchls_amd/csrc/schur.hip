@@ -60,11 +60,11 @@ __device__ __forceinline__ double sym(const double* m, int i, int j) {
   return m[a * D - a * (a - 1) / 2 + (b - a)];
 }
 
-template <typename Real, int R, int DC, int DP, bool CFIRST, int ITEMS, int MODE>
-__global__ void __launch_bounds__(TLH_THREADS) schur_tile_kernel(SchurArgs<Real> a) {
+template <typename Real, int R, int DC, int DP, bool CFIRST, int ITEMS, int MODE, int NTH>
+__global__ void __launch_bounds__(NTH) schur_tile_kernel(SchurArgs<Real> a) {
   constexpr int OC = CFIRST ? 0 : R * DP;
   constexpr int OP = CFIRST ? R * DC : 0;
-  constexpr int T = TLH_THREADS * ITEMS;
+  constexpr int T = NTH * ITEMS;
   constexpr int DMAX = DC > DP ? DC : DP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Real* stage = reinterpret_cast<Real*>(smem_raw);
@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(TLH_THREADS) schur_tile_kernel(SchurArgs<Real>
   Real* acc = sstage + T * DP;
   const bool dense = MODE != SCHUR_BACKSUB && a.mode_c == TLH_REDUCE_DENSE;
   if (dense)
-    for (int i = threadIdx.x; i < a.count_c * DC; i += TLH_THREADS) acc[i] = Real(0);
+    for (int i = threadIdx.x; i < a.count_c * DC; i += NTH) acc[i] = Real(0);
 
   const int t0 = blockIdx.x * a.tiles_per_wg;
   const int t1 = min(t0 + a.tiles_per_wg, a.ntiles);
@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(TLH_THREADS) schur_tile_kernel(SchurArgs<Real>
     int rk[ITEMS];
 #pragma unroll
     for (int it = 0; it < ITEMS; ++it) {
-      const int pos = (int)threadIdx.x + it * TLH_THREADS;
+      const int pos = (int)threadIdx.x + it * NTH;
       const int64_t n = start + min(pos, len - 1);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(TLH_THREADS) schur_tile_kernel(SchurArgs<Real>
     if (MODE != SCHUR_RHS) {
 #pragma unroll
       for (int it = 0; it < ITEMS; ++it) {
-        const int pos = (int)threadIdx.x + it * TLH_THREADS;
+        const int pos = (int)threadIdx.x + it * NTH;
         if (pos < len) {
 #pragma unroll
           for (int c = 0; c < DP; ++c) {
@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(TLH_THREADS) schur_tile_kernel(SchurArgs<Real>
     // one owner per point: s_p = C_p^-1 (...) in float64
     {
       const int sb = a.seg_ptr_p[t], se = a.seg_ptr_p[t + 1];
-      for (int k = sb + (int)threadIdx.x; k < se; k += TLH_THREADS) {
+      for (int k = sb + (int)threadIdx.x; k < se; k += NTH) {
         const int lo = a.seg_start_p[k];
         const int hi = (k + 1 < se) ? a.seg_start_p[k + 1] : len;
         const int64_t v = a.idx_p[start + lo];
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(TLH_THREADS) schur_tile_kernel(SchurArgs<Real>
     Real cc[ITEMS][DC];
 #pragma unroll
     for (int it = 0; it < ITEMS; ++it) {
-      const int pos = min((int)threadIdx.x + it * TLH_THREADS, len - 1);
+      const int pos = min((int)threadIdx.x + it * NTH, len - 1);
       Real sv[DP];
 #pragma unroll
       for (int c = 0; c < DP; ++c) sv[c] = sstage[pos * DP + c];
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(TLH_THREADS) schur_tile_kernel(SchurArgs<Real>
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < ITEMS; ++it) {
-      const int pos = (int)threadIdx.x + it * TLH_THREADS;
+      const int pos = (int)threadIdx.x + it * NTH;
       if (pos < len) {
 #pragma unroll
         for (int c = 0; c < DC; ++c) stage[rk[it] * DC + c] = cc[it][c];
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(TLH_THREADS) schur_tile_kernel(SchurArgs<Real>
     }
     __syncthreads();
     const int sb = a.seg_ptr_c[t], se = a.seg_ptr_c[t + 1];
-    for (int k = sb + (int)threadIdx.x; k < se; k += TLH_THREADS) {
+    for (int k = sb + (int)threadIdx.x; k < se; k += NTH) {
       const int lo = a.seg_start_c[k];
       const int hi = (k + 1 < se) ? a.seg_start_c[k + 1] : len;
       Real sum[DC];
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(TLH_THREADS) schur_tile_kernel(SchurArgs<Real>
     __syncthreads();
     const int64_t nn = (int64_t)a.count_c * DC;
     Real* out = a.part_c + (int64_t)(a.dense_block_c + blockIdx.x) * nn;
-    for (int64_t i = threadIdx.x; i < nn; i += TLH_THREADS) out[i] = acc[i];
+    for (int64_t i = threadIdx.x; i < nn; i += NTH) out[i] = acc[i];
   }
 }
 
@@ -566,26 +566,30 @@ SchurArgs<Real> make_schur_args(const tlh_operator_t* op, const tlh_group_t& g, 
   return a;
 }
 
+// The W-group tile kernel runs one observation per thread: 1,024 threads over a
+// float32 tile (512 for float64), so a compute unit holding one workgroup (the
+// dense camera accumulator fills most of its LDS) still has 16 (8) wavefronts.
 template <typename Real>
-constexpr int schur_items() {
-  return sizeof(Real) == 4 ? 4 : 2;
+constexpr int schur_threads() {
+  return sizeof(Real) == 4 ? 1024 : 512;
 }
 
 template <typename Real, int R, int DC, int DP, bool CFIRST, int MODE>
 int launch_schur_t(const tlh_operator_t* op, const tlh_group_t& g, int sc, int sp, const void* xc, const void* bp,
                    void* outp, const double* cinv, hipStream_t s) {
-  constexpr int ITEMS = schur_items<Real>();
-  if (g.tile_cap > TLH_THREADS * ITEMS) return TLH_EINVAL;
-  constexpr int T = TLH_THREADS * ITEMS;
+  constexpr int ITEMS = 1;
+  constexpr int NTH = schur_threads<Real>();
+  if (g.tile_cap > NTH * ITEMS) return TLH_EINVAL;
+  constexpr int T = NTH * ITEMS;
   int64_t lds = (int64_t)T * ((DC > DP ? DC : DP) + DP) * sizeof(Real);
   if (MODE != SCHUR_BACKSUB && g.slot[sc].mode == TLH_REDUCE_DENSE)
     lds += op->types[g.slot[sc].type].count * DC * (int64_t)sizeof(Real);
-  auto kernel = schur_tile_kernel<Real, R, DC, DP, CFIRST, ITEMS, MODE>;
+  auto kernel = schur_tile_kernel<Real, R, DC, DP, CFIRST, ITEMS, MODE, NTH>;
   TLH_RETURN_IF_ERROR(allow_lds((const void*)kernel, lds));
   const unsigned grid = MODE == SCHUR_BACKSUB ? (unsigned)g.ntiles : (unsigned)g.nwg;
   SchurArgs<Real> args = make_schur_args<Real>(op, g, sc, sp, xc, bp, outp, cinv);
   if (MODE == SCHUR_BACKSUB) args.tiles_per_wg = 1;
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(TLH_THREADS), (size_t)lds, s, args);
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(NTH), (size_t)lds, s, args);
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }

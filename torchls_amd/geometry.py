@@ -97,6 +97,13 @@ def se3_log(T: Tensor) -> Tensor:
     return torch.cat(((Vinv @ T[..., :3, 3:4])[..., 0], w), -1)
 
 
+def se3_inv(T: Tensor) -> Tensor:
+    """Inverse of 4x4 SE(3) matrices: [R^T, -R^T t; 0, 1]."""
+    Rt = T[..., :3, :3].transpose(-1, -2)
+    top = torch.cat((Rt, -(Rt @ T[..., :3, 3:4])), dim=-1)
+    return torch.cat((top, T[..., 3:4, :]), dim=-2)
+
+
 def _se3_retract(pose: Tensor, delta: Tensor) -> Tensor:
     return pose @ se3_exp(delta)
 

@@ -63,6 +63,15 @@ class PcgState:
 # Replays per host read after the first batch: iterations past convergence are
 # frozen by the control word but still run the operator, so later batches are short.
 SMALL_BATCH = 4
+# A restart after the recurrence converged must at least halve the true residual;
+# otherwise the operator's own rounding (a float32 matvec under float64 vectors)
+# bounds the attainable residual and the solve stops as "stagnated" (usable).
+STALL_FACTOR = 0.5
+# Systems up to this size run each iteration's vector work in one workgroup
+# (tlh_pcg_iteration_small accepts up to _lib.PCG_SMALL_MAX_N); larger ones spread
+# it over the grid, which is faster once a single compute unit's bandwidth and
+# its tree reductions dominate (12,000 unknowns: 44 us in one workgroup).
+SMALL_PREFERRED_N = 4096
 
 
 def hip_pcg(matvec, rhs, *, preconditioner, config, use_graph: bool = True, first_batch: int | None = None):
@@ -139,7 +148,7 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None):
     begin()
     reset(rhs, True)
 
-    small = (fused or apply is None) and n <= _lib.PCG_SMALL_MAX_N
+    small = (fused or apply is None) and n <= SMALL_PREFERRED_N
 
     def iteration():
         ap = matvec(p)
@@ -202,6 +211,7 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None):
     rhs_norm = threshold = None
     reason = "max_iterations"
     true_norm = math.nan
+    converged_true = math.inf  # true residual at the last check after a converged recurrence
     batch = max(1, int(first_batch)) if first_batch else config.graph_batch_iterations
     while True:
         for _ in range(max(0, min(batch, stop_at - iterations))):
@@ -239,6 +249,11 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None):
             break
         if iterations >= config.max_iterations:
             break
+        if status == 1:
+            if true_norm > STALL_FACTOR * converged_true:
+                reason = "stagnated"
+                break
+            converged_true = true_norm
         drift = math.sqrt(drift2) if drift2 >= 0 else math.nan
         if (
             status == 1

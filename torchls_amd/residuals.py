@@ -14,7 +14,7 @@ from __future__ import annotations
 import torch
 from torch import Tensor
 
-from .geometry import skew
+from .geometry import se3_inv, se3_log, skew
 
 
 def reprojection(camera: Tensor, point: Tensor, observation: Tensor) -> Tensor:
@@ -80,3 +80,19 @@ def _batched_linearize(cams: Tensor, pts: Tensor, uv: Tensor):
 reprojection.batched_residual = _batched_residual
 reprojection.batched_linearize = _batched_linearize
 reprojection.hip_residual = "reprojection"
+
+
+def between(pose_i: Tensor, pose_j: Tensor, measurement: Tensor) -> Tensor:
+    """SE(3) between factor of the pose-graph workload: Log(Z^-1 T_i^-1 T_j), 6 rows,
+    [translation, rotation] like the tangent.  The reference's geometry has no SE(3)
+    log; the logarithm here (geometry.se3_log) is the inverse of its exponential
+    (geometry.py:40-57).  Linearized by torch.func like any user residual, giving the
+    two aliased 6x6 blocks of a pose-graph edge."""
+    return se3_log(se3_inv(measurement) @ se3_inv(pose_i) @ pose_j)
+
+
+def _between_batched(pose_i: Tensor, pose_j: Tensor, measurement: Tensor) -> Tensor:
+    return se3_log(se3_inv(measurement) @ se3_inv(pose_i) @ pose_j)
+
+
+between.batched_residual = _between_batched

@@ -106,9 +106,11 @@ class HipSchurPlan:
             groups.append((rows, ltypes, self._plan_groups[k][2]))
         eprim = {self.wgroup: self.eslot}
         stage_extra = 256 * (4 if self._esize == 4 else 2) * self.edim * self._esize
+        # the W-group kernel's workgroups (1,024 / 512 threads, csrc/schur.hip) fill a compute unit
         self.plan = build_structure(self.rlayout, groups, esize=self._esize,
                                     lds_limit=int(self._info.lds_per_block) - stage_extra,
-                                    cu_count=int(self._info.cu_count), force_primary=eprim, split_primary=False)
+                                    cu_count=int(self._info.cu_count), wgs_per_cu=1, force_primary=eprim,
+                                    split_primary=False)
         # the preconditioner reduces W-group edge blocks only: its plan holds that group alone,
         # so every camera partial it lists is written by tlh_schur_precond
         self.sparse_plan = build_structure(self.rlayout, [groups[self.wgroup]], esize=self._esize,

@@ -392,7 +392,7 @@ def solve(
                             pcg_hint = max(1, result.iterations)
                             retained, iterations, converged = result.solution, result.iterations, result.converged
                             # the step's finiteness is read with the trial (one host read per trial)
-                            usable = result.reason in ("residual_tolerance", "max_iterations")
+                            usable = result.reason in ("residual_tolerance", "max_iterations", "stagnated")
                             finite_step = torch.isfinite(retained).all()
                             summary.linear_restarts += result.restarts
                             summary.linear_residual_checks += result.residual_checks
@@ -420,7 +420,7 @@ def solve(
                     summary.linear_relative_residual_history.append(
                         result.residual_norm / result.rhs_norm if result.rhs_norm else result.residual_norm)
                     delta = result.solution
-                    usable = result.reason in ("residual_tolerance", "max_iterations")
+                    usable = result.reason in ("residual_tolerance", "max_iterations", "stagnated")
                     finite_step = torch.isfinite(delta).all()
             except (LinAlgError, torch.linalg.LinAlgError):
                 usable = False

@@ -4,7 +4,7 @@ from __future__ import annotations
 
 import torch
 
-from .generators import BundleAdjustment
+from .generators import BundleAdjustment, PoseGraph
 from .residuals import reprojection_batched
 from .sparse import BlockJacobian, JacobianGroup, VariableLayout
 
@@ -55,3 +55,16 @@ def normal_matvec_bytes(n_obs: int, rows: int, dims: tuple, n: int, esize: int) 
     """Algorithmic HBM bytes of one normal matvec: blocks + two int32 ids per
     observation, plus reading x and writing y once."""
     return n_obs * (rows * sum(dims) * esize + 4 * len(dims)) + 2 * n * esize
+
+
+def pose_graph_problem(pg: PoseGraph, device, dtype=torch.float64):
+    """Problem of a generated pose graph: one SE(3) variable per pose and one
+    between factor Log(Z^-1 T_i^-1 T_j) per odometry edge and loop closure."""
+    from .geometry import SE3Var
+    from .problem import Problem
+    from .residuals import between
+
+    P = Problem({"poses": torch.tensor(pg.poses_init, dtype=dtype, device=device)}, manifolds={"poses": SE3Var})
+    P.add_residual(between, ("poses", torch.tensor(pg.i)), ("poses", torch.tensor(pg.j)),
+                   torch.tensor(pg.measurements, dtype=dtype, device=device))
+    return P

@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--repeats", type=int, default=3, help="timed solves per precision (median reported)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a 1 GiB device copy 20 times (HBM counter calibration) and exit")
+    p.add_argument("--quick", action="store_true",
+                   help="skip the 5M-observation LM and pose-graph LM legs (profiling runs)")
     p.add_argument("--dense-only", action="store_true",
                    help="only time the dense Schur MFMA Cholesky (for counter profiles) and exit")
     return p.parse_args()
@@ -365,15 +367,17 @@ def main():
             f"{100 * large['frac']:.1f}% of {HBM_PEAK_GBS:.0f} GB/s)")
 
     lm5m = None
-    if not args.no_large:
+    if not args.no_large and not args.quick:
         lm5m = lm_large(device, 3)
         log(f"LM at 5M observations: mixed {lm5m['mixed_lm_iters_per_s']} it/s, float64 "
             f"{lm5m['float64_lm_iters_per_s']} it/s (x{lm5m['mixed_over_float64']}), final-cost rel err "
             f"{lm5m['final_cost_rel_err']:.2e}")
 
-    pgraph = pose_graph_lm(device, 5)
-    log(f"pose graph ({pgraph['poses']} poses, {pgraph['factors']} factors): mixed {pgraph['mixed_lm_iters_per_s']} "
-        f"it/s, float64 {pgraph['float64_lm_iters_per_s']} it/s, final-cost rel err {pgraph['final_cost_rel_err']:.2e}")
+    pgraph = None if args.quick else pose_graph_lm(device, 5)
+    if pgraph:
+        log(f"pose graph ({pgraph['poses']} poses, {pgraph['factors']} factors): mixed {pgraph['mixed_lm_iters_per_s']} "
+            f"it/s, float64 {pgraph['float64_lm_iters_per_s']} it/s, final-cost rel err "
+            f"{pgraph['final_cost_rel_err']:.2e}")
 
     chol = dense_schur_cholesky(device, 20)
     log(f"dense Schur Cholesky n={chol['n']}: {chol['us_per_factorisation']} us, {chol['tflops']} TFLOP/s")

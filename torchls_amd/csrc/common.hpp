@@ -58,6 +58,20 @@ struct GroupArgs {
 };
 
 // Dynamic LDS above the default limit must be enabled once per kernel.
+// Largest dynamic LDS allocation of one workgroup on the current device.
+inline int64_t device_lds_max() {
+  static int64_t cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
+  if (!cached[dev]) {
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lds <= 0)
+      return 65536;
+    cached[dev] = lds;
+  }
+  return cached[dev];
+}
+
 inline hipError_t allow_lds(const void* kernel, int64_t bytes) {
   if (bytes <= 65536) return hipSuccess;
   return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);

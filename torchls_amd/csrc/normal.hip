@@ -891,20 +891,6 @@ int64_t wave_lds(const tlh_group_t& g, const tlh_type_t& ts, int64_t lds_max, bo
   return b;
 }
 
-// Largest dynamic LDS allocation of one workgroup on the current device.
-inline int64_t device_lds_max() {
-  static int64_t cached[64] = {};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
-  if (!cached[dev]) {
-    int lds = 0;
-    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lds <= 0)
-      return 65536;
-    cached[dev] = lds;
-  }
-  return cached[dev];
-}
-
 template <typename Real, int R, int D0, int D1, int P>
 int launch_wave_t(const tlh_operator_t* op, const tlh_group_t& g, const void* x, void* y, double damping,
                   hipStream_t stream) {

@@ -60,25 +60,55 @@ __device__ __forceinline__ double sym(const double* m, int i, int j) {
   return m[a * D - a * (a - 1) / 2 + (b - a)];
 }
 
-template <typename Real, int R, int DC, int DP, bool CFIRST, int ITEMS, int MODE, int NTH>
+// XL: the retained x slice is copied into LDS once per workgroup (after the
+// accumulator), so the per-observation camera gathers are LDS reads.  Each
+// tile's point owners load their point's C_p^-1 (and b_p) at the start of the
+// tile, so those dependent loads overlap the block loads instead of stalling
+// the point phase.
+template <typename Real, int R, int DC, int DP, bool CFIRST, int ITEMS, int MODE, int NTH, bool XL>
 __global__ void __launch_bounds__(NTH) schur_tile_kernel(SchurArgs<Real> a) {
   constexpr int OC = CFIRST ? 0 : R * DP;
   constexpr int OP = CFIRST ? R * DC : 0;
   constexpr int T = NTH * ITEMS;
   constexpr int DMAX = DC > DP ? DC : DP;
+  constexpr int NM = DP * (DP + 1) / 2;
+  static_assert(ITEMS == 1, "one point segment per thread");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Real* stage = reinterpret_cast<Real*>(smem_raw);
   Real* sstage = stage + T * DMAX;
   Real* acc = sstage + T * DP;
+  Real* xl = acc + (int64_t)a.count_c * DC;
   const bool dense = MODE != SCHUR_BACKSUB && a.mode_c == TLH_REDUCE_DENSE;
   if (dense)
     for (int i = threadIdx.x; i < a.count_c * DC; i += NTH) acc[i] = Real(0);
+  if constexpr (XL) {
+    for (int i = threadIdx.x; i < a.count_c * DC; i += NTH) xl[i] = a.xc[i];
+    __syncthreads();
+  }
 
   const int t0 = blockIdx.x * a.tiles_per_wg;
   const int t1 = min(t0 + a.tiles_per_wg, a.ntiles);
   for (int t = t0; t < t1; ++t) {
     const int start = a.tile_start[t];
     const int len = a.tile_start[t + 1] - start;
+    // point segment k = sb + thread: its range, variable, C_p^-1 and b_p, loaded early
+    const int sb = a.seg_ptr_p[t], se = a.seg_ptr_p[t + 1];
+    const int kseg = sb + (int)threadIdx.x;
+    const bool owner = kseg < se;
+    int lo = 0, hi = 0;
+    int64_t v = 0;
+    double mk[NM], bpv[DP];
+    if (owner) {
+      lo = a.seg_start_p[kseg];
+      hi = (kseg + 1 < se) ? a.seg_start_p[kseg + 1] : len;
+      v = a.idx_p[start + lo];
+#pragma unroll
+      for (int e = 0; e < NM; ++e) mk[e] = a.cinv[v * NM + e];
+      if (MODE != SCHUR_MATVEC) {
+#pragma unroll
+        for (int c = 0; c < DP; ++c) bpv[c] = (double)a.bp[v * DP + c];
+      }
+    }
     Real jc[ITEMS][R][DC], jp[ITEMS][R][DP], u[ITEMS][R];
     int rk[ITEMS];
 #pragma unroll
@@ -94,7 +124,7 @@ __global__ void __launch_bounds__(NTH) schur_tile_kernel(SchurArgs<Real> a) {
       }
       rk[it] = a.rank_c ? (int)a.rank_c[n] : min(pos, len - 1);
       if (MODE != SCHUR_RHS) {
-        const Real* x = a.xc + (int64_t)a.idx_c[n] * DC;
+        const Real* x = XL ? xl + a.idx_c[n] * DC : a.xc + (int64_t)a.idx_c[n] * DC;
         Real xv[DC];
 #pragma unroll
         for (int c = 0; c < DC; ++c) xv[c] = x[c];
@@ -129,12 +159,8 @@ __global__ void __launch_bounds__(NTH) schur_tile_kernel(SchurArgs<Real> a) {
     }
     __syncthreads();
     // one owner per point: s_p = C_p^-1 (...) in float64
-    {
-      const int sb = a.seg_ptr_p[t], se = a.seg_ptr_p[t + 1];
-      for (int k = sb + (int)threadIdx.x; k < se; k += NTH) {
-        const int lo = a.seg_start_p[k];
-        const int hi = (k + 1 < se) ? a.seg_start_p[k + 1] : len;
-        const int64_t v = a.idx_p[start + lo];
+    if (owner) {
+      {
         double tp[DP];
 #pragma unroll
         for (int c = 0; c < DP; ++c) tp[c] = 0.0;
@@ -145,16 +171,15 @@ __global__ void __launch_bounds__(NTH) schur_tile_kernel(SchurArgs<Real> a) {
         }
         if (MODE != SCHUR_MATVEC) {
 #pragma unroll
-          for (int c = 0; c < DP; ++c) tp[c] = (double)a.bp[v * DP + c] - tp[c];
+          for (int c = 0; c < DP; ++c) tp[c] = bpv[c] - tp[c];
           // RHS: tp = b_p; BACKSUB: tp = b_p - W^T x
         }
-        const double* m = a.cinv + v * (DP * (DP + 1) / 2);
         double sp[DP];
 #pragma unroll
         for (int i = 0; i < DP; ++i) {
           double s = 0.0;
 #pragma unroll
-          for (int j = 0; j < DP; ++j) s += sym<DP>(m, i, j) * tp[j];
+          for (int j = 0; j < DP; ++j) s += sym<DP>(mk, i, j) * tp[j];
           sp[i] = s;
         }
         if (MODE == SCHUR_BACKSUB) {
@@ -202,14 +227,14 @@ __global__ void __launch_bounds__(NTH) schur_tile_kernel(SchurArgs<Real> a) {
       }
     }
     __syncthreads();
-    const int sb = a.seg_ptr_c[t], se = a.seg_ptr_c[t + 1];
-    for (int k = sb + (int)threadIdx.x; k < se; k += NTH) {
-      const int lo = a.seg_start_c[k];
-      const int hi = (k + 1 < se) ? a.seg_start_c[k + 1] : len;
+    const int cb = a.seg_ptr_c[t], ce = a.seg_ptr_c[t + 1];
+    for (int k = cb + (int)threadIdx.x; k < ce; k += NTH) {
+      const int clo = a.seg_start_c[k];
+      const int chi = (k + 1 < ce) ? a.seg_start_c[k + 1] : len;
       Real sum[DC];
 #pragma unroll
-      for (int c = 0; c < DC; ++c) sum[c] = stage[lo * DC + c];
-      for (int i = lo + 1; i < hi; ++i)
+      for (int c = 0; c < DC; ++c) sum[c] = stage[clo * DC + c];
+      for (int i = clo + 1; i < chi; ++i)
 #pragma unroll
         for (int c = 0; c < DC; ++c) sum[c] += stage[i * DC + c];
       const int target = a.seg_target_c[k];
@@ -582,9 +607,14 @@ int launch_schur_t(const tlh_operator_t* op, const tlh_group_t& g, int sc, int s
   if (g.tile_cap > NTH * ITEMS) return TLH_EINVAL;
   constexpr int T = NTH * ITEMS;
   int64_t lds = (int64_t)T * ((DC > DP ? DC : DP) + DP) * sizeof(Real);
-  if (MODE != SCHUR_BACKSUB && g.slot[sc].mode == TLH_REDUCE_DENSE)
-    lds += op->types[g.slot[sc].type].count * DC * (int64_t)sizeof(Real);
-  auto kernel = schur_tile_kernel<Real, R, DC, DP, CFIRST, ITEMS, MODE, NTH>;
+  const int64_t slice = op->types[g.slot[sc].type].count * DC * (int64_t)sizeof(Real);
+  const bool dense = MODE != SCHUR_BACKSUB && g.slot[sc].mode == TLH_REDUCE_DENSE;
+  if (dense) lds += slice;
+  // the retained x slice goes to LDS (after the accumulator region) when it fits
+  const bool xl = MODE != SCHUR_RHS && dense && lds + slice <= device_lds_max();
+  if (xl) lds += slice;
+  auto kernel = xl ? schur_tile_kernel<Real, R, DC, DP, CFIRST, ITEMS, MODE, NTH, (MODE != SCHUR_RHS)>
+                   : schur_tile_kernel<Real, R, DC, DP, CFIRST, ITEMS, MODE, NTH, false>;
   TLH_RETURN_IF_ERROR(allow_lds((const void*)kernel, lds));
   const unsigned grid = MODE == SCHUR_BACKSUB ? (unsigned)g.ntiles : (unsigned)g.nwg;
   SchurArgs<Real> args = make_schur_args<Real>(op, g, sc, sp, xc, bp, outp, cinv);

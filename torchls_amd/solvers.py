@@ -279,9 +279,9 @@ def solve(
     normal_plan = diagonal_plan = kernels = None
     column_scale = None
 
-    def examine(lin):
-        """float64 gradient of a linearization plus one host read of
-        (finite, max |g|, 0.5 sum rho) -- the only sync per linearization."""
+    def examine_device(lin):
+        """float64 gradient of a linearization and its (nonfinite count, max |g|,
+        0.5 sum rho) as a device tensor, read by the caller with other values."""
         nonlocal diagonal_plan, kernels
         jac = lin.jacobian
         if kernels is None:
@@ -290,7 +290,12 @@ def solve(
         g = kernels.gradient(jac, lin.residual)
         cost = lin.objective if lin.objective is not None else 0.5 * lin.residual.to(torch.float64).square().sum()
         bad = lin.nonfinite if lin.nonfinite is not None else (~torch.isfinite(lin.residual)).sum().double()
-        host = torch.stack((bad.to(torch.float64), g.abs().max(), cost.to(torch.float64))).cpu()
+        return g, torch.stack((bad.to(torch.float64), g.abs().max(), cost.to(torch.float64)))
+
+    def examine(lin):
+        """examine_device plus its host read: (gradient, finite, max |g|, cost)."""
+        g, stats = examine_device(lin)
+        host = stats.cpu()
         return g, bool(host[0] == 0), float(host[1]), float(host[2])
 
     with region("linearize"):
@@ -433,13 +438,17 @@ def solve(
                     objective, reduction = problem.evaluate_trial(proposed, lin)
                 sq = kernels.squared_matvec_norm(original_jacobian, delta)
                 pred_t = -torch.dot(gradient, delta) - 0.5 * sq
-                parts = [objective, reduction, pred_t, torch.linalg.vector_norm(delta)]
-                if finite_step is not None:
-                    parts.append(finite_step.to(torch.float64))
-                host = torch.stack(parts).cpu().tolist()
+                parts = [objective, reduction, pred_t, torch.linalg.vector_norm(delta),
+                         finite_step.to(torch.float64) if finite_step is not None else pred_t.new_ones(())]
+                # The next linearization is issued before the trial is read (used if the step
+                # is accepted, dropped otherwise): its gradient and cost ride on the same read.
+                with region("linearize"):
+                    lin_next = problem.linearize(proposed)
+                g_next, stats_next = examine_device(lin_next)
+                host = torch.cat((torch.stack(parts), stats_next)).cpu().tolist()
                 proposed_cost, actual, predicted, step_norm = host[:4]
                 # a nonfinite step is unusable: rejected without a trial record
-                usable = finite_step is None or host[4] != 0
+                usable = host[4] != 0
             if usable:
                 quality = actual / predicted if predicted > 0 else -math.inf
                 accepted = math.isfinite(proposed_cost) and (
@@ -460,11 +469,10 @@ def solve(
                             rejection_factor = 2.0
                         elif quality > 0.75:
                             damping = max(trust_region.lambda_min, damping / trust_region.lambda_factor)
-                    lin = residual = jacobian = original_jacobian = None
+                    residual = jacobian = original_jacobian = None
                     normal_operator = schur_linear = system = diagonal = gradient = linear_gradient = None
-                    with region("linearize"):
-                        lin = problem.linearize(vals)
-                    head = examine(lin)
+                    lin = lin_next
+                    head = (g_next, host[5] == 0, host[6], host[7])
                     summary.gradient_norm = head[2]
                     if termination.target_cost is not None and proposed_cost <= termination.target_cost:
                         summary.termination_reason = "target_cost"

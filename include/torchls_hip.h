@@ -204,6 +204,19 @@ typedef struct tlh_pcg_state {
   uint32_t* counter;
 } tlh_pcg_state_t;
 
+/* SE(3) between factors r_f = Log(Z_f^-1 T_i^-1 T_j) (pose graph; the logarithm of
+ * torchls_amd/geometry.py, quaternion branch) with their exact 6x6 blocks for right
+ * updates T <- T Exp(delta), tangent [translation; rotation], by forward-mode dual
+ * numbers; poses and measurements row-major 4x4 (stride 16) or 3x4 (12).  Writes
+ * residual [count][6], jac_i / jac_j [count][6][6] in dtype, out2[0] = sum ||r||^2 and
+ * out2[1] = count of nonfinite entries (float64).  partials: 2*TLH_PCG_MAX_BLOCKS doubles,
+ * counter: one zeroed uint32.
+ * Replaces: the vmapped jacrev linearization (linearization.py:62-128) of this residual. */
+int tlh_between_linearize(int dtype, const void* poses, int64_t pose_stride, const int64_t* idx_i,
+                          const int64_t* idx_j, const void* meas, int64_t meas_stride, int64_t count,
+                          void* residual, void* jac_i, void* jac_j, double* out2, double* partials,
+                          uint32_t* counter, tlh_stream_t stream);
+
 /* scalars[slot] = a . b accumulated in float64 (deterministic).
  * Replaces: torch.dot in warp_pcg.py:139,147-148 and solvers.py:198-246. */
 int tlh_dot(const void* a, const void* b, int64_t n, int dtype, const tlh_pcg_state_t* st,

@@ -41,3 +41,38 @@ def test_full_system_pcg_matches_dense_qr(device):
         R, t = _relative(T, pg.i, pg.j)
         Rq, tq = _relative(T_qr, pg.i, pg.j)
         assert np.abs(R - Rq).max() <= 1e-5 and np.abs(t - tq).max() <= 1e-5
+
+
+def _plain_between(pose_i, pose_j, measurement):
+    """The same residual without the HIP marker: linearized by torch.func (vmap of jacrev)."""
+    from torchls_amd.geometry import se3_inv, se3_log
+
+    return se3_log(se3_inv(measurement) @ se3_inv(pose_i) @ pose_j)
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-11), (torch.float32, 2e-5)])
+def test_hip_between_linearization_matches_autodiff(device, dtype, tol):
+    """tlh_between_linearize (dual numbers) against torch.func's Jacobian of the same logarithm,
+    at the noisy initial poses (loop-closure residuals of 0.1-1 rad) and at rotations near pi."""
+    from torchls_amd import Problem, SE3Var
+    from torchls_amd.generators import se3_exp_np
+    from torchls_amd.residuals import between
+
+    pg = generators.pose_graph(300, seed=3, loop_fraction=0.05, loop_radius=3.0)
+    rng = np.random.default_rng(4)
+    poses = pg.poses_init.copy()
+    poses[:20] = poses[:20] @ se3_exp_np(np.concatenate((rng.normal(0, 1, (20, 3)), rng.normal(0, 1.2, (20, 3))), -1))
+    lins = []
+    for fn in (between, _plain_between):
+        P = Problem({"poses": torch.tensor(poses, dtype=dtype, device=device)}, manifolds={"poses": SE3Var})
+        P.add_residual(fn, ("poses", torch.tensor(pg.i)), ("poses", torch.tensor(pg.j)),
+                       torch.tensor(pg.measurements, dtype=dtype, device=device))
+        lins.append(P.linearize())
+    (a, b) = lins
+    print(f"between linearization dtype={dtype}", flush=True)
+    assert float(a.nonfinite) == 0
+    ra, rb = a.residual.double().cpu(), b.residual.double().cpu()
+    assert (ra - rb).abs().max() <= tol * max(1.0, rb.abs().max())
+    for ga, gb in zip(a.jacobian.groups[0].blocks, b.jacobian.groups[0].blocks):
+        ga, gb = ga.double().cpu(), gb.double().cpu()
+        assert (ga - gb).norm() <= tol * gb.norm(), float((ga - gb).norm() / gb.norm())

@@ -150,6 +150,11 @@ SIGNATURES = {
         ctypes.c_int,
         [c_void_p, c_void_p, i64, ctypes.c_int, ctypes.POINTER(PcgStateDesc), ctypes.c_int, c_void_p],
     ),
+    "tlh_between_linearize": (
+        ctypes.c_int,
+        [ctypes.c_int, c_void_p, i64, c_void_p, c_void_p, c_void_p, i64, i64, c_void_p, c_void_p, c_void_p,
+         c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
     "tlh_pcg_begin": (
         ctypes.c_int,
         [ctypes.POINTER(PcgStateDesc), ctypes.c_double, ctypes.c_double, i32, c_void_p, c_void_p],

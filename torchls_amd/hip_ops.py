@@ -201,6 +201,25 @@ def reprojection_linearize(red: Reduction, slot: int, operands, idx, weights, wa
     return raw, model, (jc, jp), red.out[slot, 0]
 
 
+def between_linearize(red: Reduction, slot: int, operands, idx):
+    """HIP linearization of one SE(3) between-factor batch (tlh_between_linearize).
+
+    Returns (residual [N, 6], (J_i, J_j) [N, 6, 6], objective) with the float64 sum of
+    ||r||^2 as a 0-d view of ``red.out[slot, 0]`` (its nonfinite count in ``red.out[slot, 1]``)."""
+    poses, meas = operands
+    n = idx[0].numel()
+    dt, dev = poses.dtype, poses.device
+    res = torch.empty((n, 6), dtype=dt, device=dev)
+    ji = torch.empty((n, 6, 6), dtype=dt, device=dev)
+    jj = torch.empty((n, 6, 6), dtype=dt, device=dev)
+    lib = _lib.load()
+    _lib.check(lib.tlh_between_linearize(
+        _lib.dtype_code(dt), poses.data_ptr(), poses[0].numel(), idx[0].data_ptr(), idx[1].data_ptr(),
+        meas.data_ptr(), meas[0].numel(), n, res.data_ptr(), ji.data_ptr(), jj.data_ptr(), red.out[slot].data_ptr(),
+        red.partials.data_ptr(), red.counter.data_ptr(), _lib.stream_handle(dev)), "tlh_between_linearize")
+    return res, (ji, jj), red.out[slot, 0]
+
+
 def reprojection_trial(red: Reduction, slot: int, operands, idx, weights, before) -> None:
     """red.out[slot] = (sum w rho(||r_new||^2), trial reduction) (tlh_reprojection_trial)."""
     cams, pts, obs, delta, n, wptr = _reprojection_common(operands, idx, weights)

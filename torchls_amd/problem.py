@@ -164,6 +164,36 @@ class _Batch:
         delta = self.loss.delta if self.loss is not None else -1.0
         return cams.contiguous(), pts.contiguous(), obs, float(delta)
 
+    def hip_between_operands(self, values, layout, device, dtype):
+        """(poses, measurements) when the HIP between-factor linearization applies, else None.
+
+        It applies to ``residuals.between`` called as fn(("poses", i), ("poses", j),
+        measurements[N, 4, 4]) on one SE(3) group, with no custom Jacobian, loss or weights."""
+        if getattr(self.fn, "hip_residual", None) != "between" or self.jacobian_fn is not None:
+            return None
+        if self.slots != (0, 1) or len(self.args) != 3 or not self.batched:
+            return None
+        if self.in_dims is not None and tuple(self.in_dims) != (0, 0, 0):
+            return None
+        if self.loss is not None or self.weights is not None:
+            return None
+        from .geometry import SE3Var
+
+        ti, tj = self.type_indices
+        if layout[ti].var_type is not SE3Var or layout[tj].var_type is not SE3Var:
+            return None
+        if self.args[0].name != self.args[1].name:
+            return None
+        poses = values[self.args[0].name]
+        if poses.shape[1:] not in ((4, 4), (3, 4)):
+            return None
+        meas = self.args[2]
+        if not isinstance(meas, Tensor) or meas.shape[0] != self.count or meas.shape[1:] not in ((4, 4), (3, 4)):
+            return None
+        if meas.device != device or meas.dtype != dtype or not meas.is_contiguous():
+            meas = meas.to(device=device, dtype=dtype).contiguous()
+        return poses.contiguous(), meas
+
     def residuals(self, values, layout, device, dtype) -> Tensor:
         batched = getattr(self.fn, "batched_residual", None)
         chunk = self.batch_size if isinstance(self.batch_size, int) else None
@@ -300,12 +330,23 @@ class Analysis:
         return terms[0], terms[1]
 
     def linearize(self, values) -> Linearization:
-        from .hip_ops import reprojection_linearize
+        from .hip_ops import between_linearize, reprojection_linearize
 
         dev, dt = _values_device_dtype(values)
         groups, residuals, raws, objectives, bad = [], [], [], [], []
         for k, b in enumerate(self.batches):
             idx, cols, w = b.device_state(dev, self.layout)
+            btw = b.hip_between_operands(values, self.layout, dev, dt)
+            if btw is not None:
+                red = self._reduction(dev)
+                r, blocks, obj = between_linearize(red, k, btw, idx)
+                if self.has_loss:  # another batch has a loss: this one contributes r^2
+                    raws.append(r.reshape(-1))
+                    objectives.append(obj)
+                bad.append(red.out[k, 1])
+                residuals.append(r.reshape(-1))
+                groups.append(JacobianGroup(blocks, cols, b.type_indices, idx))
+                continue
             hip = b.hip_operands(values, self.layout, dev, dt)
             if hip is not None:
                 red = self._reduction(dev)

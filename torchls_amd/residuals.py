@@ -96,3 +96,6 @@ def _between_batched(pose_i: Tensor, pose_j: Tensor, measurement: Tensor) -> Ten
 
 
 between.batched_residual = _between_batched
+# with one SE(3) group, no loss and no weights Problem linearizes it with the HIP
+# kernel tlh_between_linearize (forward-mode dual numbers, exact 6x12 blocks)
+between.hip_residual = "between"

@@ -334,12 +334,15 @@ int tlh_schur_precond(const tlh_operator_t* op, int wgroup, int eslot, const dou
 /* Subtract W C^-1 W^T from a dense float64 reduced matrix M (leading dimension
  * ld, holding blockdiag(D + lambda I)): one thread per camera pair key
  * (key_ci <= key_cj) summing its observation pairs pair_i/pair_j
- * [pair_ptr[k], pair_ptr[k+1]) in order (deterministic, no atomics).
+ * [pair_ptr[k], pair_ptr[k+1]) in order (deterministic, no atomics).  work
+ * (float64 [count][2 * DC * DP], may be NULL) holds W_o and W_o C_p^-1 per
+ * observation for the 2x6 / 2x3 bundle-adjustment blocks, computed once instead
+ * of once per camera pair.
  * Replaces: DampedSchurSystem.dense_matrix (schur.py:380-397). */
 int tlh_schur_dense(const tlh_operator_t* op, int wgroup, int eslot, const double* cinv,
                     const int32_t* pair_ptr, const int32_t* pair_i, const int32_t* pair_j,
                     const int32_t* key_ci, const int32_t* key_cj, int64_t npairs, double* M,
-                    int64_t ld, tlh_stream_t stream);
+                    int64_t ld, double* work, tlh_stream_t stream);
 
 /* Finish for a subset of types: y[t] = sum of partials + base_scale * base[t] for
  * t in [type_begin, type_end) (retained types of a Schur plan).  x-like vectors

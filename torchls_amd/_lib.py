@@ -216,7 +216,7 @@ SIGNATURES = {
     "tlh_schur_dense": (
         ctypes.c_int,
         [ctypes.POINTER(Operator), ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-         c_void_p, i64, c_void_p, i64, c_void_p],
+         c_void_p, i64, c_void_p, i64, c_void_p, c_void_p],
     ),
     "tlh_finish_types": (
         ctypes.c_int,

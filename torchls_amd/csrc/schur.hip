@@ -559,6 +559,100 @@ __global__ void __launch_bounds__(TLH_THREADS)
     }
 }
 
+// Fixed-shape dense path in two passes: schur_edge_kernel writes, per
+// observation, W_o = J_c^T J_p (DC x DP) and U_o = W_o C_p^-1 (float64);
+// schur_dense_pairs then sums U_i W_j^T over each camera pair's observation
+// pairs (the same products, in the same order, as schur_dense_kernel, without
+// re-reading and re-multiplying the blocks of an observation for every pair),
+// one wavefront per camera pair.
+template <typename Real, int R, int DC, int DP>
+__global__ void __launch_bounds__(TLH_THREADS)
+    schur_edge_kernel(SchurArgs<Real> a, int cfirst, int64_t count, double* __restrict__ work) {
+  const int64_t n = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
+  if (n >= count) return;
+  const int OC = cfirst ? 0 : R * DP;
+  const int OP = cfirst ? R * DC : 0;
+  double w[DC][DP];
+#pragma unroll
+  for (int c = 0; c < DC; ++c)
+#pragma unroll
+    for (int p = 0; p < DP; ++p) {
+      double s = 0.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        s += (double)a.packed[(int64_t)(OC + r * DC + c) * a.stride + n] *
+             (double)a.packed[(int64_t)(OP + r * DP + p) * a.stride + n];
+      w[c][p] = s;
+    }
+  const double* m = a.cinv + (int64_t)a.idx_p[n] * (DP * (DP + 1) / 2);
+  double* out = work + n * (2 * DC * DP);
+#pragma unroll
+  for (int c = 0; c < DC; ++c) {
+#pragma unroll
+    for (int p = 0; p < DP; ++p) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < DP; ++q) s += w[c][q] * sym<DP>(m, q, p);
+      out[c * DP + p] = w[c][p];
+      out[DC * DP + c * DP + p] = s;
+    }
+  }
+}
+
+template <int DC, int DP>
+__global__ void __launch_bounds__(TLH_THREADS)
+    schur_dense_pairs(const double* __restrict__ work, const int32_t* __restrict__ pair_ptr,
+                      const int32_t* __restrict__ pair_i, const int32_t* __restrict__ pair_j,
+                      const int32_t* __restrict__ pair_key_ci, const int32_t* __restrict__ pair_key_cj,
+                      int64_t npairs, double* __restrict__ M, int64_t ld) {
+  // one wavefront per camera pair: lane l sums entries l, l + 64, ... (a camera's
+  // own pair lists every observation of it), then a fixed xor tree adds the lanes
+  const int64_t key = ((int64_t)blockIdx.x * TLH_THREADS + threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
+  if (key >= npairs) return;
+  double acc[DC][DC];
+#pragma unroll
+  for (int i = 0; i < DC; ++i)
+#pragma unroll
+    for (int j = 0; j < DC; ++j) acc[i][j] = 0.0;
+  for (int e = pair_ptr[key] + lane; e < pair_ptr[key + 1]; e += 64) {
+    const double* ui = work + (int64_t)pair_i[e] * (2 * DC * DP) + DC * DP;
+    const double* wj = work + (int64_t)pair_j[e] * (2 * DC * DP);
+    double u[DC][DP], v[DC][DP];
+#pragma unroll
+    for (int c = 0; c < DC; ++c)
+#pragma unroll
+      for (int p = 0; p < DP; ++p) {
+        u[c][p] = ui[c * DP + p];
+        v[c][p] = wj[c * DP + p];
+      }
+#pragma unroll
+    for (int c = 0; c < DC; ++c)
+#pragma unroll
+      for (int d = 0; d < DC; ++d) {
+        double s = 0.0;
+#pragma unroll
+        for (int p = 0; p < DP; ++p) s += u[c][p] * v[d][p];
+        acc[c][d] += s;
+      }
+  }
+#pragma unroll
+  for (int c = 0; c < DC; ++c)
+#pragma unroll
+    for (int d = 0; d < DC; ++d)
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) acc[c][d] += __shfl_xor(acc[c][d], off);
+  if (lane != 0) return;
+  const int64_t ci = pair_key_ci[key], cj = pair_key_cj[key];
+#pragma unroll
+  for (int c = 0; c < DC; ++c)
+#pragma unroll
+    for (int d = 0; d < DC; ++d) {
+      M[(ci * DC + c) * ld + cj * DC + d] -= acc[c][d];
+      if (ci != cj) M[(cj * DC + d) * ld + ci * DC + c] -= acc[c][d];
+    }
+}
+
 // ------------------------------------------------------------------ host side
 template <typename Real>
 SchurArgs<Real> make_schur_args(const tlh_operator_t* op, const tlh_group_t& g, int sc, int sp, const void* xc,
@@ -747,7 +841,8 @@ extern "C" int tlh_schur_precond(const tlh_operator_t* op, int wgroup, int eslot
 
 extern "C" int tlh_schur_dense(const tlh_operator_t* op, int wgroup, int eslot, const double* cinv, const int32_t* pair_ptr,
                                const int32_t* pair_i, const int32_t* pair_j, const int32_t* key_ci,
-                               const int32_t* key_cj, int64_t npairs, double* M, int64_t ld, tlh_stream_t stream) {
+                               const int32_t* key_cj, int64_t npairs, double* M, int64_t ld, double* work,
+                               tlh_stream_t stream) {
   if (!op || wgroup < 0 || wgroup >= op->ngroups || !cinv || !M) return TLH_EINVAL;
   const tlh_group_t& g = op->groups[wgroup];
   if (g.nslots != 2 || eslot < 0 || eslot > 1 || g.slot[eslot].rank != nullptr) return TLH_EINVAL;
@@ -758,6 +853,22 @@ extern "C" int tlh_schur_dense(const tlh_operator_t* op, int wgroup, int eslot, 
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const dim3 grid((unsigned)ceil_div(npairs, TLH_THREADS)), block(TLH_THREADS);
   const bool ba = R == 2 && DC == 6 && DP == 3;
+  if (ba && work) {
+    const dim3 egrid((unsigned)ceil_div(g.count, TLH_THREADS));
+    if (op->dtype == TLH_F32) {
+      SchurArgs<float> a = make_schur_args<float>(op, g, sc, se, nullptr, nullptr, nullptr, cinv);
+      hipLaunchKernelGGL((schur_edge_kernel<float, 2, 6, 3>), egrid, block, 0, s, a, (int)(sc < se), g.count, work);
+    } else {
+      SchurArgs<double> a = make_schur_args<double>(op, g, sc, se, nullptr, nullptr, nullptr, cinv);
+      hipLaunchKernelGGL((schur_edge_kernel<double, 2, 6, 3>), egrid, block, 0, s, a, (int)(sc < se), g.count, work);
+    }
+    TLH_RETURN_IF_LAUNCH_FAILED();
+    const dim3 wgrid((unsigned)ceil_div(npairs, TLH_THREADS / 64));
+    hipLaunchKernelGGL((schur_dense_pairs<6, 3>), wgrid, block, 0, s, work, pair_ptr, pair_i, pair_j, key_ci, key_cj,
+                       npairs, M, ld);
+    TLH_RETURN_IF_LAUNCH_FAILED();
+    return 0;
+  }
   if (op->dtype == TLH_F32) {
     SchurArgs<float> a = make_schur_args<float>(op, g, sc, se, nullptr, nullptr, nullptr, cinv);
     auto kernel = ba ? schur_dense_kernel<float, 2, 6, 3> : schur_dense_kernel<float, 0, 0, 0>;

@@ -334,10 +334,14 @@ class HipSchurSystem:
             blk = self.lin.diagonal[t] + self.damping * torch.eye(d, dtype=torch.float64, device=p.device)
             idx = off + torch.arange(c, device=p.device)[:, None] * d + torch.arange(d, device=p.device)
             M[idx[:, :, None], idx[:, None, :]] = blk
+        wg = p.plan.groups[p.wgroup]
+        # per-observation W_o and W_o C_p^-1 (float64), read by the camera-pair sums
+        work = torch.empty(max(1, wg.count * 2 * wg.slots[1 - p.eslot].dim * p.edim), dtype=torch.float64,
+                           device=p.device)
         _lib.check(p.lib.tlh_schur_dense(ctypes.byref(self.lin.op), p.wgroup, p.eslot, self.cinv.data_ptr(),
                                          pairs["ptr"].data_ptr(), pairs["i"].data_ptr(), pairs["j"].data_ptr(),
                                          pairs["ci"].data_ptr(), pairs["cj"].data_ptr(), pairs["n"], M.data_ptr(), n,
-                                         _lib.stream_handle(p.device)), "tlh_schur_dense")
+                                         work.data_ptr(), _lib.stream_handle(p.device)), "tlh_schur_dense")
         return M
 
     def dense_solve(self, rhs: torch.Tensor) -> torch.Tensor:

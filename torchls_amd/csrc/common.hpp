@@ -106,6 +106,56 @@ __device__ __forceinline__ void upper_index(int k, int d, int& i, int& j) {
   j = i + k;
 }
 
+// Cross-lane moves by DPP (data-parallel primitives: register reads of another
+// lane, no LDS traffic).  A lane whose source is outside the pattern (or whose
+// row is masked off) receives 0.
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, DPP_ROW_SHR8 = 0x118;
+constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_int(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_real(float x) {
+  return __int_as_float(dpp_int<CTRL, ROWS>(__float_as_int(x)));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_real(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_int<CTRL, ROWS>((int)(b & 0xffffffffll)), hi = dpp_int<CTRL, ROWS>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// One step of the inclusive segmented scan: (f, v) of the source lane is
+// combined into this lane's unless this lane starts a segment (flag set).
+template <int CTRL, int ROWS, typename Real, int D>
+__device__ __forceinline__ void segmented_step(int& flag, Real (&v)[D]) {
+  const int fo = dpp_int<CTRL, ROWS>(flag);
+  Real vo[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) vo[c] = dpp_real<CTRL, ROWS>(v[c]);
+  if (!flag) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) v[c] = vo[c] + v[c];
+  }
+  flag |= fo;
+}
+
+// Inclusive segmented scan over the 64 lanes of a wavefront, in a fixed order:
+// Hillis-Steele within each row of 16 lanes (shifts 1, 2, 4, 8), then row 0's
+// total into row 1 and row 2's into row 3 (broadcast of lane 15 of the row
+// before), then lanes 0-31's total into rows 2 and 3 (broadcast of lane 31).
+template <typename Real, int D>
+__device__ __forceinline__ void wave_segmented_scan(int& flag, Real (&v)[D]) {
+  segmented_step<DPP_ROW_SHR1, 0xf>(flag, v);
+  segmented_step<DPP_ROW_SHR2, 0xf>(flag, v);
+  segmented_step<DPP_ROW_SHR4, 0xf>(flag, v);
+  segmented_step<DPP_ROW_SHR8, 0xf>(flag, v);
+  segmented_step<DPP_ROW_BCAST15, 0xa>(flag, v);
+  segmented_step<DPP_ROW_BCAST31, 0xc>(flag, v);
+}
+
 // True when a type's partial lists are long enough for one wavefront per
 // (variable, component); such kernels take count * W wavefronts.
 inline bool csr_heavy(int64_t csr_entries, int64_t variables) {

@@ -60,22 +60,29 @@ __device__ __forceinline__ double sym(const double* m, int i, int j) {
   return m[a * D - a * (a - 1) / 2 + (b - a)];
 }
 
-// XL: the retained x slice is copied into LDS once per workgroup (after the
-// accumulator), so the per-observation camera gathers are LDS reads.  Each
-// tile's point owners load their point's C_p^-1 (and b_p) at the start of the
-// tile, so those dependent loads overlap the block loads instead of stalling
-// the point phase.
+// One observation per thread.  The point phase is a segmented sum over the
+// tile (sorted by point): each wavefront scans its 64 observations with DPP
+// (flag = first observation of a point), lane 63 publishes the wave's tail,
+// and a lane whose point began in an earlier wavefront adds the tails of the
+// waves before it, nearest first, up to the wave where the point starts.  The
+// point's last observation then holds t_p = sum J_p^T u and forms
+// s_p = C_p^-1 (...) in float64; the scan also carries the point's first
+// position, where s_p is published for the point's other observations.  XL:
+// the retained x slice is copied into LDS once per workgroup (after the
+// accumulator), so the per-observation camera gathers are LDS reads.
 template <typename Real, int R, int DC, int DP, bool CFIRST, int ITEMS, int MODE, int NTH, bool XL>
 __global__ void __launch_bounds__(NTH) schur_tile_kernel(SchurArgs<Real> a) {
   constexpr int OC = CFIRST ? 0 : R * DP;
   constexpr int OP = CFIRST ? R * DC : 0;
   constexpr int T = NTH * ITEMS;
-  constexpr int DMAX = DC > DP ? DC : DP;
   constexpr int NM = DP * (DP + 1) / 2;
-  static_assert(ITEMS == 1, "one point segment per thread");
+  constexpr int NW = NTH / 64;
+  static_assert(ITEMS == 1, "one observation per thread");
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  Real* stage = reinterpret_cast<Real*>(smem_raw);
-  Real* sstage = stage + T * DMAX;
+  double* tail = reinterpret_cast<double*>(smem_raw);  // [NW][DP + 1]
+  int* tail_flag = reinterpret_cast<int*>(tail + NW * (DP + 1));
+  Real* stage = reinterpret_cast<Real*>(tail_flag + NW * 2);
+  Real* sstage = stage + T * DC;
   Real* acc = sstage + T * DP;
   Real* xl = acc + (int64_t)a.count_c * DC;
   const bool dense = MODE != SCHUR_BACKSUB && a.mode_c == TLH_REDUCE_DENSE;
@@ -83,147 +90,134 @@ __global__ void __launch_bounds__(NTH) schur_tile_kernel(SchurArgs<Real> a) {
     for (int i = threadIdx.x; i < a.count_c * DC; i += NTH) acc[i] = Real(0);
   if constexpr (XL) {
     for (int i = threadIdx.x; i < a.count_c * DC; i += NTH) xl[i] = a.xc[i];
-    __syncthreads();
   }
+  if (dense || XL) __syncthreads();
 
+  const int pos = (int)threadIdx.x;
+  const int wave = pos >> 6, lane = pos & 63;
   const int t0 = blockIdx.x * a.tiles_per_wg;
   const int t1 = min(t0 + a.tiles_per_wg, a.ntiles);
   for (int t = t0; t < t1; ++t) {
     const int start = a.tile_start[t];
     const int len = a.tile_start[t + 1] - start;
-    // point segment k = sb + thread: its range, variable, C_p^-1 and b_p, loaded early
-    const int sb = a.seg_ptr_p[t], se = a.seg_ptr_p[t + 1];
-    const int kseg = sb + (int)threadIdx.x;
-    const bool owner = kseg < se;
-    int lo = 0, hi = 0;
-    int64_t v = 0;
-    double mk[NM], bpv[DP];
-    if (owner) {
-      lo = a.seg_start_p[kseg];
-      hi = (kseg + 1 < se) ? a.seg_start_p[kseg + 1] : len;
-      v = a.idx_p[start + lo];
+    const bool valid = pos < len;
+    const int64_t n = start + min(pos, len - 1);
+    Real jc[R][DC], jp[R][DP];
 #pragma unroll
-      for (int e = 0; e < NM; ++e) mk[e] = a.cinv[v * NM + e];
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+      for (int c = 0; c < DC; ++c) jc[r][c] = a.packed[(int64_t)(OC + r * DC + c) * a.stride + n];
+#pragma unroll
+      for (int c = 0; c < DP; ++c) jp[r][c] = a.packed[(int64_t)(OP + r * DP + c) * a.stride + n];
+    }
+    const int rk = a.rank_c ? (int)a.rank_c[n] : min(pos, len - 1);
+    // point boundaries; a lane past the tile's end is a segment of its own
+    const int vp = a.idx_p[n];
+    const bool first = !valid || pos == 0 || a.idx_p[n - 1] != vp;
+    const bool last = valid && (pos == len - 1 || a.idx_p[n + 1] != vp);
+    double mk[NM], bpv[DP];
+    if (last) {
+#pragma unroll
+      for (int e = 0; e < NM; ++e) mk[e] = a.cinv[(int64_t)vp * NM + e];
       if (MODE != SCHUR_MATVEC) {
 #pragma unroll
-        for (int c = 0; c < DP; ++c) bpv[c] = (double)a.bp[v * DP + c];
+        for (int c = 0; c < DP; ++c) bpv[c] = (double)a.bp[(int64_t)vp * DP + c];
       }
     }
-    Real jc[ITEMS][R][DC], jp[ITEMS][R][DP], u[ITEMS][R];
-    int rk[ITEMS];
+    Real u[R];
+    if (MODE != SCHUR_RHS) {
+      const int ic = a.idx_c[n];
+      const Real* x = XL ? xl + ic * DC : a.xc + (int64_t)ic * DC;
+      Real xv[DC];
 #pragma unroll
-    for (int it = 0; it < ITEMS; ++it) {
-      const int pos = (int)threadIdx.x + it * NTH;
-      const int64_t n = start + min(pos, len - 1);
+      for (int c = 0; c < DC; ++c) xv[c] = x[c];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
+        Real s = Real(0);
 #pragma unroll
-        for (int c = 0; c < DC; ++c) jc[it][r][c] = a.packed[(int64_t)(OC + r * DC + c) * a.stride + n];
-#pragma unroll
-        for (int c = 0; c < DP; ++c) jp[it][r][c] = a.packed[(int64_t)(OP + r * DP + c) * a.stride + n];
+        for (int c = 0; c < DC; ++c) s += jc[r][c] * xv[c];
+        u[r] = s;
       }
-      rk[it] = a.rank_c ? (int)a.rank_c[n] : min(pos, len - 1);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) u[r] = Real(0);
+    }
+    // w = (J_p^T u, first position of the point), summed per point in float64
+    double w[DP + 1];
+#pragma unroll
+    for (int c = 0; c < DP; ++c) {
+      Real s = Real(0);
       if (MODE != SCHUR_RHS) {
-        const Real* x = XL ? xl + a.idx_c[n] * DC : a.xc + (int64_t)a.idx_c[n] * DC;
-        Real xv[DC];
 #pragma unroll
-        for (int c = 0; c < DC; ++c) xv[c] = x[c];
+        for (int r = 0; r < R; ++r) s += jp[r][c] * u[r];
+      }
+      w[c] = valid ? (double)s : 0.0;
+    }
+    w[DP] = first ? (double)pos : 0.0;
+    int flag = first ? 1 : 0;
+    wave_segmented_scan<double, DP + 1>(flag, w);
+    if (lane == 63) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          Real s = Real(0);
+      for (int c = 0; c <= DP; ++c) tail[wave * (DP + 1) + c] = w[c];
+      tail_flag[wave] = flag;
+    }
+    __syncthreads();
+    if (!flag) {
+      // the point began in an earlier wavefront (wave 0 always holds a start)
+      for (int k = wave - 1; k >= 0; --k) {
 #pragma unroll
-          for (int c = 0; c < DC; ++c) s += jc[it][r][c] * xv[c];
-          u[it][r] = s;
-        }
+        for (int c = 0; c <= DP; ++c) w[c] += tail[k * (DP + 1) + c];
+        if (tail_flag[k]) break;
+      }
+    }
+    const int fp = (int)w[DP];
+    if (last) {
+      double tp[DP];
+#pragma unroll
+      for (int c = 0; c < DP; ++c) {
+        if (MODE == SCHUR_MATVEC) tp[c] = w[c];
+        if (MODE == SCHUR_RHS) tp[c] = bpv[c];
+        if (MODE == SCHUR_BACKSUB) tp[c] = bpv[c] - w[c];  // b_p - W^T x
+      }
+      double sp[DP];
+#pragma unroll
+      for (int i = 0; i < DP; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < DP; ++j) s += sym<DP>(mk, i, j) * tp[j];
+        sp[i] = s;
+      }
+      if (MODE == SCHUR_BACKSUB) {
+#pragma unroll
+        for (int c = 0; c < DP; ++c) a.outp[(int64_t)vp * DP + c] = (Real)sp[c];
       } else {
 #pragma unroll
-        for (int r = 0; r < R; ++r) u[it][r] = Real(0);
+        for (int c = 0; c < DP; ++c) sstage[fp * DP + c] = (Real)sp[c];
       }
     }
-    // point contributions J_p^T u (identity order: the tile is sorted by point)
-    __syncthreads();
-    if (MODE != SCHUR_RHS) {
-#pragma unroll
-      for (int it = 0; it < ITEMS; ++it) {
-        const int pos = (int)threadIdx.x + it * NTH;
-        if (pos < len) {
-#pragma unroll
-          for (int c = 0; c < DP; ++c) {
-            Real s = Real(0);
-#pragma unroll
-            for (int r = 0; r < R; ++r) s += jp[it][r][c] * u[it][r];
-            stage[pos * DP + c] = s;
-          }
-        }
-      }
+    if (MODE == SCHUR_BACKSUB) {
+      __syncthreads();  // the tails are rewritten by the next tile
+      continue;
     }
     __syncthreads();
-    // one owner per point: s_p = C_p^-1 (...) in float64
-    if (owner) {
-      {
-        double tp[DP];
-#pragma unroll
-        for (int c = 0; c < DP; ++c) tp[c] = 0.0;
-        if (MODE != SCHUR_RHS) {
-          for (int i = lo; i < hi; ++i)
-#pragma unroll
-            for (int c = 0; c < DP; ++c) tp[c] += (double)stage[i * DP + c];
-        }
-        if (MODE != SCHUR_MATVEC) {
-#pragma unroll
-          for (int c = 0; c < DP; ++c) tp[c] = bpv[c] - tp[c];
-          // RHS: tp = b_p; BACKSUB: tp = b_p - W^T x
-        }
-        double sp[DP];
-#pragma unroll
-        for (int i = 0; i < DP; ++i) {
-          double s = 0.0;
-#pragma unroll
-          for (int j = 0; j < DP; ++j) s += sym<DP>(mk, i, j) * tp[j];
-          sp[i] = s;
-        }
-        if (MODE == SCHUR_BACKSUB) {
-#pragma unroll
-          for (int c = 0; c < DP; ++c) a.outp[v * DP + c] = (Real)sp[c];
-        } else {
-          for (int i = lo; i < hi; ++i)
-#pragma unroll
-            for (int c = 0; c < DP; ++c) sstage[i * DP + c] = (Real)sp[c];
-        }
-      }
-    }
-    if (MODE == SCHUR_BACKSUB) continue;
-    __syncthreads();
-    Real cc[ITEMS][DC];
-#pragma unroll
-    for (int it = 0; it < ITEMS; ++it) {
-      const int pos = min((int)threadIdx.x + it * NTH, len - 1);
+    if (valid) {
       Real sv[DP];
 #pragma unroll
-      for (int c = 0; c < DP; ++c) sv[c] = sstage[pos * DP + c];
+      for (int c = 0; c < DP; ++c) sv[c] = sstage[fp * DP + c];
       Real q[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        Real s = u[it][r];
+        Real s = u[r];
 #pragma unroll
-        for (int c = 0; c < DP; ++c) s -= jp[it][r][c] * sv[c];
+        for (int c = 0; c < DP; ++c) s -= jp[r][c] * sv[c];
         q[r] = s;
       }
 #pragma unroll
       for (int c = 0; c < DC; ++c) {
         Real s = Real(0);
 #pragma unroll
-        for (int r = 0; r < R; ++r) s += jc[it][r][c] * q[r];
-        cc[it][c] = s;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int it = 0; it < ITEMS; ++it) {
-      const int pos = (int)threadIdx.x + it * NTH;
-      if (pos < len) {
-#pragma unroll
-        for (int c = 0; c < DC; ++c) stage[rk[it] * DC + c] = cc[it][c];
+        for (int r = 0; r < R; ++r) s += jc[r][c] * q[r];
+        stage[rk * DC + c] = s;
       }
     }
     __syncthreads();
@@ -700,7 +694,9 @@ int launch_schur_t(const tlh_operator_t* op, const tlh_group_t& g, int sc, int s
   constexpr int NTH = schur_threads<Real>();
   if (g.tile_cap > NTH * ITEMS) return TLH_EINVAL;
   constexpr int T = NTH * ITEMS;
-  int64_t lds = (int64_t)T * ((DC > DP ? DC : DP) + DP) * sizeof(Real);
+  // wave tails (float64 sums and flags), camera stage [T][DC], point stage [T][DP]
+  int64_t lds = (int64_t)(NTH / 64) * ((DP + 1) * sizeof(double) + 2 * sizeof(int)) +
+                (int64_t)T * (DC + DP) * sizeof(Real);
   const int64_t slice = op->types[g.slot[sc].type].count * DC * (int64_t)sizeof(Real);
   const bool dense = MODE != SCHUR_BACKSUB && g.slot[sc].mode == TLH_REDUCE_DENSE;
   if (dense) lds += slice;

@@ -105,7 +105,9 @@ class HipSchurPlan:
             self.rows[k] = rows
             groups.append((rows, ltypes, self._plan_groups[k][2]))
         eprim = {self.wgroup: self.eslot}
-        stage_extra = 256 * (4 if self._esize == 4 else 2) * self.edim * self._esize
+        # beyond the camera stage the plan reserves: the point stage and the wave tails
+        threads = 1024 if self._esize == 4 else 512
+        stage_extra = threads * self.edim * self._esize + threads // 64 * ((self.edim + 1) * 8 + 8)
         # the W-group kernel's workgroups (1,024 / 512 threads, csrc/schur.hip) fill a compute unit
         self.plan = build_structure(self.rlayout, groups, esize=self._esize,
                                     lds_limit=int(self._info.lds_per_block) - stage_extra,

@@ -267,15 +267,27 @@ __device__ __forceinline__ void load_items(const T* __restrict__ base, int64_t s
   }
 }
 
-// Raw per-thread data of one tile: ids, ranks and block components.  Ranks stay
-// packed as loaded (two 16-bit ranks per word for even ITEMS) and are unpacked
-// only where they are used, after the block loads are in flight.
-template <typename Real, int R, int DP, int DS, int ITEMS>
+// Ids and ranks of one tile, loaded two tiles ahead of its reduction so that
+// the gathers they address can be issued one tile ahead.  Ranks stay packed as
+// loaded (two 16-bit ranks per word for even ITEMS) and are unpacked only
+// where they are used.
+template <int ITEMS>
+struct WaveIds {
+  static constexpr int RW = (ITEMS + 1) / 2;
+  int ip[ITEMS], is[ITEMS];
+  uint32_t rk[RW];
+};
+
+// Raw per-thread data of one tile: ids, ranks, block components and the
+// gathered primary x (and secondary x when it is not held in LDS).
+template <typename Real, int R, int DP, int DS, int ITEMS, bool XL>
 struct WaveTile {
   static constexpr int RW = (ITEMS + 1) / 2;
   int ip[ITEMS], is[ITEMS];
   uint32_t rk[RW];
   Real jp[R][DP][ITEMS], js[R][DS][ITEMS];
+  Real xp[ITEMS][DP], xs[ITEMS][XL ? 1 : DS];
+  int direct[ITEMS];
   __device__ __forceinline__ int rank(int k) const {
     return ITEMS == 1 ? (int)rk[0] : (int)((k & 1) ? (rk[k >> 1] >> 16) : (rk[k >> 1] & 0xffffu));
   }
@@ -286,9 +298,8 @@ struct WaveTile {
 // bounds branch; positions past the group's count are masked by the caller.
 // Ranks are read as single 16-bit loads (zero-extended, no unpacking) so that
 // nothing forces a wait between the id loads and the block loads.
-template <typename Real, int R, int DP, int DS, int OP, int OS, int ITEMS, int NT, bool STREAM>
-__device__ __forceinline__ void load_wave_tile(const WaveArgs<Real>& a, int t, int b,
-                                               WaveTile<Real, R, DP, DS, ITEMS>& d) {
+template <typename Real, int ITEMS, int NT>
+__device__ __forceinline__ void load_wave_ids(const WaveArgs<Real>& a, int t, int b, WaveIds<ITEMS>& d) {
   constexpr int T = NT * ITEMS;
   const int64_t start = (int64_t)t * T;
   load_items<int32_t, ITEMS>(a.idx_p, start, b, T, true, d.ip);
@@ -309,6 +320,21 @@ __device__ __forceinline__ void load_wave_tile(const WaveArgs<Real>& a, int t, i
       for (int w = 0; w < ITEMS / 2; ++w) d.rk[w] = (uint32_t)(b + 2 * w) | ((uint32_t)(b + 2 * w + 1) << 16);
     }
   }
+}
+
+// Block loads of tile t and the x gathers addressed by its (already loaded) ids.
+template <typename Real, int R, int DP, int DS, int OP, int OS, int ITEMS, int NT, bool STREAM, bool XL>
+__device__ __forceinline__ void load_wave_tile(const WaveArgs<Real>& a, int t, int b, const WaveIds<ITEMS>& ids,
+                                               WaveTile<Real, R, DP, DS, ITEMS, XL>& d) {
+  constexpr int T = NT * ITEMS;
+  const int64_t start = (int64_t)t * T;
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    d.ip[k] = ids.ip[k];
+    d.is[k] = ids.is[k];
+  }
+#pragma unroll
+  for (int w = 0; w < WaveIds<ITEMS>::RW; ++w) d.rk[w] = ids.rk[w];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
 #pragma unroll
@@ -319,6 +345,16 @@ __device__ __forceinline__ void load_wave_tile(const WaveArgs<Real>& a, int t, i
     for (int c = 0; c < DS; ++c)
       load_items<Real, ITEMS, STREAM>(a.packed + (int64_t)(OS + r * DS + c) * a.stride, start, b, T, true,
                                       d.js[r][c]);
+  }
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+#pragma unroll
+    for (int c = 0; c < DP; ++c) d.xp[k][c] = a.x_p[(int64_t)ids.ip[k] * DP + c];
+    if constexpr (!XL) {
+#pragma unroll
+      for (int c = 0; c < DS; ++c) d.xs[k][c] = a.x_s[(int64_t)ids.is[k] * DS + c];
+    }
+    d.direct[k] = a.direct_p ? (int)a.direct_p[ids.ip[k]] : 0;
   }
 }
 
@@ -372,31 +408,21 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
   // One tile: gathers and contributions from `cur`, then `prefetch` (which issues
   // the following tile's loads into the other register buffer), then the
   // primary scan and the secondary LDS reduction.
-  auto process = [&](const WaveTile<Real, R, DP, DS, ITEMS>& cur, int t, auto&& prefetch) {
+  auto process = [&](const WaveTile<Real, R, DP, DS, ITEMS, XL>& cur, int t, auto&& prefetch) {
     const int64_t start = (int64_t)t * T;
     const int len = (int)min((int64_t)T, a.count - start);
     const int left = len - b;  // items of this thread inside the tile (may be <= 0)
-    int ip[ITEMS], is[ITEMS];
+    int ip[ITEMS], is[ITEMS], direct[ITEMS];
+    Real xp[ITEMS][DP], xs[ITEMS][DS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       ip[k] = cur.ip[k];
       is[k] = cur.is[k];
-    }
-    Real xp[ITEMS][DP], xs[ITEMS][DS];
+      direct[k] = cur.direct[k];
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
+      for (int c = 0; c < DP; ++c) xp[k][c] = cur.xp[k][c];
 #pragma unroll
-      for (int c = 0; c < DP; ++c) xp[k][c] = a.x_p[(int64_t)ip[k] * DP + c];
-#pragma unroll
-      for (int c = 0; c < DS; ++c) xs[k][c] = XL ? xl[is[k] * DS + c] : a.x_s[(int64_t)is[k] * DS + c];
-    }
-    int direct[ITEMS];
-    if (a.direct_p) {
-#pragma unroll
-      for (int k = 0; k < ITEMS; ++k) direct[k] = a.direct_p[ip[k]];
-    } else {
-#pragma unroll
-      for (int k = 0; k < ITEMS; ++k) direct[k] = 0;
+      for (int c = 0; c < DS; ++c) xs[k][c] = XL ? xl[is[k] * DS + c] : cur.xs[k][XL ? 0 : c];
     }
     // contributions J_p^T q and J_s^T q with q = J_p x_p + J_s x_s
     Real cp[ITEMS][DP], cs[ITEMS][DS];
@@ -533,20 +559,32 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
     __syncthreads();
   };
   // Two register buffers used alternately (no copies between them), so the loads
-  // of tile t+1 are in flight while tile t is reduced through LDS.
-  WaveTile<Real, R, DP, DS, ITEMS> ta, tb;
-  if (t0 < t1) load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM>(a, t0, b, ta);
+  // of tile t+1 are in flight while tile t is reduced through LDS; the ids of
+  // tile t+2 are loaded behind them.
+  WaveIds<ITEMS> idn;
+  WaveTile<Real, R, DP, DS, ITEMS, XL> ta, tb;
+  if (t0 < t1) {
+    load_wave_ids<Real, ITEMS, NT>(a, t0, b, idn);
+    load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, t0, b, idn, ta);
+    if (t0 + 1 < t1) load_wave_ids<Real, ITEMS, NT>(a, t0 + 1, b, idn);
+  }
   if constexpr (XL) {
     for (int i = threadIdx.x; i < a.count_s * DS; i += NT) xl[i] = a.x_s[i];
     __syncthreads();
   }
   for (int t = t0; t < t1; t += 2) {
     process(ta, t, [&] {
-      if (t + 1 < t1) load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM>(a, t + 1, b, tb);
+      if (t + 1 < t1) {
+        load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, t + 1, b, idn, tb);
+        if (t + 2 < t1) load_wave_ids<Real, ITEMS, NT>(a, t + 2, b, idn);
+      }
     });
     if (t + 1 < t1)
       process(tb, t + 1, [&] {
-        if (t + 2 < t1) load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM>(a, t + 2, b, ta);
+        if (t + 2 < t1) {
+          load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, t + 2, b, idn, ta);
+          if (t + 3 < t1) load_wave_ids<Real, ITEMS, NT>(a, t + 3, b, idn);
+        }
       });
   }
   if (dense) {

@@ -665,36 +665,39 @@ __global__ void __launch_bounds__(TLH_THREADS)
   y[i] = sum + damping * x[i];
 }
 
-// Finish for types with many dense blocks: a workgroup owns 32 consecutive
-// columns; its 8 column lanes each sum every 8th dense block (fixed stride),
-// then the 8 lane sums are added in lane order.  Same result for every run.
+// Finish for types with many dense blocks: a workgroup owns COLS consecutive
+// columns; its 256 / COLS column lanes each sum every (256 / COLS)-th dense
+// block (fixed stride), then the lane sums are added in lane order.  Same
+// result for every run.  Few columns (a small camera set) take COLS = 8 so the
+// grid still spreads over the chip and each lane sums only a few blocks.
 constexpr int FIN_COLS = 32;
-constexpr int FIN_LANES = TLH_THREADS / FIN_COLS;
+constexpr int FIN_COLS_NARROW = 8;
 
-template <typename Real>
+template <typename Real, int COLS>
 __global__ void __launch_bounds__(TLH_THREADS)
     finish_dense_kernel(Real* __restrict__ y, const Real* __restrict__ x, Real damping,
                         const Real* __restrict__ partials, const int32_t* __restrict__ csr_ptr,
                         const int32_t* __restrict__ csr_idx, int64_t count, int dim, int dense_blocks,
                         const int32_t* __restrict__ list, int64_t nlist) {
-  __shared__ Real lane_sum[FIN_LANES][FIN_COLS];
+  constexpr int LANES = TLH_THREADS / COLS;
+  __shared__ Real lane_sum[LANES][COLS];
   (void)list;
   (void)nlist;
-  const int col = threadIdx.x % FIN_COLS;
-  const int lane = threadIdx.x / FIN_COLS;
+  const int col = threadIdx.x % COLS;
+  const int lane = threadIdx.x / COLS;
   const int64_t total = count * dim;
-  const int64_t i = (int64_t)blockIdx.x * FIN_COLS + col;
+  const int64_t i = (int64_t)blockIdx.x * COLS + col;
   Real sum = Real(0);
   if (i < total) {
-#pragma unroll 4
-    for (int b = lane; b < dense_blocks; b += FIN_LANES) sum += partials[(int64_t)b * total + i];
+#pragma unroll 8
+    for (int b = lane; b < dense_blocks; b += LANES) sum += partials[(int64_t)b * total + i];
   }
   lane_sum[lane][col] = sum;
   __syncthreads();
   if (lane == 0 && i < total) {
     Real acc = lane_sum[0][col];
 #pragma unroll
-    for (int l = 1; l < FIN_LANES; ++l) acc += lane_sum[l][col];
+    for (int l = 1; l < LANES; ++l) acc += lane_sum[l][col];
     const int64_t v = i / dim;
     const int c = (int)(i - v * dim);
     const int k1 = csr_ptr[v + 1];
@@ -989,12 +992,15 @@ int finish_impl(const tlh_operator_t* op, int t0, int t1, const void* x, double 
     if (total == 0) continue;
     const int64_t nlist = t.finish_list ? t.finish_count : t.count;
     if (nlist == 0) continue;
-    if (t.dense_blocks >= 2 * FIN_LANES && !t.finish_list)
-      hipLaunchKernelGGL(finish_dense_kernel<Real>, dim3((unsigned)ceil_div(total, FIN_COLS)),
-                         dim3(TLH_THREADS), 0, stream, static_cast<Real*>(y) + t.offset,
-                         static_cast<const Real*>(x) + t.offset, (Real)damping,
+    if (t.dense_blocks >= 2 * (TLH_THREADS / FIN_COLS) && !t.finish_list) {
+      const bool narrow = ceil_div(total, FIN_COLS) < 256;
+      const int cols = narrow ? FIN_COLS_NARROW : FIN_COLS;
+      auto kernel = narrow ? finish_dense_kernel<Real, FIN_COLS_NARROW> : finish_dense_kernel<Real, FIN_COLS>;
+      hipLaunchKernelGGL(kernel, dim3((unsigned)ceil_div(total, cols)), dim3(TLH_THREADS), 0, stream,
+                         static_cast<Real*>(y) + t.offset, static_cast<const Real*>(x) + t.offset, (Real)damping,
                          static_cast<const Real*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim,
                          t.dense_blocks, t.finish_list, nlist);
+    }
     else if (csr_heavy(t.csr_entries, nlist))
       hipLaunchKernelGGL(finish_wave_kernel<Real>, dim3((unsigned)ceil_div(nlist * t.dim, TLH_THREADS / 64)),
                          dim3(TLH_THREADS), 0, stream, static_cast<Real*>(y) + t.offset,

@@ -303,17 +303,27 @@ __global__ void __launch_bounds__(TLH_THREADS)
 constexpr int SMALL_THREADS = 1024;
 constexpr int64_t SMALL_MAX_N = 16384;
 
-template <int NV>
+template <int NV, int NTH = SMALL_THREADS>
 __device__ __forceinline__ void small_sum(double (&v)[NV], double* smem) {
-  for (int k = 0; k < NV; ++k) smem[k * SMALL_THREADS + threadIdx.x] = v[k];
+  // fixed order: an xor-shuffle tree inside each wavefront (every lane ends with
+  // the same sum), then the wavefront sums in wavefront order
+  constexpr int NWAVES = NTH / 64;
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+  const int wave = (int)threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) smem[k * NWAVES + wave] = v[k];
   __syncthreads();
-  for (int w = SMALL_THREADS / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int k = 0; k < NV; ++k) smem[k * SMALL_THREADS + threadIdx.x] += smem[k * SMALL_THREADS + threadIdx.x + w];
-    __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double sum = smem[k * NWAVES];
+    for (int w = 1; w < NWAVES; ++w) sum += smem[k * NWAVES + w];
+    v[k] = sum;
   }
-  for (int k = 0; k < NV; ++k) v[k] = smem[k * SMALL_THREADS];
-  __syncthreads();
+  __syncthreads();  // the next sum rewrites smem
 }
 
 template <typename Real>
@@ -321,7 +331,7 @@ __global__ void __launch_bounds__(SMALL_THREADS)
     small_iteration_kernel(tlh_block_diag_t pc, int identity, Real* __restrict__ x, Real* __restrict__ r,
                            Real* __restrict__ z, Real* __restrict__ p, const Real* __restrict__ ap, int64_t n,
                            int32_t* control, double* scalars) {
-  __shared__ double smem[2 * SMALL_THREADS];
+  __shared__ double smem[2 * (SMALL_THREADS / 64)];
   double cv[1] = {0.0};
   for (int64_t i = threadIdx.x; i < n; i += SMALL_THREADS) cv[0] += (double)p[i] * (double)ap[i];
   small_sum<1>(cv, smem);
@@ -394,6 +404,138 @@ __global__ void __launch_bounds__(SMALL_THREADS)
       scalars[TLH_PCG_RZ] = nrz;
     }
   }
+}
+
+// The same iteration for one block-Jacobi type of at most NTH blocks of
+// dimension D (the reduced camera system): thread b owns block b and keeps its
+// p, Ap, x, r, z and inverse in registers, all loaded at the start, so the
+// iteration makes one trip to memory instead of one per phase.
+template <typename Real, int D, int NTH>
+__global__ void __launch_bounds__(NTH)
+    small_block_iteration_kernel(const double* __restrict__ inv, int64_t count, Real* __restrict__ x,
+                                 Real* __restrict__ r, Real* __restrict__ z, Real* __restrict__ p,
+                                 const Real* __restrict__ ap, int32_t* control, double* scalars) {
+  constexpr int W = D * (D + 1) / 2;
+  __shared__ double smem[2 * (NTH / 64)];
+  const int64_t bi = threadIdx.x;
+  const bool own = bi < count;
+  const int64_t e0 = bi * D;
+  Real pv[D], av[D], xv[D], rv[D];
+  double m[W];
+  if (own) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      pv[k] = p[e0 + k];
+      av[k] = ap[e0 + k];
+      xv[k] = x[e0 + k];
+      rv[k] = r[e0 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < W; ++k) m[k] = inv[bi * W + k];
+  }
+  double cv[1] = {0.0};
+  if (own) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) cv[0] += (double)pv[k] * (double)av[k];
+  }
+  small_sum<1, NTH>(cv, smem);
+  const double c = cv[0];
+  const double rz = scalars[TLH_PCG_RZ];
+  const bool stopped = frozen(control);
+  const bool move = !stopped && finite(c) && finite(rz) && c > 0.0 && rz > 0.0;
+  const double alpha = move ? rz / c : 0.0;
+  double v[2] = {0.0, 0.0};
+  Real zv[D];
+  if (own) {
+    if (move) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        xv[k] = (Real)((double)xv[k] + alpha * (double)pv[k]);
+        rv[k] = (Real)((double)rv[k] - alpha * (double)av[k]);
+        x[e0 + k] = xv[k];
+        r[e0 + k] = rv[k];
+      }
+    }
+    double rd[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) rd[k] = (double)rv[k];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      double t = 0.0;
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        const int a = i < j ? i : j, b = i < j ? j : i;
+        t += m[a * D - a * (a - 1) / 2 + (b - a)] * rd[j];
+      }
+      zv[i] = (Real)t;
+      z[e0 + i] = zv[i];
+      v[0] += rd[i] * (double)zv[i];
+      v[1] += rd[i] * rd[i];
+    }
+  }
+  small_sum<2, NTH>(v, smem);
+  const double nrz = v[0], norm2 = v[1];
+  const bool turn = !stopped && finite(c) && finite(rz) && finite(nrz) && c > 0.0 && rz > 0.0;
+  const double beta = turn ? nrz / rz : 0.0;
+  if (own && turn) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) p[e0 + k] = (Real)((double)zv[k] + beta * (double)pv[k]);
+  }
+  if (threadIdx.x == 0) {
+    scalars[TLH_PCG_CURVATURE] = c;
+    scalars[TLH_PCG_NEW_RZ] = nrz;
+    scalars[TLH_PCG_NORM2] = norm2;
+    if (!stopped) {
+      control[0] += 1;
+      if (!finite(c) || !finite(rz) || !finite(norm2))
+        control[1] = 2;
+      else if (c <= 0.0 || rz <= 0.0)
+        control[1] = 3;
+      else if (norm2 <= scalars[TLH_PCG_THR2])
+        control[1] = 1;
+      else if (!finite(nrz))
+        control[1] = 2;
+      scalars[TLH_PCG_RZ] = nrz;
+    }
+  }
+}
+
+template <typename Real>
+int launch_small(const tlh_block_diag_t* pc, int mode, void* x, void* r, void* z, void* p, const void* ap,
+                 int64_t n, const tlh_pcg_state_t* st, hipStream_t s) {
+  Real* xr = static_cast<Real*>(x);
+  Real* rr = static_cast<Real*>(r);
+  Real* zr = static_cast<Real*>(z);
+  Real* pr = static_cast<Real*>(p);
+  const Real* ar = static_cast<const Real*>(ap);
+  // one block type in registers: 256 threads when they cover the blocks (four
+  // wavefronts, no register cap), else 1,024 (6x6 float64 blocks would spill)
+  const bool one = mode == 0 && pc->ntypes == 1 && (pc->dim[0] == 6 || pc->dim[0] == 3);
+  const int64_t cnt = one ? pc->count[0] : 0;
+  const int64_t o = one ? pc->offset[0] : 0;
+  void (*kernel)(const double*, int64_t, Real*, Real*, Real*, Real*, const Real*, int32_t*, double*) = nullptr;
+  int nth = 256;
+  if (one && cnt <= 256) {
+    kernel = pc->dim[0] == 6 ? small_block_iteration_kernel<Real, 6, 256> : small_block_iteration_kernel<Real, 3, 256>;
+  } else if (one && cnt <= SMALL_THREADS && pc->dim[0] == 3) {
+    kernel = small_block_iteration_kernel<Real, 3, SMALL_THREADS>;
+    nth = SMALL_THREADS;
+  } else if constexpr (sizeof(Real) == 4) {
+    if (one && cnt <= SMALL_THREADS) {
+      kernel = small_block_iteration_kernel<Real, 6, SMALL_THREADS>;
+      nth = SMALL_THREADS;
+    }
+  }
+  if (kernel) {
+    hipLaunchKernelGGL(kernel, dim3(1), dim3(nth), 0, s, pc->inv[0], cnt, xr + o, rr + o, zr + o, pr + o, ar + o,
+                       st->control, st->scalars);
+  } else {
+    tlh_block_diag_t empty{};
+    hipLaunchKernelGGL(small_iteration_kernel<Real>, dim3(1), dim3(SMALL_THREADS), 0, s, mode == 0 ? *pc : empty,
+                       mode, xr, rr, zr, pr, ar, n, st->control, st->scalars);
+  }
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
 }
 
 // r = src; optionally x = 0; z = M^-1 r; p = z; rz = r.z (warp_pcg.py:130-132, 169-172, 210-212).
@@ -507,20 +649,10 @@ extern "C" int tlh_pcg_iteration_small(const tlh_block_diag_t* pc, int mode, voi
   if (!x || !r || !z || !p || !ap || n < 1 || n > SMALL_MAX_N || !state_ok(st)) return TLH_EINVAL;
   if (mode != 0 && mode != 1) return TLH_EINVAL;
   if (mode == 0 && (!pc || !precond_ok(pc, n))) return TLH_EINVAL;
-  tlh_block_diag_t empty{};
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  if (dtype == TLH_F32)
-    hipLaunchKernelGGL(small_iteration_kernel<float>, dim3(1), dim3(SMALL_THREADS), 0, s, mode == 0 ? *pc : empty,
-                       mode, (float*)x, (float*)r, (float*)z, (float*)p, (const float*)ap, n, st->control,
-                       st->scalars);
-  else if (dtype == TLH_F64)
-    hipLaunchKernelGGL(small_iteration_kernel<double>, dim3(1), dim3(SMALL_THREADS), 0, s, mode == 0 ? *pc : empty,
-                       mode, (double*)x, (double*)r, (double*)z, (double*)p, (const double*)ap, n, st->control,
-                       st->scalars);
-  else
-    return TLH_EINVAL;
-  TLH_RETURN_IF_LAUNCH_FAILED();
-  return 0;
+  if (dtype == TLH_F32) return launch_small<float>(pc, mode, x, r, z, p, ap, n, st, s);
+  if (dtype == TLH_F64) return launch_small<double>(pc, mode, x, r, z, p, ap, n, st, s);
+  return TLH_EINVAL;
 }
 
 extern "C" int tlh_block_inverse(const double* blocks, int64_t count, int dim, double damping, double* inverse,

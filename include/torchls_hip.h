@@ -155,6 +155,8 @@ int tlh_normal_matvec(const tlh_operator_t* op, const void* x, void* y, double d
  * symmetric).  op must be an all-SPARSE plan sharing the normal plan's order and
  * tiles (same packed blocks); each type's `partials` is a float64 workspace of
  * [sparse_partials, dim*(dim+1)/2].  lds_limit: bytes of LDS per workgroup.
+ * out[t] == NULL skips type t (allowed only for types whose groups are
+ * two-slot 2x6 / 2x3 pairs); tlh_major_gram then supplies its blocks.
  * Replaces: WarpDiagonalPlan.diagonal_blocks (warp_diagonal.py:123-179),
  * BlockJacobian.diagonal_blocks (sparse.py:104-121). */
 int tlh_diagonal_blocks(const tlh_operator_t* op, double* const* out, int64_t lds_limit,
@@ -330,6 +332,33 @@ int tlh_schur_apply(const tlh_operator_t* op, int wgroup, int eslot, int mode, c
 int tlh_schur_precond(const tlh_operator_t* op, int wgroup, int eslot, const double* cinv,
                       const double* D, double damping, double* M, int64_t lds_limit,
                       tlh_stream_t stream);
+
+/* Variable-major diagonal blocks of one slot: out[v] (float64 [count, dim, dim],
+ * full, symmetric) = sum over k in [ptr[v], ptr[v+1]) of B_o^T B_o with
+ * o = order[k] and B_o the o-th [rows, dim] block of the raw row-major blocks
+ * ([N, rows, dim], float32 or float64, as the linearization wrote them).
+ * order: int32 observation indices sorted by the slot's variable; ptr: int32
+ * [count + 1].  One workgroup per variable, fixed summation order.  For types
+ * with few variables and many observations each (cameras): no partials.
+ * Shapes: (rows, dim) in {(2, 6), (2, 3), (6, 6)}, else TLH_ENOTFOUND.
+ * Replaces: BlockJacobian.diagonal_blocks (sparse.py:104-121) for that slot. */
+int tlh_major_gram(const void* blocks, int block_dtype, int rows, int dim, const int32_t* order,
+                   const int32_t* ptr, int64_t count, double* out, tlh_stream_t stream);
+
+/* Retained preconditioner blocks in camera order: M[c] = D[c] + lambda I -
+ * sum over edges e of camera c of W_e C_p^-1 W_e^T, W_e = sum over the edge's
+ * observations of J_c^T J_p (float64).  jc, jp: raw row-major blocks [N, rows,
+ * dc] and [N, rows, dp]; order: int32 observations sorted by (camera, point);
+ * edge_ptr: int32 [E + 1] runs of one (camera, point) pair in that order;
+ * var_edge_ptr: int32 [count + 1] edges of each camera; edge_point: int32 [E]
+ * point of each edge; cinv: float64 upper triangles of C_p^-1; D, M: float64
+ * [count, dc, dc].  Shapes: rows 2, dc 6, dp 3, else TLH_ENOTFOUND.
+ * Replaces: DampedSchurSystem.preconditioner (schur.py:330-378), like
+ * tlh_schur_precond, without per-tile camera partials. */
+int tlh_schur_precond_major(const void* jc, const void* jp, int block_dtype, int rows, int dc, int dp,
+                            const int32_t* order, const int32_t* edge_ptr, const int32_t* var_edge_ptr,
+                            const int32_t* edge_point, int64_t count, const double* cinv, const double* D,
+                            double damping, double* M, tlh_stream_t stream);
 
 /* Subtract W C^-1 W^T from a dense float64 reduced matrix M (leading dimension
  * ld, holding blockdiag(D + lambda I)): one thread per camera pair key

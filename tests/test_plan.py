@@ -48,3 +48,30 @@ def test_primary_is_largest_type_and_tiles_align():
     for b in g.tile_start[1:-1]:
         assert prim[b] != prim[b - 1]
     assert g.slots[0].mode == REDUCE_DENSE
+
+
+def test_major_order_and_edges_match_brute_force():
+    from torchls_amd.plan import major_edges, major_eligible, major_order
+
+    rng = np.random.default_rng(5)
+    n, nc, npt = 700, 6, 40
+    ic = rng.integers(0, nc - 1, n)  # camera nc - 1 is never observed
+    ip = rng.integers(0, npt, n)
+    order, ptr = major_order(ic, nc)
+    assert order.dtype == np.int32 and ptr.dtype == np.int32
+    for v in range(nc):
+        members = order[ptr[v]:ptr[v + 1]]
+        assert np.array_equal(members, np.flatnonzero(ic == v))  # stable: increasing positions
+    order, edge_ptr, var_edge_ptr, edge_point = major_edges(ic, ip, nc)
+    assert sorted(order.tolist()) == list(range(n))
+    assert var_edge_ptr[-1] == edge_point.size == edge_ptr.size - 1 and edge_ptr[-1] == n
+    assert var_edge_ptr[nc - 1] == var_edge_ptr[nc]
+    for v in range(nc):
+        pairs = sorted({int(p) for p in ip[ic == v]})
+        edges = range(var_edge_ptr[v], var_edge_ptr[v + 1])
+        assert [int(edge_point[e]) for e in edges] == pairs
+        for e in edges:
+            obs = order[edge_ptr[e]:edge_ptr[e + 1]]
+            assert np.all(ic[obs] == v) and np.all(ip[obs] == edge_point[e])
+            assert np.array_equal(obs, np.flatnonzero((ic == v) & (ip == edge_point[e])))
+    assert major_eligible(200, 500_000) and not major_eligible(50_000, 500_000) and not major_eligible(0, 10)

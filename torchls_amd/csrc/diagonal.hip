@@ -116,6 +116,7 @@ __global__ void __launch_bounds__(TLH_THREADS) diag_pair_kernel(GroupArgs<Real> 
     const int W = s == 0 ? W0 : W1;
     const int off = s == 0 ? 0 : R * D0;
     double* out = reinterpret_cast<double*>(sl.partials);
+    if (!out) continue;  // type computed elsewhere (uniform over the workgroup)
     for (int c0 = 0; c0 < W; c0 += chunk) {
       const int cw = min(chunk, W - c0);
       __syncthreads();
@@ -214,11 +215,13 @@ int diag_impl(const tlh_operator_t* op, double* const* out, hipStream_t stream, 
     a.nslots = g.nslots;
     int alias[3] = {0, 0, 0};
     int wmax = 1;
+    bool skipped = false;
     for (int s = 0; s < g.nslots; ++s) {
       const tlh_slot_t& sl = g.slot[s];
       if (sl.mode != TLH_REDUCE_SPARSE) return TLH_EINVAL;
       SlotArgs<Real>& o = a.slot[s];
-      o.partials = static_cast<Real*>(op->types[sl.type].partials);
+      skipped = skipped || !out[sl.type];
+      o.partials = out[sl.type] ? static_cast<Real*>(op->types[sl.type].partials) : nullptr;
       o.index = sl.index;
       o.rank = sl.rank;
       o.seg_ptr = sl.seg_ptr;
@@ -247,6 +250,7 @@ int diag_impl(const tlh_operator_t* op, double* const* out, hipStream_t stream, 
       TLH_RETURN_IF_LAUNCH_FAILED();
       continue;
     }
+    if (skipped) return TLH_EINVAL;  // only the two-slot pair kernel skips a type
     TLH_RETURN_IF_ERROR(allow_lds((const void*)diag_tile_kernel<Real>, lds));
     hipLaunchKernelGGL(diag_tile_kernel<Real>, dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds, stream,
                        a, chunk, alias[0], alias[1], alias[2]);
@@ -255,7 +259,7 @@ int diag_impl(const tlh_operator_t* op, double* const* out, hipStream_t stream, 
   for (int ti = 0; ti < op->ntypes; ++ti) {
     const tlh_type_t& t = op->types[ti];
     const int64_t total = t.count * (t.dim * (t.dim + 1) / 2);
-    if (total == 0) continue;
+    if (total == 0 || !out[ti]) continue;
     if (t.dense_blocks != 0) return TLH_EINVAL;
     if (csr_heavy(t.csr_entries, t.count))
       hipLaunchKernelGGL(diag_finish_wave, dim3((unsigned)ceil_div(total, TLH_THREADS / 64)), dim3(TLH_THREADS), 0,

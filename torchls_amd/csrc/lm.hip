@@ -224,6 +224,57 @@ __global__ void __launch_bounds__(TLH_THREADS)
   grid_sum2(v, partials, counter, out, smem);
 }
 
+// sqnorm_kernel for two-slot groups of a fixed shape: the same products in the
+// same order, fully unrolled, so an observation's block and x loads are all in
+// flight together instead of one runtime-indexed load at a time.
+template <int R, int D0, int D1, typename BT, typename XT>
+__global__ void __launch_bounds__(TLH_THREADS)
+    sqnorm_pair_kernel(RawGroup g, const void* x, double* out, double* partials, uint32_t* counter) {
+  __shared__ double smem[2 * TLH_THREADS];
+  const BT* __restrict__ b0 = static_cast<const BT*>(g.blocks[0]);
+  const BT* __restrict__ b1 = static_cast<const BT*>(g.blocks[1]);
+  const XT* __restrict__ xv = static_cast<const XT*>(x);
+  double v[2] = {0.0, 0.0};
+  for (int64_t o = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; o < g.count;
+       o += (int64_t)gridDim.x * TLH_THREADS) {
+    const int64_t c0 = g.offsets[0] + g.index[0][o] * D0;
+    const int64_t c1 = g.offsets[1] + g.index[1][o] * D1;
+    double x0[D0], x1[D1], q[R];
+#pragma unroll
+    for (int c = 0; c < D0; ++c) x0[c] = (double)xv[c0 + c];
+#pragma unroll
+    for (int c = 0; c < D1; ++c) x1[c] = (double)xv[c1 + c];
+#pragma unroll
+    for (int r = 0; r < R; ++r) q[r] = 0.0;
+#pragma unroll
+    for (int c = 0; c < D0; ++c)
+#pragma unroll
+      for (int r = 0; r < R; ++r) q[r] += (double)b0[(o * R + r) * D0 + c] * x0[c];
+#pragma unroll
+    for (int c = 0; c < D1; ++c)
+#pragma unroll
+      for (int r = 0; r < R; ++r) q[r] += (double)b1[(o * R + r) * D1 + c] * x1[c];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[0] += q[r] * q[r];
+  }
+  grid_sum2(v, partials, counter, out, smem);
+}
+
+template <int R, int D0, int D1>
+bool launch_sqnorm_pair(const RawGroup& g, int block_dtype, const void* x, int x_dtype, double* out2,
+                        double* partials, uint32_t* counter, hipStream_t s) {
+  if (g.nslots != 2 || g.rows != R || g.dims[0] != D0 || g.dims[1] != D1) return false;
+  void (*kernel)(RawGroup, const void*, double*, double*, uint32_t*);
+  if (block_dtype == TLH_F64)
+    kernel = x_dtype == TLH_F64 ? sqnorm_pair_kernel<R, D0, D1, double, double>
+                                : sqnorm_pair_kernel<R, D0, D1, double, float>;
+  else
+    kernel = x_dtype == TLH_F64 ? sqnorm_pair_kernel<R, D0, D1, float, double>
+                                : sqnorm_pair_kernel<R, D0, D1, float, float>;
+  hipLaunchKernelGGL(kernel, dim3(red_grid(g.count)), dim3(TLH_THREADS), 0, s, g, x, out2, partials, counter);
+  return true;
+}
+
 // y[o*R + r] = sum_s J_s[o] x_s in float64, written in the output dtype (sparse.py:31-41).
 __global__ void __launch_bounds__(TLH_THREADS)
     block_matvec_kernel(RawGroup g, int block_dtype, const void* x, int x_dtype, void* y, int y_dtype) {
@@ -374,8 +425,12 @@ extern "C" int tlh_squared_matvec_norm(int64_t count, int rows, int nslots, cons
     g.index[s] = index[s];
     g.blocks[s] = blocks[s];
   }
-  hipLaunchKernelGGL(sqnorm_kernel, dim3(red_grid(count)), dim3(TLH_THREADS), 0,
-                     reinterpret_cast<hipStream_t>(stream), g, block_dtype, x, x_dtype, out2, partials, counter);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (!launch_sqnorm_pair<2, 6, 3>(g, block_dtype, x, x_dtype, out2, partials, counter, s) &&
+      !launch_sqnorm_pair<2, 3, 6>(g, block_dtype, x, x_dtype, out2, partials, counter, s) &&
+      !launch_sqnorm_pair<6, 6, 6>(g, block_dtype, x, x_dtype, out2, partials, counter, s))
+    hipLaunchKernelGGL(sqnorm_kernel, dim3(red_grid(count)), dim3(TLH_THREADS), 0, s, g, block_dtype, x, x_dtype,
+                       out2, partials, counter);
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }

@@ -21,6 +21,7 @@ import torch
 
 from . import _lib
 from .hip_normal import DeviceStructure, check_columns, packed_blocks, plan_for
+from .plan import major_eligible, major_order
 
 
 class LinAlgError(RuntimeError):
@@ -44,11 +45,36 @@ class HipDiagonalPlan:
             for g in jacobian.groups
         )
         self.plan, self.dev = plan_for(jacobian, self.dtype, allow_dense=False)
+        self.major = self._major_slots(jacobian)
         self.partials = [
             torch.empty(max(1, t.sparse_partials * t.dim * (t.dim + 1) // 2), dtype=torch.float64,
                         device=self.device)
             for t in self.plan.types
         ]
+
+    def _major_slots(self, jacobian) -> dict:
+        """Types whose diagonal blocks come from tlh_major_gram: a type with few
+        variables and many observations (cameras) seen by exactly one residual
+        group, a two-slot 2x6 / 2x3 pair of different types.  type -> (group,
+        slot, order, ptr)."""
+        groups_of = {}
+        for gi, g in enumerate(jacobian.groups):
+            for t in set(g.type_indices):
+                groups_of.setdefault(t, []).append(gi)
+        out = {}
+        for gi, g in enumerate(jacobian.groups):
+            shape = [tuple(int(v) for v in b.shape[1:]) for b in g.blocks]
+            if len(g.blocks) != 2 or g.type_indices[0] == g.type_indices[1] or \
+                    sorted(shape) != [(2, 3), (2, 6)]:
+                continue
+            n = int(g.blocks[0].shape[0])
+            for s, t in enumerate(g.type_indices):
+                count = jacobian.layout[t].count
+                if groups_of[t] != [gi] or not major_eligible(count, n):
+                    continue
+                order, ptr = major_order(g.variable_indices[s].detach().reshape(-1).cpu().numpy(), count)
+                out[t] = (gi, s, torch.from_numpy(order).to(self.device), torch.from_numpy(ptr).to(self.device))
+        return out
 
     def compatible_with(self, normal_plan) -> bool:
         """True when a HipNormalPlan packs observations in this plan's order and tiles."""
@@ -90,19 +116,29 @@ class HipDiagonalPlan:
                                         pack))
         return packed
 
-    def diagonal_blocks_f64(self, packed: list) -> tuple:
-        """float64 diagonal blocks from packed blocks laid out in this plan's order."""
+    def diagonal_blocks_f64(self, packed: list, jacobian=None) -> tuple:
+        """float64 diagonal blocks from packed blocks laid out in this plan's order.
+
+        With ``jacobian`` (the one ``packed`` was made from), the types in
+        ``self.major`` are summed in variable order from its raw blocks instead."""
         outs = tuple(torch.empty((c, d, d), dtype=torch.float64, device=self.device) for c, d in self.layout)
+        major = self.major if jacobian is not None else {}
         op = self.dev.operator(self.dtype, packed, self.partials)
-        ptrs = (ctypes.c_void_p * len(outs))(*[o.data_ptr() for o in outs])
+        ptrs = (ctypes.c_void_p * len(outs))(*[None if t in major else o.data_ptr() for t, o in enumerate(outs)])
         lds = _lib.device_info(self.device).lds_per_block
-        _lib.check(self.lib.tlh_diagonal_blocks(ctypes.byref(op), ptrs, lds, _lib.stream_handle(self.device)),
-                   "tlh_diagonal_blocks")
+        stream = _lib.stream_handle(self.device)
+        _lib.check(self.lib.tlh_diagonal_blocks(ctypes.byref(op), ptrs, lds, stream), "tlh_diagonal_blocks")
+        for t, (gi, s, order, ptr) in major.items():
+            blocks = jacobian.groups[gi].blocks[s].contiguous()
+            _, rows, dim = blocks.shape
+            _lib.check(self.lib.tlh_major_gram(blocks.data_ptr(), _lib.dtype_code(blocks.dtype), rows, dim,
+                                               order.data_ptr(), ptr.data_ptr(), outs[t].shape[0],
+                                               outs[t].data_ptr(), stream), "tlh_major_gram")
         return outs
 
     def diagonal_blocks(self, jacobian) -> tuple:
         self._check(jacobian)
-        outs = self.diagonal_blocks_f64(self._pack(jacobian))
+        outs = self.diagonal_blocks_f64(self._pack(jacobian), jacobian)
         return tuple(o.to(self.block_dtype) for o in outs)
 
 

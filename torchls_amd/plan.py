@@ -395,6 +395,46 @@ def build_structure(
     return StructurePlan(types, gplans, esize, cap)
 
 
+# Variable-major accumulation (csrc/major.hip) pays off for a type with few
+# variables observed many times each: its blocks then need no per-tile partials.
+MAJOR_MAX_VARIABLES = 65536
+MAJOR_MIN_MEAN = 64
+
+
+def major_eligible(variables: int, observations: int) -> bool:
+    return 0 < variables <= MAJOR_MAX_VARIABLES and observations >= MAJOR_MIN_MEAN * variables \
+        and observations < 2**31
+
+
+def major_order(index: np.ndarray, count: int):
+    """Observations sorted by variable (stable) and each variable's range: (order, ptr), int32."""
+    index = np.asarray(index, dtype=np.int64).reshape(-1)
+    order = np.argsort(index, kind="stable")
+    ptr = np.zeros(count + 1, dtype=np.int64)
+    np.cumsum(np.bincount(index, minlength=count), out=ptr[1:])
+    return order.astype(np.int32), ptr.astype(np.int32)
+
+
+def major_edges(index_c: np.ndarray, index_p: np.ndarray, count_c: int):
+    """Observations sorted by (camera, point, position) and their (camera, point) runs:
+    (order, edge_ptr [E+1], var_edge_ptr [count_c+1], edge_point [E]), all int32."""
+    ic = np.asarray(index_c, dtype=np.int64).reshape(-1)
+    ip = np.asarray(index_p, dtype=np.int64).reshape(-1)
+    n = ic.size
+    order = np.lexsort((np.arange(n), ip, ic))
+    sc, sp = ic[order], ip[order]
+    head = np.ones(n, dtype=bool)
+    if n:
+        head[1:] = (sc[1:] != sc[:-1]) | (sp[1:] != sp[:-1])
+    starts = np.flatnonzero(head)
+    edge_ptr = np.append(starts, n)
+    edge_cam = sc[starts]
+    var_edge_ptr = np.zeros(count_c + 1, dtype=np.int64)
+    np.cumsum(np.bincount(edge_cam, minlength=count_c), out=var_edge_ptr[1:])
+    return (order.astype(np.int32), edge_ptr.astype(np.int32), var_edge_ptr.astype(np.int32),
+            sp[starts].astype(np.int32))
+
+
 def emulate_normal_matvec(plan: StructurePlan, packed: list, x: np.ndarray, damping: float):
     """Execute the plan's data flow on the CPU (numpy), exactly as the kernels do.
 

@@ -27,7 +27,7 @@ import torch
 from . import _lib
 from .hip_diagonal import LinAlgError
 from .hip_normal import DeviceStructure, packed_blocks
-from .plan import build_structure
+from .plan import build_structure, major_edges, major_eligible
 
 
 class HipSchurPlan:
@@ -125,6 +125,16 @@ class HipSchurPlan:
             raise NotImplementedError("an eliminated variable has more observations than one tile holds")
         self.dev = DeviceStructure(self.plan, self.device)
         self.sdev = DeviceStructure(self.sparse_plan, self.device)
+        # camera-order preconditioner (csrc/major.hip) for few, heavily observed retained variables
+        self.major = None
+        grp = jacobian.groups[self.groups[self.wgroup][0]]
+        sr = 1 - self.eslot
+        rcount = self.rlayout[self.groups[self.wgroup][2][sr]][0]
+        if (tuple(grp.blocks[sr].shape[1:]) == (2, 6) and tuple(grp.blocks[self.eslot].shape[1:]) == (2, 3)
+                and major_eligible(rcount, int(grp.blocks[0].shape[0]))):
+            idx = self._plan_groups[self.wgroup][2]
+            arrays = major_edges(idx[sr], idx[self.eslot], rcount)
+            self.major = tuple(torch.from_numpy(a).to(self.device) for a in arrays)
         nt = len(self.retained)
         self.partials = [torch.empty(max(1, t.partial_count * t.dim), dtype=self.dtype, device=self.device)
                          for t in self.plan.types]
@@ -308,7 +318,17 @@ class HipSchurSystem:
         lds = _lib.device_info(p.device).lds_per_block
         for k, t in enumerate(p.retained):
             D = self.lin.diagonal[t]
-            if k == wtype:
+            if k == wtype and p.major is not None:
+                M = torch.empty_like(D)
+                order, edge_ptr, var_edge_ptr, edge_point = p.major
+                raw = self.lin._keep[p.wgroup]
+                jc, jp = raw[1 - p.eslot], raw[p.eslot]
+                _lib.check(p.lib.tlh_schur_precond_major(
+                    jc.data_ptr(), jp.data_ptr(), _lib.dtype_code(jc.dtype), 2, 6, 3, order.data_ptr(),
+                    edge_ptr.data_ptr(), var_edge_ptr.data_ptr(), edge_point.data_ptr(), D.shape[0],
+                    self.cinv.data_ptr(), D.data_ptr(), self.damping, M.data_ptr(), _lib.stream_handle(p.device)),
+                    "tlh_schur_precond_major")
+            elif k == wtype:
                 M = torch.empty_like(D)
                 _lib.check(p.lib.tlh_schur_precond(ctypes.byref(self.lin.sop), 0, p.eslot, self.cinv.data_ptr(),
                                                    D.data_ptr(), self.damping, M.data_ptr(), lds,

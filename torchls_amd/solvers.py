@@ -330,7 +330,7 @@ def solve(
             packed_unscaled = None
             if needs_diagonal or diagonal_damping or (jacobi_scaling and column_scale is None):
                 packed_unscaled = diagonal_plan._pack(jacobian)
-                diagonal = diagonal_plan.diagonal_blocks_f64(packed_unscaled)
+                diagonal = diagonal_plan.diagonal_blocks_f64(packed_unscaled, jacobian)
             if jacobi_scaling and column_scale is None:
                 column_scale = torch.cat(
                     [(1 / (1 + b.diagonal(dim1=-2, dim2=-1).clamp_min(0).sqrt())).reshape(-1) for b in diagonal]

@@ -273,7 +273,8 @@ int tlh_pcg_true_residual(const void* rhs, const void* ax, const void* r, void* 
  * from the reference-layout blocks (blocks[s]: [count, rows, dim_s], block_dtype)
  * and v ([count*rows] in original observation order, v_dtype).  Each type's
  * `partials` must be a float64 [sparse_partials, dim] workspace.  Call once per
- * group, then tlh_rmatvec_finish_f64.
+ * group, then tlh_rmatvec_finish_f64.  blocks[s] == NULL skips slot s (two-slot
+ * 2x6 / 2x3 float64 groups only); tlh_major_rmatvec then supplies its type.
  * Replaces: BlockJacobian.rmatvec (sparse.py:43-55), Linearization.gradient
  * (problem.py:40-42). */
 int tlh_rmatvec_f64(const tlh_operator_t* op, int group, const void* const* blocks, int block_dtype,
@@ -281,6 +282,11 @@ int tlh_rmatvec_f64(const tlh_operator_t* op, int group, const void* const* bloc
 
 /* out[offset_t + ...] = sum of each variable's rmatvec partials (float64 [size]). */
 int tlh_rmatvec_finish_f64(const tlh_operator_t* op, double* out, tlh_stream_t stream);
+
+/* tlh_rmatvec_finish_f64 for the types [type_begin, type_end) only (the others
+ * are left untouched, e.g. types summed by tlh_major_rmatvec). */
+int tlh_rmatvec_finish_range_f64(const tlh_operator_t* op, int type_begin, int type_end, double* out,
+                                 tlh_stream_t stream);
 
 /* out2[0] = ||J_g x||^2 of one residual group in float64 (slots summed before
  * squaring); index[s] are the group's int64 variable indices (original order),
@@ -344,6 +350,15 @@ int tlh_schur_precond(const tlh_operator_t* op, int wgroup, int eslot, const dou
  * Replaces: BlockJacobian.diagonal_blocks (sparse.py:104-121) for that slot. */
 int tlh_major_gram(const void* blocks, int block_dtype, int rows, int dim, const int32_t* order,
                    const int32_t* ptr, int64_t count, double* out, tlh_stream_t stream);
+
+/* Variable-major float64 J^T v of one slot: out[var * dim + c] (float64
+ * [count, dim]) = sum over k in [ptr[var], ptr[var+1]) of sum_r B_o[r][c] v[o *
+ * rows + r], o = order[k]; blocks as in tlh_major_gram, v in original row order
+ * (float32 or float64).  Shapes as tlh_major_gram.
+ * Replaces: BlockJacobian.rmatvec (sparse.py:43-55) for that slot, used by
+ * Linearization.gradient (problem.py:40-42). */
+int tlh_major_rmatvec(const void* blocks, int block_dtype, const void* v, int v_dtype, int rows, int dim,
+                      const int32_t* order, const int32_t* ptr, int64_t count, double* out, tlh_stream_t stream);
 
 /* Retained preconditioner blocks in camera order: M[c] = D[c] + lambda I -
  * sum over edges e of camera c of W_e C_p^-1 W_e^T, W_e = sum over the edge's

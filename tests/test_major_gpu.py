@@ -115,3 +115,57 @@ def test_schur_precond_major_repeated_edges(device):
     finally:
         plan.major = saved
     assert rel_err(major.cpu().numpy(), tile.cpu().numpy()) <= 1e-12
+
+
+@pytest.mark.parametrize("rows,dim", [(2, 6), (2, 3), (6, 6)])
+def test_major_rmatvec_matches_numpy(device, rows, dim):
+    print(f"major rmatvec rows={rows} dim={dim}", flush=True)
+    rng = np.random.default_rng(40 + rows + dim)
+    count, n = 7, 2500
+    idx = rng.integers(0, count, n)
+    blocks = rng.standard_normal((n, rows, dim))
+    v = rng.standard_normal(n * rows)
+    order, ptr = major_order(idx, count)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+    b, vv, o, p = dev(blocks), dev(v), dev(order), dev(ptr)
+    out = torch.full((count, dim), np.nan, dtype=torch.float64, device=device)
+    lib = _lib.load()
+    _lib.check(lib.tlh_major_rmatvec(b.data_ptr(), _lib.TLH_F64, vv.data_ptr(), _lib.TLH_F64, rows, dim, o.data_ptr(),
+                                     p.data_ptr(), count, out.data_ptr(), _lib.stream_handle(device)),
+               "tlh_major_rmatvec")
+    want = np.zeros((count, dim))
+    np.add.at(want, idx, np.einsum("nrc,nr->nc", blocks, v.reshape(n, rows)))
+    assert rel_err(out.cpu().numpy(), want) <= 1e-12
+
+
+def test_gradient_major_matches_tile_path(device):
+    print("gradient major vs tile", flush=True)
+    from torchls_amd.hip_ops import LinearizationKernels
+
+    P = build(generators.bundle_adjustment(16, 500, 8, seed=26), device)
+    lin = P._analyze().linearize(P.parameters)
+    kernels = LinearizationKernels(lin.jacobian)
+    assert 0 in kernels.diag_plan.major
+    r = lin.residual.to(torch.float64).contiguous()
+    g_major = kernels.gradient(lin.jacobian, r)
+    saved, kernels.diag_plan.major = kernels.diag_plan.major, {}
+    try:
+        g_tile = kernels.gradient(lin.jacobian, r)
+    finally:
+        kernels.diag_plan.major = saved
+    assert rel_err(g_major.cpu().numpy(), g_tile.cpu().numpy()) <= 1e-12
+    J = lin.jacobian
+    dense = torch.zeros(J.shape, dtype=torch.float64)
+    row = 0
+    for g in J.groups:
+        n, rows = g.blocks[0].shape[:2]
+        for b, t, vi in zip(g.blocks, g.type_indices, g.variable_indices):
+            ent = J.layout[t]
+            bd = b.double().cpu()
+            vic = vi.reshape(-1).cpu().numpy()
+            for k in range(n):
+                c0 = ent.offset + int(vic[k]) * ent.dim
+                dense[row + k * rows: row + (k + 1) * rows, c0: c0 + ent.dim] += bd[k]
+        row += n * rows
+    want = (dense.T @ r.cpu()).numpy()
+    assert rel_err(g_major.cpu().numpy(), want) <= 1e-12

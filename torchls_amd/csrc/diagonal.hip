@@ -231,7 +231,7 @@ int diag_impl(const tlh_operator_t* op, double* const* out, hipStream_t stream, 
       o.mode = sl.mode;
       for (int q = 0; q < s; ++q)
         if (g.slot[q].type == sl.type) alias[s] |= 1 << q;
-      wmax = std::max(wmax, sl.dim * (sl.dim + 1) / 2);
+      if (out[sl.type]) wmax = std::max(wmax, sl.dim * (sl.dim + 1) / 2);  // LDS sized for computed slots
     }
     int chunk = std::min<int64_t>(wmax, lds_limit / ((int64_t)g.tile_cap * 8));
     if (chunk < 1) return TLH_ELDS;

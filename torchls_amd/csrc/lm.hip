@@ -100,8 +100,6 @@ __global__ void __launch_bounds__(TLH_THREADS) rmatvec_pair_kernel(RmatvecArgs a
     const int64_t o = a.order[start + pos];
     double vr[R], j0[K0], j1[K1];
     const v2d* pv = reinterpret_cast<const v2d*>(static_cast<const double*>(a.v) + o * R);
-    const v2d* p0 = reinterpret_cast<const v2d*>(static_cast<const double*>(a.blocks[0]) + o * K0);
-    const v2d* p1 = reinterpret_cast<const v2d*>(static_cast<const double*>(a.blocks[1]) + o * K1);
 #pragma unroll
     for (int k = 0; k < R / 2; ++k) {
       const v2d v = pv[k];
@@ -109,16 +107,26 @@ __global__ void __launch_bounds__(TLH_THREADS) rmatvec_pair_kernel(RmatvecArgs a
       vr[2 * k + 1] = v[1];
     }
 #pragma unroll
-    for (int k = 0; k < K0 / 2; ++k) {
-      const v2d v = p0[k];
-      j0[2 * k] = v[0];
-      j0[2 * k + 1] = v[1];
-    }
+    for (int k = 0; k < K0; ++k) j0[k] = 0.0;
 #pragma unroll
-    for (int k = 0; k < K1 / 2; ++k) {
-      const v2d v = p1[k];
-      j1[2 * k] = v[0];
-      j1[2 * k + 1] = v[1];
+    for (int k = 0; k < K1; ++k) j1[k] = 0.0;
+    if (a.blocks[0]) {
+      const v2d* p0 = reinterpret_cast<const v2d*>(static_cast<const double*>(a.blocks[0]) + o * K0);
+#pragma unroll
+      for (int k = 0; k < K0 / 2; ++k) {
+        const v2d v = p0[k];
+        j0[2 * k] = v[0];
+        j0[2 * k + 1] = v[1];
+      }
+    }
+    if (a.blocks[1]) {
+      const v2d* p1 = reinterpret_cast<const v2d*>(static_cast<const double*>(a.blocks[1]) + o * K1);
+#pragma unroll
+      for (int k = 0; k < K1 / 2; ++k) {
+        const v2d v = p1[k];
+        j1[2 * k] = v[0];
+        j1[2 * k + 1] = v[1];
+      }
     }
 #pragma unroll
     for (int c = 0; c < D0; ++c) {
@@ -138,6 +146,7 @@ __global__ void __launch_bounds__(TLH_THREADS) rmatvec_pair_kernel(RmatvecArgs a
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     const int D = s == 0 ? D0 : D1;
+    if (!a.blocks[s]) continue;  // slot reduced elsewhere (uniform over the workgroup)
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < PER; ++it) {
@@ -344,17 +353,19 @@ extern "C" int tlh_rmatvec_f64(const tlh_operator_t* op, int group, const void* 
   if (g.rows > 16) return TLH_EINVAL;
   RmatvecArgs a{};
   int dmax = 1;
+  bool skipped = false;
   for (int s = 0; s < g.nslots; ++s) {
     const tlh_slot_t& sl = g.slot[s];
     if (sl.mode != TLH_REDUCE_SPARSE) return TLH_EINVAL;
     a.blocks[s] = blocks[s];
+    skipped = skipped || !blocks[s];
     a.dims[s] = sl.dim;
     a.rank[s] = sl.rank;
     a.seg_ptr[s] = sl.seg_ptr;
     a.seg_start[s] = sl.seg_start;
     a.seg_target[s] = sl.seg_target;
     a.partials[s] = static_cast<double*>(op->types[sl.type].partials);
-    dmax = sl.dim > dmax ? sl.dim : dmax;
+    if (blocks[s]) dmax = sl.dim > dmax ? sl.dim : dmax;
   }
   a.v = v;
   a.order = g.order;
@@ -383,6 +394,7 @@ extern "C" int tlh_rmatvec_f64(const tlh_operator_t* op, int group, const void* 
     TLH_RETURN_IF_LAUNCH_FAILED();
     return 0;
   }
+  if (skipped) return TLH_EINVAL;  // only the two-slot pair kernel skips a slot
   TLH_RETURN_IF_ERROR(allow_lds((const void*)rmatvec_tile_kernel, lds));
   hipLaunchKernelGGL(rmatvec_tile_kernel, dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds,
                      reinterpret_cast<hipStream_t>(stream), a);
@@ -391,8 +403,14 @@ extern "C" int tlh_rmatvec_f64(const tlh_operator_t* op, int group, const void* 
 }
 
 extern "C" int tlh_rmatvec_finish_f64(const tlh_operator_t* op, double* out, tlh_stream_t stream) {
-  if (!op || !out) return TLH_EINVAL;
-  for (int ti = 0; ti < op->ntypes; ++ti) {
+  if (!op) return TLH_EINVAL;
+  return tlh_rmatvec_finish_range_f64(op, 0, op->ntypes, out, stream);
+}
+
+extern "C" int tlh_rmatvec_finish_range_f64(const tlh_operator_t* op, int type_begin, int type_end, double* out,
+                                            tlh_stream_t stream) {
+  if (!op || !out || type_begin < 0 || type_end > op->ntypes || type_begin > type_end) return TLH_EINVAL;
+  for (int ti = type_begin; ti < type_end; ++ti) {
     const tlh_type_t& t = op->types[ti];
     const int64_t total = t.count * t.dim;
     if (total == 0) continue;

@@ -12,6 +12,7 @@
 // row is read once and nothing but the final blocks is written.
 //
 //   tlh_major_gram          D_v = sum_{obs of v} J^T J            (sparse.py:104-121)
+//   tlh_major_rmatvec       g_v = sum_{obs of v} J^T r            (sparse.py:43-55, problem.py:40-42)
 //   tlh_schur_precond_major M_c = D_c + lambda I - sum_{edges e of c} W_e C_p^-1 W_e^T,
 //                           W_e = sum_{obs in e} J_c^T J_p        (schur.py:353-362)
 //
@@ -166,6 +167,55 @@ __global__ void __launch_bounds__(MAJOR_THREADS)
   }
 }
 
+// g_v = sum over the variable's observations of J^T v (float64), one workgroup
+// per variable, the same fixed order as major_gram_kernel.
+template <typename BT, typename VT, int R, int D>
+__global__ void __launch_bounds__(MAJOR_THREADS)
+    major_rmatvec_kernel(const BT* __restrict__ blocks, const VT* __restrict__ vec, const int32_t* __restrict__ order,
+                         const int32_t* __restrict__ ptr, double* __restrict__ out) {
+  __shared__ double smem[(MAJOR_THREADS / 64 + 1) * D];
+  const int v = blockIdx.x;
+  const int k0 = ptr[v], k1 = ptr[v + 1];
+  double acc[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) acc[c] = 0.0;
+#pragma unroll 4
+  for (int k = k0 + (int)threadIdx.x; k < k1; k += MAJOR_THREADS) {
+    const int64_t o = order[k];
+    const BT* row = blocks + o * (R * D);
+    double vr[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) vr[r] = (double)vec[o * R + r];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) s += (double)row[r * D + c] * vr[r];
+      acc[c] += s;
+    }
+  }
+  major_block_sum<D>(acc, smem);
+  if ((int)threadIdx.x < D) out[(int64_t)v * D + threadIdx.x] = smem[(MAJOR_THREADS / 64) * D + threadIdx.x];
+}
+
+template <int R, int D>
+void launch_rmatvec(const void* blocks, int block_dtype, const void* vec, int v_dtype, const int32_t* order,
+                    const int32_t* ptr, int64_t count, double* out, hipStream_t s) {
+  const dim3 grid((unsigned)count), block(MAJOR_THREADS);
+  if (block_dtype == TLH_F64 && v_dtype == TLH_F64)
+    hipLaunchKernelGGL((major_rmatvec_kernel<double, double, R, D>), grid, block, 0, s,
+                       static_cast<const double*>(blocks), static_cast<const double*>(vec), order, ptr, out);
+  else if (block_dtype == TLH_F64)
+    hipLaunchKernelGGL((major_rmatvec_kernel<double, float, R, D>), grid, block, 0, s,
+                       static_cast<const double*>(blocks), static_cast<const float*>(vec), order, ptr, out);
+  else if (v_dtype == TLH_F64)
+    hipLaunchKernelGGL((major_rmatvec_kernel<float, double, R, D>), grid, block, 0, s,
+                       static_cast<const float*>(blocks), static_cast<const double*>(vec), order, ptr, out);
+  else
+    hipLaunchKernelGGL((major_rmatvec_kernel<float, float, R, D>), grid, block, 0, s,
+                       static_cast<const float*>(blocks), static_cast<const float*>(vec), order, ptr, out);
+}
+
 template <int R, int D>
 bool launch_gram(const void* blocks, int block_dtype, const int32_t* order, const int32_t* ptr, int64_t count,
                  double* out, hipStream_t s) {
@@ -215,6 +265,26 @@ extern "C" int tlh_schur_precond_major(const void* jc, const void* jp, int block
     hipLaunchKernelGGL((major_precond_kernel<float, 2, 6, 3>), dim3((unsigned)count), dim3(MAJOR_THREADS), 0, s,
                        static_cast<const float*>(jc), static_cast<const float*>(jp), order, edge_ptr, var_edge_ptr,
                        edge_point, cinv, D, damping, M);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+extern "C" int tlh_major_rmatvec(const void* blocks, int block_dtype, const void* v, int v_dtype, int rows, int dim,
+                                 const int32_t* order, const int32_t* ptr, int64_t count, double* out,
+                                 tlh_stream_t stream) {
+  const bool dtypes_ok = (block_dtype == TLH_F64 || block_dtype == TLH_F32) && (v_dtype == TLH_F64 || v_dtype == TLH_F32);
+  if (count < 0 || count > INT32_MAX || !dtypes_ok) return TLH_EINVAL;
+  if (count == 0) return 0;
+  if (!blocks || !v || !order || !ptr || !out) return TLH_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (rows == 2 && dim == 6)
+    launch_rmatvec<2, 6>(blocks, block_dtype, v, v_dtype, order, ptr, count, out, s);
+  else if (rows == 2 && dim == 3)
+    launch_rmatvec<2, 3>(blocks, block_dtype, v, v_dtype, order, ptr, count, out, s);
+  else if (rows == 6 && dim == 6)
+    launch_rmatvec<6, 6>(blocks, block_dtype, v, v_dtype, order, ptr, count, out, s);
+  else
+    return TLH_ENOTFOUND;
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }

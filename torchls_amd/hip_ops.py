@@ -62,22 +62,47 @@ class LinearizationKernels:
         op = self.diag_plan.dev.operator(torch.float64, [None] * len(jacobian.groups), self.grad_partials)
         stream = _lib.stream_handle(self.device)
         lds = _lib.device_info(self.device).lds_per_block
+        # types with few, heavily observed variables (cameras) are summed in variable order when
+        # their group takes the two-slot float64 kernel (16-byte aligned rows), which can skip a slot
+        slot_of = {(gi, s): t for t, (gi, s, _, _) in self.diag_plan.major.items()}
+        used = {}
         offset = 0
         keep = []
         for gi, group in enumerate(jacobian.groups):
             count, rows = group.blocks[0].shape[:2]
             blocks = [b.contiguous() for b in group.blocks]
             keep.append(blocks)
-            bptr = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in blocks] + [None] * (3 - len(blocks)))
             v = residual[offset : offset + count * rows]
             offset += count * rows
+            pair = (len(blocks) == 2 and rows == 2 and v.dtype == torch.float64 and v.data_ptr() % 16 == 0
+                    and all(b.dtype == torch.float64 and b.data_ptr() % 16 == 0 for b in blocks))
+            ptrs = []
+            for s, blk in enumerate(blocks):
+                if pair and (gi, s) in slot_of:
+                    used[slot_of[(gi, s)]] = (blk, v)
+                    ptrs.append(None)
+                else:
+                    ptrs.append(blk.data_ptr())
+            bptr = (ctypes.c_void_p * 3)(*ptrs + [None] * (3 - len(blocks)))
             _lib.check(
                 self.lib.tlh_rmatvec_f64(ctypes.byref(op), gi, bptr, _lib.dtype_code(blocks[0].dtype), v.data_ptr(),
                                          _lib.dtype_code(v.dtype), lds, stream),
                 "tlh_rmatvec_f64",
             )
         out = torch.empty(self.size, dtype=torch.float64, device=self.device)
-        _lib.check(self.lib.tlh_rmatvec_finish_f64(ctypes.byref(op), out.data_ptr(), stream), "tlh_rmatvec_finish_f64")
+        start = 0
+        for t in sorted(used) + [len(self.layout)]:
+            if start < t:
+                _lib.check(self.lib.tlh_rmatvec_finish_range_f64(ctypes.byref(op), start, t, out.data_ptr(), stream),
+                           "tlh_rmatvec_finish_range_f64")
+            start = t + 1
+        for t, (blk, v) in used.items():
+            _, _, order, ptr = self.diag_plan.major[t]
+            ent = self.layout[t]
+            _lib.check(self.lib.tlh_major_rmatvec(blk.data_ptr(), _lib.dtype_code(blk.dtype), v.data_ptr(),
+                                                  _lib.dtype_code(v.dtype), blk.shape[1], ent.dim, order.data_ptr(),
+                                                  ptr.data_ptr(), ent.count, out[ent.offset:].data_ptr(), stream),
+                       "tlh_major_rmatvec")
         return out
 
     def squared_matvec_norm(self, jacobian, x: torch.Tensor) -> torch.Tensor:

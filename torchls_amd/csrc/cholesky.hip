@@ -3,7 +3,7 @@
 // Replaces torch.linalg.cholesky / cholesky_solve on the dense Schur matrix
 // (solvers.py:516-518).  Blocked right-looking algorithm on the lower
 // triangle, panel width 64:
-//   1. panel kernel (one wavefront per workgroup, a row per lane): every
+//   1. panel kernel (four wavefronts per workgroup, a row per lane): every
 //      workgroup factors the 64x64 diagonal block (redundantly, no
 //      inter-workgroup sync); workgroup 0 writes L_kk, the others solve one
 //      64-row block of the panel below: A_ik <- A_ik L_kk^-T;
@@ -47,38 +47,53 @@ __device__ __forceinline__ double dot_sub(const double (&v)[SUB], const double* 
   return (p[0] + p[1]) + (p[2] + p[3]);
 }
 
+// Panel workgroups hold four wavefronts: wavefront 0 runs the pivot (and
+// triangular) steps of each 16-column sub-panel, lane r owning row r, and all
+// four then share the update of the columns right of the sub-panel, wavefront
+// w taking columns c0 + 16 + w, c0 + 20 + w, ... (every (row, column) update is
+// the same dot_sub as with one wavefront, so the factor is unchanged).
+constexpr int PANEL_WAVES = 4;
+constexpr int PANEL_THREADS = 64 * PANEL_WAVES;
+
 // Cholesky of the 64x64 block in W (row r = lane; lower triangle; padded rows
 // hold the identity).  Right-looking by sub-panels of 16 columns.
-__device__ __forceinline__ void factor_block(double (*W)[NB + 1], int lane, bool& ok) {
+__device__ __forceinline__ void factor_block(double (*W)[NB + 1], int wave, int lane, bool& ok) {
 #pragma unroll
   for (int sp = 0; sp < NB / SUB; ++sp) {
     const int c0 = sp * SUB;
-    double v[SUB];
+    if (wave == 0) {
+      double v[SUB];
 #pragma unroll
-    for (int c = 0; c < SUB; ++c) v[c] = W[lane][c0 + c];
+      for (int c = 0; c < SUB; ++c) v[c] = W[lane][c0 + c];
 #pragma unroll
-    for (int j = 0; j < SUB; ++j) {
-      const int pr = c0 + j;
-      const double dj = readlane_f64(v[j], pr);
-      const bool good = dj > 0.0 && isfinite(dj);
-      if (!good) ok = false;
-      const double p = sqrt(good ? dj : 1.0);
-      const double inv = 1.0 / p;
-      if (lane == pr) v[j] = p;
-      else if (lane > pr) v[j] *= inv;
+      for (int j = 0; j < SUB; ++j) {
+        const int pr = c0 + j;
+        const double dj = readlane_f64(v[j], pr);
+        const bool good = dj > 0.0 && isfinite(dj);
+        if (!good) ok = false;
+        const double p = sqrt(good ? dj : 1.0);
+        const double inv = 1.0 / p;
+        if (lane == pr) v[j] = p;
+        else if (lane > pr) v[j] *= inv;
 #pragma unroll
-      for (int c = j + 1; c < SUB; ++c) {
-        const double lc = readlane_f64(v[j], c0 + c);  // L[c0 + c][pr]
-        if (lane >= c0 + c) v[c] -= v[j] * lc;
+        for (int c = j + 1; c < SUB; ++c) {
+          const double lc = readlane_f64(v[j], c0 + c);  // L[c0 + c][pr]
+          if (lane >= c0 + c) v[c] -= v[j] * lc;
+        }
       }
-    }
 #pragma unroll
-    for (int c = 0; c < SUB; ++c) W[lane][c0 + c] = v[c];
-    __syncthreads();
-    for (int c = c0 + SUB; c < NB; ++c) {
-      if (lane >= c) W[lane][c] -= dot_sub(v, &W[c][c0]);
+      for (int c = 0; c < SUB; ++c) W[lane][c0 + c] = v[c];
     }
     __syncthreads();
+    if (c0 + SUB < NB) {
+      double v[SUB];
+#pragma unroll
+      for (int c = 0; c < SUB; ++c) v[c] = W[lane][c0 + c];
+      for (int c = c0 + SUB + wave; c < NB; c += PANEL_WAVES) {
+        if (lane >= c) W[lane][c] -= dot_sub(v, &W[c][c0]);
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -86,55 +101,70 @@ __device__ __forceinline__ void factor_block(double (*W)[NB + 1], int lane, bool
 // inter-workgroup sync); workgroup 0 writes L_kk, workgroup w >= 1 solves
 // X L_kk^T = A_ik for the w-th 64-row block below, lane r forming row r of X
 // independently (sub-panel by sub-panel, L_kk read from LDS by every lane at
-// the same address).
-__global__ void __launch_bounds__(PT)
+// the same address).  The rows of A_ik are loaded before the factorisation so
+// those loads overlap it.
+__global__ void __launch_bounds__(PANEL_THREADS)
     chol_panel_kernel(double* __restrict__ A, int64_t n, int64_t ld, int64_t k, int32_t* status) {
   __shared__ double W[NB][NB + 1];
   __shared__ double X[NB][NB + 1];
   __shared__ double inv_diag[NB];
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
   const int kb = (int)min((int64_t)NB, n - k);
-#pragma unroll 16
-  for (int r = 0; r < NB; ++r)  // coalesced: lane = column; unrolled so the row loads overlap
+  const int64_t r0 = k + kb + (int64_t)((int)blockIdx.x - 1) * NB;
+  const int rows = blockIdx.x == 0 ? 0 : (int)max((int64_t)0, min((int64_t)NB, n - r0));
+  if (blockIdx.x != 0 && rows == 0) return;  // uniform over the workgroup
+#pragma unroll
+  for (int i = 0; i < NB / PANEL_WAVES; ++i) {  // coalesced: lane = column
+    const int r = wave + PANEL_WAVES * i;
     W[r][lane] = (r < kb && lane <= r) ? A[(k + r) * ld + k + lane] : (r == lane ? 1.0 : 0.0);
+    if (rows > 0) X[r][lane] = (r < rows && lane < kb) ? A[(r0 + r) * ld + k + lane] : 0.0;
+  }
   __syncthreads();
   bool ok = true;
-  factor_block(W, lane, ok);
-  const bool all_ok = __builtin_amdgcn_readfirstlane((int)__any(!ok)) == 0;
+  factor_block(W, wave, lane, ok);
   if (blockIdx.x == 0) {
-#pragma unroll 16
-    for (int r = 0; r < kb; ++r)
-      if (lane <= r) A[(k + r) * ld + k + lane] = W[r][lane];
-    if (lane == 0 && !all_ok) __hip_atomic_fetch_or(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const bool all_ok = __builtin_amdgcn_readfirstlane((int)__any(!ok)) == 0;
+#pragma unroll
+    for (int i = 0; i < NB / PANEL_WAVES; ++i) {
+      const int r = wave + PANEL_WAVES * i;
+      if (r < kb && lane <= r) A[(k + r) * ld + k + lane] = W[r][lane];
+    }
+    if (threadIdx.x == 0 && !all_ok) __hip_atomic_fetch_or(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
-  const int64_t r0 = k + kb + (int64_t)(blockIdx.x - 1) * NB;
-  const int rows = (int)min((int64_t)NB, n - r0);
-  if (rows <= 0) return;
-  inv_diag[lane] = 1.0 / W[lane][lane];
-#pragma unroll 16
-  for (int r = 0; r < NB; ++r) X[r][lane] = (r < rows && lane < kb) ? A[(r0 + r) * ld + k + lane] : 0.0;
+  if (wave == 0) inv_diag[lane] = 1.0 / W[lane][lane];
   __syncthreads();
 #pragma unroll
   for (int sp = 0; sp < NB / SUB; ++sp) {
     const int c0 = sp * SUB;
-    double v[SUB];
+    if (wave == 0) {
+      double v[SUB];
 #pragma unroll
-    for (int c = 0; c < SUB; ++c) v[c] = X[lane][c0 + c];
+      for (int c = 0; c < SUB; ++c) v[c] = X[lane][c0 + c];
 #pragma unroll
-    for (int j = 0; j < SUB; ++j) {
-      v[j] *= inv_diag[c0 + j];
+      for (int j = 0; j < SUB; ++j) {
+        v[j] *= inv_diag[c0 + j];
 #pragma unroll
-      for (int c = j + 1; c < SUB; ++c) v[c] -= v[j] * W[c0 + c][c0 + j];
+        for (int c = j + 1; c < SUB; ++c) v[c] -= v[j] * W[c0 + c][c0 + j];
+      }
+#pragma unroll
+      for (int c = 0; c < SUB; ++c) X[lane][c0 + c] = v[c];
     }
+    __syncthreads();
+    if (c0 + SUB < NB) {
+      double v[SUB];
 #pragma unroll
-    for (int c = 0; c < SUB; ++c) X[lane][c0 + c] = v[c];
-    for (int c = c0 + SUB; c < NB; ++c) X[lane][c] -= dot_sub(v, &W[c][c0]);
+      for (int c = 0; c < SUB; ++c) v[c] = X[lane][c0 + c];
+      for (int c = c0 + SUB + wave; c < NB; c += PANEL_WAVES) X[lane][c] -= dot_sub(v, &W[c][c0]);
+      __syncthreads();
+    }
   }
-  __syncthreads();
-#pragma unroll 16
-  for (int r = 0; r < rows; ++r)
-    if (lane < kb) A[(r0 + r) * ld + k + lane] = X[r][lane];
+#pragma unroll
+  for (int i = 0; i < NB / PANEL_WAVES; ++i) {
+    const int r = wave + PANEL_WAVES * i;
+    if (r < rows && lane < kb) A[(r0 + r) * ld + k + lane] = X[r][lane];
+  }
 }
 
 // Trailing update of the lower blocks (bi >= bj) right of panel k.
@@ -288,7 +318,8 @@ extern "C" int tlh_cholesky_f64(double* A, int64_t n, int64_t ld, int32_t* statu
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   for (int64_t k = 0; k < n; k += NB) {
     const int64_t below = n - k - NB > 0 ? ceil_div(n - k - NB, NB) : 0;
-    hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)(1 + below)), dim3(PT), 0, s, A, n, ld, k, status);
+    hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)(1 + below)), dim3(PANEL_THREADS), 0, s, A, n, ld, k,
+                       status);
     TLH_RETURN_IF_LAUNCH_FAILED();
     if (below > 0) {
       const int64_t blocks = below * (below + 1) / 2;

@@ -3,7 +3,7 @@
 // Replaces torch.linalg.cholesky / cholesky_solve on the dense Schur matrix
 // (solvers.py:516-518).  Blocked right-looking algorithm on the lower
 // triangle, panel width 64:
-//   1. panel kernel (four wavefronts per workgroup, a row per lane): every
+//   1. panel kernel (eight wavefronts per workgroup, a row per lane): every
 //      workgroup factors the 64x64 diagonal block (redundantly, no
 //      inter-workgroup sync); workgroup 0 writes L_kk, the others solve one
 //      64-row block of the panel below: A_ik <- A_ik L_kk^-T;
@@ -47,12 +47,12 @@ __device__ __forceinline__ double dot_sub(const double (&v)[SUB], const double* 
   return (p[0] + p[1]) + (p[2] + p[3]);
 }
 
-// Panel workgroups hold four wavefronts: wavefront 0 runs the pivot (and
+// Panel workgroups hold eight wavefronts: wavefront 0 runs the pivot (and
 // triangular) steps of each 16-column sub-panel, lane r owning row r, and all
-// four then share the update of the columns right of the sub-panel, wavefront
-// w taking columns c0 + 16 + w, c0 + 20 + w, ... (every (row, column) update is
+// eight then share the update of the columns right of the sub-panel, wavefront
+// w taking columns c0 + 16 + w, c0 + 24 + w, ... (every (row, column) update is
 // the same dot_sub as with one wavefront, so the factor is unchanged).
-constexpr int PANEL_WAVES = 4;
+constexpr int PANEL_WAVES = 8;
 constexpr int PANEL_THREADS = 64 * PANEL_WAVES;
 
 // Cholesky of the 64x64 block in W (row r = lane; lower triangle; padded rows

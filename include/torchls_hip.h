@@ -399,6 +399,15 @@ int tlh_finish_types(const tlh_operator_t* op, int type_begin, int type_end, con
  * J^T J x contributions to its types' partials. */
 int tlh_group_tiles(const tlh_operator_t* op, int group, const void* x, tlh_stream_t stream);
 
+/* dst[di(i)] = scale * src[si(i)] for i < count, si(i) = src_index ? src_index[i]
+ * : src_offset + i (likewise di), converted through float64 (float32/float64 on
+ * either side): the gathers and scatters of a Schur step in one launch each.
+ * Replaces: the slicing, index gathers/scatters and casts of
+ * DampedSchurSystem.reduced_rhs / backsubstitute (schur.py:285-301). */
+int tlh_copy_cast(const void* src, int src_dtype, const int64_t* src_index, int64_t src_offset, void* dst,
+                  int dst_dtype, const int64_t* dst_index, int64_t dst_offset, int64_t count, double scale,
+                  tlh_stream_t stream);
+
 /* ---------------------------------------------------------------- dense Cholesky */
 /* In-place lower Cholesky A = L L^T (row-major, leading dimension ld, float64):
  * blocked right-looking, 64-wide panels, MFMA (v_mfma_f64_16x16x4_f64) trailing

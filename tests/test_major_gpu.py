@@ -169,3 +169,28 @@ def test_gradient_major_matches_tile_path(device):
         row += n * rows
     want = (dense.T @ r.cpu()).numpy()
     assert rel_err(g_major.cpu().numpy(), want) <= 1e-12
+
+
+@pytest.mark.parametrize("src_dt,dst_dt", [(torch.float64, torch.float32), (torch.float32, torch.float64),
+                                           (torch.float64, torch.float64)])
+def test_copy_cast_gathers_and_scatters(device, src_dt, dst_dt):
+    print(f"copy_cast {src_dt} -> {dst_dt}", flush=True)
+    lib = _lib.load()
+    g = torch.Generator().manual_seed(7)
+    src = torch.randn(1000, generator=g, dtype=torch.float64).to(src_dt).to(device)
+    idx = torch.randperm(1000, generator=g)[:300].to(device)
+    out = torch.full((300,), np.nan, dtype=dst_dt, device=device)
+    stream = _lib.stream_handle(device)
+    # gather with a scale
+    _lib.check(lib.tlh_copy_cast(src.data_ptr(), _lib.dtype_code(src_dt), idx.data_ptr(), 0, out.data_ptr(),
+                                 _lib.dtype_code(dst_dt), None, 0, 300, -1.0, stream), "tlh_copy_cast")
+    want = (-src.double()[idx]).to(dst_dt)
+    assert torch.equal(out, want)
+    # scatter into an offset-free destination, then a contiguous copy at an offset
+    dst = torch.full((1000,), np.nan, dtype=dst_dt, device=device)
+    _lib.check(lib.tlh_copy_cast(out.data_ptr(), _lib.dtype_code(dst_dt), None, 0, dst.data_ptr(),
+                                 _lib.dtype_code(dst_dt), idx.data_ptr(), 0, 300, 1.0, stream), "tlh_copy_cast")
+    assert torch.equal(dst[idx], out)
+    _lib.check(lib.tlh_copy_cast(src.data_ptr(), _lib.dtype_code(src_dt), None, 10, dst.data_ptr(),
+                                 _lib.dtype_code(dst_dt), None, 500, 100, 1.0, stream), "tlh_copy_cast")
+    assert torch.equal(dst[500:600], src[10:110].double().to(dst_dt))

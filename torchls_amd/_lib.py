@@ -213,6 +213,11 @@ SIGNATURES = {
         [ctypes.POINTER(Operator), ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, ctypes.c_double, c_void_p, i64,
          c_void_p],
     ),
+    "tlh_copy_cast": (
+        ctypes.c_int,
+        [c_void_p, ctypes.c_int, c_void_p, i64, c_void_p, ctypes.c_int, c_void_p, i64, i64, ctypes.c_double,
+         c_void_p],
+    ),
     "tlh_major_rmatvec": (
         ctypes.c_int,
         [c_void_p, ctypes.c_int, c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, i64,

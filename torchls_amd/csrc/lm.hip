@@ -343,6 +343,19 @@ __global__ void __launch_bounds__(TLH_THREADS)
   grid_sum2(v, partials, counter, out, smem);
 }
 
+// dst[di(i)] = scale * src[si(i)], si(i) = sidx ? sidx[i] : soff + i (likewise di),
+// converted through float64.
+template <typename S, typename D>
+__global__ void __launch_bounds__(TLH_THREADS)
+    copy_cast_kernel(const S* __restrict__ src, const int64_t* __restrict__ sidx, int64_t soff, D* __restrict__ dst,
+                     const int64_t* __restrict__ didx, int64_t doff, int64_t count, double scale) {
+  const int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
+  if (i >= count) return;
+  const int64_t si = sidx ? sidx[i] : soff + i;
+  const int64_t di = didx ? didx[i] : doff + i;
+  dst[di] = (D)(scale * (double)src[si]);
+}
+
 }  // namespace
 
 extern "C" int tlh_rmatvec_f64(const tlh_operator_t* op, int group, const void* const* blocks, int block_dtype,
@@ -482,6 +495,31 @@ extern "C" int tlh_block_matvec(int64_t count, int rows, int nslots, const int32
   }
   hipLaunchKernelGGL(block_matvec_kernel, dim3((unsigned)ceil_div(count, TLH_THREADS)), dim3(TLH_THREADS), 0,
                      reinterpret_cast<hipStream_t>(stream), g, block_dtype, x, x_dtype, y, y_dtype);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+extern "C" int tlh_copy_cast(const void* src, int src_dtype, const int64_t* src_index, int64_t src_offset, void* dst,
+                             int dst_dtype, const int64_t* dst_index, int64_t dst_offset, int64_t count, double scale,
+                             tlh_stream_t stream) {
+  const bool ok = (src_dtype == TLH_F32 || src_dtype == TLH_F64) && (dst_dtype == TLH_F32 || dst_dtype == TLH_F64);
+  if (count < 0 || !ok) return TLH_EINVAL;
+  if (count == 0) return 0;
+  if (!src || !dst) return TLH_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const dim3 grid((unsigned)ceil_div(count, TLH_THREADS)), block(TLH_THREADS);
+  if (src_dtype == TLH_F64 && dst_dtype == TLH_F64)
+    hipLaunchKernelGGL((copy_cast_kernel<double, double>), grid, block, 0, s, static_cast<const double*>(src),
+                       src_index, src_offset, static_cast<double*>(dst), dst_index, dst_offset, count, scale);
+  else if (src_dtype == TLH_F64)
+    hipLaunchKernelGGL((copy_cast_kernel<double, float>), grid, block, 0, s, static_cast<const double*>(src),
+                       src_index, src_offset, static_cast<float*>(dst), dst_index, dst_offset, count, scale);
+  else if (dst_dtype == TLH_F64)
+    hipLaunchKernelGGL((copy_cast_kernel<float, double>), grid, block, 0, s, static_cast<const float*>(src),
+                       src_index, src_offset, static_cast<double*>(dst), dst_index, dst_offset, count, scale);
+  else
+    hipLaunchKernelGGL((copy_cast_kernel<float, float>), grid, block, 0, s, static_cast<const float*>(src),
+                       src_index, src_offset, static_cast<float*>(dst), dst_index, dst_offset, count, scale);
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }

@@ -51,6 +51,8 @@ class PcgState:
         self.desc = _lib.PcgStateDesc(self.control.data_ptr(), self.scalars.data_ptr(),
                                       self.partials.data_ptr(), self.counter.data_ptr())
         self.host = torch.zeros(8, dtype=torch.float64).pin_memory()
+        # staging of the control word's window rewrites (host -> device, non-blocking)
+        self.host_control = torch.zeros(4, dtype=torch.int32).pin_memory()
 
     @classmethod
     def scratch(cls, device) -> "PcgState":
@@ -58,6 +60,18 @@ class PcgState:
         if key not in cls._scratch:
             cls._scratch[key] = PcgState(device)
         return cls._scratch[key]
+
+    _solver: dict = {}
+
+    @classmethod
+    def for_solve(cls, device) -> "PcgState":
+        """The state of this thread's PCG solves on a device, reused by every solve:
+        tlh_pcg_begin and tlh_pcg_reset initialise it, every grid reduction leaves
+        its counter at zero, and a solve reads it back before it returns."""
+        key = (threading.get_ident(), str(device))
+        if key not in cls._solver:
+            cls._solver[key] = PcgState(device)
+        return cls._solver[key]
 
 
 # Replays per host read after the first batch: iterations past convergence are
@@ -102,7 +116,7 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None):
     n = rhs.numel()
     code = _lib.dtype_code(rhs.dtype)
     rhs = rhs.contiguous()
-    st = PcgState(device)
+    st = PcgState.for_solve(device)
     sd = ctypes.byref(st.desc)
     stream = torch.cuda.current_stream(device)
     sh = stream.cuda_stream
@@ -125,7 +139,7 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None):
         _lib.check(lib.tlh_pcg_begin(sd, float(config.atol), float(config.rtol), window_end(0),
                                      None if pd_flag is None else pd_flag.data_ptr(), sh), "tlh_pcg_begin")
 
-    x = torch.zeros_like(rhs)
+    x = torch.empty_like(rhs)  # zeroed by the reset below
     r = torch.empty_like(rhs)
     z = torch.empty_like(rhs)
     p = torch.empty_like(rhs)
@@ -205,7 +219,7 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None):
         true2, drift2, rhs2, it, status = host.tolist()
         return rhs2, true2, drift2, int(it), int(status)
 
-    host_control = torch.zeros(4, dtype=torch.int32).pin_memory()
+    host_control = st.host_control
     iterations = checks = restarts = 0
     stop_at = window_end(0)
     rhs_norm = threshold = None

@@ -378,13 +378,15 @@ class Analysis:
             blocks = tuple(x.contiguous() for x in blocks)
             bad.append(sum((~torch.isfinite(x)).sum() for x in (r, *blocks)).to(torch.float64))
             groups.append(JacobianGroup(blocks, cols, b.type_indices, idx))
-        residual = torch.cat(residuals)
+        # one batch (the common case): its tensors are used as they are, no concatenation copies
+        one = len(residuals) == 1
+        residual = residuals[0] if one else torch.cat(residuals)
         J = BlockJacobian(tuple(groups), self.layout, (residual.numel(), self.tangent_dim))
         return Linearization(
             residual, J,
-            torch.cat(raws) if self.has_loss else None,
-            0.5 * torch.stack(objectives).sum() if self.has_loss else None,
-            torch.stack(bad).sum(),
+            (raws[0] if one else torch.cat(raws)) if self.has_loss else None,
+            0.5 * (objectives[0] if one else torch.stack(objectives).sum()) if self.has_loss else None,
+            bad[0].clone() if one else torch.stack(bad).sum(),  # bad[0] views a reused reduction slot
         )
 
     def retract(self, values, delta: Tensor, *, kernel: bool = False):

@@ -278,13 +278,27 @@ class HipSchurSystem:
         return y
 
     # -- DampedSchurSystem interface
-    def reduced_rhs(self, rhs: torch.Tensor) -> torch.Tensor:
+    def _copy(self, src, src_index, src_offset, dst, dst_index, dst_offset, count, scale=1.0):
         p = self.plan
-        rhs = rhs.to(p.dtype)
-        self._bp = rhs[p.eoffset: p.eoffset + p.ecount * p.edim].contiguous()
-        b_ret = rhs[p.global_indices].contiguous()
+        _lib.check(p.lib.tlh_copy_cast(src.data_ptr(), _lib.dtype_code(src.dtype),
+                                       None if src_index is None else src_index.data_ptr(), src_offset,
+                                       dst.data_ptr(), _lib.dtype_code(dst.dtype),
+                                       None if dst_index is None else dst_index.data_ptr(), dst_offset, count,
+                                       float(scale), _lib.stream_handle(p.device)), "tlh_copy_cast")
+
+    def reduced_rhs(self, rhs: torch.Tensor, *, scale: float = 1.0) -> torch.Tensor:
+        """Reduced right-hand side of ``scale * rhs`` (the LM passes the gradient and -1)."""
+        p = self.plan
+        if rhs.dtype not in (torch.float32, torch.float64):
+            rhs = rhs.to(torch.float64)
+        rhs = rhs.contiguous()
+        self._bp = torch.empty(p.ecount * p.edim, dtype=p.dtype, device=p.device)
+        self._copy(rhs, None, p.eoffset, self._bp, None, 0, self._bp.numel(), scale)
+        b_ret = torch.empty(p.dim, dtype=p.dtype, device=p.device)
+        self._copy(rhs, p.global_indices, 0, b_ret, None, 0, p.dim, scale)
         self._apply(_lib_mode("RHS"), None, self._bp, None)
-        self._others(torch.zeros_like(b_ret))
+        if len(p.groups) > 1:
+            self._others(torch.zeros_like(b_ret))
         return self._finish(b_ret, 1.0)
 
     def matvec(self, vector: torch.Tensor) -> torch.Tensor:
@@ -303,10 +317,11 @@ class HipSchurSystem:
         xr = retained.to(p.dtype).contiguous()
         xe = torch.empty(p.ecount * p.edim, dtype=p.dtype, device=p.device)
         self._apply(_lib_mode("BACKSUB"), xr, self._bp, xe)
+        # the retained indices and the eliminated range cover the whole tangent vector
         total = sum(e.count * e.dim for e in p.layout)
-        out = torch.zeros(total, dtype=torch.float64, device=p.device)
-        out[p.global_indices] = xr.to(torch.float64)
-        out[p.eoffset: p.eoffset + p.ecount * p.edim] = xe.to(torch.float64)
+        out = torch.empty(total, dtype=torch.float64, device=p.device)
+        self._copy(xr, None, 0, out, p.global_indices, 0, p.dim)
+        self._copy(xe, None, 0, out, None, p.eoffset, xe.numel())
         return out
 
     def preconditioner_blocks(self):

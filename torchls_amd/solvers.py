@@ -383,7 +383,7 @@ def solve(
                         # blocks' positive definiteness (LinAlgError before any iteration)
                         system = schur_linear.damped(damping, defer_check=linear_solver == "schur_cg" and
                                                      conjugate_gradient.preconditioner is not None)
-                        rhs = system.reduced_rhs(-linear_gradient)
+                        rhs = system.reduced_rhs(linear_gradient, scale=-1.0)
                     with region("linear_solve"):
                         if linear_solver == "schur_dense":
                             retained = system.dense_solve(rhs)
@@ -437,18 +437,20 @@ def solve(
                 with region("trial_residual"):
                     objective, reduction = problem.evaluate_trial(proposed, lin)
                 sq = kernels.squared_matvec_norm(original_jacobian, delta)
-                pred_t = -torch.dot(gradient, delta) - 0.5 * sq
-                parts = [objective, reduction, pred_t, torch.linalg.vector_norm(delta),
-                         finite_step.to(torch.float64) if finite_step is not None else pred_t.new_ones(())]
+                gd = torch.dot(gradient, delta)
+                parts = [objective, reduction, gd, sq, torch.linalg.vector_norm(delta),
+                         finite_step.to(torch.float64) if finite_step is not None else gd.new_ones(())]
                 # The next linearization is issued before the trial is read (used if the step
                 # is accepted, dropped otherwise): its gradient and cost ride on the same read.
                 with region("linearize"):
                     lin_next = problem.linearize(proposed)
                 g_next, stats_next = examine_device(lin_next)
-                host = torch.cat((torch.stack(parts), stats_next)).cpu().tolist()
-                proposed_cost, actual, predicted, step_norm = host[:4]
+                host = torch.stack([*parts, *stats_next.unbind()]).cpu().tolist()
+                proposed_cost, actual, dot_gd, sq_jd, step_norm = host[:5]
+                predicted = -dot_gd - 0.5 * sq_jd  # the model decrease, in float64 on the host
                 # a nonfinite step is unusable: rejected without a trial record
-                usable = host[4] != 0
+                usable = host[5] != 0
+                stats_host = host[6:9]  # the next linearization's (nonfinite count, max |g|, cost)
             if usable:
                 quality = actual / predicted if predicted > 0 else -math.inf
                 accepted = math.isfinite(proposed_cost) and (
@@ -472,7 +474,7 @@ def solve(
                     residual = jacobian = original_jacobian = None
                     normal_operator = schur_linear = system = diagonal = gradient = linear_gradient = None
                     lin = lin_next
-                    head = (g_next, host[5] == 0, host[6], host[7])
+                    head = (g_next, stats_host[0] == 0, stats_host[1], stats_host[2])
                     summary.gradient_norm = head[2]
                     if termination.target_cost is not None and proposed_cost <= termination.target_cost:
                         summary.termination_reason = "target_cost"

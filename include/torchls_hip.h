@@ -192,7 +192,9 @@ int tlh_block_inverse(const double* blocks, int64_t count, int dim, double dampi
 #define TLH_PCG_TRUE2 5     /* ||rhs - A x||^2 */
 #define TLH_PCG_DRIFT2 6    /* ||(rhs - A x) - r||^2 */
 #define TLH_PCG_AUX 7       /* free slot for tlh_dot */
-#define TLH_PCG_NSCALARS 8
+#define TLH_PCG_ITERATIONS 8 /* control[0] as float64, written by tlh_pcg_true_residual */
+#define TLH_PCG_STATUS 9     /* control[1] as float64, written by tlh_pcg_true_residual */
+#define TLH_PCG_NSCALARS 10
 
 /* Device-resident PCG state (warp_pcg.py:131-132): control = {iterations,
  * status, stop_at, spare}; status 0 active, 1 recurrence tolerance, 2
@@ -262,7 +264,9 @@ int tlh_pcg_iteration_small(const tlh_block_diag_t* pc, int mode, void* x, void*
                             const void* ap, int64_t n, int dtype, const tlh_pcg_state_t* st,
                             tlh_stream_t stream);
 
-/* true_r = rhs - ax; scalars[TRUE2] = ||true_r||^2, scalars[DRIFT2] = ||true_r - r||^2.
+/* true_r = rhs - ax; scalars[TRUE2] = ||true_r||^2, scalars[DRIFT2] = ||true_r - r||^2,
+ * and scalars[ITERATIONS], scalars[STATUS] = control[0], control[1] (as float64), so
+ * the host reads a check's values as one slice [TRUE2, STATUS].
  * Replaces the true-residual check of warp_pcg.py:189-204. */
 int tlh_pcg_true_residual(const void* rhs, const void* ax, const void* r, void* true_r, int64_t n,
                           int dtype, const tlh_pcg_state_t* st, tlh_stream_t stream);

@@ -106,3 +106,18 @@ def test_hip_lm_against_reference_golden(device, name):
     assert abs(s.iterations - int(g[name + "_counts"][0])) <= 1
     assert math.isclose(s.final_cost, float(g[name + "_cost"][-1]), rel_tol=1e-8)
     assert np.allclose(sol["points"].cpu().numpy(), g[name + "_pts"], rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_step_finiteness_flag_propagates_nan_and_inf(device, dtype):
+    """The LM loop reads max |step| (one float64 reduction) as the step's finiteness flag."""
+    import math
+
+    print(f"finiteness flag dtype={dtype}", flush=True)
+    x = torch.randn(1_000_003, dtype=dtype, device=device)
+    assert math.isfinite(torch.linalg.vector_norm(x, ord=math.inf, dtype=torch.float64).item())
+    for pos in (0, 517_231, 1_000_002):
+        for bad in (float("nan"), float("inf"), -float("inf")):
+            y = x.clone()
+            y[pos] = bad
+            assert not math.isfinite(torch.linalg.vector_norm(y, ord=math.inf, dtype=torch.float64).item())

@@ -105,7 +105,8 @@ class Operator(ctypes.Structure):
 TLH_MAX_TYPES = 8
 TLH_PCG_MAX_BLOCKS = 1024
 PCG_RZ, PCG_THR2, PCG_CURVATURE, PCG_NEW_RZ, PCG_NORM2, PCG_TRUE2, PCG_DRIFT2, PCG_AUX = range(8)
-PCG_NSCALARS = 8
+PCG_ITERATIONS, PCG_STATUS = 8, 9
+PCG_NSCALARS = 10
 
 
 class BlockDiag(ctypes.Structure):

@@ -579,8 +579,8 @@ __global__ void __launch_bounds__(TLH_THREADS)
 template <typename Real>
 __global__ void __launch_bounds__(TLH_THREADS)
     true_residual_kernel(const Real* __restrict__ rhs, const Real* __restrict__ ax, const Real* __restrict__ r,
-                         Real* __restrict__ true_r, int64_t n, double* scalars, double* partials,
-                         uint32_t* counter) {
+                         Real* __restrict__ true_r, int64_t n, const int32_t* control, double* scalars,
+                         double* partials, uint32_t* counter) {
   __shared__ double smem[2 * TLH_THREADS];
   double v[2] = {0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * TLH_THREADS) {
@@ -593,6 +593,9 @@ __global__ void __launch_bounds__(TLH_THREADS)
   if (grid_sum<2>(v, partials, counter, smem) && threadIdx.x == 0) {
     scalars[TLH_PCG_TRUE2] = v[0];
     scalars[TLH_PCG_DRIFT2] = v[1];
+    // the control word beside them, so the host reads one contiguous slice
+    scalars[TLH_PCG_ITERATIONS] = (double)control[0];
+    scalars[TLH_PCG_STATUS] = (double)control[1];
   }
 }
 
@@ -775,10 +778,11 @@ extern "C" int tlh_pcg_true_residual(const void* rhs, const void* ax, const void
   const dim3 grid(grid_for(n)), block(TLH_THREADS);
   if (dtype == TLH_F32)
     hipLaunchKernelGGL(true_residual_kernel<float>, grid, block, 0, s, (const float*)rhs, (const float*)ax,
-                       (const float*)r, (float*)true_r, n, st->scalars, st->partials, st->counter);
+                       (const float*)r, (float*)true_r, n, st->control, st->scalars, st->partials, st->counter);
   else if (dtype == TLH_F64)
     hipLaunchKernelGGL(true_residual_kernel<double>, grid, block, 0, s, (const double*)rhs, (const double*)ax,
-                       (const double*)r, (double*)true_r, n, st->scalars, st->partials, st->counter);
+                       (const double*)r, (double*)true_r, n, st->control, st->scalars, st->partials,
+                       st->counter);
   else
     return TLH_EINVAL;
   TLH_RETURN_IF_LAUNCH_FAILED();

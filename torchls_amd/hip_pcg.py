@@ -215,7 +215,7 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None):
         ax = matvec(x)
         _lib.check(lib.tlh_pcg_true_residual(rhs.data_ptr(), ax.data_ptr(), r.data_ptr(), true_r.data_ptr(), n,
                                              code, sd, sh), "tlh_pcg_true_residual")
-        host = torch.cat((st.scalars[_lib.PCG_TRUE2: _lib.PCG_AUX + 1], st.control[:2].to(torch.float64))).cpu()
+        host = st.scalars[_lib.PCG_TRUE2: _lib.PCG_STATUS + 1].cpu()  # the kernel copied the control word
         true2, drift2, rhs2, it, status = host.tolist()
         return rhs2, true2, drift2, int(it), int(status)
 

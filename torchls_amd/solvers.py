@@ -290,7 +290,8 @@ def solve(
         g = kernels.gradient(jac, lin.residual)
         cost = lin.objective if lin.objective is not None else 0.5 * lin.residual.to(torch.float64).square().sum()
         bad = lin.nonfinite if lin.nonfinite is not None else (~torch.isfinite(lin.residual)).sum().double()
-        return g, torch.stack((bad.to(torch.float64), g.abs().max(), cost.to(torch.float64)))
+        gmax = torch.linalg.vector_norm(g, ord=math.inf)  # max |g| in one reduction
+        return g, torch.stack((bad.to(torch.float64), gmax, cost.to(torch.float64)))
 
     def examine(lin):
         """examine_device plus its host read: (gradient, finite, max |g|, cost)."""
@@ -368,7 +369,9 @@ def solve(
                 summary.termination_reason = "time_limit"
                 break
             summary.lambda_history.append(damping)
-            finite_step = None  # device flag of the iterative solves, read with the trial
+            # max |step| of the iterative solves (float64, one reduction), read with the trial:
+            # finite exactly when every entry is
+            finite_step = None
             try:
                 if linear_solver == "dense_qr":
                     dense = jacobian.to_dense()
@@ -398,7 +401,8 @@ def solve(
                             retained, iterations, converged = result.solution, result.iterations, result.converged
                             # the step's finiteness is read with the trial (one host read per trial)
                             usable = result.reason in ("residual_tolerance", "max_iterations", "stagnated")
-                            finite_step = torch.isfinite(retained).all()
+                            finite_step = torch.linalg.vector_norm(retained, ord=math.inf,
+                                                                   dtype=torch.float64)
                             summary.linear_restarts += result.restarts
                             summary.linear_residual_checks += result.residual_checks
                             summary.linear_relative_residual_history.append(
@@ -426,7 +430,7 @@ def solve(
                         result.residual_norm / result.rhs_norm if result.rhs_norm else result.residual_norm)
                     delta = result.solution
                     usable = result.reason in ("residual_tolerance", "max_iterations", "stagnated")
-                    finite_step = torch.isfinite(delta).all()
+                    finite_step = torch.linalg.vector_norm(delta, ord=math.inf, dtype=torch.float64)
             except (LinAlgError, torch.linalg.LinAlgError):
                 usable = False
             if usable:
@@ -439,7 +443,7 @@ def solve(
                 sq = kernels.squared_matvec_norm(original_jacobian, delta)
                 gd = torch.dot(gradient, delta)
                 parts = [objective, reduction, gd, sq, torch.linalg.vector_norm(delta),
-                         finite_step.to(torch.float64) if finite_step is not None else gd.new_ones(())]
+                         finite_step if finite_step is not None else gd.new_zeros(())]
                 # The next linearization is issued before the trial is read (used if the step
                 # is accepted, dropped otherwise): its gradient and cost ride on the same read.
                 with region("linearize"):
@@ -449,7 +453,7 @@ def solve(
                 proposed_cost, actual, dot_gd, sq_jd, step_norm = host[:5]
                 predicted = -dot_gd - 0.5 * sq_jd  # the model decrease, in float64 on the host
                 # a nonfinite step is unusable: rejected without a trial record
-                usable = host[5] != 0
+                usable = math.isfinite(host[5])
                 stats_host = host[6:9]  # the next linearization's (nonfinite count, max |g|, cost)
             if usable:
                 quality = actual / predicted if predicted > 0 else -math.inf

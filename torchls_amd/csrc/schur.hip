@@ -111,6 +111,17 @@ __global__ void __launch_bounds__(NTH) schur_tile_kernel(SchurArgs<Real> a) {
       for (int c = 0; c < DP; ++c) jp[r][c] = a.packed[(int64_t)(OP + r * DP + c) * a.stride + n];
     }
     const int rk = a.rank_c ? (int)a.rank_c[n] : min(pos, len - 1);
+    // this thread's camera segment (a tile of at most NTH observations has at most NTH),
+    // loaded here so the reduction after the barriers does not wait on memory
+    int cseg = -1, clo = 0, chi = 0;
+    if (MODE != SCHUR_BACKSUB) {
+      const int cb = a.seg_ptr_c[t], ce = a.seg_ptr_c[t + 1];
+      if (cb + pos < ce) {
+        cseg = a.seg_target_c[cb + pos];
+        clo = a.seg_start_c[cb + pos];
+        chi = (cb + pos + 1 < ce) ? a.seg_start_c[cb + pos + 1] : len;
+      }
+    }
     // point boundaries; a lane past the tile's end is a segment of its own
     const int vp = a.idx_p[n];
     const bool first = !valid || pos == 0 || a.idx_p[n - 1] != vp;
@@ -221,17 +232,14 @@ __global__ void __launch_bounds__(NTH) schur_tile_kernel(SchurArgs<Real> a) {
       }
     }
     __syncthreads();
-    const int cb = a.seg_ptr_c[t], ce = a.seg_ptr_c[t + 1];
-    for (int k = cb + (int)threadIdx.x; k < ce; k += NTH) {
-      const int clo = a.seg_start_c[k];
-      const int chi = (k + 1 < ce) ? a.seg_start_c[k + 1] : len;
+    if (cseg >= 0) {
       Real sum[DC];
 #pragma unroll
       for (int c = 0; c < DC; ++c) sum[c] = stage[clo * DC + c];
       for (int i = clo + 1; i < chi; ++i)
 #pragma unroll
         for (int c = 0; c < DC; ++c) sum[c] += stage[i * DC + c];
-      const int target = a.seg_target_c[k];
+      const int target = cseg;
       if (dense) {
 #pragma unroll
         for (int c = 0; c < DC; ++c) acc[target * DC + c] += sum[c];

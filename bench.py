@@ -277,6 +277,31 @@ def dense_schur_cholesky(device, reps):
             "us_per_solve": round(t_solve * 1e6, 1)}
 
 
+TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "bench_hbm_traffic.md")
+
+
+def profiled_traffic(kernels):
+    """HBM bytes per launch of the given (kernel name prefix, grid) launches, summed, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this benchmark (copy-calibrated);
+    None when the summary or a kernel is missing."""
+    try:
+        lines = open(TRAFFIC_SUMMARY).read().splitlines()
+    except OSError:
+        return None
+    rows = []
+    for line in lines:
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[1].isdigit() and cells[2].isdigit():
+            rows.append((cells[0], int(cells[1]), int(cells[2]), float(cells[5])))
+    total = 0.0
+    for name, grid in kernels:
+        match = [r for r in rows if r[0] == name and (grid is None or r[1] == grid)]
+        if not match:
+            return None
+        total += max(match, key=lambda r: r[2])[3] * 1e6  # the most frequent launch of that kernel
+    return total
+
+
 def event_time(fn, reps):
     """Average seconds per call: HIP events on the current stream around ``reps`` calls."""
     for _ in range(5):
@@ -354,6 +379,8 @@ def main():
     # implicit Schur matvec algorithmic bytes: blocks + two int32 ids per observation,
     # C^-1 (float64 upper 3x3) per point, camera vector in/out
     schur_bytes = N * (2 * 9 * 4 + 8) + n_p * 6 * 8 + 2 * 6 * n_c * 4
+    schur_traffic = profiled_traffic([("void schur_tile_kernel<float, 2, 6, 3, true, 1, 0, 1024, true>", None),
+                                      ("void finish_dense_kernel<float, 8>", None)])
     schur_gbs = schur_bytes / t_schur / 1e9
     t_normal = normal_matvec_timing(P, torch.float32, args.kernel_reps)
     n = 6 * n_c + 3 * n_p
@@ -411,7 +438,8 @@ def main():
         "normal_matvec_gbs": round(normal_gbs, 1), "normal_matvec_us": round(t_normal * 1e6, 2),
         "first_solve_extra_s": round(plan_s, 4),
         "roofline": {"bound": "hbm", "achieved": round(schur_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(schur_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(schur_gbs / HBM_PEAK_GBS, 4), "traffic": schur_traffic,
+                     "traffic_source": "profiles/bench_hbm_traffic.md (rocprofv3 FETCH_SIZE, WRITE_SIZE)",
                      "kernel": "implicit Schur matvec (schur_tile_kernel + finish)",
                      "us_per_launch": round(t_schur * 1e6, 2), "bytes_per_launch": schur_bytes},
         "normal_matvec_large": large, "lm_large": lm5m, "pose_graph": pgraph,

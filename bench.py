@@ -379,8 +379,9 @@ def main():
     # implicit Schur matvec algorithmic bytes: blocks + two int32 ids per observation,
     # C^-1 (float64 upper 3x3) per point, camera vector in/out
     schur_bytes = N * (2 * 9 * 4 + 8) + n_p * 6 * 8 + 2 * 6 * n_c * 4
+    # the camera finish of the Schur matvec: 8 columns per 256-thread workgroup
     schur_traffic = profiled_traffic([("void schur_tile_kernel<float, 2, 6, 3, true, 1, 0, 1024, true>", None),
-                                      ("void finish_dense_kernel<float, 8>", None)])
+                                      ("void finish_jobs_kernel<float>", -(-n_c * 6 // 8) * 256)])
     schur_gbs = schur_bytes / t_schur / 1e9
     t_normal = normal_matvec_timing(P, torch.float32, args.kernel_reps)
     n = 6 * n_c + 3 * n_p

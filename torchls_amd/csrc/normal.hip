@@ -646,84 +646,115 @@ __global__ void __launch_bounds__(TLH_THREADS)
   flush_dense(g, acc);
 }
 
-template <typename Real>
-__global__ void __launch_bounds__(TLH_THREADS)
-    finish_kernel(Real* __restrict__ y, const Real* __restrict__ x, Real damping,
-                  const Real* __restrict__ partials, const int32_t* __restrict__ csr_ptr,
-                  const int32_t* __restrict__ csr_idx, int64_t count, int dim, int dense_blocks,
-                  const int32_t* __restrict__ list, int64_t nlist) {
-  const int64_t e = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
-  if (e >= nlist * dim) return;
-  const int64_t le = e / dim;
-  const int c = (int)(e - le * dim);
-  const int64_t v = list ? (int64_t)list[le] : le;
-  const int64_t i = v * dim + c;
-  Real sum = Real(0);
-  for (int b = 0; b < dense_blocks; ++b) sum += partials[((int64_t)b * count + v) * dim + c];
-  const int k1 = csr_ptr[v + 1];
-  for (int k = csr_ptr[v]; k < k1; ++k) sum += partials[(int64_t)csr_idx[k] * dim + c];
-  y[i] = sum + damping * x[i];
-}
-
-// Finish for types with many dense blocks: a workgroup owns COLS consecutive
-// columns; its 256 / COLS column lanes each sum every (256 / COLS)-th dense
-// block (fixed stride), then the lane sums are added in lane order.  Same
-// result for every run.  Few columns (a small camera set) take COLS = 8 so the
-// grid still spreads over the chip and each lane sums only a few blocks.
+// Finishes: y[v] = sum of the variable's partials (dense blocks, then its CSR
+// list) + damping * x[v].  Every type of one finish call is a job of a single
+// launch (the jobs write disjoint slices of y), each job taking one of three
+// shapes below; a workgroup finds its job from the jobs' block counts.
 constexpr int FIN_COLS = 32;
 constexpr int FIN_COLS_NARROW = 8;
+enum FinishRole { FIN_LIST = 0, FIN_WAVE = 1, FIN_DENSE = 2, FIN_DENSE_NARROW = 3 };
 
+template <typename Real>
+struct FinishJob {
+  Real* y;
+  const Real* x;
+  const Real* partials;
+  const int32_t* csr_ptr;
+  const int32_t* csr_idx;
+  const int32_t* list;
+  int64_t count;
+  int64_t nlist;
+  int32_t dim;
+  int32_t dense_blocks;
+  int32_t role;
+  int32_t blocks;  // workgroups of this job
+};
+
+template <typename Real>
+struct FinishJobs {
+  FinishJob<Real> job[TLH_MAX_TYPES];
+  int32_t njobs;
+  Real damping;
+};
+
+// One thread per (listed variable, component).
+template <typename Real>
+__device__ __forceinline__ void finish_list(const FinishJob<Real>& j, Real damping, int64_t bid) {
+  const int64_t e = bid * TLH_THREADS + threadIdx.x;
+  if (e >= j.nlist * j.dim) return;
+  const int64_t le = e / j.dim;
+  const int c = (int)(e - le * j.dim);
+  const int64_t v = j.list ? (int64_t)j.list[le] : le;
+  const int64_t i = v * j.dim + c;
+  Real sum = Real(0);
+  for (int b = 0; b < j.dense_blocks; ++b) sum += j.partials[((int64_t)b * j.count + v) * j.dim + c];
+  const int k1 = j.csr_ptr[v + 1];
+  for (int k = j.csr_ptr[v]; k < k1; ++k) sum += j.partials[(int64_t)j.csr_idx[k] * j.dim + c];
+  j.y[i] = sum + damping * j.x[i];
+}
+
+// Types with many dense blocks: a workgroup owns COLS consecutive columns; its
+// 256 / COLS column lanes each sum every (256 / COLS)-th dense block (fixed
+// stride), then the lane sums are added in lane order.  Few columns (a small
+// camera set) take COLS = 8 so the grid still spreads over the chip.
 template <typename Real, int COLS>
-__global__ void __launch_bounds__(TLH_THREADS)
-    finish_dense_kernel(Real* __restrict__ y, const Real* __restrict__ x, Real damping,
-                        const Real* __restrict__ partials, const int32_t* __restrict__ csr_ptr,
-                        const int32_t* __restrict__ csr_idx, int64_t count, int dim, int dense_blocks,
-                        const int32_t* __restrict__ list, int64_t nlist) {
+__device__ __forceinline__ void finish_dense(const FinishJob<Real>& j, Real damping, int64_t bid, Real* lane_sum) {
   constexpr int LANES = TLH_THREADS / COLS;
-  __shared__ Real lane_sum[LANES][COLS];
-  (void)list;
-  (void)nlist;
   const int col = threadIdx.x % COLS;
   const int lane = threadIdx.x / COLS;
-  const int64_t total = count * dim;
-  const int64_t i = (int64_t)blockIdx.x * COLS + col;
+  const int64_t total = j.count * j.dim;
+  const int64_t i = bid * COLS + col;
   Real sum = Real(0);
   if (i < total) {
 #pragma unroll 8
-    for (int b = lane; b < dense_blocks; b += LANES) sum += partials[(int64_t)b * total + i];
+    for (int b = lane; b < j.dense_blocks; b += LANES) sum += j.partials[(int64_t)b * total + i];
   }
-  lane_sum[lane][col] = sum;
+  lane_sum[lane * COLS + col] = sum;
   __syncthreads();
   if (lane == 0 && i < total) {
-    Real acc = lane_sum[0][col];
+    Real acc = lane_sum[col];
 #pragma unroll
-    for (int l = 1; l < LANES; ++l) acc += lane_sum[l][col];
-    const int64_t v = i / dim;
-    const int c = (int)(i - v * dim);
-    const int k1 = csr_ptr[v + 1];
-    for (int k = csr_ptr[v]; k < k1; ++k) acc += partials[(int64_t)csr_idx[k] * dim + c];
-    y[i] = acc + damping * x[i];
+    for (int l = 1; l < LANES; ++l) acc += lane_sum[l * COLS + col];
+    const int64_t v = i / j.dim;
+    const int c = (int)(i - v * j.dim);
+    const int k1 = j.csr_ptr[v + 1];
+    for (int k = j.csr_ptr[v]; k < k1; ++k) acc += j.partials[(int64_t)j.csr_idx[k] * j.dim + c];
+    j.y[i] = acc + damping * j.x[i];
   }
 }
 
 // One wavefront per (variable, component) for types with long partial lists.
 template <typename Real>
-__global__ void __launch_bounds__(TLH_THREADS)
-    finish_wave_kernel(Real* __restrict__ y, const Real* __restrict__ x, Real damping,
-                       const Real* __restrict__ partials, const int32_t* __restrict__ csr_ptr,
-                       const int32_t* __restrict__ csr_idx, int64_t count, int dim, int dense_blocks,
-                       const int32_t* __restrict__ list, int64_t nlist) {
+__device__ __forceinline__ void finish_wave(const FinishJob<Real>& j, Real damping, int64_t bid) {
   const int lane = threadIdx.x & 63;
-  const int64_t e = (int64_t)blockIdx.x * (TLH_THREADS / 64) + (threadIdx.x >> 6);
-  if (e >= nlist * dim) return;
-  const int64_t le = e / dim;
-  const int c = (int)(e - le * dim);
-  const int64_t v = list ? (int64_t)list[le] : le;
-  const Real sum = csr_wave_sum<Real>(partials, csr_idx, csr_ptr[v], csr_ptr[v + 1], dim, c, lane);
+  const int64_t e = bid * (TLH_THREADS / 64) + (threadIdx.x >> 6);
+  if (e >= j.nlist * j.dim) return;
+  const int64_t le = e / j.dim;
+  const int c = (int)(e - le * j.dim);
+  const int64_t v = j.list ? (int64_t)j.list[le] : le;
+  const Real sum = csr_wave_sum<Real>(j.partials, j.csr_idx, j.csr_ptr[v], j.csr_ptr[v + 1], j.dim, c, lane);
   if (lane == 0) {
     Real d = Real(0);
-    for (int b = 0; b < dense_blocks; ++b) d += partials[((int64_t)b * count + v) * dim + c];
-    y[v * dim + c] = (d + sum) + damping * x[v * dim + c];
+    for (int b = 0; b < j.dense_blocks; ++b) d += j.partials[((int64_t)b * j.count + v) * j.dim + c];
+    j.y[v * j.dim + c] = (d + sum) + damping * j.x[v * j.dim + c];
+  }
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(TLH_THREADS) finish_jobs_kernel(FinishJobs<Real> jobs) {
+  __shared__ Real lane_sum[TLH_THREADS];
+  int64_t bid = blockIdx.x;
+  int k = 0;
+  while (k + 1 < jobs.njobs && bid >= jobs.job[k].blocks) {
+    bid -= jobs.job[k].blocks;
+    ++k;
+  }
+  const FinishJob<Real>& j = jobs.job[k];
+  switch (j.role) {  // uniform over the workgroup
+    case FIN_DENSE: finish_dense<Real, FIN_COLS>(j, jobs.damping, bid, lane_sum); break;
+    case FIN_DENSE_NARROW: finish_dense<Real, FIN_COLS_NARROW>(j, jobs.damping, bid, lane_sum); break;
+    case FIN_WAVE: finish_wave<Real>(j, jobs.damping, bid); break;
+    default: finish_list<Real>(j, jobs.damping, bid); break;
   }
 }
 
@@ -986,35 +1017,46 @@ int normal_matvec_impl(const tlh_operator_t* op, const void* x, void* y, double 
 template <typename Real>
 int finish_impl(const tlh_operator_t* op, int t0, int t1, const void* x, double damping, void* y,
                 hipStream_t stream) {
+  FinishJobs<Real> jobs{};
+  jobs.damping = (Real)damping;
+  int64_t blocks = 0;
   for (int ti = t0; ti < t1; ++ti) {
     const tlh_type_t& t = op->types[ti];
     const int64_t total = t.count * t.dim;
     if (total == 0) continue;
     const int64_t nlist = t.finish_list ? t.finish_count : t.count;
     if (nlist == 0) continue;
+    FinishJob<Real>& j = jobs.job[jobs.njobs++];
+    j.y = static_cast<Real*>(y) + t.offset;
+    j.x = static_cast<const Real*>(x) + t.offset;
+    j.partials = static_cast<const Real*>(t.partials);
+    j.csr_ptr = t.csr_ptr;
+    j.csr_idx = t.csr_idx;
+    j.list = t.finish_list;
+    j.count = t.count;
+    j.nlist = nlist;
+    j.dim = t.dim;
+    j.dense_blocks = t.dense_blocks;
+    int64_t nb;
     if (t.dense_blocks >= 2 * (TLH_THREADS / FIN_COLS) && !t.finish_list) {
       const bool narrow = ceil_div(total, FIN_COLS) < 256;
-      const int cols = narrow ? FIN_COLS_NARROW : FIN_COLS;
-      auto kernel = narrow ? finish_dense_kernel<Real, FIN_COLS_NARROW> : finish_dense_kernel<Real, FIN_COLS>;
-      hipLaunchKernelGGL(kernel, dim3((unsigned)ceil_div(total, cols)), dim3(TLH_THREADS), 0, stream,
-                         static_cast<Real*>(y) + t.offset, static_cast<const Real*>(x) + t.offset, (Real)damping,
-                         static_cast<const Real*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim,
-                         t.dense_blocks, t.finish_list, nlist);
+      j.role = narrow ? FIN_DENSE_NARROW : FIN_DENSE;
+      nb = ceil_div(total, narrow ? FIN_COLS_NARROW : FIN_COLS);
+    } else if (csr_heavy(t.csr_entries, nlist)) {
+      j.role = FIN_WAVE;
+      nb = ceil_div(nlist * t.dim, TLH_THREADS / 64);
+    } else {
+      j.role = FIN_LIST;
+      nb = ceil_div(nlist * t.dim, TLH_THREADS);
     }
-    else if (csr_heavy(t.csr_entries, nlist))
-      hipLaunchKernelGGL(finish_wave_kernel<Real>, dim3((unsigned)ceil_div(nlist * t.dim, TLH_THREADS / 64)),
-                         dim3(TLH_THREADS), 0, stream, static_cast<Real*>(y) + t.offset,
-                         static_cast<const Real*>(x) + t.offset, (Real)damping,
-                         static_cast<const Real*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim,
-                         t.dense_blocks, t.finish_list, nlist);
-    else
-      hipLaunchKernelGGL(finish_kernel<Real>, dim3((unsigned)ceil_div(nlist * t.dim, TLH_THREADS)),
-                         dim3(TLH_THREADS), 0, stream, static_cast<Real*>(y) + t.offset,
-                         static_cast<const Real*>(x) + t.offset, (Real)damping,
-                         static_cast<const Real*>(t.partials), t.csr_ptr, t.csr_idx, t.count, t.dim,
-                         t.dense_blocks, t.finish_list, nlist);
-    TLH_RETURN_IF_LAUNCH_FAILED();
+    if (nb > INT32_MAX) return TLH_EINVAL;
+    j.blocks = (int32_t)nb;
+    blocks += nb;
   }
+  if (jobs.njobs == 0) return 0;
+  if (blocks > INT32_MAX) return TLH_EINVAL;
+  hipLaunchKernelGGL(finish_jobs_kernel<Real>, dim3((unsigned)blocks), dim3(TLH_THREADS), 0, stream, jobs);
+  TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }
 

@@ -231,6 +231,52 @@ def pose_graph_lm(device, steps):
             "pcg_iterations_mixed": s32.linear_iterations, "pcg_iterations_float64": s64.linear_iterations}
 
 
+def implicit_large(device, lm_steps):
+    """Differentiable solve at 1M observations (500 cameras, 100,000 points): LM with the
+    full-system PCG, then the implicit gradient of sum ||log(T_true^-1 T)||^2 over the
+    cameras w.r.t. every observation by one adjoint PCG solve.  Forward and backward are
+    timed separately; the finite-difference check of these gradients runs at a small size
+    in the GPU tests (float64 Hessian polishing does not scale to this size)."""
+    from torchls_amd import ConjugateGradientConfig, HuberLoss, Problem, SE3Var, TerminationConfig, generators
+    from torchls_amd.geometry import se3_log
+    from torchls_amd.implicit import solve_implicit
+    from torchls_amd.residuals import reprojection
+
+    prob = generators.ba_differentiable()
+    obs = torch.tensor(prob.observations, device=device, requires_grad=True)
+    P = Problem({"cams": torch.tensor(prob.cameras_init, device=device),
+                 "points": torch.tensor(prob.points_init, device=device)}, manifolds={"cams": SE3Var})
+    P.add_residual(reprojection, ("cams", torch.tensor(prob.cam_idx)), ("points", torch.tensor(prob.pt_idx)), obs,
+                   loss=HuberLoss(1e-2))
+    # gauge: priors on one camera and one point
+    P.add_residual(lambda c, r: (c[:3, :] - r[:3, :]).reshape(-1), ("cams", 0),
+                   torch.tensor(prob.cameras_true[0], device=device))
+    P.add_residual(lambda q, r: q - r, ("points", 0), torch.tensor(prob.points_true[0], device=device))
+    truth = torch.tensor(prob.cameras_true, device=device)
+    opts = dict(conjugate_gradient=ConjugateGradientConfig(matvec_dtype=torch.float32),
+                termination=TerminationConfig(max_iterations=lm_steps, cost_tolerance=0.0, gradient_tolerance=0.0,
+                                              parameter_tolerance=0.0))
+    init = {k: v.clone() for k, v in P.parameters.items()}
+    out = {}
+    for rep in range(2):  # the first run pays the symbolic analysis
+        P.parameters.update({k: v.clone() for k, v in init.items()})
+        obs.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sol = solve_implicit(P, obs, **opts)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss = se3_log(torch.linalg.solve(truth, sol["cams"])).square().sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        out = {"forward_s": round(t1 - t0, 4), "backward_s": round(t2 - t1, 4)}
+    g = obs.grad
+    return {"workload": "ba_differentiable", "cameras": prob.n_cameras, "points": prob.n_points,
+            "observations": prob.n_obs, "seed": prob.seed, "lm_steps": lm_steps, **out,
+            "outer_loss": float(loss.detach()), "grad_norm": float(g.norm()), "grad_finite": bool(torch.isfinite(g).all())}
+
+
 def dense_schur_cholesky(device, reps):
     """MFMA float64 Cholesky of the 1,536 x 1,536 reduced camera system of a 256-camera
     problem (dense Schur path): HIP-event time per factorisation and n^3/3 flop rate."""
@@ -411,6 +457,11 @@ def main():
     chol = dense_schur_cholesky(device, 20)
     log(f"dense Schur Cholesky n={chol['n']}: {chol['us_per_factorisation']} us, {chol['tflops']} TFLOP/s")
 
+    implicit = None if args.quick else implicit_large(device, 10)
+    if implicit:
+        log(f"implicit gradient at {implicit['observations']} observations: forward {implicit['forward_s']} s, "
+            f"backward (adjoint) {implicit['backward_s']} s, |grad| {implicit['grad_norm']:.3e}")
+
     cpu = None
     oracle_rel = None
     if not args.no_cpu:
@@ -444,7 +495,7 @@ def main():
                      "kernel": "implicit Schur matvec (schur_tile_kernel + finish)",
                      "us_per_launch": round(t_schur * 1e6, 2), "bytes_per_launch": schur_bytes},
         "normal_matvec_large": large, "lm_large": lm5m, "pose_graph": pgraph,
-        "dense_schur_cholesky": chol,
+        "dense_schur_cholesky": chol, "implicit_gradient": implicit,
         "cpu_baseline": cpu,
     }), flush=True)
 

@@ -64,75 +64,90 @@ __device__ __forceinline__ double loss_value(double s, double delta, double& sqr
   return huber_value(s, delta);
 }
 
+// Block rows leave through LDS: a workgroup's 256 consecutive observations own
+// contiguous [256, 2, 6] and [256, 2, 3] slices, written with coalesced stores
+// instead of one strided row per lane.  Padded index: one extra word every 32.
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
+
 template <typename Real>
 __global__ void __launch_bounds__(TLH_THREADS)
     linearize_kernel(ReprojArgs<Real> a, Real* __restrict__ raw, Real* __restrict__ model, Real* __restrict__ jc,
                      Real* __restrict__ jp, double* out, double* partials, uint32_t* counter) {
   __shared__ double smem[2 * TLH_THREADS];
+  constexpr int NJ = TLH_THREADS * 18;
+  __shared__ Real sj[NJ + NJ / 32];
   double v[2] = {0.0, 0.0};
-  for (int64_t o = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; o < a.count;
-       o += (int64_t)gridDim.x * TLH_THREADS) {
-    const Real* p = a.pts + a.pi[o] * 3;
-    Projection<Real> q;
-    project(a, o, p, q);
-    const Real inv = Real(1) / q.l[2];
-    const Real D[2][3] = {{inv, Real(0), -q.l[0] * inv * inv}, {Real(0), inv, -q.l[1] * inv * inv}};
-    Real DR[2][3];
+  for (int64_t base = (int64_t)blockIdx.x * TLH_THREADS; base < a.count; base += (int64_t)gridDim.x * TLH_THREADS) {
+    const int rows = (int)min((int64_t)TLH_THREADS, a.count - base);
+    const int64_t o = base + threadIdx.x;
+    if ((int)threadIdx.x < rows) {
+      const Real* p = a.pts + a.pi[o] * 3;
+      Projection<Real> q;
+      project(a, o, p, q);
+      const Real inv = Real(1) / q.l[2];
+      const Real D[2][3] = {{inv, Real(0), -q.l[0] * inv * inv}, {Real(0), inv, -q.l[1] * inv * inv}};
+      Real DR[2][3];
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+      for (int r = 0; r < 2; ++r)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) DR[r][j] = D[r][0] * q.R[0][j] + D[r][1] * q.R[1][j] + D[r][2] * q.R[2][j];
-    const double s = (double)q.r[0] * (double)q.r[0] + (double)q.r[1] * (double)q.r[1];
-    double sf;
-    double value = loss_value(s, a.delta, sf);
-    const Real wl = (Real)sf;
-    Real ws = Real(1);
-    if (a.weights) {
-      value *= a.weights[o];
-      ws = (Real)sqrt(a.weights[o]);
+        for (int j = 0; j < 3; ++j) DR[r][j] = D[r][0] * q.R[0][j] + D[r][1] * q.R[1][j] + D[r][2] * q.R[2][j];
+      const double s = (double)q.r[0] * (double)q.r[0] + (double)q.r[1] * (double)q.r[1];
+      double sf;
+      double value = loss_value(s, a.delta, sf);
+      const Real wl = (Real)sf;
+      Real ws = Real(1);
+      if (a.weights) {
+        value *= a.weights[o];
+        ws = (Real)sqrt(a.weights[o]);
+      }
+      v[0] += value;
+      const Real m0 = q.r[0] * wl * ws, m1 = q.r[1] * wl * ws;
+      Real c[2][6], e[2][3];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        // -(D R [p]x) columns: [p]x = [[0,-p2,p1],[p2,0,-p0],[-p1,p0,0]]
+        const Real k0 = DR[r][1] * p[2] - DR[r][2] * p[1];
+        const Real k1 = DR[r][2] * p[0] - DR[r][0] * p[2];
+        const Real k2 = DR[r][0] * p[1] - DR[r][1] * p[0];
+        c[r][0] = DR[r][0] * wl * ws;
+        c[r][1] = DR[r][1] * wl * ws;
+        c[r][2] = DR[r][2] * wl * ws;
+        c[r][3] = -k0 * wl * ws;
+        c[r][4] = -k1 * wl * ws;
+        c[r][5] = -k2 * wl * ws;
+        e[r][0] = DR[r][0] * wl * ws;
+        e[r][1] = DR[r][1] * wl * ws;
+        e[r][2] = DR[r][2] * wl * ws;
+      }
+      // observations with a nonfinite residual or block entry, for the solver's finiteness check
+      bool ok = isfinite(q.r[0]) && isfinite(q.r[1]) && isfinite(m0) && isfinite(m1);
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) ok = ok && isfinite(c[r][j]);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) ok = ok && isfinite(e[r][j]);
+      }
+      v[1] += ok ? 0.0 : 1.0;
+      if (raw) {
+        raw[2 * o] = q.r[0];
+        raw[2 * o + 1] = q.r[1];
+      }
+      model[2 * o] = m0;
+      model[2 * o + 1] = m1;
+      // stage: [0, 12*256) camera rows, [12*256, 18*256) point rows
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) sj[lds_pad(threadIdx.x * 12 + r * 6 + j)] = c[r][j];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) sj[lds_pad(12 * TLH_THREADS + threadIdx.x * 6 + r * 3 + j)] = e[r][j];
+      }
     }
-    v[0] += value;
-    const Real m0 = q.r[0] * wl * ws, m1 = q.r[1] * wl * ws;
-    Real c[2][6], e[2][3];
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      // -(D R [p]x) columns: [p]x = [[0,-p2,p1],[p2,0,-p0],[-p1,p0,0]]
-      const Real k0 = DR[r][1] * p[2] - DR[r][2] * p[1];
-      const Real k1 = DR[r][2] * p[0] - DR[r][0] * p[2];
-      const Real k2 = DR[r][0] * p[1] - DR[r][1] * p[0];
-      c[r][0] = DR[r][0] * wl * ws;
-      c[r][1] = DR[r][1] * wl * ws;
-      c[r][2] = DR[r][2] * wl * ws;
-      c[r][3] = -k0 * wl * ws;
-      c[r][4] = -k1 * wl * ws;
-      c[r][5] = -k2 * wl * ws;
-      e[r][0] = DR[r][0] * wl * ws;
-      e[r][1] = DR[r][1] * wl * ws;
-      e[r][2] = DR[r][2] * wl * ws;
-    }
-    // observations with a nonfinite residual or block entry, for the solver's finiteness check
-    bool ok = isfinite(q.r[0]) && isfinite(q.r[1]) && isfinite(m0) && isfinite(m1);
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-#pragma unroll
-      for (int j = 0; j < 6; ++j) ok = ok && isfinite(c[r][j]);
-#pragma unroll
-      for (int j = 0; j < 3; ++j) ok = ok && isfinite(e[r][j]);
-    }
-    v[1] += ok ? 0.0 : 1.0;
-    if (raw) {
-      raw[2 * o] = q.r[0];
-      raw[2 * o + 1] = q.r[1];
-    }
-    model[2 * o] = m0;
-    model[2 * o + 1] = m1;
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-#pragma unroll
-      for (int j = 0; j < 6; ++j) jc[o * 12 + r * 6 + j] = c[r][j];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) jp[o * 6 + r * 3 + j] = e[r][j];
-    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < rows * 12; i += TLH_THREADS) jc[base * 12 + i] = sj[lds_pad(i)];
+    for (int i = threadIdx.x; i < rows * 6; i += TLH_THREADS) jp[base * 6 + i] = sj[lds_pad(12 * TLH_THREADS + i)];
+    __syncthreads();  // the stage is rewritten by the next chunk
   }
   grid_sum2(v, partials, counter, out, smem);
 }

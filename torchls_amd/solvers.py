@@ -290,8 +290,7 @@ def solve(
         g = kernels.gradient(jac, lin.residual)
         cost = lin.objective if lin.objective is not None else 0.5 * lin.residual.to(torch.float64).square().sum()
         bad = lin.nonfinite if lin.nonfinite is not None else (~torch.isfinite(lin.residual)).sum().double()
-        gmax = torch.linalg.vector_norm(g, ord=math.inf)  # max |g| in one reduction
-        return g, torch.stack((bad.to(torch.float64), gmax, cost.to(torch.float64)))
+        return g, torch.stack((bad.to(torch.float64), g.abs().max(), cost.to(torch.float64)))
 
     def examine(lin):
         """examine_device plus its host read: (gradient, finite, max |g|, cost)."""

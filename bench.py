@@ -309,13 +309,17 @@ def dense_schur_cholesky(device, reps):
         raise RuntimeError("dense Schur matrix is not positive definite")
     rhs = torch.randn(n, dtype=torch.float64, device=device)
     x = rhs.clone()
+    flags = torch.empty(2 * -(-n // 64), dtype=torch.int32, device=device)
 
     def solve():
         x.copy_(rhs)
-        _lib.check(lib.tlh_cholesky_solve_f64(work.data_ptr(), n, n, x.data_ptr(), _lib.stream_handle(device)),
-                   "tlh_cholesky_solve_f64")
+        _lib.check(lib.tlh_cholesky_solve_blocks_f64(work.data_ptr(), n, n, x.data_ptr(), flags.data_ptr(),
+                                                     status.data_ptr(), _lib.stream_handle(device)),
+                   "tlh_cholesky_solve_blocks_f64")
 
     t_solve = max(event_time(solve, reps) - event_time(lambda: x.copy_(rhs), reps), 1e-9)
+    if int(status.item()) != 0:
+        raise RuntimeError("dense Schur triangular solve did not complete")
     flops = n ** 3 / 3
     return {"workload": "ba_dense_schur", "cameras": prob.n_cameras, "n": n, "us_per_factorisation": round(t * 1e6, 1),
             "tflops": round(flops / t / 1e12, 3), "peak_tflops_fp64_mfma": 78.6,

@@ -11,7 +11,7 @@ from oracle.lm import Termination, solve as oracle_solve
 from oracle.schur import SchurSystem
 from oracle.sparse import NpGroup, NpJacobian
 from tests.helpers import rel_err
-from torchls_amd import HuberLoss, Problem, SE3Var, TerminationConfig, generators
+from torchls_amd import HuberLoss, Problem, SE3Var, TerminationConfig, _lib, generators
 from torchls_amd.cholesky import cholesky_f64, cholesky_solve_f64
 from torchls_amd.hip_diagonal import LinAlgError
 from torchls_amd.residuals import reprojection
@@ -75,7 +75,7 @@ def test_schur_operators(device, priors, dtype, tol):
     assert rel_err(np.tril(M), np.tril(Md)) <= tol
 
 
-@pytest.mark.parametrize("n", [5, 70, 200, 1536])
+@pytest.mark.parametrize("n", [5, 64, 70, 200, 1536])
 def test_cholesky(device, n):
     print(f"cholesky n={n}", flush=True)
     rng = np.random.default_rng(n)
@@ -87,8 +87,16 @@ def test_cholesky(device, n):
     b = rng.standard_normal(n)
     x = cholesky_solve_f64(torch.tensor(M, device=device), torch.tensor(b, device=device)).cpu().numpy()
     assert rel_err(x, np.linalg.solve(M, b)) <= 1e-10
+    # the one-workgroup substitution gives the same solution
+    xs = torch.tensor(b, device=device)
+    Lt = torch.tensor(L, device=device)
+    _lib.check(_lib.load().tlh_cholesky_solve_f64(Lt.data_ptr(), n, n, xs.data_ptr(), _lib.stream_handle(device)),
+               "tlh_cholesky_solve_f64")
+    assert rel_err(x, xs.cpu().numpy()) <= 1e-12
     with pytest.raises(LinAlgError):
         cholesky_f64(torch.tensor(-M, device=device))
+    with pytest.raises(LinAlgError):
+        cholesky_solve_f64(torch.tensor(-M, device=device), torch.tensor(b, device=device))
 
 
 @pytest.mark.parametrize("solver", ["schur_cg", "schur_dense"])

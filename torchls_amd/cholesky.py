@@ -25,10 +25,26 @@ def cholesky_f64(M: torch.Tensor) -> torch.Tensor:
 
 
 def cholesky_solve_f64(M: torch.Tensor, rhs: torch.Tensor) -> torch.Tensor:
-    """Solve M x = rhs for SPD M through cholesky_f64 and two triangular solves."""
-    L = cholesky_f64(M)
+    """Solve M x = rhs for SPD M: MFMA Cholesky factorisation, then forward and backward
+    substitution over one workgroup per 64-row block.  One status read covers both
+    (LinAlgError on a non-positive pivot, as torch.linalg.cholesky raises)."""
+    if M.ndim != 2 or M.shape[0] != M.shape[1] or M.dtype != torch.float64 or M.device.type != "cuda":
+        raise ValueError("cholesky_solve_f64 expects a square float64 GPU matrix")
+    n = M.shape[0]
+    if rhs.shape != (n,):
+        raise ValueError(f"rhs must have shape ({n},)")
+    L = M.contiguous().clone()
     x = rhs.to(torch.float64).contiguous().clone()
+    status = torch.zeros(1, dtype=torch.int32, device=M.device)
+    flags = torch.empty(2 * max(1, -(-n // 64)), dtype=torch.int32, device=M.device)
     lib = _lib.load()
-    _lib.check(lib.tlh_cholesky_solve_f64(L.data_ptr(), L.shape[0], L.shape[0], x.data_ptr(),
-                                          _lib.stream_handle(M.device)), "tlh_cholesky_solve_f64")
+    stream = _lib.stream_handle(M.device)
+    _lib.check(lib.tlh_cholesky_f64(L.data_ptr(), n, n, status.data_ptr(), stream), "tlh_cholesky_f64")
+    _lib.check(lib.tlh_cholesky_solve_blocks_f64(L.data_ptr(), n, n, x.data_ptr(), flags.data_ptr(),
+                                                 status.data_ptr(), stream), "tlh_cholesky_solve_blocks_f64")
+    code = int(status.item())
+    if code & 1:
+        raise LinAlgError("matrix is not positive definite")
+    if code & 2:
+        raise RuntimeError("tlh_cholesky_solve_blocks_f64: a block wait gave up")
     return x

@@ -311,6 +311,152 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
   for (int64_t i = t; i < n; i += SOLVE_THREADS) b[i] = xs[i];
 }
 
+// Blocked substitution over many workgroups.  Workgroup w owns one 64-row block;
+// a block's off-diagonal updates need the solutions of the blocks before it
+// (forward) or after it (backward), which their workgroups publish through
+// flags.  Each update loads its L tile before waiting for the flag, so the
+// serial chain is only the 64-step diagonal solves.  Workgroups are dispatched
+// in index order and a block waits only on blocks with lower workgroup
+// indices, so every wait is on a workgroup that is already running; a wait
+// still gives up after SOLVE_SPIN_MAX polls, setting *status |= 2.
+constexpr int SOLVE_WG = 256;
+constexpr int SOLVE_SPIN_MAX = 1 << 24;
+
+__global__ void __launch_bounds__(TLH_THREADS) solve_flags_reset(int32_t* flags, int count) {
+  const int i = blockIdx.x * TLH_THREADS + threadIdx.x;
+  if (i < count) flags[i] = 0;
+}
+
+__device__ __forceinline__ void wait_flag(const int32_t* flag, int32_t* status) {
+  if (threadIdx.x == 0) {
+    int polls = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+      if (++polls > SOLVE_SPIN_MAX) {
+        __hip_atomic_fetch_or(status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+__device__ __forceinline__ void publish_flag(int32_t* flag) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Forward: block i solves L_ii y_i = b_i - sum_{j<i} L_ij y_j.
+__global__ void __launch_bounds__(SOLVE_WG)
+    chol_forward_blocks(const double* __restrict__ L, int64_t n, int64_t ld, double* b, int32_t* flags,
+                        int32_t* status) {
+  __shared__ double T[NB][NB + 1];
+  __shared__ double D[NB][NB + 1];
+  __shared__ double xs[NB], acc[NB];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int i = blockIdx.x;
+  const int64_t i0 = (int64_t)i * NB;
+  const int m = (int)min((int64_t)NB, n - i0);
+  for (int e = t; e < NB * NB; e += SOLVE_WG) {
+    const int r = e / NB, c = e % NB;
+    D[r][c] = (r < m && c <= r) ? L[(i0 + r) * ld + i0 + c] : (r == c ? 1.0 : 0.0);
+  }
+  if (t < NB) acc[t] = t < m ? b[i0 + t] : 0.0;
+  __syncthreads();
+  // wave 0 keeps row `lane` of the diagonal block in registers for the substitution
+  double drow[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) drow[c] = D[lane][c];
+  const double dinv = 1.0 / D[lane][lane];
+  const int r = t >> 2, q = t & 3;  // four lanes per row, 16 columns each
+  for (int j = 0; j < i; ++j) {
+    for (int e = t; e < NB * NB; e += SOLVE_WG) {
+      const int rr = e / NB, c = e % NB;
+      T[rr][c] = rr < m ? L[(i0 + rr) * ld + (int64_t)j * NB + c] : 0.0;
+    }
+    wait_flag(flags + j, status);
+    if (t < NB) xs[t] = b[(int64_t)j * NB + t];
+    __syncthreads();
+    double sacc = 0.0;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) sacc += T[r][q * 16 + c] * xs[q * 16 + c];
+    sacc += __shfl_xor(sacc, 1);
+    sacc += __shfl_xor(sacc, 2);
+    if (q == 0 && r < m) acc[r] -= sacc;
+    __syncthreads();
+  }
+  if (wave == 0) {
+    double v = lane < m ? acc[lane] : 0.0;
+#pragma unroll
+    for (int jj = 0; jj < NB; ++jj) {
+      if (jj < m) {
+        const double xj = readlane_f64(v, jj) * readlane_f64(dinv, jj);
+        v = lane == jj ? xj : (lane > jj ? v - drow[jj] * xj : v);
+      }
+    }
+    if (lane < m) b[i0 + lane] = v;
+  }
+  publish_flag(flags + i);
+}
+
+// Backward: workgroup w solves block k = nb - 1 - w: L_kk^T x_k = y_k - sum_{j>k} L_jk^T x_j.
+__global__ void __launch_bounds__(SOLVE_WG)
+    chol_backward_blocks(const double* __restrict__ L, int64_t n, int64_t ld, double* b, int32_t* flags,
+                         int32_t* status) {
+  __shared__ double T[NB][NB + 1];
+  __shared__ double D[NB][NB + 1];
+  __shared__ double xs[NB], acc[NB];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int nb = (int)gridDim.x;
+  const int k = nb - 1 - (int)blockIdx.x;
+  const int64_t k0 = (int64_t)k * NB;
+  const int m = (int)min((int64_t)NB, n - k0);
+  for (int e = t; e < NB * NB; e += SOLVE_WG) {
+    const int rr = e / NB, c = e % NB;
+    D[rr][c] = (rr < m && c <= rr) ? L[(k0 + rr) * ld + k0 + c] : (rr == c ? 1.0 : 0.0);
+  }
+  if (t < NB) acc[t] = t < m ? b[k0 + t] : 0.0;
+  __syncthreads();
+  // wave 0 keeps column `lane` of the diagonal block in registers for the substitution
+  double dcol[NB];
+#pragma unroll
+  for (int rr = 0; rr < NB; ++rr) dcol[rr] = D[rr][lane];
+  const double dinv = 1.0 / D[lane][lane];
+  const int c = t >> 2, q = t & 3;  // four lanes per column, 16 rows each
+  for (int jb = nb - 1; jb > k; --jb) {
+    const int64_t j0 = (int64_t)jb * NB;
+    const int mj = (int)min((int64_t)NB, n - j0);
+    for (int e = t; e < NB * NB; e += SOLVE_WG) {
+      const int rr = e / NB, cc = e % NB;
+      T[rr][cc] = (rr < mj && cc < m) ? L[(j0 + rr) * ld + k0 + cc] : 0.0;
+    }
+    wait_flag(flags + nb + jb, status);
+    if (t < NB) xs[t] = t < mj ? b[j0 + t] : 0.0;
+    __syncthreads();
+    double sacc = 0.0;
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) sacc += T[q * 16 + rr][c] * xs[q * 16 + rr];
+    sacc += __shfl_xor(sacc, 1);
+    sacc += __shfl_xor(sacc, 2);
+    if (q == 0 && c < m) acc[c] -= sacc;
+    __syncthreads();
+  }
+  if (wave == 0) {
+    double v = lane < m ? acc[lane] : 0.0;
+#pragma unroll
+    for (int jj = NB - 1; jj >= 0; --jj) {
+      if (jj < m) {
+        const double xj = readlane_f64(v, jj) * readlane_f64(dinv, jj);
+        v = lane == jj ? xj : (lane < jj ? v - dcol[jj] * xj : v);
+      }
+    }
+    if (lane < m) b[k0 + lane] = v;
+  }
+  publish_flag(flags + nb + k);
+}
+
 }  // namespace
 
 extern "C" int tlh_cholesky_f64(double* A, int64_t n, int64_t ld, int32_t* status, tlh_stream_t stream) {
@@ -338,6 +484,23 @@ extern "C" int tlh_cholesky_solve_f64(const double* L, int64_t n, int64_t ld, do
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(SOLVE_THREADS), (size_t)lds, reinterpret_cast<hipStream_t>(stream),
                      L, n, ld, b);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+extern "C" int tlh_cholesky_solve_blocks_f64(const double* L, int64_t n, int64_t ld, double* b, int32_t* flags,
+                                             int32_t* status, tlh_stream_t stream) {
+  if (!L || !b || !flags || !status || n < 0 || ld < n) return TLH_EINVAL;
+  if (n == 0) return 0;
+  const int64_t nb = ceil_div(n, NB);
+  if (nb > (1 << 20)) return TLH_EINVAL;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(solve_flags_reset, dim3((unsigned)ceil_div(2 * nb, TLH_THREADS)), dim3(TLH_THREADS), 0, s, flags,
+                     (int)(2 * nb));
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  hipLaunchKernelGGL(chol_forward_blocks, dim3((unsigned)nb), dim3(SOLVE_WG), 0, s, L, n, ld, b, flags, status);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  hipLaunchKernelGGL(chol_backward_blocks, dim3((unsigned)nb), dim3(SOLVE_WG), 0, s, L, n, ld, b, flags, status);
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }

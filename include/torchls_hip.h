@@ -415,7 +415,8 @@ int tlh_copy_cast(const void* src, int src_dtype, const int64_t* src_index, int6
 /* ---------------------------------------------------------------- dense Cholesky */
 /* In-place lower Cholesky A = L L^T (row-major, leading dimension ld, float64):
  * blocked right-looking, 64-wide panels, MFMA (v_mfma_f64_16x16x4_f64) trailing
- * updates.  Only the lower triangle is read and written meaningfully; the upper
+ * updates, one step of look-ahead (the next panel's columns are updated first,
+ * then that panel runs beside the rest of the trailing update).  Only the lower triangle is read and written meaningfully; the upper
  * triangle of the diagonal blocks is left undefined.  *status |= 1 on a
  * non-positive pivot.  Replaces: torch.linalg.cholesky (solvers.py:517). */
 int tlh_cholesky_f64(double* A, int64_t n, int64_t ld, int32_t* status, tlh_stream_t stream);

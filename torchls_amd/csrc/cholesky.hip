@@ -71,8 +71,8 @@ __device__ __forceinline__ void factor_block(double (*W)[NB + 1], int wave, int 
         const double dj = readlane_f64(v[j], pr);
         const bool good = dj > 0.0 && isfinite(dj);
         if (!good) ok = false;
-        const double p = sqrt(good ? dj : 1.0);
-        const double inv = 1.0 / p;
+        const double inv = rsqrt(good ? dj : 1.0);  // one reciprocal square root, no divide
+        const double p = (good ? dj : 1.0) * inv;
         if (lane == pr) v[j] = p;
         else if (lane > pr) v[j] *= inv;
 #pragma unroll
@@ -97,33 +97,39 @@ __device__ __forceinline__ void factor_block(double (*W)[NB + 1], int wave, int 
   }
 }
 
-// Panel step k: every workgroup factors the diagonal block (redundantly, no
-// inter-workgroup sync); workgroup 0 writes L_kk, workgroup w >= 1 solves
-// X L_kk^T = A_ik for the w-th 64-row block below, lane r forming row r of X
-// independently (sub-panel by sub-panel, L_kk read from LDS by every lane at
-// the same address).  The rows of A_ik are loaded before the factorisation so
-// those loads overlap it.
-__global__ void __launch_bounds__(PANEL_THREADS)
-    chol_panel_kernel(double* __restrict__ A, int64_t n, int64_t ld, int64_t k, int32_t* status) {
-  __shared__ double W[NB][NB + 1];
-  __shared__ double X[NB][NB + 1];
-  __shared__ double inv_diag[NB];
+// Panel k, job wg: every job factors the diagonal block A_kk itself (no
+// inter-workgroup sync).  Job 0 writes L_kk over A_kk and reports a bad pivot;
+// job w >= 1 solves X L_kk^T = A_ik for the w-th 64-row block below, lane r
+// forming row r of X independently (sub-panel by sub-panel, L_kk read from LDS
+// by every lane at the same address).  Job 0 never runs in the same launch as
+// jobs w >= 1 of its panel, which read A_kk.  The rows of A_ik are loaded
+// before the factorisation so those loads overlap it.  W and X are the two
+// 64x65 LDS tiles of the caller.
+__device__ __forceinline__ void panel_body(double* __restrict__ A, int64_t n, int64_t ld, int64_t k, int32_t* status,
+                                           int wg, double (*W)[NB + 1], double (*X)[NB + 1], double* inv_diag) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int kb = (int)min((int64_t)NB, n - k);
-  const int64_t r0 = k + kb + (int64_t)((int)blockIdx.x - 1) * NB;
-  const int rows = blockIdx.x == 0 ? 0 : (int)max((int64_t)0, min((int64_t)NB, n - r0));
-  if (blockIdx.x != 0 && rows == 0) return;  // uniform over the workgroup
+  const int64_t r0 = k + kb + (int64_t)(wg - 1) * NB;
+  const int rows = wg == 0 ? 0 : (int)max((int64_t)0, min((int64_t)NB, n - r0));
+  if (wg != 0 && rows == 0) return;  // uniform over the workgroup
+  double gw[NB / PANEL_WAVES], gx[NB / PANEL_WAVES];  // all loads in flight before the LDS stores
 #pragma unroll
   for (int i = 0; i < NB / PANEL_WAVES; ++i) {  // coalesced: lane = column
     const int r = wave + PANEL_WAVES * i;
-    W[r][lane] = (r < kb && lane <= r) ? A[(k + r) * ld + k + lane] : (r == lane ? 1.0 : 0.0);
-    if (rows > 0) X[r][lane] = (r < rows && lane < kb) ? A[(r0 + r) * ld + k + lane] : 0.0;
+    gw[i] = (r < kb && lane <= r) ? A[(k + r) * ld + k + lane] : (r == lane ? 1.0 : 0.0);
+    gx[i] = (rows > 0 && r < rows && lane < kb) ? A[(r0 + r) * ld + k + lane] : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < NB / PANEL_WAVES; ++i) {
+    const int r = wave + PANEL_WAVES * i;
+    W[r][lane] = gw[i];
+    X[r][lane] = gx[i];
   }
   __syncthreads();
   bool ok = true;
   factor_block(W, wave, lane, ok);
-  if (blockIdx.x == 0) {
+  if (wg == 0) {
     const bool all_ok = __builtin_amdgcn_readfirstlane((int)__any(!ok)) == 0;
 #pragma unroll
     for (int i = 0; i < NB / PANEL_WAVES; ++i) {
@@ -167,29 +173,49 @@ __global__ void __launch_bounds__(PANEL_THREADS)
   }
 }
 
-// Trailing update of the lower blocks (bi >= bj) right of panel k.
-__global__ void __launch_bounds__(TLH_THREADS)
-    chol_trailing_kernel(double* __restrict__ A, int64_t n, int64_t ld, int64_t k) {
-  __shared__ double Pi[NB][NB + 1];
-  __shared__ double Pj[NB][NB + 1];
-  // triangular block index: blockIdx.x -> (bi, bj), bi >= bj
-  const int64_t b = blockIdx.x;
-  int64_t bi = (int64_t)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
-  while ((bi + 1) * (bi + 2) / 2 <= b) ++bi;
-  while (bi * (bi + 1) / 2 > b) --bi;
-  const int64_t bj = b - bi * (bi + 1) / 2;
+// Trailing update of one lower tile (bi >= bj) right of panel k:
+// A_ij -= L_ik L_jk^T with v_mfma_f64_16x16x4_f64, the first four wavefronts
+// each computing 16 rows of the 64x64 tile (any further wavefronts only help
+// load the two panel tiles).
+template <int NTH>
+__device__ __forceinline__ void trailing_tile(double* __restrict__ A, int64_t n, int64_t ld, int64_t k, int64_t bi,
+                                              int64_t bj, double (*Pi)[NB + 1], double (*Pj)[NB + 1]) {
   const int64_t base = k + NB;
   const int64_t ri = base + bi * NB, rj = base + bj * NB;
   const int kb = (int)min((int64_t)NB, n - k);
-#pragma unroll 16
-  for (int e = threadIdx.x; e < NB * NB; e += TLH_THREADS) {
-    const int i = e / NB, j = e % NB;
-    Pi[i][j] = (ri + i < n && j < kb) ? A[(ri + i) * ld + k + j] : 0.0;
-    Pj[i][j] = (rj + i < n && j < kb) ? A[(rj + i) * ld + k + j] : 0.0;
-  }
-  __syncthreads();
   const int lane = threadIdx.x % 64;
   const int w = threadIdx.x / 64;
+  // every load is issued before the first LDS store (one memory round trip),
+  // including the tile being updated, held in the MFMA output layout
+  constexpr int PER = NB * NB / NTH;
+  double ga[PER], gb[PER], cold[4][4];
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    const int e = (int)threadIdx.x + m * NTH;
+    const int i = e / NB, j = e % NB;
+    ga[m] = (ri + i < n && j < kb) ? A[(ri + i) * ld + k + j] : 0.0;
+    gb[m] = (rj + i < n && j < kb) ? A[(rj + i) * ld + k + j] : 0.0;
+  }
+  if (w < 4) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t row = ri + 16 * w + 4 * v + lane / 16;
+        const int64_t col = rj + 16 * t + (lane % 16);
+        cold[t][v] = (row < n && col < n && col <= row) ? A[row * ld + col] : 0.0;
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    const int e = (int)threadIdx.x + m * NTH;
+    const int i = e / NB, j = e % NB;
+    Pi[i][j] = ga[m];
+    Pj[i][j] = gb[m];
+  }
+  __syncthreads();
+  if (w >= 4) return;
   v4d acc[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
@@ -207,9 +233,61 @@ __global__ void __launch_bounds__(TLH_THREADS)
     for (int v = 0; v < 4; ++v) {
       const int64_t row = ri + 16 * w + 4 * v + lane / 16;
       const int64_t col = rj + 16 * t + (lane % 16);
-      if (row < n && col < n && col <= row) A[row * ld + col] -= acc[t][v];
+      if (row < n && col < n && col <= row) A[row * ld + col] = cold[t][v] - acc[t][v];
     }
   }
+}
+
+// Tile b of a lower block triangle numbered row by row: b -> (bi, bj), bi >= bj.
+__device__ __forceinline__ void triangle_index(int64_t b, int64_t& bi, int64_t& bj) {
+  bi = (int64_t)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+  while ((bi + 1) * (bi + 2) / 2 <= b) ++bi;
+  while (bi * (bi + 1) / 2 > b) --bi;
+  bj = b - bi * (bi + 1) / 2;
+}
+
+// Panel k: jobs first_job + blockIdx.x.
+__global__ void __launch_bounds__(PANEL_THREADS)
+    chol_panel_kernel(double* __restrict__ A, int64_t n, int64_t ld, int64_t k, int first_job, int32_t* status) {
+  __shared__ double S[2][NB][NB + 1];
+  __shared__ double inv_diag[NB];
+  panel_body(A, n, ld, k, status, first_job + (int)blockIdx.x, S[0], S[1], inv_diag);
+}
+
+// Look-ahead, part 1 of step k: the trailing update of the next panel's column
+// block only (tiles (bi, 0)).
+__global__ void __launch_bounds__(TLH_THREADS) chol_trailing_col_kernel(double* __restrict__ A, int64_t n, int64_t ld,
+                                                                        int64_t k) {
+  __shared__ double S[2][NB][NB + 1];
+  trailing_tile<TLH_THREADS>(A, n, ld, k, (int64_t)blockIdx.x, 0, S[0], S[1]);
+}
+
+// Look-ahead, part 2 of step k, by workgroup:
+//   0                 job 0 of panel k (writes L_kk; A_kk is read by no other job here)
+//   1 .. npanel       panel k + 64: its jobs 1 .. npanel (the row blocks below it), or
+//                     its job 0 when it is the last panel (no row blocks below)
+//   npanel + 1 ..     the rest of step k's trailing update (tiles with bj >= 1)
+// Part 1 finished panel k + 64's columns.  The sets touch disjoint tiles: job 0
+// reads and writes the diagonal block of rows [k, k + 64); the panel jobs read
+// its diagonal block and write rows >= k + 128 of columns [k + 64, k + 128); the
+// trailing tiles read columns [k, k + 64) of rows >= k + 128 and write columns
+// >= k + 128.
+__global__ void __launch_bounds__(PANEL_THREADS)
+    chol_step_kernel(double* __restrict__ A, int64_t n, int64_t ld, int64_t k, int npanel, int last, int32_t* status) {
+  __shared__ double S[2][NB][NB + 1];
+  __shared__ double inv_diag[NB];
+  const int g = (int)blockIdx.x;
+  if (g == 0) {
+    panel_body(A, n, ld, k, status, 0, S[0], S[1], inv_diag);
+    return;
+  }
+  if (g <= npanel) {
+    panel_body(A, n, ld, k + NB, status, last ? 0 : g, S[0], S[1], inv_diag);
+    return;
+  }
+  int64_t bi, bj;
+  triangle_index(g - 1 - npanel, bi, bj);
+  trailing_tile<PANEL_THREADS>(A, n, ld, k, bi + 1, bj + 1, S[0], S[1]);
 }
 
 // Forward (L y = b) then backward (L^T x = y) substitution, one workgroup of
@@ -461,17 +539,25 @@ __global__ void __launch_bounds__(SOLVE_WG)
 
 extern "C" int tlh_cholesky_f64(double* A, int64_t n, int64_t ld, int32_t* status, tlh_stream_t stream) {
   if (!A || !status || n < 0 || ld < n) return TLH_EINVAL;
+  if (n == 0) return 0;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  for (int64_t k = 0; k < n; k += NB) {
-    const int64_t below = n - k - NB > 0 ? ceil_div(n - k - NB, NB) : 0;
-    hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)(1 + below)), dim3(PANEL_THREADS), 0, s, A, n, ld, k,
-                       status);
+  // panel 0's row blocks (or, for n <= 64, its diagonal block), then per step k:
+  // the next panel's columns, then L_kk beside the next panel and the rest of
+  // step k's trailing update
+  const int64_t below0 = n > NB ? ceil_div(n - NB, NB) : 0;
+  hipLaunchKernelGGL(chol_panel_kernel, dim3((unsigned)(below0 > 0 ? below0 : 1)), dim3(PANEL_THREADS), 0, s, A, n, ld,
+                     (int64_t)0, below0 > 0 ? 1 : 0, status);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  for (int64_t k = 0; k + NB < n; k += NB) {
+    const int64_t below = ceil_div(n - k - NB, NB);  // block rows below panel k (>= 1)
+    hipLaunchKernelGGL(chol_trailing_col_kernel, dim3((unsigned)below), dim3(TLH_THREADS), 0, s, A, n, ld, k);
     TLH_RETURN_IF_LAUNCH_FAILED();
-    if (below > 0) {
-      const int64_t blocks = below * (below + 1) / 2;
-      hipLaunchKernelGGL(chol_trailing_kernel, dim3((unsigned)blocks), dim3(TLH_THREADS), 0, s, A, n, ld, k);
-      TLH_RETURN_IF_LAUNCH_FAILED();
-    }
+    const int last = below == 1;                    // panel k + 64 has no row blocks below
+    const int64_t npanel = last ? 1 : below - 1;
+    const int64_t rest = (below - 1) * below / 2;   // tiles 1 <= bj <= bi <= below - 1
+    hipLaunchKernelGGL(chol_step_kernel, dim3((unsigned)(1 + npanel + rest)), dim3(PANEL_THREADS), 0, s, A, n, ld, k,
+                       (int)npanel, last, status);
+    TLH_RETURN_IF_LAUNCH_FAILED();
   }
   return 0;
 }

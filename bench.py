@@ -309,11 +309,11 @@ def dense_schur_cholesky(device, reps):
         raise RuntimeError("dense Schur matrix is not positive definite")
     rhs = torch.randn(n, dtype=torch.float64, device=device)
     x = rhs.clone()
-    flags = torch.empty(2 * -(-n // 64), dtype=torch.int32, device=device)
+    swork = torch.empty(2 * n, dtype=torch.float64, device=device)
 
     def solve():
         x.copy_(rhs)
-        _lib.check(lib.tlh_cholesky_solve_blocks_f64(work.data_ptr(), n, n, x.data_ptr(), flags.data_ptr(),
+        _lib.check(lib.tlh_cholesky_solve_blocks_f64(work.data_ptr(), n, n, x.data_ptr(), swork.data_ptr(),
                                                      status.data_ptr(), _lib.stream_handle(device)),
                    "tlh_cholesky_solve_blocks_f64")
 

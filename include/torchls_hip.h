@@ -426,13 +426,13 @@ int tlh_cholesky_f64(double* A, int64_t n, int64_t ld, int32_t* status, tlh_stre
 int tlh_cholesky_solve_f64(const double* L, int64_t n, int64_t ld, double* b, tlh_stream_t stream);
 
 /* Same result as tlh_cholesky_solve_f64, spread over one workgroup per 64-row
- * block: forward then backward substitution, each block waiting on the flags
- * of the blocks it depends on.  flags: int32 workspace of 2 * ceil(n / 64)
- * entries, cleared by the call itself (a fill kernel, graph-capturable).
- * *status |= 2 if a wait gave up (the result is then not defined).
+ * block: forward then backward substitution, each block waiting on the solved
+ * values of the blocks it depends on.  work: float64 workspace of 2 * n entries,
+ * filled by the call itself (a fill kernel, graph-capturable).  *status |= 2 if
+ * a wait gave up (the result is then not defined).
  * Replaces: torch.cholesky_solve (solvers.py:518). */
-int tlh_cholesky_solve_blocks_f64(const double* L, int64_t n, int64_t ld, double* b, int32_t* flags,
-                                  int32_t* status, tlh_stream_t stream);
+int tlh_cholesky_solve_blocks_f64(const double* L, int64_t n, int64_t ld, double* b, double* work, int32_t* status,
+                                  tlh_stream_t stream);
 
 /* ---------------------------------------------------------------- reprojection residual */
 /* One reprojection residual batch r_i = pi(R_c p + t_c) - obs_i (pi(x,y,z) = (x/z, y/z);

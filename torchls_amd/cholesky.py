@@ -36,11 +36,11 @@ def cholesky_solve_f64(M: torch.Tensor, rhs: torch.Tensor) -> torch.Tensor:
     L = M.contiguous().clone()
     x = rhs.to(torch.float64).contiguous().clone()
     status = torch.zeros(1, dtype=torch.int32, device=M.device)
-    flags = torch.empty(2 * max(1, -(-n // 64)), dtype=torch.int32, device=M.device)
+    work = torch.empty(2 * max(1, n), dtype=torch.float64, device=M.device)
     lib = _lib.load()
     stream = _lib.stream_handle(M.device)
     _lib.check(lib.tlh_cholesky_f64(L.data_ptr(), n, n, status.data_ptr(), stream), "tlh_cholesky_f64")
-    _lib.check(lib.tlh_cholesky_solve_blocks_f64(L.data_ptr(), n, n, x.data_ptr(), flags.data_ptr(),
+    _lib.check(lib.tlh_cholesky_solve_blocks_f64(L.data_ptr(), n, n, x.data_ptr(), work.data_ptr(),
                                                  status.data_ptr(), stream), "tlh_cholesky_solve_blocks_f64")
     code = int(status.item())
     if code & 1:

@@ -391,46 +391,59 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
 
 // Blocked substitution over many workgroups.  Workgroup w owns one 64-row block;
 // a block's off-diagonal updates need the solutions of the blocks before it
-// (forward) or after it (backward), which their workgroups publish through
-// flags.  Each update loads its L tile before waiting for the flag, so the
-// serial chain is only the 64-step diagonal solves.  Workgroups are dispatched
-// in index order and a block waits only on blocks with lower workgroup
-// indices, so every wait is on a workgroup that is already running; a wait
-// still gives up after SOLVE_SPIN_MAX polls, setting *status |= 2.
+// (forward) or after it (backward).  Each solved block is also stored, value by
+// value with agent-scope atomic stores, into a workspace that the launch first
+// fills with a signalling-NaN pattern; a waiting workgroup polls those values
+// with agent-scope atomic loads until none holds the pattern, so the values are
+// their own ready flags (no fences, no separate flag round trip).  Arithmetic
+// results are never signalling NaNs, so a solved value cannot look unsolved.
+// Each update loads its L slice before waiting, so the serial chain is the
+// 64-step diagonal solves and the polls.  Workgroups are dispatched in index
+// order and a block waits only on blocks with lower workgroup indices, so every
+// wait is on a workgroup that is already running; a wait still gives up after
+// SOLVE_SPIN_MAX polls, setting *status |= 2.
 constexpr int SOLVE_WG = 256;
 constexpr int SOLVE_SPIN_MAX = 1 << 24;
+constexpr long long SOLVE_EMPTY = 0x7ff4dead5eed1234ll;  // a signalling NaN
 
-__global__ void __launch_bounds__(TLH_THREADS) solve_flags_reset(int32_t* flags, int count) {
-  const int i = blockIdx.x * TLH_THREADS + threadIdx.x;
-  if (i < count) flags[i] = 0;
+__global__ void __launch_bounds__(TLH_THREADS) solve_work_reset(long long* work, int64_t count) {
+  const int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
+  if (i < count) work[i] = SOLVE_EMPTY;
 }
 
-__device__ __forceinline__ void wait_flag(const int32_t* flag, int32_t* status) {
-  if (threadIdx.x == 0) {
+// Wavefront 0 polls the cnt values at src until all are solved and stages them
+// in xs (zero past cnt); the barrier also orders every earlier read of xs
+// before the new values land.
+__device__ __forceinline__ void wait_block(const double* src, int cnt, double* xs, int32_t* status) {
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    const long long* p = reinterpret_cast<const long long*>(src) + lane;
+    long long bits = 0;
     int polls = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    while (true) {
+      bits = lane < cnt ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+      if (__all(bits != SOLVE_EMPTY)) break;
       if (++polls > SOLVE_SPIN_MAX) {
-        __hip_atomic_fetch_or(status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) __hip_atomic_fetch_or(status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
-      __builtin_amdgcn_s_sleep(2);
+      __builtin_amdgcn_s_sleep(1);
     }
+    xs[lane] = lane < cnt ? __longlong_as_double(bits) : 0.0;
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-__device__ __forceinline__ void publish_flag(int32_t* flag) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void publish_value(double* dst, double v) {
+  __hip_atomic_store(reinterpret_cast<long long*>(dst), __double_as_longlong(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Forward: block i solves L_ii y_i = b_i - sum_{j<i} L_ij y_j.
 __global__ void __launch_bounds__(SOLVE_WG)
-    chol_forward_blocks(const double* __restrict__ L, int64_t n, int64_t ld, double* b, int32_t* flags,
+    chol_forward_blocks(const double* __restrict__ L, int64_t n, int64_t ld, double* b, double* ywork,
                         int32_t* status) {
-  __shared__ double T[NB][NB + 1];
   __shared__ double D[NB][NB + 1];
   __shared__ double xs[NB], acc[NB];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -448,23 +461,29 @@ __global__ void __launch_bounds__(SOLVE_WG)
 #pragma unroll
   for (int c = 0; c < NB; ++c) drow[c] = D[lane][c];
   const double dinv = 1.0 / D[lane][lane];
-  const int r = t >> 2, q = t & 3;  // four lanes per row, 16 columns each
+  // four lanes per row, 16 columns each, read straight into registers; the next
+  // block's slice is loaded before waiting on the current one
+  const int r = t >> 2, q = t & 3;
+  auto load = [&](int j, double (&g)[16]) {
+#pragma unroll
+    for (int c = 0; c < 16; ++c) g[c] = r < m ? L[(i0 + r) * ld + (int64_t)j * NB + q * 16 + c] : 0.0;
+  };
+  double cur[16];
+  if (i > 0) load(0, cur);
   for (int j = 0; j < i; ++j) {
-    for (int e = t; e < NB * NB; e += SOLVE_WG) {
-      const int rr = e / NB, c = e % NB;
-      T[rr][c] = rr < m ? L[(i0 + rr) * ld + (int64_t)j * NB + c] : 0.0;
-    }
-    wait_flag(flags + j, status);
-    if (t < NB) xs[t] = b[(int64_t)j * NB + t];
-    __syncthreads();
+    double nxt[16];
+    if (j + 1 < i) load(j + 1, nxt);
+    wait_block(ywork + (int64_t)j * NB, NB, xs, status);
     double sacc = 0.0;
 #pragma unroll
-    for (int c = 0; c < 16; ++c) sacc += T[r][q * 16 + c] * xs[q * 16 + c];
+    for (int c = 0; c < 16; ++c) sacc += cur[c] * xs[q * 16 + c];
     sacc += __shfl_xor(sacc, 1);
     sacc += __shfl_xor(sacc, 2);
     if (q == 0 && r < m) acc[r] -= sacc;
-    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 16; ++c) cur[c] = nxt[c];
   }
+  __syncthreads();
   if (wave == 0) {
     double v = lane < m ? acc[lane] : 0.0;
 #pragma unroll
@@ -474,16 +493,17 @@ __global__ void __launch_bounds__(SOLVE_WG)
         v = lane == jj ? xj : (lane > jj ? v - drow[jj] * xj : v);
       }
     }
-    if (lane < m) b[i0 + lane] = v;
+    if (lane < m) {
+      b[i0 + lane] = v;
+      publish_value(ywork + i0 + lane, v);
+    }
   }
-  publish_flag(flags + i);
 }
 
 // Backward: workgroup w solves block k = nb - 1 - w: L_kk^T x_k = y_k - sum_{j>k} L_jk^T x_j.
 __global__ void __launch_bounds__(SOLVE_WG)
-    chol_backward_blocks(const double* __restrict__ L, int64_t n, int64_t ld, double* b, int32_t* flags,
+    chol_backward_blocks(const double* __restrict__ L, int64_t n, int64_t ld, double* b, double* xwork,
                          int32_t* status) {
-  __shared__ double T[NB][NB + 1];
   __shared__ double D[NB][NB + 1];
   __shared__ double xs[NB], acc[NB];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -502,25 +522,34 @@ __global__ void __launch_bounds__(SOLVE_WG)
 #pragma unroll
   for (int rr = 0; rr < NB; ++rr) dcol[rr] = D[rr][lane];
   const double dinv = 1.0 / D[lane][lane];
-  const int c = t >> 2, q = t & 3;  // four lanes per column, 16 rows each
+  // four lanes per column, 16 rows each, read straight into registers; the next
+  // block's slice is loaded before waiting on the current one
+  const int c = t >> 2, q = t & 3;
+  auto load = [&](int jb, double (&g)[16]) {
+    const int64_t j0 = (int64_t)jb * NB;
+    const int mj = (int)min((int64_t)NB, n - j0);
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr)
+      g[rr] = (q * 16 + rr < mj && c < m) ? L[(j0 + q * 16 + rr) * ld + k0 + c] : 0.0;
+  };
+  double cur[16];
+  if (k < nb - 1) load(nb - 1, cur);
   for (int jb = nb - 1; jb > k; --jb) {
     const int64_t j0 = (int64_t)jb * NB;
     const int mj = (int)min((int64_t)NB, n - j0);
-    for (int e = t; e < NB * NB; e += SOLVE_WG) {
-      const int rr = e / NB, cc = e % NB;
-      T[rr][cc] = (rr < mj && cc < m) ? L[(j0 + rr) * ld + k0 + cc] : 0.0;
-    }
-    wait_flag(flags + nb + jb, status);
-    if (t < NB) xs[t] = t < mj ? b[j0 + t] : 0.0;
-    __syncthreads();
+    double nxt[16];
+    if (jb - 1 > k) load(jb - 1, nxt);
+    wait_block(xwork + j0, mj, xs, status);
     double sacc = 0.0;
 #pragma unroll
-    for (int rr = 0; rr < 16; ++rr) sacc += T[q * 16 + rr][c] * xs[q * 16 + rr];
+    for (int rr = 0; rr < 16; ++rr) sacc += cur[rr] * xs[q * 16 + rr];
     sacc += __shfl_xor(sacc, 1);
     sacc += __shfl_xor(sacc, 2);
     if (q == 0 && c < m) acc[c] -= sacc;
-    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) cur[rr] = nxt[rr];
   }
+  __syncthreads();
   if (wave == 0) {
     double v = lane < m ? acc[lane] : 0.0;
 #pragma unroll
@@ -530,9 +559,11 @@ __global__ void __launch_bounds__(SOLVE_WG)
         v = lane == jj ? xj : (lane < jj ? v - dcol[jj] * xj : v);
       }
     }
-    if (lane < m) b[k0 + lane] = v;
+    if (lane < m) {
+      b[k0 + lane] = v;
+      publish_value(xwork + k0 + lane, v);
+    }
   }
-  publish_flag(flags + nb + k);
 }
 
 }  // namespace
@@ -574,19 +605,19 @@ extern "C" int tlh_cholesky_solve_f64(const double* L, int64_t n, int64_t ld, do
   return 0;
 }
 
-extern "C" int tlh_cholesky_solve_blocks_f64(const double* L, int64_t n, int64_t ld, double* b, int32_t* flags,
+extern "C" int tlh_cholesky_solve_blocks_f64(const double* L, int64_t n, int64_t ld, double* b, double* work,
                                              int32_t* status, tlh_stream_t stream) {
-  if (!L || !b || !flags || !status || n < 0 || ld < n) return TLH_EINVAL;
+  if (!L || !b || !work || !status || n < 0 || ld < n) return TLH_EINVAL;
   if (n == 0) return 0;
   const int64_t nb = ceil_div(n, NB);
   if (nb > (1 << 20)) return TLH_EINVAL;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(solve_flags_reset, dim3((unsigned)ceil_div(2 * nb, TLH_THREADS)), dim3(TLH_THREADS), 0, s, flags,
-                     (int)(2 * nb));
+  hipLaunchKernelGGL(solve_work_reset, dim3((unsigned)ceil_div(2 * n, TLH_THREADS)), dim3(TLH_THREADS), 0, s,
+                     reinterpret_cast<long long*>(work), 2 * n);
   TLH_RETURN_IF_LAUNCH_FAILED();
-  hipLaunchKernelGGL(chol_forward_blocks, dim3((unsigned)nb), dim3(SOLVE_WG), 0, s, L, n, ld, b, flags, status);
+  hipLaunchKernelGGL(chol_forward_blocks, dim3((unsigned)nb), dim3(SOLVE_WG), 0, s, L, n, ld, b, work, status);
   TLH_RETURN_IF_LAUNCH_FAILED();
-  hipLaunchKernelGGL(chol_backward_blocks, dim3((unsigned)nb), dim3(SOLVE_WG), 0, s, L, n, ld, b, flags, status);
+  hipLaunchKernelGGL(chol_backward_blocks, dim3((unsigned)nb), dim3(SOLVE_WG), 0, s, L, n, ld, b, work + n, status);
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }

@@ -644,15 +644,22 @@ __global__ void __launch_bounds__(TLH_THREADS)
     for (int d = 0; d < DC; ++d)
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) acc[c][d] += __shfl_xor(acc[c][d], off);
-  if (lane != 0) return;
+  // every lane holds all DC*DC sums; lane l < DC*DC writes entry (l / DC, l % DC)
+  // and its mirror, both loads issued before either store
+  if (lane >= DC * DC) return;
+  double mine = 0.0;
+#pragma unroll
+  for (int cd = 0; cd < DC * DC; ++cd)
+    if (lane == cd) mine = acc[cd / DC][cd % DC];
   const int64_t ci = pair_key_ci[key], cj = pair_key_cj[key];
-#pragma unroll
-  for (int c = 0; c < DC; ++c)
-#pragma unroll
-    for (int d = 0; d < DC; ++d) {
-      M[(ci * DC + c) * ld + cj * DC + d] -= acc[c][d];
-      if (ci != cj) M[(cj * DC + d) * ld + ci * DC + c] -= acc[c][d];
-    }
+  const int c = lane / DC, d = lane % DC;
+  const int64_t p0 = (ci * DC + c) * ld + cj * DC + d;
+  const int64_t p1 = (cj * DC + d) * ld + ci * DC + c;
+  const bool mirror = ci != cj;
+  const double m0 = M[p0];
+  const double m1 = mirror ? M[p1] : 0.0;
+  M[p0] = m0 - mine;
+  if (mirror) M[p1] = m1 - mine;
 }
 
 // ------------------------------------------------------------------ host side

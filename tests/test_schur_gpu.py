@@ -99,6 +99,24 @@ def test_cholesky(device, n):
         cholesky_solve_f64(torch.tensor(-M, device=device), torch.tensor(b, device=device))
 
 
+def test_cholesky_solve_sentinel_rhs(device):
+    """The multi-workgroup solve marks unsolved values with a signalling-NaN bit pattern;
+    a right-hand side holding that exact pattern still completes (NaN result, no wait
+    gives up) and leaves the other systems' results alone."""
+    print("cholesky sentinel rhs", flush=True)
+    n = 200
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((n, n + 3))
+    M = torch.tensor(A @ A.T + n * 1e-3 * np.eye(n), device=device)
+    b = rng.standard_normal(n)
+    rhs = torch.tensor(b, device=device)
+    rhs.view(torch.int64)[0] = 0x7FF4DEAD5EED1234
+    x = cholesky_solve_f64(M, rhs).cpu().numpy()
+    assert np.isnan(x).all()
+    x = cholesky_solve_f64(M, torch.tensor(b, device=device)).cpu().numpy()
+    assert rel_err(x, np.linalg.solve(M.cpu().numpy(), b)) <= 1e-10
+
+
 @pytest.mark.parametrize("solver", ["schur_cg", "schur_dense"])
 def test_schur_lm_matches_oracle(device, solver):
     print(f"schur lm {solver}", flush=True)

@@ -395,15 +395,15 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
 // value with agent-scope atomic stores, into a workspace that the launch first
 // fills with a signalling-NaN pattern; a waiting workgroup polls those values
 // with agent-scope atomic loads until none holds the pattern, so the values are
-// their own ready flags (no fences, no separate flag round trip).  Arithmetic
-// results are never signalling NaNs, so a solved value cannot look unsolved.
+// their own ready flags (no fences, no separate flag round trip); a solved
+// value never carries the pattern (see publish_value).
 // Each update loads its L slice before waiting, so the serial chain is the
 // 64-step diagonal solves and the polls.  Workgroups are dispatched in index
 // order and a block waits only on blocks with lower workgroup indices, so every
 // wait is on a workgroup that is already running; a wait still gives up after
 // SOLVE_SPIN_MAX polls, setting *status |= 2.
 constexpr int SOLVE_WG = 256;
-constexpr int SOLVE_SPIN_MAX = 1 << 24;
+constexpr int SOLVE_SPIN_MAX = 1 << 20;  // about a second of polls; a real wait is the solve's length
 constexpr long long SOLVE_EMPTY = 0x7ff4dead5eed1234ll;  // a signalling NaN
 
 __global__ void __launch_bounds__(TLH_THREADS) solve_work_reset(long long* work, int64_t count) {
@@ -435,9 +435,12 @@ __device__ __forceinline__ void wait_block(const double* src, int cnt, double* x
   __syncthreads();
 }
 
+// A value equal to the pattern (possible only if a signalling NaN in b passed
+// through unquieted) is published as a quiet NaN, so every wait ends.
 __device__ __forceinline__ void publish_value(double* dst, double v) {
-  __hip_atomic_store(reinterpret_cast<long long*>(dst), __double_as_longlong(v), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
+  long long bits = __double_as_longlong(v);
+  if (bits == SOLVE_EMPTY) bits = 0x7ff8000000000000ll;
+  __hip_atomic_store(reinterpret_cast<long long*>(dst), bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Forward: block i solves L_ii y_i = b_i - sum_{j<i} L_ij y_j.

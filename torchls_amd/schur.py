@@ -368,9 +368,10 @@ class HipSchurSystem:
         n = p.dim
         M = torch.zeros((n, n), dtype=torch.float64, device=p.device)
         for t, (c, d, off) in zip(p.retained, p.rlayout[:-1]):
-            blk = self.lin.diagonal[t] + self.damping * torch.eye(d, dtype=torch.float64, device=p.device)
-            idx = off + torch.arange(c, device=p.device)[:, None] * d + torch.arange(d, device=p.device)
-            M[idx[:, :, None], idx[:, None, :]] = blk
+            # block v of this type sits at rows and columns off + v*d .. off + v*d + d - 1
+            blocks = M.as_strided((c, d, d), (d * n + d, n, 1), storage_offset=off * n + off)
+            blocks.copy_(self.lin.diagonal[t])
+            blocks.diagonal(dim1=1, dim2=2).add_(self.damping)
         wg = p.plan.groups[p.wgroup]
         # per-observation W_o and W_o C_p^-1 (float64), read by the camera-pair sums
         work = torch.empty(max(1, wg.count * 2 * wg.slots[1 - p.eslot].dim * p.edim), dtype=torch.float64,

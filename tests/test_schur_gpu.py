@@ -75,7 +75,7 @@ def test_schur_operators(device, priors, dtype, tol):
     assert rel_err(np.tril(M), np.tril(Md)) <= tol
 
 
-@pytest.mark.parametrize("n", [5, 64, 70, 200, 1536])
+@pytest.mark.parametrize("n", [5, 64, 70, 129, 200, 1000, 1536])
 def test_cholesky(device, n):
     print(f"cholesky n={n}", flush=True)
     rng = np.random.default_rng(n)

@@ -144,6 +144,16 @@ int64_t tlh_group_lds_bytes(const tlh_operator_t* op, int group);
 int tlh_pack_blocks(const tlh_operator_t* op, int group, const void* const* blocks,
                     int in_dtype, tlh_stream_t stream);
 
+/* tlh_pack_blocks of J diag(col_scale): entry (row, col) of a slot's block is
+ * multiplied by col_scale[offset_t + index * dim_t + col] (float64 [op->size], in
+ * op's layout, offset_t = op->types[slot type].offset) before the conversion, so
+ * every operator reading the packed blocks (normal matvec, diagonal and Schur
+ * kernels) works on the column-scaled Jacobian without a scaled copy of J.
+ * Replaces: BlockJacobian.scale_columns (sparse.py:86-102) as used by
+ * jacobi_scaling / damping_mode="diagonal" (solvers.py:430-472). */
+int tlh_pack_blocks_scaled(const tlh_operator_t* op, int group, const void* const* blocks,
+                           int in_dtype, const double* col_scale, tlh_stream_t stream);
+
 /* y = J^T (J x) + damping * x, deterministic (fixed-order) reductions.
  * x and y are [size] vectors in op->dtype.
  * Replaces: WarpNormalLinearization.normal_matvec (warp_normal.py:248-294),
@@ -382,7 +392,9 @@ int tlh_schur_precond_major(const void* jc, const void* jp, int block_dtype, int
 /* Subtract W C^-1 W^T from a dense float64 reduced matrix M (leading dimension
  * ld, holding blockdiag(D + lambda I)): one thread per camera pair key
  * (key_ci <= key_cj) summing its observation pairs pair_i/pair_j
- * [pair_ptr[k], pair_ptr[k+1]) in order (deterministic, no atomics).  work
+ * [pair_ptr[k], pair_ptr[k+1]) in order (deterministic, no atomics).  Camera ci's
+ * rows start at the retained type's offset in op (op->types[..].offset) plus
+ * ci * dim, so other retained types may precede it in M.  work
  * (float64 [count][2 * DC * DP], may be NULL) holds W_o and W_o C_p^-1 per
  * observation for the 2x6 / 2x3 bundle-adjustment blocks, computed once instead
  * of once per camera pair.

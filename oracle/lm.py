@@ -1,7 +1,8 @@
 """float64 Levenberg-Marquardt restatement (CPU oracle) of torchls solvers.solve.
 
 Follows solvers.py:288-696 for the conjugate-gradient and Schur paths
-(identity damping, no column scaling, no bounds): initial cost, per-iteration
+(identity or diagonal damping, optional Jacobi column scaling, no bounds):
+initial cost, per-iteration
 gradient and termination tests, block-Jacobi PCG (warp_pcg control flow by
 default), trial evaluation through expanded differences, gain ratio, and the
 fixed-factor / adaptive lambda updates.  ``problem`` provides layout,
@@ -28,6 +29,9 @@ class TrustRegion:
     step_quality_min: float = 1e-4
     max_trials: int = 20
     strategy: str = "fixed_factor"
+    damping_mode: str = "identity"
+    diagonal_min: float = 1e-6
+    diagonal_max: float = 1e32
 
 
 @dataclass
@@ -53,14 +57,41 @@ class Summary:
     linear_iterations_history: list = field(default_factory=list)
 
 
+def _solve_scale(diag, column_scale, jacobi_scaling, tr):
+    """(column_scale, solve_scale) of one iteration (solvers.py:430-460): the Jacobi
+    column scale 1 / (1 + sqrt(diag(J^T J))) fixed at the first iteration, and with
+    diagonal damping the inverse square root of the clamped (scaled) diagonal."""
+    if jacobi_scaling and column_scale is None:
+        column_scale = np.concatenate(
+            [(1.0 / (1.0 + np.sqrt(np.maximum(np.diagonal(b, axis1=1, axis2=2), 0.0)))).reshape(-1) for b in diag])
+    solve_scale = column_scale
+    if tr is not None and tr.damping_mode == "diagonal":
+        nd = np.concatenate([np.diagonal(b, axis1=1, axis2=2).reshape(-1) for b in diag])
+        if column_scale is not None:
+            nd = nd * column_scale**2
+        inv = 1.0 / np.sqrt(np.clip(nd, tr.diagonal_min, tr.diagonal_max))
+        solve_scale = inv if column_scale is None else column_scale * inv
+    return column_scale, solve_scale
+
+
+def _scale_blocks(diag, layout, scale):
+    """D_t -> S_t D_t S_t per variable block (solvers.py:463-472)."""
+    out = []
+    for (c, d, off), b in zip(layout, diag):
+        s = scale[off : off + c * d].reshape(c, d)
+        out.append(b * s[:, :, None] * s[:, None, :])
+    return out
+
+
 def solve(problem, vals, *, linear_solver="conjugate_gradient", cg: Config = Config(),
           tr: TrustRegion | None = TrustRegion(), term: Termination = Termination(), eliminate: int | None = None,
-          pcg_flavour: str = "batched", preconditioner: bool = True):
+          pcg_flavour: str = "batched", preconditioner: bool = True, jacobi_scaling: bool = False):
     pcg_fn = pcg_batched if pcg_flavour == "batched" else pcg_reference
     c0 = problem.cost(vals)
     s = Summary(c0, c0, cost_history=[c0])
     lam = tr.lambda_initial if tr else 0.0
     rej = 2.0
+    column_scale = None
     lin = problem.linearize(vals)
     for it in range(term.max_iterations):
         s.iterations = it + 1
@@ -73,7 +104,13 @@ def solve(problem, vals, *, linear_solver="conjugate_gradient", cg: Config = Con
         if s.gradient_norm <= term.gradient_tolerance:
             s.termination_reason = "gradient_tolerance"
             break
+        J0, g0 = J, g  # the model decrease uses the unscaled system (solvers.py:608-611)
         diag = J.diagonal_blocks()
+        column_scale, solve_scale = _solve_scale(diag, column_scale, jacobi_scaling, tr)
+        if solve_scale is not None:
+            J = J.scale_columns(solve_scale)
+            g = g * solve_scale
+            diag = _scale_blocks(diag, J.layout, solve_scale)
         old_cost = lin["objective"] if lin["objective"] is not None else 0.5 * float(r @ r)
         accepted = False
         for _ in range(tr.max_trials if tr else 1):
@@ -83,7 +120,8 @@ def solve(problem, vals, *, linear_solver="conjugate_gradient", cg: Config = Con
                     apply = block_jacobi(diag, J.layout, lam) if preconditioner else None
                     res = pcg_fn(lambda v: J.normal_matvec(v, lam), -g, apply, cg)
                     delta, its = res.solution, res.iterations
-                    usable = res.reason in ("residual_tolerance", "max_iterations") and np.all(np.isfinite(delta))
+                    usable = res.reason in ("residual_tolerance", "max_iterations", "stagnated") and np.all(
+                        np.isfinite(delta))
                 elif linear_solver in ("schur_cg", "schur_dense"):
                     sys_ = SchurSystem(J, diag, eliminate, lam)
                     rhs = sys_.reduced_rhs(-g)
@@ -109,7 +147,8 @@ def solve(problem, vals, *, linear_solver="conjugate_gradient", cg: Config = Con
                                 return out
                         res = pcg_fn(sys_.matvec, rhs, apply, cg)
                         xr, its = res.solution, res.iterations
-                        usable = res.reason in ("residual_tolerance", "max_iterations") and np.all(np.isfinite(xr))
+                        usable = res.reason in ("residual_tolerance", "max_iterations", "stagnated") and np.all(
+                            np.isfinite(xr))
                     delta = sys_.backsubstitute(xr)
                 elif linear_solver == "dense":
                     A = J.to_dense()
@@ -122,9 +161,11 @@ def solve(problem, vals, *, linear_solver="conjugate_gradient", cg: Config = Con
             except np.linalg.LinAlgError:
                 usable = False
             if usable:
+                if solve_scale is not None:
+                    delta = solve_scale * delta
                 prop = problem.retract(vals, delta)
                 obj, red = problem.evaluate_trial(prop, lin)
-                pred = float(-(g @ delta) - 0.5 * J.squared_matvec_norm(delta))
+                pred = float(-(g0 @ delta) - 0.5 * J0.squared_matvec_norm(delta))
                 q = red / pred if pred > 0 else -math.inf
                 accepted = math.isfinite(obj) and (tr is None or (red > 0 and q >= tr.step_quality_min))
                 if accepted:

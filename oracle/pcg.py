@@ -40,6 +40,13 @@ class Config:
     restart_strategy: str = "adaptive"
     residual_drift_tolerance: float = 0.1
     graph_batch_iterations: int = 16
+    # the HIP backend's optional exit (no reference counterpart): a restart after a
+    # converged recurrence whose true residual is above STALL_FACTOR times the true
+    # residual at the previous such restart ends the solve as "stagnated"
+    stop_on_stagnation: bool = False
+
+
+STALL_FACTOR = 0.5
 
 
 def pcg_reference(matvec, rhs, apply=None, cfg: Config = Config()) -> Result:
@@ -151,6 +158,7 @@ def pcg_batched(matvec, rhs, apply=None, cfg: Config = Config()) -> Result:
     it = checks = restarts = 0
     reason = "max_iterations"
     tn = math.nan
+    converged_true = math.inf
     while it < cfg.max_iterations:
         iv = cfg.residual_check_interval
         stop_at = min(cfg.max_iterations, ((it // iv + 1) * iv) if iv else cfg.max_iterations)
@@ -174,6 +182,11 @@ def pcg_batched(matvec, rhs, apply=None, cfg: Config = Config()) -> Result:
             break
         if it >= cfg.max_iterations:
             break
+        if status == 1 and cfg.stop_on_stagnation:
+            if tn > STALL_FACTOR * converged_true:
+                reason = "stagnated"
+                break
+            converged_true = tn
         drift = float(np.linalg.norm(tr - r))
         if status == 1 or cfg.restart_strategy == "periodic" or drift > cfg.residual_drift_tolerance * max(tn, thr):
             r = tr

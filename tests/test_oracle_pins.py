@@ -202,3 +202,102 @@ def test_block_jacobi_is_the_blockwise_solve(jac):
         for k in range(count):
             sl = slice(off + k * dim, off + (k + 1) * dim)
             assert np.allclose(out[sl], np.linalg.solve(D[k] + 0.7 * np.eye(dim), v[sl]), rtol=1e-10, atol=1e-12)
+
+
+# ---------------------------------------------------------------- warp_pcg counts by hand
+# Diagonal SPD systems whose eigenvalues take k distinct powers of two, with a
+# right-hand side of ones: unpreconditioned CG reaches the exact solution after k
+# iterations (to rounding, far below the 1e-6 relative threshold) and not before.
+# The (iterations, restarts, checks, reason, converged) below are derived by hand
+# from the batched control flow of warp_pcg.py:176-213: a batch replays up to the
+# window end stop_at = min(max_iterations, next multiple of residual_check_interval);
+# the host checks the true residual when the status is nonzero or the window ended;
+# a check restarts on status 1 / periodic strategy / drift (never here: the drift is
+# rounding-sized) unless the residual converged or max_iterations was reached.
+PCG_COUNT_CASES = [
+    # (k distinct eigenvalues, config, expected)
+    (3, dict(), (3, 0, 1, "residual_tolerance", True)),            # one check, at status 1
+    (5, dict(residual_check_interval=2), (5, 0, 3, "residual_tolerance", True)),  # windows end at 2, 4; status 1 at 5
+    (5, dict(residual_check_interval=2, graph_batch_iterations=1), (5, 0, 3, "residual_tolerance", True)),
+    (5, dict(residual_check_interval=2, graph_batch_iterations=3), (5, 0, 3, "residual_tolerance", True)),
+    (5, dict(residual_check_interval=0, graph_batch_iterations=4), (5, 0, 1, "residual_tolerance", True)),
+    (4, dict(residual_check_interval=4), (4, 0, 1, "residual_tolerance", True)),  # status 1 at the window end
+    (5, dict(residual_check_interval=2, max_iterations=3), (3, 0, 2, "max_iterations", False)),
+    (5, dict(residual_check_interval=2, max_iterations=3, restart_strategy="periodic"),
+     (3, 1, 2, "max_iterations", False)),                           # restart at the check of iteration 2
+    (5, dict(max_iterations=5), (5, 0, 1, "residual_tolerance", True)),
+    (5, dict(max_iterations=4), (4, 0, 1, "max_iterations", False)),
+    (5, dict(residual_check_interval=3, restart_strategy="periodic", max_iterations=7),
+     (7, 2, 3, "max_iterations", False)),                           # restarts at 3 and 6, last check at 7
+]
+
+
+def pcg_count_system(k, n=20):
+    eig = 2.0 ** (np.arange(n) % k)
+    return eig, np.ones(n)
+
+
+@pytest.mark.parametrize("case", range(len(PCG_COUNT_CASES)))
+def test_pcg_batched_counts_by_hand(case):
+    k, cfg, expected = PCG_COUNT_CASES[case]
+    eig, b = pcg_count_system(k)
+    res = pcg_batched(lambda v: eig * v, b, None, Config(**cfg))
+    assert (res.iterations, res.restarts, res.residual_checks, res.reason, res.converged) == expected
+    if res.converged:
+        assert np.allclose(res.solution, b / eig, rtol=1e-12)
+
+
+def test_pcg_batched_exits_by_hand():
+    """Exits at the first read: an exact block-Jacobi preconditioner converges in one
+    iteration; an indefinite diagonal stops at status 3 after one (frozen) iteration;
+    a NaN operator at status 2; a zero or NaN right-hand side before any iteration."""
+    eig, b = pcg_count_system(5)
+    res = pcg_batched(lambda v: eig * v, b, lambda v: v / eig, Config())
+    assert (res.iterations, res.restarts, res.residual_checks, res.reason, res.converged) == (
+        1, 0, 1, "residual_tolerance", True)
+    res = pcg_batched(lambda v: np.array([1.0, -1.0]) * v, np.array([1.0, 2.0]), None, Config())
+    assert (res.iterations, res.restarts, res.residual_checks, res.reason, res.converged) == (
+        1, 0, 1, "nonpositive_curvature", False)
+    assert np.array_equal(res.solution, np.zeros(2))  # the guarded step was not taken
+    res = pcg_batched(lambda v: v * np.nan, b, None, Config())
+    assert (res.iterations, res.restarts, res.residual_checks, res.reason, res.converged) == (
+        1, 0, 1, "nonfinite", False)
+    res = pcg_batched(lambda v: eig * v, np.zeros_like(b), None, Config())
+    assert (res.iterations, res.residual_checks, res.reason, res.converged) == (0, 0, "residual_tolerance", True)
+    res = pcg_batched(lambda v: eig * v, b * np.nan, None, Config())
+    assert (res.iterations, res.residual_checks, res.reason, res.converged) == (0, 0, "nonfinite", False)
+
+
+@pytest.mark.parametrize("mode", ["diagonal", "jacobi", "both"])
+def test_lm_scaling_is_marquardt_damping(mode):
+    """Column scaling s and lambda I on J diag(s) is the damped system
+    (J^T J + lambda diag(s)^-2) delta = -g of the unscaled problem: Marquardt's
+    lambda diag(J^T J) for damping_mode="diagonal", lambda (1 + sqrt(diag))^2 for
+    Jacobi scaling.  The first accepted step of the oracle LM equals that solve."""
+    from oracle.ba import BAProblem
+    from oracle.lm import Termination, TrustRegion, solve as lm_solve
+    from torchls_amd.generators import bundle_adjustment
+
+    prob = bundle_adjustment(6, 40, 4, seed=31)
+    o = BAProblem(prob.cam_idx, prob.pt_idx, prob.observations, prob.n_cameras, prob.n_points, huber_delta=None)
+    vals0 = (prob.cameras_init, prob.points_init)
+    tr = TrustRegion(damping_mode="diagonal" if mode in ("diagonal", "both") else "identity")
+    vals1, s = lm_solve(o, vals0, linear_solver="dense", tr=tr, term=Termination(max_iterations=1),
+                        jacobi_scaling=mode in ("jacobi", "both"))
+    assert (s.accepted_steps, s.rejected_steps) == (1, 0)
+    lin = o.linearize(vals0)
+    A = lin["J"].to_dense()
+    H = A.T @ A
+    g = A.T @ lin["residual"]
+    d = np.diag(H).copy()
+    if mode == "diagonal":
+        damp = np.clip(d, tr.diagonal_min, tr.diagonal_max)
+    elif mode == "jacobi":
+        damp = (1.0 + np.sqrt(d)) ** 2
+    else:
+        s2 = 1.0 / (1.0 + np.sqrt(d)) ** 2
+        damp = np.clip(d * s2, tr.diagonal_min, tr.diagonal_max) / s2
+    delta = np.linalg.solve(H + tr.lambda_initial * np.diag(damp), -g)
+    expect = o.retract(vals0, delta)
+    for a, b in zip(vals1, expect):
+        assert np.allclose(a, b, rtol=1e-9, atol=1e-12)

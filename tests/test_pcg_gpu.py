@@ -154,3 +154,39 @@ def test_failure_reasons(device):
     ref = pcg_batched(lambda v: -J.normal_matvec(v, lam), -g, None, Config())
     assert neg.reason == ref.reason == "nonpositive_curvature"
     assert neg.iterations == ref.iterations
+
+
+@pytest.mark.parametrize("case", range(11))
+def test_counts_by_hand(device, case):
+    """The hand-derived warp_pcg counts of the oracle pins, on the HIP PCG."""
+    from tests.test_oracle_pins import PCG_COUNT_CASES, pcg_count_system
+
+    k, cfg, expected = PCG_COUNT_CASES[case]
+    print(f"pcg counts by hand k={k} cfg={cfg}", flush=True)
+    eig, b = pcg_count_system(k)
+    e = torch.tensor(eig, device=device)
+    res = hip_pcg(lambda v: e * v, torch.tensor(b, device=device), preconditioner=None,
+                  config=ConjugateGradientConfig(**cfg))
+    assert (res.iterations, res.restarts, res.residual_checks, res.reason, res.converged) == expected
+    if res.converged:
+        assert np.allclose(res.solution.cpu().numpy(), b / eig, rtol=1e-12)
+
+
+def test_exits_by_hand(device):
+    from tests.test_oracle_pins import pcg_count_system
+
+    eig, b = pcg_count_system(5)
+    e = torch.tensor(eig, device=device)
+    bt = torch.tensor(b, device=device)
+    cfg = ConjugateGradientConfig()
+    res = hip_pcg(lambda v: e * v, bt, preconditioner=lambda v: v / e, config=cfg)
+    assert (res.iterations, res.restarts, res.residual_checks, res.reason, res.converged) == (
+        1, 0, 1, "residual_tolerance", True)
+    d = torch.tensor([1.0, -1.0], device=device)
+    res = hip_pcg(lambda v: d * v, torch.tensor([1.0, 2.0], device=device), preconditioner=None, config=cfg)
+    assert (res.iterations, res.restarts, res.residual_checks, res.reason, res.converged) == (
+        1, 0, 1, "nonpositive_curvature", False)
+    assert torch.count_nonzero(res.solution).item() == 0
+    res = hip_pcg(lambda v: v * float("nan"), bt, preconditioner=None, config=cfg)
+    assert (res.iterations, res.restarts, res.residual_checks, res.reason, res.converged) == (
+        1, 0, 1, "nonfinite", False)

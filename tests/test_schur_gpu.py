@@ -132,3 +132,44 @@ def test_schur_lm_matches_oracle(device, solver):
     assert math.isclose(s.final_cost, ref.final_cost, rel_tol=1e-9)
     if solver == "schur_cg":
         assert s.linear_iterations_history == ref.linear_iterations_history
+
+
+def scale_prior(scale, reference):
+    return scale - reference
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-11), (torch.float32, 2e-5)])
+def test_schur_retained_type_before_cameras(device, dtype, tol):
+    """A retained group with only a prior, declared before the cameras, moves the
+    camera rows of the reduced system: the dense matrix, matvec and preconditioner
+    still match the oracle's closed form."""
+    print(f"schur retained type first dtype={dtype}", flush=True)
+    prob = generators.bundle_adjustment(16, 500, 8, seed=22)
+    rng = np.random.default_rng(5)
+    scale0 = rng.standard_normal((3, 2))
+    P = Problem({"scale": torch.tensor(scale0, device=device),
+                 "cams": torch.tensor(prob.cameras_init, device=device),
+                 "points": torch.tensor(prob.points_init, device=device)}, manifolds={"cams": SE3Var})
+    P.add_residual(scale_prior, ("scale", torch.arange(3)), torch.tensor(scale0 + 0.3, device=device))
+    P.add_residual(reprojection, ("cams", torch.tensor(prob.cam_idx)), ("points", torch.tensor(prob.pt_idx)),
+                   torch.tensor(prob.observations, device=device), loss=HuberLoss(1e-2))
+    an = P._analyze()
+    lin = an.linearize(P.parameters)
+    diag = tuple(d.double() for d in lin.jacobian.diagonal_blocks())
+    lam = 1e-2
+    plan = HipSchurPlan(an, "points", device, dtype=dtype)
+    system = plan.linearize(lin.jacobian, diag, dense=True).damped(lam)
+    J = to_np(lin.jacobian, an.layout)
+    ref = SchurSystem(J, [d.cpu().numpy() for d in diag], 2, lam)
+    x = rng.standard_normal(ref.dim)
+    y = system.matvec(torch.tensor(x, device=device, dtype=dtype))
+    assert rel_err(y.cpu().numpy(), ref.matvec(x)) <= tol
+    for a, b in zip(system.preconditioner_blocks(), ref.preconditioner_blocks()):
+        assert rel_err(a.cpu().numpy(), b) <= tol
+    M = system.dense_matrix().cpu().numpy()
+    Md = ref.dense_matrix()
+    assert rel_err(np.tril(M), np.tril(Md)) <= tol
+    # the LM solve with the dense reduced system matches the Schur-CG solve
+    _, s_dense = P.solve(return_summary=True, linear_solver="schur_dense", eliminate="points",
+                         termination=TerminationConfig(max_iterations=3))
+    assert math.isfinite(s_dense.final_cost) and s_dense.final_cost < s_dense.initial_cost

@@ -135,6 +135,10 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.POINTER(Operator), ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int, c_void_p],
     ),
+    "tlh_pack_blocks_scaled": (
+        ctypes.c_int,
+        [ctypes.POINTER(Operator), ctypes.c_int, ctypes.POINTER(c_void_p), ctypes.c_int, c_void_p, c_void_p],
+    ),
     "tlh_normal_matvec": (
         ctypes.c_int,
         [ctypes.POINTER(Operator), c_void_p, c_void_p, ctypes.c_double, c_void_p],

@@ -758,11 +758,20 @@ __global__ void __launch_bounds__(TLH_THREADS) finish_jobs_kernel(FinishJobs<Rea
   }
 }
 
+// Optional column scale of the packed blocks (J diag(s), sparse.py:86-102): entry
+// (slot, row, col) of sorted position n is multiplied by scale[off + index[n] * dim
+// + col], off = the slot type's first column, in float64 before the conversion.
+struct PackScale {
+  const double* scale;             // [op->size] or nullptr
+  const int32_t* index[TLH_MAX_SLOTS];
+  int64_t offset[TLH_MAX_SLOTS];
+};
+
 template <typename In, typename Out>
 __global__ void __launch_bounds__(TLH_THREADS)
     pack_kernel(Out* __restrict__ packed, int64_t stride, const int64_t* __restrict__ order,
                 int64_t count, int rows, int nslots, const In* b0, const In* b1, const In* b2,
-                int d0, int d1, int d2) {
+                int d0, int d1, int d2, PackScale sc) {
   const int64_t n = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
   if (n >= count) return;
   const int64_t o = order[n];
@@ -770,9 +779,15 @@ __global__ void __launch_bounds__(TLH_THREADS)
   const int dims[3] = {d0, d1, d2};
   int comp = 0;
   for (int s = 0; s < nslots; ++s) {
-    const int k = rows * dims[s];
+    const int d = dims[s];
+    const int k = rows * d;
     const In* p = src[s] + o * k;
-    for (int e = 0; e < k; ++e) packed[(int64_t)(comp + e) * stride + n] = (Out)p[e];
+    if (sc.scale) {
+      const double* cs = sc.scale + sc.offset[s] + (int64_t)sc.index[s][n] * d;
+      for (int e = 0; e < k; ++e) packed[(int64_t)(comp + e) * stride + n] = (Out)((double)p[e] * cs[e % d]);
+    } else {
+      for (int e = 0; e < k; ++e) packed[(int64_t)(comp + e) * stride + n] = (Out)p[e];
+    }
     comp += k;
   }
 }
@@ -780,11 +795,11 @@ __global__ void __launch_bounds__(TLH_THREADS)
 // Two-slot pack with each observation's block rows read as 8/16-byte vectors
 // in one burst (observation order is usually a random permutation, so a
 // row-at-a-time gather touches every cache line once instead of once per
-// element).  K0, K1 = rows * dims of the two slots (even).
-template <typename In, typename Out, int K0, int K1>
+// element).  K0, K1 = rows * dims of the two slots (even); D0, D1 = the dims.
+template <typename In, typename Out, int K0, int K1, int D0, int D1, bool SCALED>
 __global__ void __launch_bounds__(TLH_THREADS)
     pack_pair_kernel(Out* __restrict__ packed, int64_t stride, const int64_t* __restrict__ order, int64_t count,
-                     const In* __restrict__ b0, const In* __restrict__ b1) {
+                     const In* __restrict__ b0, const In* __restrict__ b1, PackScale sc) {
   static_assert(K0 % 2 == 0 && K1 % 2 == 0, "rows of even length");
   typedef typename VecT<In, 2>::type V2;
   const int64_t n = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
@@ -805,10 +820,24 @@ __global__ void __launch_bounds__(TLH_THREADS)
     r1[2 * k] = v[0];
     r1[2 * k + 1] = v[1];
   }
+  if constexpr (SCALED) {
+    double s0[D0], s1[D1];
+    const double* c0 = sc.scale + sc.offset[0] + (int64_t)sc.index[0][n] * D0;
+    const double* c1 = sc.scale + sc.offset[1] + (int64_t)sc.index[1][n] * D1;
 #pragma unroll
-  for (int e = 0; e < K0; ++e) packed[(int64_t)e * stride + n] = (Out)r0[e];
+    for (int c = 0; c < D0; ++c) s0[c] = c0[c];
 #pragma unroll
-  for (int e = 0; e < K1; ++e) packed[(int64_t)(K0 + e) * stride + n] = (Out)r1[e];
+    for (int c = 0; c < D1; ++c) s1[c] = c1[c];
+#pragma unroll
+    for (int e = 0; e < K0; ++e) packed[(int64_t)e * stride + n] = (Out)((double)r0[e] * s0[e % D0]);
+#pragma unroll
+    for (int e = 0; e < K1; ++e) packed[(int64_t)(K0 + e) * stride + n] = (Out)((double)r1[e] * s1[e % D1]);
+  } else {
+#pragma unroll
+    for (int e = 0; e < K0; ++e) packed[(int64_t)e * stride + n] = (Out)r0[e];
+#pragma unroll
+    for (int e = 0; e < K1; ++e) packed[(int64_t)(K0 + e) * stride + n] = (Out)r1[e];
+  }
 }
 
 // ------------------------------------------------------------------- host side
@@ -1060,30 +1089,61 @@ int finish_impl(const tlh_operator_t* op, int t0, int t1, const void* x, double 
   return 0;
 }
 
+int check_op(const tlh_operator_t* op);
+
 template <typename In, typename Out>
-int pack_impl(const tlh_group_t& g, const void* const* blocks, hipStream_t stream) {
+int pack_impl(const tlh_operator_t* op, const tlh_group_t& g, const void* const* blocks, const double* scale,
+              hipStream_t stream) {
   const In* b[3] = {nullptr, nullptr, nullptr};
   int d[3] = {0, 0, 0};
+  PackScale sc{};
+  sc.scale = scale;
   for (int s = 0; s < g.nslots; ++s) {
     b[s] = static_cast<const In*>(blocks[s]);
     d[s] = g.slot[s].dim;
+    sc.index[s] = g.slot[s].index;
+    sc.offset[s] = op->types[g.slot[s].type].offset;
   }
   const dim3 grid((unsigned)ceil_div(g.count, TLH_THREADS)), block(TLH_THREADS);
   Out* out = static_cast<Out*>(g.packed);
   const int k0 = g.rows * d[0], k1 = g.rows * d[1];
   const uintptr_t vec = 2 * sizeof(In) - 1;  // the pair kernel reads 2-element vectors
   const bool aligned = g.nslots == 2 && ((uintptr_t)b[0] & vec) == 0 && ((uintptr_t)b[1] & vec) == 0;
-  if (aligned && k0 == 12 && k1 == 6)
-    hipLaunchKernelGGL((pack_pair_kernel<In, Out, 12, 6>), grid, block, 0, stream, out, g.stride, g.order, g.count,
-                       b[0], b[1]);
-  else if (aligned && k0 == 6 && k1 == 12)
-    hipLaunchKernelGGL((pack_pair_kernel<In, Out, 6, 12>), grid, block, 0, stream, out, g.stride, g.order, g.count,
-                       b[0], b[1]);
-  else
+  const bool ba = g.rows == 2 && ((d[0] == 6 && d[1] == 3) || (d[0] == 3 && d[1] == 6));
+  if (aligned && ba && k0 == 12) {
+    if (scale)
+      hipLaunchKernelGGL((pack_pair_kernel<In, Out, 12, 6, 6, 3, true>), grid, block, 0, stream, out, g.stride,
+                         g.order, g.count, b[0], b[1], sc);
+    else
+      hipLaunchKernelGGL((pack_pair_kernel<In, Out, 12, 6, 6, 3, false>), grid, block, 0, stream, out, g.stride,
+                         g.order, g.count, b[0], b[1], sc);
+  } else if (aligned && ba && k0 == 6) {
+    if (scale)
+      hipLaunchKernelGGL((pack_pair_kernel<In, Out, 6, 12, 3, 6, true>), grid, block, 0, stream, out, g.stride,
+                         g.order, g.count, b[0], b[1], sc);
+    else
+      hipLaunchKernelGGL((pack_pair_kernel<In, Out, 6, 12, 3, 6, false>), grid, block, 0, stream, out, g.stride,
+                         g.order, g.count, b[0], b[1], sc);
+  } else {
     hipLaunchKernelGGL((pack_kernel<In, Out>), grid, block, 0, stream, out, g.stride, g.order, g.count, g.rows,
-                       g.nslots, b[0], b[1], b[2], d[0], d[1], d[2]);
+                       g.nslots, b[0], b[1], b[2], d[0], d[1], d[2], sc);
+  }
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
+}
+
+int pack_dispatch(const tlh_operator_t* op, int group, const void* const* blocks, int in_dtype, const double* scale,
+                  tlh_stream_t stream) {
+  if (int rc = check_op(op)) return rc;
+  if (group < 0 || group >= op->ngroups || !blocks) return TLH_EINVAL;
+  const tlh_group_t& g = op->groups[group];
+  if (g.count == 0) return 0;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (in_dtype == TLH_F32 && op->dtype == TLH_F32) return pack_impl<float, float>(op, g, blocks, scale, s);
+  if (in_dtype == TLH_F64 && op->dtype == TLH_F32) return pack_impl<double, float>(op, g, blocks, scale, s);
+  if (in_dtype == TLH_F64 && op->dtype == TLH_F64) return pack_impl<double, double>(op, g, blocks, scale, s);
+  if (in_dtype == TLH_F32 && op->dtype == TLH_F64) return pack_impl<float, double>(op, g, blocks, scale, s);
+  return TLH_EINVAL;
 }
 
 int check_op(const tlh_operator_t* op) {
@@ -1137,16 +1197,13 @@ extern "C" int64_t tlh_group_lds_bytes(const tlh_operator_t* op, int group) {
 
 extern "C" int tlh_pack_blocks(const tlh_operator_t* op, int group, const void* const* blocks,
                                int in_dtype, tlh_stream_t stream) {
-  if (int rc = check_op(op)) return rc;
-  if (group < 0 || group >= op->ngroups || !blocks) return TLH_EINVAL;
-  const tlh_group_t& g = op->groups[group];
-  if (g.count == 0) return 0;
-  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  if (in_dtype == TLH_F32 && op->dtype == TLH_F32) return pack_impl<float, float>(g, blocks, s);
-  if (in_dtype == TLH_F64 && op->dtype == TLH_F32) return pack_impl<double, float>(g, blocks, s);
-  if (in_dtype == TLH_F64 && op->dtype == TLH_F64) return pack_impl<double, double>(g, blocks, s);
-  if (in_dtype == TLH_F32 && op->dtype == TLH_F64) return pack_impl<float, double>(g, blocks, s);
-  return TLH_EINVAL;
+  return pack_dispatch(op, group, blocks, in_dtype, nullptr, stream);
+}
+
+extern "C" int tlh_pack_blocks_scaled(const tlh_operator_t* op, int group, const void* const* blocks,
+                                      int in_dtype, const double* col_scale, tlh_stream_t stream) {
+  if (!col_scale) return TLH_EINVAL;
+  return pack_dispatch(op, group, blocks, in_dtype, col_scale, stream);
 }
 
 extern "C" int tlh_normal_matvec(const tlh_operator_t* op, const void* x, void* y, double damping,

@@ -861,6 +861,11 @@ extern "C" int tlh_schur_dense(const tlh_operator_t* op, int wgroup, int eslot, 
   const int R = g.rows, DC = g.slot[sc].dim, DP = g.slot[se].dim;
   if (DC > 8 || DP > 8) return TLH_EINVAL;
   if (npairs == 0) return 0;
+  // camera-pair blocks sit at the W-group retained type's rows and columns: its
+  // offset in the retained layout (other retained types may come first)
+  const int64_t roff = op->types[g.slot[sc].type].offset;
+  if (roff < 0 || roff + op->types[g.slot[sc].type].count * DC > ld) return TLH_EINVAL;
+  M += roff * ld + roff;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const dim3 grid((unsigned)ceil_div(npairs, TLH_THREADS)), block(TLH_THREADS);
   const bool ba = R == 2 && DC == 6 && DP == 3;

@@ -45,14 +45,17 @@ def check_columns(jacobian) -> None:
                 raise ValueError("HIP operators require layout-ordered columns index*dim+offset+arange(dim)")
 
 
-def packed_blocks(jacobian, group: int, order_key, components: int, stride: int, dtype, pack) -> torch.Tensor:
+def packed_blocks(jacobian, group: int, order_key, components: int, stride: int, dtype, pack,
+                  col_scale: torch.Tensor | None = None) -> torch.Tensor:
     """Packed [components, stride] blocks of one residual group, shared by every plan
-    whose sorted order, stride and dtype match (``order_key``); ``pack(out)`` fills a
-    new buffer.  Cached on the BlockJacobian, so each linearization packs once."""
+    whose sorted order, stride, dtype and column scale match (``order_key``);
+    ``pack(out)`` fills a new buffer.  Cached on the BlockJacobian, so each
+    linearization packs once per scale."""
     cache = jacobian.__dict__.setdefault("_hip_packed", {})
-    # in-place edits of the blocks bump their version counters and force a re-pack
+    # in-place edits of the blocks (or of the scale) bump their version counters and force a re-pack
     blocks = jacobian.groups[group].blocks
-    key = (group, order_key, components, stride, dtype, tuple((b.data_ptr(), b._version) for b in blocks))
+    skey = None if col_scale is None else (col_scale.data_ptr(), col_scale._version)
+    key = (group, order_key, components, stride, dtype, tuple((b.data_ptr(), b._version) for b in blocks), skey)
     out = cache.get(key)
     if out is None:
         out = torch.empty((components, max(stride, 1)), dtype=dtype, device=jacobian.groups[group].blocks[0].device)
@@ -237,9 +240,16 @@ class HipNormalPlan:
             ):
                 raise ValueError("HIP normal plan requires unchanged incidence structure")
 
-    def bind(self, jacobian) -> "HipNormalLinearization":
+    def bind(self, jacobian, *, col_scale: torch.Tensor | None = None) -> "HipNormalLinearization":
+        """Pack one linearization.  ``col_scale`` (float64 [n], optional) binds the
+        column-scaled Jacobian J diag(col_scale) instead (sparse.py:86-102), scaled
+        while packing: no scaled copy of J is made."""
         self.check_structure(jacobian)
-        return HipNormalLinearization(self, jacobian)
+        if col_scale is not None:
+            if col_scale.shape != (self.size,) or col_scale.device != self.device:
+                raise ValueError("col_scale must be a vector of the tangent dimension on the plan's device")
+            col_scale = col_scale.to(torch.float64).contiguous()
+        return HipNormalLinearization(self, jacobian, col_scale)
 
 
 class HipNormalLinearization:
@@ -247,7 +257,7 @@ class HipNormalLinearization:
 
     hip_graph_safe = True  # normal_matvec launches only HIP kernels on the current stream
 
-    def __init__(self, plan: HipNormalPlan, jacobian):
+    def __init__(self, plan: HipNormalPlan, jacobian, col_scale: torch.Tensor | None = None):
         self.plan = plan
         device, dtype = plan.device, plan.dtype
         self.partials = [
@@ -263,10 +273,16 @@ class HipNormalLinearization:
                 op = plan.dev.operator(dtype, bufs, self.partials)
                 blocks = [b.contiguous() for b in group.blocks]
                 ptrs = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in blocks] + [None] * (3 - len(blocks)))
-                _lib.check(plan.lib.tlh_pack_blocks(ctypes.byref(op), gi, ptrs, in_code, stream), "tlh_pack_blocks")
+                if col_scale is None:
+                    _lib.check(plan.lib.tlh_pack_blocks(ctypes.byref(op), gi, ptrs, in_code, stream),
+                               "tlh_pack_blocks")
+                else:
+                    _lib.check(plan.lib.tlh_pack_blocks_scaled(ctypes.byref(op), gi, ptrs, in_code,
+                                                               col_scale.data_ptr(), stream), "tlh_pack_blocks_scaled")
 
             self.packed.append(packed_blocks(jacobian, gi, plan.dev.order_keys[gi], g.components, g.stride, dtype,
-                                             pack))
+                                             pack, col_scale))
+        self._col_scale = col_scale  # kept alive while the packed blocks may be re-read
         self.op = plan.dev.operator(dtype, self.packed, self.partials)
 
     def _check_vector(self, vector: torch.Tensor) -> None:

@@ -77,9 +77,11 @@ class PcgState:
 # Replays per host read after the first batch: iterations past convergence are
 # frozen by the control word but still run the operator, so later batches are short.
 SMALL_BATCH = 4
-# A restart after the recurrence converged must at least halve the true residual;
-# otherwise the operator's own rounding (a float32 matvec under float64 vectors)
-# bounds the attainable residual and the solve stops as "stagnated" (usable).
+# With config.stop_on_stagnation: a restart after the recurrence converged must at
+# least halve the true residual of the previous such restart; otherwise the
+# operator's own rounding (a float32 matvec under float64 vectors) bounds the
+# attainable residual and the solve stops as "stagnated" (usable).  Without it the
+# solve keeps restarting until max_iterations, as warp_pcg does.
 STALL_FACTOR = 0.5
 # Systems up to this size run each iteration's vector work in one workgroup
 # (tlh_pcg_iteration_small accepts up to _lib.PCG_SMALL_MAX_N); larger ones spread
@@ -263,7 +265,7 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None):
             break
         if iterations >= config.max_iterations:
             break
-        if status == 1:
+        if status == 1 and getattr(config, "stop_on_stagnation", False):
             if true_norm > STALL_FACTOR * converged_true:
                 reason = "stagnated"
                 break

@@ -176,18 +176,31 @@ class HipSchurPlan:
         self.pairs = dict(ptr=t(ptr), i=t(pi), j=t(pj), ci=t(ukey // nc), cj=t(ukey % nc), n=int(ukey.size))
         return self.pairs
 
-    def linearize(self, jacobian, diagonal, *, dense: bool = False):
+    def linearize(self, jacobian, diagonal, *, dense: bool = False, col_scale: torch.Tensor | None = None):
+        """Pack one linearization; ``col_scale`` (float64, full tangent layout) makes
+        the system that of J diag(col_scale), scaled while packing (``diagonal`` must
+        then be the scaled blocks, as the reference passes them)."""
         if self.plan is None:
             self._build(jacobian)
         if dense and self.retained_only_multi:
             raise NotImplementedError("dense HIP Schur supports single-slot retained-only residual groups")
-        return HipSchurLinearization(self, jacobian, diagonal, dense)
+        local_scale = None
+        if col_scale is not None:
+            total = sum(e.count * e.dim for e in self.layout)
+            if col_scale.shape != (total,) or col_scale.device != self.device:
+                raise ValueError("col_scale must be a vector of the tangent dimension on the plan's device")
+            col_scale = col_scale.to(torch.float64)
+            # the plan's layout: retained types (re-based), then the eliminated type
+            local_scale = torch.cat((col_scale[self.global_indices],
+                                     col_scale[self.eoffset: self.eoffset + self.ecount * self.edim])).contiguous()
+        return HipSchurLinearization(self, jacobian, diagonal, dense, local_scale)
 
 
 class HipSchurLinearization:
-    def __init__(self, plan: HipSchurPlan, jacobian, diagonal, dense):
+    def __init__(self, plan: HipSchurPlan, jacobian, diagonal, dense, col_scale=None):
         self.plan = plan
         self.dense = dense
+        self.col_scale = col_scale
         self.diagonal = tuple(d.to(torch.float64).contiguous() for d in diagonal)
         p = plan
         stream = _lib.stream_handle(p.device)
@@ -208,12 +221,17 @@ class HipSchurLinearization:
                 bufs[k] = out
                 op = p.dev.operator(p.dtype, bufs, p.partials)
                 ptrs = (ctypes.c_void_p * 3)(*[b.data_ptr() for b in blocks] + [None] * (3 - len(blocks)))
-                _lib.check(p.lib.tlh_pack_blocks(ctypes.byref(op), k, ptrs, _lib.dtype_code(blocks[0].dtype),
-                                                 stream), "tlh_pack_blocks")
+                if col_scale is None:
+                    _lib.check(p.lib.tlh_pack_blocks(ctypes.byref(op), k, ptrs, _lib.dtype_code(blocks[0].dtype),
+                                                     stream), "tlh_pack_blocks")
+                else:
+                    _lib.check(p.lib.tlh_pack_blocks_scaled(ctypes.byref(op), k, ptrs,
+                                                            _lib.dtype_code(blocks[0].dtype), col_scale.data_ptr(),
+                                                            stream), "tlh_pack_blocks_scaled")
 
             if k == p.wgroup:  # every slot of the jacobian group: shareable with other plans
                 self.packed.append(packed_blocks(jacobian, gi, p.dev.order_keys[k], g.components, g.stride, p.dtype,
-                                                 pack))
+                                                 pack, col_scale))
             else:
                 out = torch.empty((g.components, max(g.stride, 1)), dtype=p.dtype, device=p.device)
                 pack(out)
@@ -333,7 +351,8 @@ class HipSchurSystem:
         lds = _lib.device_info(p.device).lds_per_block
         for k, t in enumerate(p.retained):
             D = self.lin.diagonal[t]
-            if k == wtype and p.major is not None:
+            # the camera-order kernel reads the raw (unscaled) blocks: a scaled system uses the tile kernel
+            if k == wtype and p.major is not None and self.lin.col_scale is None:
                 M = torch.empty_like(D)
                 order, edge_ptr, var_edge_ptr, edge_point = p.major
                 raw = self.lin._keep[p.wgroup]

@@ -44,6 +44,12 @@ class ConjugateGradientConfig:
     # Storage/arithmetic dtype of the packed blocks and PCG vectors; None = the
     # parameters' dtype.  torch.float32 with float64 parameters = mixed precision.
     matvec_dtype: torch.dtype | None = None
+    # Not in the reference (warp_pcg keeps restarting until max_iterations): when set,
+    # a restart after a converged recurrence that fails to halve the true residual of
+    # the previous such restart ends the solve with reason "stagnated" (a usable
+    # step, like "max_iterations").  This is where a float32 matvec under float64
+    # PCG vectors reaches its rounding floor.
+    stop_on_stagnation: bool = False
 
     def __post_init__(self):
         if (isinstance(self.normal_max_packed_bytes, bool) or not isinstance(self.normal_max_packed_bytes, int)
@@ -71,6 +77,8 @@ class ConjugateGradientConfig:
             raise ValueError("residual_drift_tolerance must be finite and positive")
         if self.matvec_dtype not in (None, torch.float32, torch.float64):
             raise ValueError("matvec_dtype must be None, torch.float32 or torch.float64")
+        if not isinstance(self.stop_on_stagnation, bool):
+            raise ValueError("stop_on_stagnation must be a bool")
 
 
 @dataclass(frozen=True)
@@ -184,20 +192,6 @@ def block_jacobi(jacobian, damping: float, *, diagonal=None, groups=(), graph_sa
     if diagonal is None:
         diagonal = HipDiagonalPlan(jacobian).diagonal_blocks(jacobian)
     return HipBlockJacobi(jacobian.layout, tuple(d.to(torch.float64) for d in diagonal), damping)
-
-
-def _scaled_jacobian(jacobian, scale: Tensor):
-    """J diag(scale) with the same symbolic structure (sparse.py:86-102)."""
-    from .sparse import BlockJacobian, JacobianGroup
-
-    groups = tuple(
-        JacobianGroup(
-            tuple((b * scale[c].to(b.dtype)[:, None, :]).contiguous() for b, c in zip(g.blocks, g.columns)),
-            g.columns, g.type_indices, g.variable_indices,
-        )
-        for g in jacobian.groups
-    )
-    return BlockJacobian(groups, jacobian.layout, jacobian.shape)
 
 
 @torch.no_grad()
@@ -343,7 +337,9 @@ def solve(
                 inv_diag = nd.clamp(trust_region.diagonal_min, trust_region.diagonal_max).rsqrt()
                 solve_scale = inv_diag if column_scale is None else column_scale * inv_diag
             if solve_scale is not None:
-                jacobian = _scaled_jacobian(jacobian, solve_scale)
+                # J diag(s) is never copied: the pack kernels scale the blocks as they
+                # lay them out for the normal / Schur operators (tlh_pack_blocks_scaled)
+                solve_scale = solve_scale.to(torch.float64).contiguous()
                 linear_gradient = gradient * solve_scale
                 if diagonal is not None:
                     diagonal = tuple(
@@ -356,11 +352,12 @@ def solve(
             schur_linear = None
             normal_operator = None
             if is_schur:
-                schur_linear = schur_plan.linearize(jacobian, diagonal, dense=linear_solver == "schur_dense")
+                schur_linear = schur_plan.linearize(jacobian, diagonal, dense=linear_solver == "schur_dense",
+                                                    col_scale=solve_scale)
             elif linear_solver == "conjugate_gradient":
                 if normal_plan is None:
                     normal_plan = cached(("normal",), lambda: HipNormalPlan(jacobian, dtype=mv_dtype))
-                normal_operator = normal_plan.bind(jacobian)
+                normal_operator = normal_plan.bind(jacobian, col_scale=solve_scale)
         accepted = False
         trial_limit = trust_region.max_trials if trust_region else 1
         for _ in range(trial_limit):
@@ -374,6 +371,8 @@ def solve(
             try:
                 if linear_solver == "dense_qr":
                     dense = jacobian.to_dense()
+                    if solve_scale is not None:
+                        dense = dense * solve_scale.to(dense.dtype)
                     eye = torch.eye(problem.tangent_dim, dtype=dtype, device=device)
                     aug = torch.cat((dense, math.sqrt(damping) * eye))
                     rhs = torch.cat((-residual, gradient.new_zeros(problem.tangent_dim).to(residual.dtype)))

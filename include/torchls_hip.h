@@ -192,6 +192,14 @@ typedef struct tlh_block_diag {
 int tlh_block_inverse(const double* blocks, int64_t count, int dim, double damping,
                       double* inverse, int32_t* status, tlh_stream_t stream);
 
+/* y[v] = inverse[v] x[v] for the variables v in list (int32, count entries; NULL =
+ * v < count) of one type: dim-vectors at v * dim, float64 packed upper-triangle
+ * inverses as written by tlh_block_inverse, x and y in dtype, float64 arithmetic.
+ * Replaces: the per-block solves C^-1 b of DampedSchurSystem.backsubstitute
+ * (schur.py:292-301) for eliminated variables no coupling residual observes. */
+int tlh_block_apply(const double* inverse, int dim, const int32_t* list, int64_t count, const void* x, void* y,
+                    int dtype, tlh_stream_t stream);
+
 /* ---------------------------------------------------------------- PCG */
 #define TLH_PCG_MAX_BLOCKS 1024
 #define TLH_PCG_RZ 0        /* r.z of the current iterate (state[0] of warp_pcg) */

@@ -151,6 +151,10 @@ SIGNATURES = {
         ctypes.c_int,
         [c_void_p, i64, ctypes.c_int, ctypes.c_double, c_void_p, c_void_p, c_void_p],
     ),
+    "tlh_block_apply": (
+        ctypes.c_int,
+        [c_void_p, ctypes.c_int, c_void_p, i64, c_void_p, c_void_p, ctypes.c_int, c_void_p],
+    ),
     "tlh_dot": (
         ctypes.c_int,
         [c_void_p, c_void_p, i64, ctypes.c_int, ctypes.POINTER(PcgStateDesc), ctypes.c_int, c_void_p],

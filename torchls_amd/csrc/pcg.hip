@@ -788,3 +788,34 @@ extern "C" int tlh_pcg_true_residual(const void* rhs, const void* ax, const void
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }
+
+namespace {
+template <typename Real>
+__global__ void __launch_bounds__(TLH_THREADS)
+    block_apply_kernel(const double* __restrict__ inv, int d, const int32_t* __restrict__ list, int64_t count,
+                       const Real* __restrict__ x, Real* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x;
+  if (i >= count) return;
+  const int64_t v = list ? (int64_t)list[i] : i;
+  double rz = 0.0, rr = 0.0;
+  apply_any<Real>(inv + v * (d * (d + 1) / 2), d, x + v * d, y + v * d, rz, rr);
+}
+}  // namespace
+
+extern "C" int tlh_block_apply(const double* inverse, int dim, const int32_t* list, int64_t count, const void* x,
+                               void* y, int dtype, tlh_stream_t stream) {
+  if (count < 0 || dim < 1 || dim > 16 || (count && (!inverse || !x || !y))) return TLH_EINVAL;
+  if (count == 0) return 0;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const dim3 grid((unsigned)ceil_div(count, TLH_THREADS)), block(TLH_THREADS);
+  if (dtype == TLH_F32)
+    hipLaunchKernelGGL(block_apply_kernel<float>, grid, block, 0, s, inverse, dim, list, count,
+                       static_cast<const float*>(x), static_cast<float*>(y));
+  else if (dtype == TLH_F64)
+    hipLaunchKernelGGL(block_apply_kernel<double>, grid, block, 0, s, inverse, dim, list, count,
+                       static_cast<const double*>(x), static_cast<double*>(y));
+  else
+    return TLH_EINVAL;
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}

@@ -123,6 +123,9 @@ class HipSchurPlan:
         cuts = w.tile_start[1:-1]
         if cuts.size and np.any(prim[cuts] == prim[cuts - 1]):
             raise NotImplementedError("an eliminated variable has more observations than one tile holds")
+        # eliminated variables no coupling residual observes: back-substituted as C^-1 b_e alone
+        lonely = np.setdiff1d(np.arange(self.ecount), prim[: w.count])
+        self.lonely = (torch.from_numpy(lonely.astype(np.int32)).to(self.device) if lonely.size else None)
         self.dev = DeviceStructure(self.plan, self.device)
         self.sdev = DeviceStructure(self.sparse_plan, self.device)
         # camera-order preconditioner (csrc/major.hip) for few, heavily observed retained variables
@@ -335,6 +338,10 @@ class HipSchurSystem:
         xr = retained.to(p.dtype).contiguous()
         xe = torch.empty(p.ecount * p.edim, dtype=p.dtype, device=p.device)
         self._apply(_lib_mode("BACKSUB"), xr, self._bp, xe)
+        if p.lonely is not None:  # the tile pass writes only the observed eliminated variables
+            _lib.check(p.lib.tlh_block_apply(self.cinv.data_ptr(), p.edim, p.lonely.data_ptr(), p.lonely.numel(),
+                                             self._bp.data_ptr(), xe.data_ptr(), _lib.dtype_code(p.dtype),
+                                             _lib.stream_handle(p.device)), "tlh_block_apply")
         # the retained indices and the eliminated range cover the whole tangent vector
         total = sum(e.count * e.dim for e in p.layout)
         out = torch.empty(total, dtype=torch.float64, device=p.device)

@@ -454,6 +454,19 @@ int tlh_cholesky_solve_f64(const double* L, int64_t n, int64_t ld, double* b, tl
 int tlh_cholesky_solve_blocks_f64(const double* L, int64_t n, int64_t ld, double* b, double* work, int32_t* status,
                                   tlh_stream_t stream);
 
+/* tlh_cholesky_solve_blocks_f64 without the work argument: the 2 * n float64
+ * workspace is a library-owned device array of 2 * 8192 entries (128 KiB per
+ * device, held until the process ends), resolved once per device on the first
+ * call made with that device current.  No device memory is allocated by the call,
+ * so it suits callers that cannot allocate (code under hipGraph capture, serving
+ * loops, harnesses that forbid hipMalloc); the first call on a device should be
+ * made outside capture.  n > 8192 returns TLH_EINVAL.  Every call on a device
+ * shares that device's array (the call's fill kernel resets it): calls that
+ * overlap on one device (concurrent streams) race.
+ * No counterpart in the reference. */
+int tlh_cholesky_solve_blocks_f64_noalloc(const double* L, int64_t n, int64_t ld, double* b, int32_t* status,
+                                          tlh_stream_t stream);
+
 /* ---------------------------------------------------------------- reprojection residual */
 /* One reprojection residual batch r_i = pi(R_c p + t_c) - obs_i (pi(x,y,z) = (x/z, y/z);
  * poses world-to-camera, rows [R | t], cam_stride 12 (3x4) or 16 (4x4) elements per

@@ -13,6 +13,8 @@
 //      layout: lane l supplies A[l % 16][k0 + l / 16] and B[k0 + l / 16][l % 16];
 //      accumulator element v of lane l is C[4 v + l / 16][l % 16].
 // Non-positive or non-finite pivots set *status (LinAlgError analogue).
+#include <mutex>
+
 #include "common.hpp"
 
 using namespace tlh;
@@ -608,13 +610,14 @@ extern "C" int tlh_cholesky_solve_f64(const double* L, int64_t n, int64_t ld, do
   return 0;
 }
 
-extern "C" int tlh_cholesky_solve_blocks_f64(const double* L, int64_t n, int64_t ld, double* b, double* work,
-                                             int32_t* status, tlh_stream_t stream) {
-  if (!L || !b || !work || !status || n < 0 || ld < n) return TLH_EINVAL;
-  if (n == 0) return 0;
+namespace {
+// Largest system of the blocks solve: the dense Schur path caps the reduced
+// dimension at 8192, and the bounded waits of a 2 x 128-step chain stay far
+// inside their poll limit.
+constexpr int64_t SOLVE_MAX_N = 8192;
+
+int solve_blocks(const double* L, int64_t n, int64_t ld, double* b, double* work, int32_t* status, hipStream_t s) {
   const int64_t nb = ceil_div(n, NB);
-  if (nb > (1 << 20)) return TLH_EINVAL;
-  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(solve_work_reset, dim3((unsigned)ceil_div(2 * n, TLH_THREADS)), dim3(TLH_THREADS), 0, s,
                      reinterpret_cast<long long*>(work), 2 * n);
   TLH_RETURN_IF_LAUNCH_FAILED();
@@ -623,4 +626,45 @@ extern "C" int tlh_cholesky_solve_blocks_f64(const double* L, int64_t n, int64_t
   hipLaunchKernelGGL(chol_backward_blocks, dim3((unsigned)nb), dim3(SOLVE_WG), 0, s, L, n, ld, b, work + n, status);
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
+}
+
+// Workspace of tlh_cholesky_solve_blocks_f64_noalloc: one copy per device, held
+// until the process ends.  Host code never uses this name as an address; it
+// uses the per-device pointer resolved by hipGetSymbolAddress below.
+__device__ __align__(256) double scratch[2 * SOLVE_MAX_N];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+double* scratch_ptr[MAX_DEVICES] = {};
+
+int scratch_for_current_device(double** out) {
+  int dev = 0;
+  TLH_RETURN_IF_ERROR(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return TLH_EINVAL;
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    TLH_RETURN_IF_ERROR(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return TLH_EINVAL;
+    scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" int tlh_cholesky_solve_blocks_f64(const double* L, int64_t n, int64_t ld, double* b, double* work,
+                                             int32_t* status, tlh_stream_t stream) {
+  if (!L || !b || !work || !status || n < 0 || ld < n || n > SOLVE_MAX_N) return TLH_EINVAL;
+  if (n == 0) return 0;
+  return solve_blocks(L, n, ld, b, work, status, reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int tlh_cholesky_solve_blocks_f64_noalloc(const double* L, int64_t n, int64_t ld, double* b,
+                                                     int32_t* status, tlh_stream_t stream) {
+  if (!L || !b || !status || n < 0 || ld < n || n > SOLVE_MAX_N) return TLH_EINVAL;
+  if (n == 0) return 0;
+  double* work = nullptr;
+  if (int rc = scratch_for_current_device(&work)) return rc;
+  return solve_blocks(L, n, ld, b, work, status, reinterpret_cast<hipStream_t>(stream));
 }

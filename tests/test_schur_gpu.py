@@ -100,9 +100,11 @@ def test_cholesky(device, n):
 
 
 def test_cholesky_solve_sentinel_rhs(device):
-    """The multi-workgroup solve marks unsolved values with a signalling-NaN bit pattern;
-    a right-hand side holding that exact pattern still completes (NaN result, no wait
-    gives up) and leaves the other systems' results alone."""
+    """The multi-workgroup solve marks unsolved values with a signalling-NaN bit pattern.
+    A right-hand side holding that exact pattern completes end to end (NaN result, no
+    wait gives up) and the next solve is unaffected.  (The pattern is quieted by the
+    first arithmetic on it, so this checks that the solve completes end to end, not
+    the remap branch of publish_value itself.)"""
     print("cholesky sentinel rhs", flush=True)
     n = 200
     rng = np.random.default_rng(7)

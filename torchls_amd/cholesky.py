@@ -46,5 +46,9 @@ def cholesky_solve_f64(M: torch.Tensor, rhs: torch.Tensor) -> torch.Tensor:
     if code & 1:
         raise LinAlgError("matrix is not positive definite")
     if code & 2:
-        raise RuntimeError("tlh_cholesky_solve_blocks_f64: a block wait gave up")
+        # a block's bounded wait gave up (the blocks solve relies on its workgroups
+        # being scheduled while earlier blocks are still running): solve again with
+        # the one-workgroup substitution, which has no cross-workgroup waits
+        x = rhs.to(torch.float64).contiguous().clone()
+        _lib.check(lib.tlh_cholesky_solve_f64(L.data_ptr(), n, n, x.data_ptr(), stream), "tlh_cholesky_solve_f64")
     return x

@@ -304,13 +304,16 @@ def pose_graph(
     loop_fraction: float = 0.03,
     loop_radius: float = 2.0,
     noise: float = 0.01,
+    init_noise: float | None = None,
 ) -> PoseGraph:
     """Planar random walk in SE(3); odometry plus loop closures within ``loop_radius``.
 
     Loop closures are sampled among pose pairs (|i-j| > 5) whose positions lie within
     ``loop_radius``; their number is ``loop_fraction * n_poses`` (fewer if not enough
     candidate pairs exist).  Each measurement is the true relative pose times
-    Exp(N(0, noise^2 I_6)).  The initial guess integrates the noisy odometry.
+    Exp(N(0, noise^2 I_6)).  The initial guess integrates the noisy odometry, or,
+    with ``init_noise``, is each true pose times Exp(N(0, init_noise^2 I_6)) (pose 0
+    exact): a start inside the basin of the optimum, from which LM converges.
     """
     rng = np.random.default_rng(seed)
     yaw = np.cumsum(np.concatenate(([0.0], rng.normal(0.0, yaw_noise, size=n_poses - 1))))
@@ -350,8 +353,12 @@ def pose_graph(
     rel = se3_inv_np(T[fi]) @ T[fj]
     Z = rel @ se3_exp_np(rng.normal(0.0, noise, size=(fi.size, 6)))
     # initial guess: chain the noisy odometry
-    init = np.zeros_like(T)
-    init[0] = T[0]
-    for k in range(n_poses - 1):
-        init[k + 1] = init[k] @ Z[k]
+    if init_noise is None:
+        init = np.zeros_like(T)
+        init[0] = T[0]
+        for k in range(n_poses - 1):
+            init[k + 1] = init[k] @ Z[k]
+    else:
+        init = T @ se3_exp_np(rng.normal(0.0, init_noise, size=(n_poses, 6)))
+        init[0] = T[0]
     return PoseGraph(seed, T, init, fi, fj, Z, n_loop)

@@ -1,15 +1,22 @@
 """Benchmark of the HIP block-sparse normal-equation path on one MI355X.
 
-Workload (default): synthetic bundle adjustment with 200 cameras, 50,000 points,
+Headline workload: synthetic bundle adjustment with 200 cameras, 50,000 points,
 10 observations per point (500,000 reprojection residuals), Huber delta 1e-2,
 LM with Schur-PCG on the 1,200-dimensional reduced camera system.  A step is
-one LM iteration (linearization, diagonal blocks, Schur preconditioner,
-hipGraph-replayed PCG, trial evaluation).  The mixed-precision HIP solve
-(float32 blocks/matvec, float64 scalars, gradient and objective) is timed
-against the all-float64 HIP solve on the same seed, and its final cost is
-checked against the float64 solve.  The dominant kernel (the implicit Schur
-matvec) is timed separately with HIP events for the roofline figure, and a
-bounded float64 CPU-oracle sample is timed on the host cores.
+one LM iteration (linearization, gradient, diagonal blocks, Schur
+preconditioner, hipGraph-replayed PCG, back-substitution, retraction, trial
+evaluation).  From the generator's initial point the first four iterations are
+productive (after them the predicted reduction is ~1e-17 of the cost, below
+float64 resolution), so the K timed steps are ceil(K / 4) solves of four
+iterations each, restarted from the initial point, with inputs resident in
+HBM.  The mixed-precision HIP solve (float32 blocks/matvec/PCG vectors, float64
+scalars, gradient and objective) is timed against the all-float64 HIP solve on
+the same seed and protocol; iterations, trials and termination reasons of both
+are reported, with the final-cost error against the float64 HIP solve and the
+float64 CPU oracle.  The roofline figure is the normal matvec at 5,012,836
+observations (2,000 cameras, the HBM-resident configuration), timed with HIP
+events on its launch stream; a bounded float64 CPU-oracle sample runs on the
+host cores.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -39,15 +46,16 @@ def log(*a):
 def parse():
     p = argparse.ArgumentParser(description=__doc__)
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=4,
-                   help="timed LM iterations (from the generator's initial point the first 4 reduce the cost by "
-                        "more than float64 resolves; from the 5th on the predicted reduction is ~1e-17 of the cost "
-                        "and acceptance is rounding noise)")
-    p.add_argument("--warmup", type=int, default=2, help="LM iterations of the untimed warm-up solve")
+    p.add_argument("--steps", type=int, default=20, help="timed LM iterations (in solves of --segment iterations)")
+    p.add_argument("--segment", type=int, default=4,
+                   help="LM iterations per timed solve from the initial point (the productive ones: from the 5th "
+                        "on the predicted reduction is ~1e-17 of the cost and acceptance is rounding noise)")
+    p.add_argument("--warmup", type=int, default=5, help="LM iterations of the untimed warm-up solves")
     p.add_argument("--workload", default="ba_medium", choices=["ba_small", "ba_medium", "ba_large"])
     p.add_argument("--kernel-reps", type=int, default=50)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--no-large", action="store_true", help="skip the 5M-observation normal-matvec measurement")
+    p.add_argument("--matvec-only", action="store_true",
+                   help="only time the 5M-observation normal matvec (for counter profiles) and exit")
     p.add_argument("--repeats", type=int, default=3, help="timed solves per precision (median reported)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a 1 GiB device copy 20 times (HBM counter calibration) and exit")
@@ -69,11 +77,17 @@ def build_problem(prob, device):
     return P
 
 
+# Both precisions run the same solver options.  stop_on_stagnation ends a PCG solve whose
+# restarts stop reducing the true residual (a float32 operator's rounding floor) instead of
+# restarting until max_iterations; float64 solves converge before it can apply.
+CG_OPTIONS = dict(stop_on_stagnation=True)
+
+
 def timed_solve(P, init, iters, mv_dtype):
     from torchls_amd import ConjugateGradientConfig, TerminationConfig
 
     P.parameters.update({k: v.clone() for k, v in init.items()})
-    cg = ConjugateGradientConfig(matvec_dtype=mv_dtype)
+    cg = ConjugateGradientConfig(matvec_dtype=mv_dtype, **CG_OPTIONS)
     term = TerminationConfig(max_iterations=iters, cost_tolerance=0.0, gradient_tolerance=0.0,
                              parameter_tolerance=0.0)
     torch.cuda.synchronize()
@@ -82,6 +96,32 @@ def timed_solve(P, init, iters, mv_dtype):
                    termination=term)
     torch.cuda.synchronize()
     return time.perf_counter() - t0, s
+
+
+def segmented_run(P, init, steps, segment, mv_dtype):
+    """``steps`` LM iterations as solves of ``segment`` iterations from ``init`` (each
+    solve bracketed by device synchronizations, the parameter reset outside the
+    clock): (seconds, per-solve summaries)."""
+    total, sums, left = 0.0, [], steps
+    while left > 0:
+        t, s = timed_solve(P, init, min(segment, left), mv_dtype)
+        total += t
+        sums.append(s)
+        left -= min(segment, left)
+    return total, sums
+
+
+def run_stats(t, sums):
+    from collections import Counter
+
+    it = sum(s.iterations for s in sums)
+    acc = sum(s.accepted_steps for s in sums)
+    rej = sum(s.rejected_steps for s in sums)
+    return {"seconds": round(t, 5), "iterations": it, "accepted": acc, "rejected": rej, "trials": acc + rej,
+            "lm_iters_per_s": round(it / t, 3), "trials_per_s": round((acc + rej) / t, 3),
+            "pcg_iterations": sum(s.linear_iterations for s in sums),
+            "termination": dict(Counter(s.termination_reason for s in sums)),
+            "final_cost": sums[-1].final_cost}
 
 
 def schur_matvec_timing(P, mv_dtype, reps):
@@ -106,49 +146,72 @@ def schur_matvec_timing(P, mv_dtype, reps):
     return start.elapsed_time(stop) / reps / 1e3
 
 
-def normal_matvec_timing(P, mv_dtype, reps):
-    from torchls_amd.hip_normal import HipNormalPlan
-
-    an = P._analyze()
-    lin = an.linearize(P.parameters)
-    op = HipNormalPlan(lin.jacobian, dtype=mv_dtype).bind(lin.jacobian)
-    x = torch.randn(lin.jacobian.shape[1], dtype=mv_dtype, device=lin.residual.device)
-    y = torch.empty_like(x)
-    for _ in range(5):
-        op.normal_matvec_into(x, y, 1e-3)
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record()
-    for _ in range(reps):
-        op.normal_matvec_into(x, y, 1e-3)
-    stop.record()
-    torch.cuda.synchronize()
-    return start.elapsed_time(stop) / reps / 1e3
+def host_cores():
+    """Host threads this process may use (the scheduler affinity, capped by OMP_NUM_THREADS
+    where the environment sets the machine's share)."""
+    n = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    return min(n, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else n
 
 
-def cpu_baseline(prob, iters):
-    """float64 numpy oracle LM (Schur-PCG) on the same workload and initial point, host cores:
-    the same number of LM iterations as the timed GPU solves, so its final cost is the
-    float64 reference for the mixed-precision result."""
+def cpu_baseline(prob, iters, linear_solver="schur_cg"):
+    """float64 numpy oracle LM on the same workload and initial point, BLAS on every host
+    core this process may use: its final cost is the float64 reference for the HIP result."""
     from oracle.ba import BAProblem
     from oracle.lm import Termination, solve
 
     from threadpoolctl import threadpool_limits
 
+    cores = host_cores()
     o = BAProblem(prob.cam_idx, prob.pt_idx, prob.observations, prob.n_cameras, prob.n_points,
                   huber_delta=prob.huber_delta)
-    with threadpool_limits(limits=1):  # one host core: the reported core count
+    with threadpool_limits(limits=cores):
         t0 = time.perf_counter()
-        _, s = solve(o, (prob.cameras_init, prob.points_init), linear_solver="schur_cg", eliminate=1,
+        _, s = solve(o, (prob.cameras_init, prob.points_init), linear_solver=linear_solver,
+                     eliminate=1 if linear_solver.startswith("schur") else None,
                      term=Termination(max_iterations=iters, cost_tolerance=0, gradient_tolerance=0,
                                       parameter_tolerance=0))
         dt = time.perf_counter() - t0
-    return s.iterations / dt, s
+    return s.iterations / dt, s, cores
+
+
+def small_cpu_vs_hip(device, iters=20):
+    """The reference's CPU-runnable configuration: 16 cameras, 500 points, 4,000 residuals,
+    float64, LM with full-system PCG + block Jacobi, 20 iterations -- the float64 CPU oracle
+    (host cores) against the float64 HIP solve (median of 3 after a warm-up solve)."""
+    from torchls_amd import ConjugateGradientConfig, TerminationConfig, generators
+
+    prob = generators.ba_small()
+    P = build_problem(prob, device)
+    init = {k: v.clone() for k, v in P.parameters.items()}
+    term = TerminationConfig(max_iterations=iters, cost_tolerance=0.0, gradient_tolerance=0.0,
+                             parameter_tolerance=0.0)
+    times = []
+    for rep in range(4):
+        P.parameters.update({k: v.clone() for k, v in init.items()})
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, s = P.solve(return_summary=True, termination=term, conjugate_gradient=ConjugateGradientConfig())
+        torch.cuda.synchronize()
+        if rep:
+            times.append(time.perf_counter() - t0)
+    t = sorted(times)[1]
+    rate, ref, cores = cpu_baseline(prob, iters, "conjugate_gradient")
+    return {"workload": "ba_small", "cameras": prob.n_cameras, "points": prob.n_points,
+            "observations": prob.n_obs, "seed": prob.seed, "iterations": s.iterations,
+            "accepted_rejected": [s.accepted_steps, s.rejected_steps], "termination": s.termination_reason,
+            "hip_float64_lm_iters_per_s": round(s.iterations / t, 2),
+            "cpu_oracle_lm_iters_per_s": round(rate, 2), "cpu_cores": cores,
+            "hip_over_cpu": round((s.iterations / t) / rate, 2),
+            "final_cost_hip": s.final_cost, "final_cost_cpu_oracle": ref.final_cost,
+            "final_cost_rel_err": abs(s.final_cost - ref.final_cost) / abs(ref.final_cost),
+            "pcg_iterations_equal": s.linear_iterations_history == ref.linear_iterations_history}
 
 
 def large_normal_matvec(device, reps):
     """Normal matvec (float32 blocks) on the 2,000-camera / 500,000-point / 5M-observation
-    workload: HIP-event time per launch (matvec kernel + finish kernels, launch stream)."""
+    workload: HIP-event time per launch (matvec kernel + finish kernels) on the stream the
+    kernels are launched on (the current stream)."""
     from torchls_amd import generators
     from torchls_amd.hip_normal import HipNormalPlan
     from torchls_amd.workloads import normal_matvec_bytes
@@ -162,16 +225,24 @@ def large_normal_matvec(device, reps):
     y = torch.empty_like(x)
     t = event_time(lambda: op.normal_matvec_into(x, y, 1e-3), reps)
     nbytes = normal_matvec_bytes(prob.n_obs, 2, (6, 3), n, 4)
-    return {"workload": "ba_large", "observations": prob.n_obs, "cameras": prob.n_cameras,
-            "points": prob.n_points, "seed": prob.seed, "us_per_launch": round(t * 1e6, 2),
-            "bytes_per_launch": nbytes, "achieved": round(nbytes / t / 1e9, 1), "unit": "GB/s",
-            "peak": HBM_PEAK_GBS, "frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4)}
+    gbs = nbytes / t / 1e9
+    wave = op.plan.plan.groups[0]
+    traffic = profiled_traffic(MATVEC_TRAFFIC, [("void normal_wave_kernel<float, 2, 6, 3, 1, 2, 512, true, true>", None),
+                                                ("void finish_jobs_kernel<float>", None)])
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": "profiles/normal_matvec_5M_hbm_traffic.md (rocprofv3 FETCH_SIZE, WRITE_SIZE)",
+            "kernel": "normal matvec J^T J x + lambda x, float32 (normal_wave_kernel + finish_jobs_kernel)",
+            "workload": "ba_large", "observations": prob.n_obs, "cameras": prob.n_cameras, "points": prob.n_points,
+            "seed": prob.seed, "us_per_launch": round(t * 1e6, 2), "bytes_per_launch": nbytes,
+            "bytes_per_unit": "80 B per observation (2x6 + 2x3 float32 blocks, two int32 ids) + 8 B per "
+                              "tangent entry (x read, y written)", "workgroups": int(wave.nwg)}
 
 
 def lm_large(device, steps):
     """LM iterations/s of the mixed-precision and float64 HIP Schur-PCG at 5M observations
     (2,000 cameras, 500,000 points, 5 % outliers), where the blocks no longer fit the last-level
-    cache: the same fixed-iteration protocol as the headline line, one warm-up solve each."""
+    cache: ``steps`` productive iterations from the initial point, one warm-up solve each."""
     from torchls_amd import generators
 
     prob = generators.ba_large()
@@ -182,26 +253,24 @@ def lm_large(device, steps):
     res = {}
     for dt in (torch.float32, None):
         timed_solve(P, init, 1, dt)
-        res[dt] = timed_solve(P, init, steps, dt)
-    (t32, s32), (t64, s64) = res[torch.float32], res[None]
-    out.update({"mixed_lm_iters_per_s": round(s32.iterations / t32, 3),
-                "float64_lm_iters_per_s": round(s64.iterations / t64, 3),
-                "mixed_over_float64": round((s32.iterations / t32) / (s64.iterations / t64), 3),
-                "final_cost_mixed": s32.final_cost, "final_cost_float64": s64.final_cost,
-                "final_cost_rel_err": abs(s32.final_cost - s64.final_cost) / abs(s64.final_cost),
-                "pcg_iterations_mixed": s32.linear_iterations, "pcg_iterations_float64": s64.linear_iterations,
-                "accepted_rejected_mixed": [s32.accepted_steps, s32.rejected_steps],
-                "accepted_rejected_float64": [s64.accepted_steps, s64.rejected_steps]})
+        t, s = timed_solve(P, init, steps, dt)
+        res[dt] = run_stats(t, [s])
+    m, f = res[torch.float32], res[None]
+    out.update({"mixed": m, "float64": f, "mixed_over_float64": round(m["lm_iters_per_s"] / f["lm_iters_per_s"], 3),
+                "final_cost_rel_err": abs(m["final_cost"] - f["final_cost"]) / abs(f["final_cost"])})
     return out
 
 
 def pose_graph_lm(device, steps):
-    """LM iterations/s on the 10,000-pose SE(3) pose graph (odometry + 3 % loop closures,
-    6x6 aliased blocks): full-system PCG with block-Jacobi, mixed precision vs float64."""
+    """LM on the 10,000-pose SE(3) pose graph (odometry + 3 % loop closures, 6x6 aliased
+    blocks, started at the true poses perturbed by 0.01 per tangent coordinate): full-system
+    PCG with block Jacobi, mixed precision vs float64.  The first three LM iterations are
+    the ones whose PCG solves meet rtol within 3,000 iterations (the chain's Laplacian has a
+    condition number of order n^2), so the mixed result is compared with float64 there."""
     from torchls_amd import ConjugateGradientConfig, TerminationConfig, generators
     from torchls_amd.workloads import pose_graph_problem
 
-    pg = generators.pose_graph()
+    pg = generators.pose_graph(init_noise=0.01)
     P = pose_graph_problem(pg, device)
     init = {k: v.clone() for k, v in P.parameters.items()}
 
@@ -212,23 +281,26 @@ def pose_graph_lm(device, steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         _, s = P.solve(return_summary=True, linear_solver="conjugate_gradient",
-                       conjugate_gradient=ConjugateGradientConfig(matvec_dtype=dt), termination=term)
+                       conjugate_gradient=ConjugateGradientConfig(matvec_dtype=dt, max_iterations=3000,
+                                                                  **CG_OPTIONS),
+                       termination=term)
         torch.cuda.synchronize()
         return time.perf_counter() - t0, s
 
     res = {}
     for dt in (torch.float32, None):
         run(dt, 1)
-        res[dt] = run(dt, steps)
-    (t32, s32), (t64, s64) = res[torch.float32], res[None]
+        t, s = run(dt, steps)
+        res[dt] = (run_stats(t, [s]), s)
+    (m, s32), (f, s64) = res[torch.float32], res[None]
     return {"workload": "pose_graph", "poses": pg.n_poses, "factors": pg.n_factors, "loop_closures": pg.n_loop,
-            "seed": pg.seed, "steps": steps, "linear_solver": "conjugate_gradient (block-Jacobi)",
-            "mixed_lm_iters_per_s": round(s32.iterations / t32, 3),
-            "float64_lm_iters_per_s": round(s64.iterations / t64, 3),
-            "mixed_over_float64": round((s32.iterations / t32) / (s64.iterations / t64), 3),
-            "initial_cost": s64.initial_cost, "final_cost_mixed": s32.final_cost, "final_cost_float64": s64.final_cost,
+            "seed": pg.seed, "init": "true poses x Exp(N(0, 0.01^2 I))", "steps": steps,
+            "linear_solver": "conjugate_gradient (block-Jacobi, max 3,000 iterations)", "mixed": m, "float64": f,
+            "mixed_over_float64": round(m["lm_iters_per_s"] / f["lm_iters_per_s"], 3),
+            "initial_cost": s64.initial_cost,
             "final_cost_rel_err": abs(s32.final_cost - s64.final_cost) / abs(s64.final_cost),
-            "pcg_iterations_mixed": s32.linear_iterations, "pcg_iterations_float64": s64.linear_iterations}
+            "pcg_history_mixed": s32.linear_iterations_history, "pcg_history_float64": s64.linear_iterations_history,
+            "pcg_converged_float64": all(s64.linear_converged_history)}
 
 
 def implicit_large(device, lm_steps):
@@ -327,15 +399,16 @@ def dense_schur_cholesky(device, reps):
             "us_per_solve": round(t_solve * 1e6, 1)}
 
 
-TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "bench_hbm_traffic.md")
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+MATVEC_TRAFFIC = os.path.join(PROFILES, "normal_matvec_5M_hbm_traffic.md")
 
 
-def profiled_traffic(kernels):
-    """HBM bytes per launch of the given (kernel name prefix, grid) launches, summed, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this benchmark (copy-calibrated);
-    None when the summary or a kernel is missing."""
+def profiled_traffic(summary, kernels):
+    """HBM bytes per launch of the given (kernel name, grid or None) launches, summed, from a
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None when the
+    summary or a kernel is missing."""
     try:
-        lines = open(TRAFFIC_SUMMARY).read().splitlines()
+        lines = open(summary).read().splitlines()
     except OSError:
         return None
     rows = []
@@ -387,8 +460,10 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
               "described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.matvec_only:
+        print(json.dumps(large_normal_matvec(torch.device("cuda:0"), args.kernel_reps)), flush=True)
+        return
     from torchls_amd import generators
-    from torchls_amd.workloads import normal_matvec_bytes
 
     device = torch.device("cuda:0")
     torch.manual_seed(0)
@@ -397,31 +472,28 @@ def main():
     P = build_problem(prob, device)
     init = {k: v.clone() for k, v in P.parameters.items()}
 
-    # warm-up solves: symbolic plans (cached on the problem's analysis and reused
-    # by later solves, like a serving loop), kernel loading, graph capture paths
+    # warm-up solves: symbolic plans (cached on the problem's analysis and reused by
+    # later solves, like a serving loop), kernel loading, graph capture paths
     cold = {}
     for dt in (torch.float32, None):
-        cold[dt] = timed_solve(P, init, max(1, args.warmup), dt)[0]
-    warm = timed_solve(P, init, max(1, args.warmup), torch.float32)[0]
+        cold[dt] = segmented_run(P, init, max(1, args.warmup), args.segment, dt)[0]
+    warm = segmented_run(P, init, max(1, args.warmup), args.segment, torch.float32)[0]
     plan_s = cold[torch.float32] - warm
-    log(f"warm-up done: first mixed solve {cold[torch.float32]:.3f}s, warm {warm:.3f}s "
+    log(f"warm-up done: first mixed run {cold[torch.float32]:.3f}s, warm {warm:.3f}s "
         f"(symbolic analysis and first-call cost {plan_s:.3f}s)")
     runs = {torch.float32: [], None: []}
     for _ in range(max(1, args.repeats)):
         for dt in (torch.float32, None):
-            runs[dt].append(timed_solve(P, init, args.steps, dt))
-    t_mixed, s_mixed = sorted(runs[torch.float32], key=lambda r: r[0])[len(runs[torch.float32]) // 2]
-    t_f64, s_f64 = sorted(runs[None], key=lambda r: r[0])[len(runs[None]) // 2]
-    log("timed solves (s): mixed " + " ".join(f"{r[0]:.4f}" for r in runs[torch.float32]) + "; float64 " +
+            runs[dt].append(segmented_run(P, init, args.steps, args.segment, dt))
+    med = {dt: sorted(r, key=lambda x: x[0])[len(r) // 2] for dt, r in runs.items()}
+    mixed, f64 = run_stats(*med[torch.float32]), run_stats(*med[None])
+    log("timed runs (s): mixed " + " ".join(f"{r[0]:.4f}" for r in runs[torch.float32]) + "; float64 " +
         " ".join(f"{r[0]:.4f}" for r in runs[None]))
-    log(f"mixed: {args.steps} LM iterations in {t_mixed:.4f}s (median of {len(runs[None])}) "
-        f"final_cost={s_mixed.final_cost:.12e}")
-    log(f"float64: {args.steps} LM iterations in {t_f64:.4f}s final_cost={s_f64.final_cost:.12e}")
-    rel = abs(s_mixed.final_cost - s_f64.final_cost) / abs(s_f64.final_cost)
-    iters_mixed = s_mixed.iterations
-    iters_f64 = s_f64.iterations
-    lm_mixed = iters_mixed / t_mixed
-    lm_f64 = iters_f64 / t_f64
+    for name, st in (("mixed", mixed), ("float64", f64)):
+        log(f"{name}: {st['iterations']} LM iterations ({st['accepted']} accepted, {st['rejected']} rejected, "
+            f"{st['termination']}) in {st['seconds']:.4f}s: {st['lm_iters_per_s']} it/s, "
+            f"{st['trials_per_s']} trials/s, final_cost={st['final_cost']:.12e}")
+    rel = abs(mixed["final_cost"] - f64["final_cost"]) / abs(f64["final_cost"])
 
     P.parameters.update({k: v.clone() for k, v in init.items()})
     t_schur = schur_matvec_timing(P, torch.float32, args.kernel_reps)
@@ -429,34 +501,25 @@ def main():
     # implicit Schur matvec algorithmic bytes: blocks + two int32 ids per observation,
     # C^-1 (float64 upper 3x3) per point, camera vector in/out
     schur_bytes = N * (2 * 9 * 4 + 8) + n_p * 6 * 8 + 2 * 6 * n_c * 4
-    # the camera finish of the Schur matvec: 8 columns per 256-thread workgroup
-    schur_traffic = profiled_traffic([("void schur_tile_kernel<float, 2, 6, 3, true, 1, 0, 1024, true>", None),
-                                      ("void finish_jobs_kernel<float>", -(-n_c * 6 // 8) * 256)])
     schur_gbs = schur_bytes / t_schur / 1e9
-    t_normal = normal_matvec_timing(P, torch.float32, args.kernel_reps)
-    n = 6 * n_c + 3 * n_p
-    normal_gbs = normal_matvec_bytes(N, 2, (6, 3), n, 4) / t_normal / 1e9
-    log(f"schur matvec {t_schur * 1e6:.2f} us ({schur_gbs:.1f} GB/s); normal matvec {t_normal * 1e6:.2f} us "
-        f"({normal_gbs:.1f} GB/s)")
+    log(f"schur matvec {t_schur * 1e6:.2f} us ({schur_gbs:.1f} GB/s effective, MALL-resident)")
 
-    large = None
-    if not args.no_large:
-        large = large_normal_matvec(device, args.kernel_reps)
-        log(f"normal matvec at 5M observations: {large['us_per_launch']} us ({large['achieved']} GB/s, "
-            f"{100 * large['frac']:.1f}% of {HBM_PEAK_GBS:.0f} GB/s)")
+    roofline = large_normal_matvec(device, args.kernel_reps)
+    log(f"normal matvec at 5M observations: {roofline['us_per_launch']} us ({roofline['achieved']} GB/s, "
+        f"{100 * roofline['frac']:.1f}% of {HBM_PEAK_GBS:.0f} GB/s), traffic {roofline['traffic']}")
 
     lm5m = None
-    if not args.no_large and not args.quick:
+    if not args.quick:
         lm5m = lm_large(device, 3)
-        log(f"LM at 5M observations: mixed {lm5m['mixed_lm_iters_per_s']} it/s, float64 "
-            f"{lm5m['float64_lm_iters_per_s']} it/s (x{lm5m['mixed_over_float64']}), final-cost rel err "
+        log(f"LM at 5M observations: mixed {lm5m['mixed']['lm_iters_per_s']} it/s, float64 "
+            f"{lm5m['float64']['lm_iters_per_s']} it/s (x{lm5m['mixed_over_float64']}), final-cost rel err "
             f"{lm5m['final_cost_rel_err']:.2e}")
 
-    pgraph = None if args.quick else pose_graph_lm(device, 5)
+    pgraph = None if args.quick else pose_graph_lm(device, 3)
     if pgraph:
-        log(f"pose graph ({pgraph['poses']} poses, {pgraph['factors']} factors): mixed {pgraph['mixed_lm_iters_per_s']} "
-            f"it/s, float64 {pgraph['float64_lm_iters_per_s']} it/s, final-cost rel err "
-            f"{pgraph['final_cost_rel_err']:.2e}")
+        log(f"pose graph ({pgraph['poses']} poses, {pgraph['factors']} factors): mixed "
+            f"{pgraph['mixed']['lm_iters_per_s']} it/s, float64 {pgraph['float64']['lm_iters_per_s']} it/s, "
+            f"final-cost rel err {pgraph['final_cost_rel_err']:.2e}")
 
     chol = dense_schur_cholesky(device, 20)
     log(f"dense Schur Cholesky n={chol['n']}: {chol['us_per_factorisation']} us, {chol['tflops']} TFLOP/s")
@@ -466,40 +529,43 @@ def main():
         log(f"implicit gradient at {implicit['observations']} observations: forward {implicit['forward_s']} s, "
             f"backward (adjoint) {implicit['backward_s']} s, |grad| {implicit['grad_norm']:.3e}")
 
+    small = None
     cpu = None
     oracle_rel = None
     if not args.no_cpu:
-        rate, ref = cpu_baseline(prob, args.steps)
-        oracle_rel = abs(s_mixed.final_cost - ref.final_cost) / abs(ref.final_cost)
-        log(f"float64 CPU oracle: {ref.iterations} LM iterations at {rate:.4f} it/s, final_cost="
+        small = small_cpu_vs_hip(device)
+        log(f"16 cameras / 4,000 residuals, 20 LM iterations: HIP float64 {small['hip_float64_lm_iters_per_s']} it/s, "
+            f"CPU oracle {small['cpu_oracle_lm_iters_per_s']} it/s on {small['cpu_cores']} cores, final-cost rel err "
+            f"{small['final_cost_rel_err']:.2e}")
+        rate, ref, cores = cpu_baseline(prob, args.segment)
+        oracle_rel = abs(mixed["final_cost"] - ref.final_cost) / abs(ref.final_cost)
+        log(f"float64 CPU oracle: {ref.iterations} LM iterations at {rate:.4f} it/s on {cores} cores, final_cost="
             f"{ref.final_cost:.12e}; mixed HIP rel err {oracle_rel:.2e}")
-        cpu = {"value": round(rate, 4), "unit": "LM iterations/s", "cores": 1, "kind": "port",
-               "sample": f"{ref.iterations} LM iterations (Schur-PCG, float64 numpy oracle, BLAS limited to one "
-                         "thread) of the same 500,000-observation workload from the same initial point"}
+        cpu = {"value": round(rate, 4), "unit": "LM iterations/s", "cores": cores, "kind": "port",
+               "sample": f"{ref.iterations} LM iterations (one solve of the timed protocol: Schur-PCG from the same "
+                         f"initial point) of the same 500,000-observation workload by the float64 numpy oracle, "
+                         f"BLAS allowed {cores} threads (its einsum/scatter work runs on one)"}
     print(json.dumps({
         "metric": "LM iterations/s (mixed-precision HIP Schur-PCG), with matvec HBM GB/s",
-        "value": round(lm_mixed, 3), "unit": "LM iterations/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(1e3 * t_mixed / max(1, iters_mixed), 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 blocks/matvec, f64 scalars",
+        "value": mixed["lm_iters_per_s"], "unit": "LM iterations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * mixed["seconds"] / max(1, mixed["iterations"]), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 blocks/matvec/PCG vectors, f64 scalars, gradient and objective",
         "data": "synthetic (seeded generator)",
         "config": {"workload": args.workload, "cameras": n_c, "points": n_p, "observations": N, "seed": prob.seed,
-                   "linear_solver": "schur_cg", "huber_delta": prob.huber_delta},
-        "float64_lm_iters_per_s": round(lm_f64, 3), "mixed_over_float64": round(lm_mixed / lm_f64, 3),
-        "final_cost_mixed": s_mixed.final_cost, "final_cost_float64": s_f64.final_cost,
+                   "linear_solver": "schur_cg", "huber_delta": prob.huber_delta,
+                   "protocol": f"{args.steps} LM iterations as solves of {args.segment} from the initial point",
+                   "cg_options": CG_OPTIONS},
+        "mixed": mixed, "float64": f64,
+        "mixed_over_float64": round(mixed["lm_iters_per_s"] / f64["lm_iters_per_s"], 3),
+        "mixed_over_float64_per_trial": round(mixed["trials_per_s"] / f64["trials_per_s"], 3),
         "final_cost_rel_err": rel, "final_cost_rel_err_vs_cpu_oracle": oracle_rel,
-        "accepted_rejected_mixed": [s_mixed.accepted_steps, s_mixed.rejected_steps],
-        "accepted_rejected_float64": [s_f64.accepted_steps, s_f64.rejected_steps],
-        "pcg_iterations_mixed": s_mixed.linear_iterations,
-        "pcg_iterations_float64": s_f64.linear_iterations,
-        "normal_matvec_gbs": round(normal_gbs, 1), "normal_matvec_us": round(t_normal * 1e6, 2),
+        "schur_matvec_medium": {"us_per_launch": round(t_schur * 1e6, 2), "bytes_per_launch": schur_bytes,
+                                "effective_gbs": round(schur_gbs, 1),
+                                "note": "blocks fit the 256 MB last-level cache: effective, not HBM, bandwidth"},
         "first_solve_extra_s": round(plan_s, 4),
-        "roofline": {"bound": "hbm", "achieved": round(schur_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(schur_gbs / HBM_PEAK_GBS, 4), "traffic": schur_traffic,
-                     "traffic_source": "profiles/bench_hbm_traffic.md (rocprofv3 FETCH_SIZE, WRITE_SIZE)",
-                     "kernel": "implicit Schur matvec (schur_tile_kernel + finish)",
-                     "us_per_launch": round(t_schur * 1e6, 2), "bytes_per_launch": schur_bytes},
-        "normal_matvec_large": large, "lm_large": lm5m, "pose_graph": pgraph,
-        "dense_schur_cholesky": chol, "implicit_gradient": implicit,
+        "roofline": roofline, "lm_large": lm5m, "pose_graph": pgraph,
+        "dense_schur_cholesky": chol, "implicit_gradient": implicit, "small_cpu_vs_hip": small,
         "cpu_baseline": cpu,
     }), flush=True)
 

@@ -520,6 +520,27 @@ int tlh_reprojection_obs_vjp(const double* cams, const double* pts, const double
                              int64_t cam_offset, int64_t pt_offset, double huber_delta, int64_t count,
                              double* grad, tlh_stream_t stream);
 
+/* Per-observation exact Hessian-vector products of f = 1/2 sum_i w_i rho(||r_i||^2) for the
+ * reprojection residual at delta = 0 of the solver's retraction (poses T Exp(delta_c),
+ * points p + delta_p): out[i][0..5] (camera tangent) and out[i][6..8] (point tangent) =
+ * d/de grad f_i(e v) for the tangent vector v (float64, cam_offset / pt_offset the first
+ * column of each type), i.e. J_i^T (rho' I + 2 rho'' r r^T) J_i v plus the second-order
+ * residual and retraction terms rho' sum_k r_k d2 r_k[v, .].  Evaluated in hyper-dual
+ * arithmetic from cams [n_c,4,4], pts [n_p,3], obs [N,2] (float64); weights may be NULL;
+ * huber_delta <= 0 means no loss.  Sum the rows per variable with tlh_segment_sum_f64.
+ * No counterpart in the reference (its solve is not differentiated: solvers.py:288);
+ * it replaces the torch.func Hessian-vector products of an exact-Hessian adjoint solve. */
+int tlh_reprojection_hvp(const double* cams, const double* pts, const double* obs, const int64_t* cam_idx,
+                         const int64_t* pt_idx, const double* v, int64_t cam_offset, int64_t pt_offset,
+                         double huber_delta, const double* weights, int64_t count, double* out, tlh_stream_t stream);
+
+/* out[var * dim + c] = sum over k in [ptr[var], ptr[var+1]) of values[order[k] * stride + col0 + c]
+ * (float64, fixed summation order: one wavefront per variable, lane-strided sums and a
+ * fixed xor tree).  order: int32 rows grouped by variable; ptr: int32 [count + 1].
+ * Reduces per-observation rows (e.g. of tlh_reprojection_hvp) into per-variable vectors. */
+int tlh_segment_sum_f64(const double* values, int64_t stride, int col0, int dim, const int32_t* order,
+                        const int32_t* ptr, int64_t count, double* out, tlh_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -269,6 +269,15 @@ SIGNATURES = {
          c_void_p, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     ),
     "tlh_se3_retract": (ctypes.c_int, [ctypes.c_int, c_void_p, ctypes.c_int, i64, c_void_p, i64, c_void_p, c_void_p]),
+    "tlh_reprojection_hvp": (
+        ctypes.c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, i64, i64, ctypes.c_double, c_void_p, i64,
+         c_void_p, c_void_p],
+    ),
+    "tlh_segment_sum_f64": (
+        ctypes.c_int,
+        [c_void_p, i64, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, i64, c_void_p, c_void_p],
+    ),
     "tlh_reprojection_obs_vjp": (
         ctypes.c_int,
         [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, i64, i64, ctypes.c_double, i64, c_void_p,

@@ -11,10 +11,13 @@ Backward (no reference counterpart; the reference solve is
 2. one adjoint PCG solve ``(J~^T J~ + lambda I) w = dL/dtheta`` on the solver's
    own HIP operator at the optimum (HIP normal matvec, block Jacobi, HIP PCG);
    ``refine > 0`` continues with up to ``refine`` conjugate-gradient steps on
-   the exact Hessian (``torch.func`` Hessian-vector products of the float64
-   objective), preconditioned by that Gauss-Newton solve, which removes the
-   Gauss-Newton approximation error (large for Huber outliers, whose exact
-   curvature along their residual vanishes);
+   the exact Hessian, preconditioned by that Gauss-Newton solve, which removes
+   the Gauss-Newton approximation error (large for Huber outliers, whose exact
+   curvature along their residual vanishes).  The exact Hessian-vector product
+   of the reprojection batches runs in the HIP kernel ``tlh_reprojection_hvp``
+   (hyper-dual numbers through the retraction and the projection, summed per
+   variable by ``tlh_segment_sum_f64``); other residual batches (priors and
+   user residuals) use ``torch.func``;
 3. the HIP backward kernel ``tlh_reprojection_obs_vjp`` recomputes each
    observation's Jacobian blocks and applies the residual-vs-observation
    Jacobian transpose: ``dL/dobs_i = (rho' I + 2 rho'' r r^T)(J_i w)``.
@@ -63,12 +66,15 @@ def _tangent_gradient(analysis, values, grads):
     return torch.cat(parts)
 
 
-def _objective(analysis, values, delta):
-    """float64 0.5 * sum rho(||r||^2) at retract(values, delta) (differentiable)."""
+def _objective(analysis, values, delta, batches=None):
+    """float64 0.5 * sum rho(||r||^2) at retract(values, delta) (differentiable), over
+    ``batches`` (indices; default all)."""
     moved = analysis.retract(values, delta)
     first = next(iter(moved.values()))
     total = torch.zeros((), dtype=torch.float64, device=first.device)
-    for b in analysis.batches:
+    for k, b in enumerate(analysis.batches):
+        if batches is not None and k not in batches:
+            continue
         r = b.residuals(moved, analysis.layout, first.device, first.dtype).to(torch.float64)
         s = r.square().sum(-1)
         v = s if b.loss is None else b.loss.value(s)
@@ -77,6 +83,107 @@ def _objective(analysis, values, delta):
             v = v * w
         total = total + 0.5 * v.sum()
     return total
+
+
+def _csr(index: torch.Tensor, count: int):
+    """(order, ptr) int32 grouping of rows by variable index (stable)."""
+    order = torch.argsort(index, stable=True)
+    ptr = torch.zeros(count + 1, dtype=torch.int64, device=index.device)
+    ptr[1:] = torch.bincount(index, minlength=count).cumsum(0)
+    return order.to(torch.int32).contiguous(), ptr.to(torch.int32).contiguous()
+
+
+class ExactHessian:
+    """v -> (H + damping I) v, H the exact Hessian of the float64 objective
+    0.5 sum w rho(||r||^2) composed with the retraction, at ``values`` (delta = 0).
+
+    Reprojection batches with float64 4x4 poses run in HIP (``tlh_reprojection_hvp``
+    per observation, ``tlh_segment_sum_f64`` per camera and point, fixed order);
+    every other batch through ``torch.func`` (forward-over-reverse)."""
+
+    def __init__(self, analysis, values, damping: float = 0.0):
+        self.analysis, self.values, self.damping = analysis, values, float(damping)
+        first = next(iter(values.values()))
+        self.device = first.device
+        self.lib = _lib.load()
+        self.hip, others = [], []
+        for k, b in enumerate(analysis.batches):
+            ops = b.hip_operands(values, analysis.layout, first.device, first.dtype)
+            if ops is None or first.dtype != torch.float64 or ops[0].shape[1:] != (4, 4):
+                others.append(k)
+                continue
+            cams, pts, obs, delta = ops
+            idx, _, w = b.device_state(first.device, analysis.layout)
+            tc, tp = b.type_indices
+            ec, ep = analysis.layout[tc], analysis.layout[tp]
+            self.hip.append(dict(cams=cams, pts=pts, obs=obs, delta=delta, ci=idx[0].contiguous(),
+                                 pi=idx[1].contiguous(), w=None if w is None else w.contiguous(), count=b.count,
+                                 cam=(ec.offset, ec.count) + _csr(idx[0], ec.count),
+                                 pt=(ep.offset, ep.count) + _csr(idx[1], ep.count)))
+        self.others = others
+
+    def __call__(self, v: torch.Tensor) -> torch.Tensor:
+        v = v.to(torch.float64).contiguous()
+        out = self.damping * v
+        stream = _lib.stream_handle(self.device)
+        for h in self.hip:
+            rows = torch.empty((h["count"], 9), dtype=torch.float64, device=self.device)
+            _lib.check(self.lib.tlh_reprojection_hvp(
+                h["cams"].data_ptr(), h["pts"].data_ptr(), h["obs"].data_ptr(), h["ci"].data_ptr(),
+                h["pi"].data_ptr(), v.data_ptr(), h["cam"][0], h["pt"][0], h["delta"],
+                None if h["w"] is None else h["w"].data_ptr(), h["count"], rows.data_ptr(), stream),
+                "tlh_reprojection_hvp")
+            for (off, count, order, ptr), col0, dim in ((h["cam"], 0, 6), (h["pt"], 6, 3)):
+                part = torch.empty(count * dim, dtype=torch.float64, device=self.device)
+                _lib.check(self.lib.tlh_segment_sum_f64(rows.data_ptr(), 9, col0, dim, order.data_ptr(),
+                                                        ptr.data_ptr(), count, part.data_ptr(), stream),
+                           "tlh_segment_sum_f64")
+                out[off: off + count * dim] += part
+        if self.others:
+            zero = torch.zeros_like(v)
+            batches = set(self.others)
+            with torch.enable_grad():
+                hv = torch.func.jvp(torch.func.grad(lambda d: _objective(self.analysis, self.values, d, batches)),
+                                    (zero,), (v,))[1]
+            out = out + hv
+        return out
+
+
+def _gn_solver(analysis, values, damping, config):
+    """v -> (J~^T J~ + damping I)^-1 v by HIP PCG (float64 normal matvec, block Jacobi) at values.
+    The symbolic plans are cached on the analysis, like the solver's."""
+    lin = analysis.linearize(values)
+    J = lin.jacobian
+    key = ("implicit", str(J.groups[0].blocks[0].device))
+    if key not in analysis.plans:
+        analysis.plans[key] = (HipNormalPlan(J, dtype=torch.float64), HipDiagonalPlan(J))
+    normal_plan, diagonal_plan = analysis.plans[key]
+    op = normal_plan.bind(J)
+    diag = diagonal_plan.diagonal_blocks(J)
+    pre = HipBlockJacobi(J.layout, tuple(d.double() for d in diag), damping)
+    matvec = lambda v: op.normal_matvec(v, damping)  # noqa: E731
+    return lin, (lambda v: hip_pcg(matvec, v, preconditioner=pre, config=config).solution)
+
+
+def newton_polish(problem, *, steps: int = 3, iterations: int = 50, rtol: float = 1e-13,
+                  config: ConjugateGradientConfig | None = None):
+    """Refine the problem's parameters (near an optimum) by Newton steps on the exact
+    float64 Hessian: each step solves H delta = -g to relative residual ``rtol`` by
+    conjugate gradients with ``ExactHessian`` products, preconditioned by the
+    Gauss-Newton HIP PCG solve, then retracts.  Returns (and stores) the refined
+    parameters."""
+    analysis = problem._analyze()
+    cfg = config or ConjugateGradientConfig(rtol=rtol, atol=0.0, max_iterations=2000, residual_check_interval=25)
+    vals = {k: v.detach() for k, v in problem.parameters.items()}
+    with torch.no_grad():
+        for _ in range(steps):
+            lin, gn = _gn_solver(analysis, vals, 0.0, cfg)
+            g = lin.jacobian.rmatvec(lin.residual).to(torch.float64)
+            H = ExactHessian(analysis, vals)
+            delta = _exact_adjoint(H, gn, -g, gn(-g), iterations, rtol=rtol)
+            vals = analysis.retract(vals, delta, kernel=True)
+    problem.parameters.update(vals)
+    return dict(vals)
 
 
 def _exact_adjoint(hvp, precondition, b, w, iterations, rtol=1e-12):
@@ -107,6 +214,8 @@ class _ImplicitSolve(torch.autograd.Function):
         problem, options = state["problem"], state["options"]
         with torch.no_grad():
             sol, summary = problem.solve(return_summary=True, **options)
+            if state["polish"] > 0:
+                sol = newton_polish(problem, steps=state["polish"])
         state["summary"] = summary
         state["solution"] = {k: v.detach().clone() for k, v in sol.items()}
         ctx.state = state
@@ -122,25 +231,11 @@ class _ImplicitSolve(torch.autograd.Function):
         values = st["solution"]
         grad_map = {k: g for k, g in zip(st["names"], grads) if g is not None}
         b = _tangent_gradient(analysis, values, grad_map)
-        lin = analysis.linearize(values)
-        J = lin.jacobian
         lam = st["adjoint_damping"]
-        op = HipNormalPlan(J, dtype=torch.float64).bind(J)
-        diag = HipDiagonalPlan(J).diagonal_blocks(J)
-        pre = HipBlockJacobi(J.layout, tuple(d.double() for d in diag), lam)
-        cfg = st["adjoint_config"]
-        matvec = lambda v: op.normal_matvec(v, lam)  # noqa: E731
-        gn_solve = lambda v: hip_pcg(matvec, v, preconditioner=pre, config=cfg).solution  # noqa: E731
+        _, gn_solve = _gn_solver(analysis, values, lam, st["adjoint_config"])
         w = gn_solve(b)
         if st["refine"] > 0:
-            zero = torch.zeros_like(w)
-
-            def hvp(v):
-                with torch.enable_grad():
-                    hv = torch.func.jvp(torch.func.grad(lambda d: _objective(analysis, values, d)), (zero,), (v,))[1]
-                return hv + lam * v
-
-            w = _exact_adjoint(hvp, gn_solve, b, w, st["refine"])
+            w = _exact_adjoint(ExactHessian(analysis, values, lam), gn_solve, b, w, st["refine"])
         k, pos, opts = st["batch"], st["arg"], st["opts"]
         batch = analysis.batches[k]
         cams = values[analysis.layout[batch.type_indices[0]].name].to(torch.float64).contiguous()
@@ -166,19 +261,21 @@ class _ImplicitSolve(torch.autograd.Function):
 
 
 def solve_implicit(problem, observations: torch.Tensor, *, adjoint_damping: float = 0.0, refine: int = 0,
-                   adjoint_config: ConjugateGradientConfig | None = None, **options) -> dict:
+                   polish: int = 0, adjoint_config: ConjugateGradientConfig | None = None, **options) -> dict:
     """Solve ``problem`` and return {name: tensor} differentiable w.r.t. ``observations``.
 
     ``observations`` must be the tensor given to ``add_residual`` for a
     reprojection batch.  The returned tensors carry a grad_fn whose backward
     runs the adjoint solve described in the module docstring.  The problem
     should be gauge-fixed (e.g. priors on one camera and one point) so that
-    the optimum depends smoothly on the observations.
+    the optimum depends smoothly on the observations.  ``polish`` Newton steps on the
+    exact Hessian (``newton_polish``) follow the LM solve, so that the solution and
+    the adjoint are evaluated at the optimum to float64 rounding.
     """
     k, pos, opts = _find_batch(problem, observations)
     names = list(problem.parameters)
     state = dict(problem=problem, options=options, batch=k, arg=pos, opts=opts, observations=observations,
-                 adjoint_damping=float(adjoint_damping), refine=int(refine),
+                 adjoint_damping=float(adjoint_damping), refine=int(refine), polish=int(polish),
                  adjoint_config=adjoint_config or ConjugateGradientConfig(rtol=1e-12, atol=0.0, max_iterations=2000,
                                                                          residual_check_interval=25))
     outs = _ImplicitSolve.apply(observations, state)

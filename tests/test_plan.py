@@ -3,8 +3,15 @@
 import numpy as np
 import pytest
 
+import torch
+
 from oracle.sparse import random_jacobian
-from torchls_amd.plan import REDUCE_DENSE, build_structure, emulate_normal_matvec, pack_numpy
+from tests.plan_emulation import emulate_normal_matvec, pack_numpy
+from torchls_amd.plan import REDUCE_DENSE, build_structure
+
+
+def _t(a):
+    return torch.as_tensor(np.asarray(a, dtype=np.int64))
 
 SHAPES = [
     ([(7, 6), (300, 3), (5, 1)], [(1500, 2, [0, 1]), (40, 3, [1]), (200, 6, [0, 0]), (50, 2, [2, 1, 2])]),
@@ -20,16 +27,15 @@ def test_plan_emulation_matches_oracle(shape, cap, piece, lds):
     layout, groups = SHAPES[shape]
     J = random_jacobian(rng, layout, groups)
     plan = build_structure(
-        J.layout, [(g.rows, g.types, g.indices) for g in J.groups],
+        J.layout, [(g.rows, g.types, [_t(i) for i in g.indices]) for g in J.groups],
         esize=8, tile_cap=cap, piece=piece, lds_limit=lds,
     )
     for g in plan.groups:
-        assert g.tile_start[-1] == g.count
-        assert np.all(np.diff(g.tile_start) <= cap)
+        assert int(g.tile_start[-1]) == g.count
+        assert bool((torch.diff(g.tile_start) <= cap).all())
         for s in g.slots:
             if s.mode != REDUCE_DENSE:
-                lens = np.diff(np.append(s.seg_start, 0))  # loose: piece bound checked below
-                assert s.seg_start.size == s.seg_target.size
+                assert s.seg_start.numel() == s.seg_target.numel()
     x = rng.standard_normal(J.n)
     y = emulate_normal_matvec(plan, pack_numpy(plan, [g.blocks for g in J.groups]), x, 0.25)
     dense = J.to_dense()
@@ -40,12 +46,13 @@ def test_plan_emulation_matches_oracle(shape, cap, piece, lds):
 def test_primary_is_largest_type_and_tiles_align():
     rng = np.random.default_rng(3)
     J = random_jacobian(rng, [(10, 6), (400, 3)], [(3000, 2, [0, 1])])
-    plan = build_structure(J.layout, [(g.rows, g.types, g.indices) for g in J.groups], esize=4, tile_cap=256)
+    plan = build_structure(J.layout, [(g.rows, g.types, [_t(i) for i in g.indices]) for g in J.groups], esize=4,
+                           tile_cap=256)
     g = plan.groups[0]
     assert g.primary == 1 and g.slots[1].rank is None and g.slots[0].rank is not None
-    prim = g.slots[1].index
+    prim = g.slots[1].index.numpy()
     # every tile boundary falls between two different points (runs fit in a tile)
-    for b in g.tile_start[1:-1]:
+    for b in g.tile_start[1:-1].tolist():
         assert prim[b] != prim[b - 1]
     assert g.slots[0].mode == REDUCE_DENSE
 
@@ -57,12 +64,13 @@ def test_major_order_and_edges_match_brute_force():
     n, nc, npt = 700, 6, 40
     ic = rng.integers(0, nc - 1, n)  # camera nc - 1 is never observed
     ip = rng.integers(0, npt, n)
-    order, ptr = major_order(ic, nc)
-    assert order.dtype == np.int32 and ptr.dtype == np.int32
+    order, ptr = major_order(_t(ic), nc)
+    assert order.dtype == torch.int32 and ptr.dtype == torch.int32
+    order, ptr = order.numpy(), ptr.numpy()
     for v in range(nc):
         members = order[ptr[v]:ptr[v + 1]]
         assert np.array_equal(members, np.flatnonzero(ic == v))  # stable: increasing positions
-    order, edge_ptr, var_edge_ptr, edge_point = major_edges(ic, ip, nc)
+    order, edge_ptr, var_edge_ptr, edge_point = (a.numpy() for a in major_edges(_t(ic), _t(ip), nc))
     assert sorted(order.tolist()) == list(range(n))
     assert var_edge_ptr[-1] == edge_point.size == edge_ptr.size - 1 and edge_ptr[-1] == n
     assert var_edge_ptr[nc - 1] == var_edge_ptr[nc]

@@ -16,7 +16,6 @@ from __future__ import annotations
 
 import ctypes
 
-import numpy as np
 import torch
 
 from . import _lib
@@ -72,15 +71,15 @@ class HipDiagonalPlan:
                 count = jacobian.layout[t].count
                 if groups_of[t] != [gi] or not major_eligible(count, n):
                     continue
-                order, ptr = major_order(g.variable_indices[s].detach().reshape(-1).cpu().numpy(), count)
-                out[t] = (gi, s, torch.from_numpy(order).to(self.device), torch.from_numpy(ptr).to(self.device))
+                order, ptr = major_order(g.variable_indices[s].detach().reshape(-1).to(self.device), count)
+                out[t] = (gi, s, order, ptr)
         return out
 
     def compatible_with(self, normal_plan) -> bool:
         """True when a HipNormalPlan packs observations in this plan's order and tiles."""
         a, b = self.plan.groups, normal_plan.plan.groups
         return len(a) == len(b) and all(
-            x.count == y.count and np.array_equal(x.order, y.order) and np.array_equal(x.tile_start, y.tile_start)
+            x.count == y.count and torch.equal(x.order, y.order) and torch.equal(x.tile_start, y.tile_start)
             for x, y in zip(a, b)
         )
 

@@ -15,19 +15,27 @@ caller keeping float64 outer quantities.
 from __future__ import annotations
 
 import ctypes
-import hashlib
 
-import numpy as np
 import torch
 
 from . import _lib
 from .plan import REDUCE_DENSE, StructurePlan, build_structure
 
 
-def _np_to_device(a: np.ndarray, device) -> torch.Tensor:
-    if a.dtype == np.uint16:
-        a = a.view(np.int16)
-    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+def _to_device(a: torch.Tensor, device) -> torch.Tensor:
+    return a.to(device).contiguous()
+
+
+def order_key(order: torch.Tensor) -> tuple:
+    """Identity of a sorted order, computed on the device: its length and two wrapped
+    int64 position-weighted sums (one host read)."""
+    n = order.numel()
+    if n == 0:
+        return (0, 0, 0)
+    o = order.to(torch.int64)
+    w = torch.arange(n, dtype=torch.int64, device=o.device)
+    s = torch.stack(((o * (2 * w + 1)).sum(), ((o + 1) * (o + 1) * (w + 7)).sum()))
+    return (n, *s.tolist())
 
 
 def _ptr(t: torch.Tensor | None) -> int | None:
@@ -72,9 +80,8 @@ class DeviceStructure:
         self.device = device
         self.keep = []
         # identity of each group's sorted order: plans with equal keys share packed blocks
-        self.order_keys = [hashlib.blake2b(np.ascontiguousarray(g.order).tobytes(), digest_size=16).hexdigest()
-                           for g in plan.groups]
-        put = lambda a: self._keep(_np_to_device(a, device))  # noqa: E731
+        self.order_keys = [order_key(g.order) for g in plan.groups]
+        put = lambda a: self._keep(_to_device(a, device))  # noqa: E731
         self.groups = (_lib.Group * max(1, len(plan.groups)))()
         for gi, g in enumerate(plan.groups):
             d = self.groups[gi]
@@ -97,18 +104,19 @@ class DeviceStructure:
                 ds.dim = s.dim
                 ds.mode = s.mode
                 ds.dense_block = s.dense_block
-                index, rank = s.index, s.rank
+                # ranks are below the tile size (<= 2,048): stored as 16-bit values
+                index, rank = s.index, None if s.rank is None else s.rank.to(torch.int16)
                 if g.kernel == 1:  # wave groups read ids and ranks of whole tiles: pad to ntiles * tile_cap
                     pad = g.ntiles * g.tile_cap - g.count
-                    index = np.concatenate((index, np.full(pad, index[-1], dtype=index.dtype)))
-                    rank = None if rank is None else np.concatenate((rank, np.zeros(pad, dtype=rank.dtype)))
+                    index = torch.cat((index, index[-1:].expand(pad)))
+                    rank = None if rank is None else torch.cat((rank, rank.new_zeros(pad)))
                 ds.index = put(index).data_ptr()
                 ds.rank = None if rank is None else put(rank).data_ptr()
                 ds.seg_ptr = put(s.seg_ptr).data_ptr()
                 ds.seg_start = put(s.seg_start).data_ptr()
                 ds.seg_target = put(s.seg_target).data_ptr()
                 ds.pos_base = s.pos_base
-                ds.rounds = None if s.rounds is None else put(s.rounds.astype(np.int32)).data_ptr()
+                ds.rounds = None if s.rounds is None else put(s.rounds.to(torch.int32)).data_ptr()
         self.types = (_lib.Type * len(plan.types))()
         for ti, t in enumerate(plan.types):
             d = self.types[ti]
@@ -118,13 +126,13 @@ class DeviceStructure:
             d.offset = t.offset
             d.sparse_partials = t.sparse_partials
             d.csr_ptr = put(t.csr_ptr).data_ptr()
-            d.csr_idx = put(t.csr_idx if t.csr_idx.size else np.zeros(1, np.int32)).data_ptr()
+            d.csr_idx = put(t.csr_idx if t.csr_idx.numel() else torch.zeros(1, dtype=torch.int32)).data_ptr()
             d.partials = None
             d.direct = None if t.direct is None else put(t.direct).data_ptr()
             d.finish_list = None if t.finish_list is None else put(
-                t.finish_list if t.finish_list.size else np.zeros(1, np.int32)).data_ptr()
-            d.finish_count = t.count if t.finish_list is None else int(t.finish_list.size)
-            d.csr_entries = int(t.csr_idx.size)
+                t.finish_list if t.finish_list.numel() else torch.zeros(1, dtype=torch.int32)).data_ptr()
+            d.finish_count = t.count if t.finish_list is None else int(t.finish_list.numel())
+            d.csr_entries = int(t.csr_idx.numel())
         self.sorted_index = None
 
     def _keep(self, t: torch.Tensor) -> torch.Tensor:
@@ -168,7 +176,7 @@ def plan_for(jacobian, dtype: torch.dtype, *, allow_dense: bool = True, tile_cap
         (
             int(g.blocks[0].shape[1]),
             list(g.type_indices),
-            [i.detach().reshape(-1).to("cpu", torch.int64).numpy() for i in g.variable_indices],
+            [i.detach().reshape(-1).to(device=device, dtype=torch.int64) for i in g.variable_indices],
         )
         for g in jacobian.groups
     ]

@@ -1,4 +1,4 @@
-"""Symbolic plan of the deterministic tile reduction (host side, numpy).
+"""Symbolic plan of the deterministic tile reduction, built on the device.
 
 Built once per solve from the incidence structure only -- the variable
 indices of every slot -- mirroring the reference's split between a symbolic
@@ -20,6 +20,14 @@ summation order for every variable:
   accumulator of the whole type (DENSE, when the type fits in LDS);
 * every variable's partials are finally summed in increasing partial id order
   (dense blocks first, then its CSR list).
+
+Every array is a torch tensor on the device of the index arrays it is built
+from, and every O(N) step runs there as a device primitive (stable radix
+sorts, scans, histograms, gathers and scatters) -- the reference's plans are
+built the same way from torch ops on the device (warp_normal.py:155-176,
+schur.py:83-136).  The host reads back only scalars (sizes, maxima, flags) and,
+for groups whose tiles are cut at primary-variable runs, the run starts that
+the greedy tile cut walks.
 """
 
 from __future__ import annotations
@@ -27,6 +35,7 @@ from __future__ import annotations
 from dataclasses import dataclass, field
 
 import numpy as np
+import torch
 
 REDUCE_SPARSE = 0
 REDUCE_DENSE = 1
@@ -52,18 +61,18 @@ class SlotPlan:
     type: int
     dim: int
     mode: int
-    index: np.ndarray  # int32 [N] sorted order
-    rank: np.ndarray | None  # uint16 [N] or None (identity)
-    seg_ptr: np.ndarray  # int32 [ntiles+1]
-    seg_start: np.ndarray  # int32 [nseg] tile-local
-    seg_var: np.ndarray  # int64 [nseg] variable of each segment
-    seg_target: np.ndarray | None = None  # int32 [nseg] partial id / variable (-1: direct write)
+    index: torch.Tensor  # int32 [N] sorted order
+    rank: torch.Tensor | None  # int32 [N] (values < 65,536) or None (identity)
+    seg_ptr: torch.Tensor  # int32 [ntiles+1]
+    seg_start: torch.Tensor  # int32 [nseg] tile-local
+    seg_var: torch.Tensor  # int64 [nseg] variable of each segment
+    seg_target: torch.Tensor | None = None  # int32 [nseg] partial id / variable (-1: direct write)
     dense_block: int = 0
     pos_base: int = 0  # wave groups: first positional partial id of this slot
     # wave groups, DENSE secondary slot reduced in occurrence rounds: rank then holds
     # each position's occurrence index (earlier positions of the tile with the same
     # variable) and rounds[t] = 1 + the largest occurrence index of tile t
-    rounds: np.ndarray | None = None  # int32 [ntiles]
+    rounds: torch.Tensor | None = None  # int32 [ntiles]
 
 
 @dataclass
@@ -71,8 +80,8 @@ class GroupPlan:
     count: int
     rows: int
     primary: int
-    order: np.ndarray  # int64 [N] original observation at each sorted position
-    tile_start: np.ndarray  # int32 [ntiles+1]
+    order: torch.Tensor  # int64 [N] original observation at each sorted position
+    tile_start: torch.Tensor  # int32 [ntiles+1]
     tile_cap: int
     nwg: int
     tiles_per_wg: int
@@ -81,7 +90,7 @@ class GroupPlan:
 
     @property
     def ntiles(self) -> int:
-        return int(self.tile_start.size - 1)
+        return int(self.tile_start.numel() - 1)
 
     @property
     def stride(self) -> int:
@@ -100,10 +109,10 @@ class TypePlan:
     offset: int
     dense_blocks: int = 0
     sparse_partials: int = 0
-    csr_ptr: np.ndarray = field(default_factory=lambda: np.zeros(1, np.int32))
-    csr_idx: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
-    direct: np.ndarray | None = None  # uint8 [count]: written by its only wave piece
-    finish_list: np.ndarray | None = None  # int32 variables the finish kernel writes (None: all)
+    csr_ptr: torch.Tensor | None = None  # int32 [count+1]
+    csr_idx: torch.Tensor | None = None  # int32 partial ids
+    direct: torch.Tensor | None = None  # uint8 [count]: written by its only wave piece
+    finish_list: torch.Tensor | None = None  # int32 variables the finish kernel writes (None: all)
 
     @property
     def partial_count(self) -> int:
@@ -122,11 +131,42 @@ class StructurePlan:
         return sum(t.count * t.dim for t in self.types)
 
 
-def _cut_tiles(primary_sorted: np.ndarray, cap: int) -> np.ndarray:
-    n = primary_sorted.size
+def _arange(n, device):
+    return torch.arange(n, dtype=torch.int64, device=device)
+
+
+def _flatnonzero(mask):
+    return torch.nonzero(mask).reshape(-1)
+
+
+def _counts_ptr(values, count, device):
+    """int64 [count+1] prefix sums of the histogram of ``values`` (CSR row pointers)."""
+    ptr = torch.zeros(count + 1, dtype=torch.int64, device=device)
+    if values.numel():
+        ptr[1:] = torch.cumsum(torch.bincount(values, minlength=count), 0)
+    return ptr
+
+
+def _stable_order(keys):
+    """Positions sorted lexicographically by ``keys`` (most significant first), ties in
+    position order: successive stable sorts from the least significant key."""
+    order = _arange(keys[0].numel(), keys[0].device)
+    for k in reversed(keys):
+        order = order[torch.sort(k[order], stable=True).indices]
+    return order
+
+
+def _cut_tiles(primary_sorted: torch.Tensor, cap: int) -> torch.Tensor:
+    """Tiles of at most ``cap`` positions cut at primary-variable run starts where the run
+    fits (greedy, one tile after the other): the run starts are read back and walked on
+    the host."""
+    n = primary_sorted.numel()
+    device = primary_sorted.device
     if n == 0:
-        return np.zeros(1, dtype=np.int32)
-    run_starts = np.concatenate(([0], np.flatnonzero(np.diff(primary_sorted)) + 1))
+        return torch.zeros(1, dtype=torch.int64, device=device)
+    head = torch.ones(n, dtype=torch.bool, device=device)
+    head[1:] = primary_sorted[1:] != primary_sorted[:-1]
+    run_starts = _flatnonzero(head).cpu().numpy()
     starts = [0]
     s = 0
     while s < n:
@@ -140,43 +180,50 @@ def _cut_tiles(primary_sorted: np.ndarray, cap: int) -> np.ndarray:
                 end = target
         starts.append(end)
         s = end
-    return np.asarray(starts, dtype=np.int64)
+    return torch.tensor(starts, dtype=torch.int64, device=device)
 
 
-def _slot_segments(index_sorted: np.ndarray, tile_start: np.ndarray, piece: int | None):
+def _tile_of(tile_start: torch.Tensor, n: int) -> torch.Tensor:
+    ntiles = tile_start.numel() - 1
+    return torch.repeat_interleave(_arange(ntiles, tile_start.device), torch.diff(tile_start), output_size=n)
+
+
+def _slot_segments(index_sorted: torch.Tensor, tile_start: torch.Tensor, piece: int | None):
     """Within-tile order by variable; rank, segment starts/tiles/variables."""
-    n = index_sorted.size
-    ntiles = tile_start.size - 1
-    lens = np.diff(tile_start)
-    tile_of = np.repeat(np.arange(ntiles, dtype=np.int64), lens)
-    pos = np.arange(n, dtype=np.int64)
-    perm = np.lexsort((pos, index_sorted, tile_of))
+    device = index_sorted.device
+    n = index_sorted.numel()
+    ntiles = tile_start.numel() - 1
+    tile_of = _tile_of(tile_start, n)
+    pos = _arange(n, device)
+    idx = index_sorted.to(torch.int64)
+    perm = _stable_order([tile_of, idx])
     local = pos - tile_start[tile_of[perm]]  # position k in tile order (perm index k)
-    rank = np.empty(n, dtype=np.int64)
+    rank = torch.empty(n, dtype=torch.int64, device=device)
     rank[perm] = local
-    v = index_sorted[perm]
+    v = idx[perm]
     t = tile_of[perm]
-    boundary = np.ones(n, dtype=bool)
+    boundary = torch.ones(n, dtype=torch.bool, device=device)
     boundary[1:] = (v[1:] != v[:-1]) | (t[1:] != t[:-1])
     if piece is not None and n:
-        run_id = np.cumsum(boundary) - 1
-        run_begin = np.flatnonzero(boundary)
+        run_id = torch.cumsum(boundary.to(torch.int64), 0) - 1
+        run_begin = _flatnonzero(boundary)
         in_run = pos - run_begin[run_id]
         boundary |= (in_run % piece) == 0
-    heads = np.flatnonzero(boundary)
-    seg_tile = t[heads]
-    seg_ptr = np.zeros(ntiles + 1, dtype=np.int64)
-    np.cumsum(np.bincount(seg_tile, minlength=ntiles), out=seg_ptr[1:])
-    identity = bool(np.all(rank == (pos - tile_start[tile_of])))
+    heads = _flatnonzero(boundary)
+    seg_ptr = _counts_ptr(t[heads], ntiles, device)
+    identity = bool(torch.equal(rank, pos - tile_start[tile_of]))
     return rank, identity, local[heads], seg_ptr, v[heads]
 
 
 WAVE = 64
 
 
-def _aligned_tiles(n: int, cap: int) -> np.ndarray:
+def _aligned_tiles(n: int, cap: int, device) -> torch.Tensor:
     """Tiles of exactly ``cap`` sorted positions (the last one may be short)."""
-    return np.append(np.arange(0, n, cap, dtype=np.int64), n) if n else np.zeros(1, dtype=np.int64)
+    if n == 0:
+        return torch.zeros(1, dtype=torch.int64, device=device)
+    starts = torch.arange(0, n, cap, dtype=torch.int64, device=device)
+    return torch.cat((starts, torch.tensor([n], dtype=torch.int64, device=device)))
 
 
 # Most rounds a tile may take in the occurrence-round reduction; groups with a
@@ -184,44 +231,47 @@ def _aligned_tiles(n: int, cap: int) -> np.ndarray:
 MAX_ROUNDS = 8
 
 
-def _occurrences(index_sorted: np.ndarray, tile_start: np.ndarray):
+def _occurrences(index_sorted: torch.Tensor, tile_start: torch.Tensor):
     """Occurrence index of each position's variable inside its tile (0 for the
     first position of the tile with that variable, in position order) and the
     number of rounds (1 + largest occurrence index) of each tile."""
-    n = index_sorted.size
-    ntiles = tile_start.size - 1
+    device = index_sorted.device
+    n = index_sorted.numel()
+    ntiles = tile_start.numel() - 1
     if n == 0:
-        return np.zeros(0, np.int64), np.zeros(ntiles, np.int32)
-    tile = np.repeat(np.arange(ntiles, dtype=np.int64), np.diff(tile_start))
-    key = tile * (int(index_sorted.max()) + 1) + index_sorted.astype(np.int64)
-    order = np.argsort(key, kind="stable")
-    ks = key[order]
-    first = np.r_[True, ks[1:] != ks[:-1]]
-    start = np.maximum.accumulate(np.where(first, np.arange(n), 0))
-    occ = np.empty(n, np.int64)
-    occ[order] = np.arange(n) - start
-    rounds = np.zeros(ntiles, np.int64)
-    np.maximum.at(rounds, tile, occ + 1)
-    return occ, rounds.astype(np.int32)
+        return torch.zeros(0, dtype=torch.int64, device=device), torch.zeros(ntiles, dtype=torch.int32, device=device)
+    tile = _tile_of(tile_start, n)
+    idx = index_sorted.to(torch.int64)
+    order = _stable_order([tile, idx])
+    kt, kv = tile[order], idx[order]
+    first = torch.ones(n, dtype=torch.bool, device=device)
+    first[1:] = (kt[1:] != kt[:-1]) | (kv[1:] != kv[:-1])
+    pos = _arange(n, device)
+    start = torch.cummax(torch.where(first, pos, torch.zeros_like(pos)), 0).values
+    occ = torch.empty(n, dtype=torch.int64, device=device)
+    occ[order] = pos - start
+    rounds = torch.zeros(ntiles, dtype=torch.int64, device=device)
+    rounds.scatter_reduce_(0, tile, occ + 1, reduce="amax")
+    return occ, rounds.to(torch.int32)
 
 
-def _wave_pieces(index_sorted: np.ndarray, tile_start: np.ndarray, span: int):
+def _wave_pieces(index_sorted: torch.Tensor, tile_start: torch.Tensor, span: int):
     """Primary-slot pieces of the wave kernel: runs of one variable, also cut at
     every ``span``-position boundary of each tile (one wavefront covers 64
     threads x ITEMS consecutive positions)."""
-    n = index_sorted.size
-    ntiles = tile_start.size - 1
-    tile_of = np.repeat(np.arange(ntiles, dtype=np.int64), np.diff(tile_start))
-    pos = np.arange(n, dtype=np.int64)
+    device = index_sorted.device
+    n = index_sorted.numel()
+    ntiles = tile_start.numel() - 1
+    tile_of = _tile_of(tile_start, n)
+    pos = _arange(n, device)
     local = pos - tile_start[tile_of]
-    head = np.ones(n, dtype=bool)
+    head = torch.ones(n, dtype=torch.bool, device=device)
     head[1:] = index_sorted[1:] != index_sorted[:-1]
     head |= (local % span) == 0
-    heads = np.flatnonzero(head)
-    ends = np.append(heads[1:], n) - 1
-    seg_ptr = np.zeros(ntiles + 1, dtype=np.int64)
-    np.cumsum(np.bincount(tile_of[heads], minlength=ntiles), out=seg_ptr[1:])
-    return local[heads], seg_ptr, index_sorted[heads], ends
+    heads = _flatnonzero(head)
+    ends = torch.cat((heads[1:], torch.tensor([n], dtype=torch.int64, device=device))) - 1
+    seg_ptr = _counts_ptr(tile_of[heads], ntiles, device)
+    return local[heads], seg_ptr, index_sorted[heads].to(torch.int64), ends
 
 
 def build_structure(
@@ -242,31 +292,33 @@ def build_structure(
     """Plan the deterministic reduction.
 
     ``layout``: [(count, dim, offset)] per variable type.
-    ``groups``: [(rows, [type per slot], [int64 index array per slot])].
+    ``groups``: [(rows, [type per slot], [int64 index tensor per slot])], every index
+    tensor on the device the plan is built on (all groups on one device).
     """
     cap = tile_cap or THREADS * items_for(esize)
     types = [TypePlan(int(c), int(d), int(o)) for c, d, o in layout]
+    device = next(torch.as_tensor(ix).device for _, _, idx in groups for ix in idx)
     gplans = []
     for gi, (rows, slot_types, slot_idx) in enumerate(groups):
         ns = len(slot_types)
         if not 1 <= ns <= 3:
             raise ValueError("the hip backend supports 1 to 3 variable slots per residual")
-        count = int(slot_idx[0].size)
+        slot_idx = [torch.as_tensor(ix).to(device=device, dtype=torch.int64).reshape(-1) for ix in slot_idx]
+        count = int(slot_idx[0].numel())
         sizes = [types[t].count for t in slot_types]
         primary = int(np.argmax(sizes))
         if force_primary and gi in force_primary:
             primary = int(force_primary[gi])
-        keys = [np.arange(count)]
-        keys += [slot_idx[s] for s in reversed(range(ns)) if s != primary]
-        keys.append(slot_idx[primary])
-        order = np.lexsort(keys).astype(np.int64) if count else np.zeros(0, np.int64)
-        sorted_idx = [np.asarray(slot_idx[s])[order] for s in range(ns)]
+        # sorted by the primary slot, then the other slots in slot order, then position
+        keys = [slot_idx[primary]] + [slot_idx[s] for s in range(ns) if s != primary]
+        order = _stable_order(keys) if count else torch.zeros(0, dtype=torch.int64, device=device)
+        sorted_idx = [slot_idx[s][order] for s in range(ns)]
         dmax = max(types[t].dim for t in slot_types)
         shape = (int(rows), *(types[t].dim for t in slot_types))
         wave = ns == 2 and shape in wave_shapes and count > 0
         gcap = WAVE_THREADS * wave_items(esize) if wave else cap
-        tile_start = _aligned_tiles(count, gcap) if wave else _cut_tiles(sorted_idx[primary], cap)
-        ntiles = tile_start.size - 1
+        tile_start = _aligned_tiles(count, gcap, device) if wave else _cut_tiles(sorted_idx[primary], cap)
+        ntiles = tile_start.numel() - 1
         stage = gcap * dmax * esize + (gcap * 4 if wave else 0)
         dense_budget = lds_limit - stage
         slots = []
@@ -289,19 +341,19 @@ def build_structure(
                     t,
                     types[t].dim,
                     REDUCE_DENSE if dense else REDUCE_SPARSE,
-                    sorted_idx[s].astype(np.int32),
-                    None if identity else rank.astype(np.uint16),
-                    seg_ptr.astype(np.int32),
-                    seg_start.astype(np.int32),
-                    seg_var.astype(np.int64),
+                    sorted_idx[s].to(torch.int32),
+                    None if identity else rank.to(torch.int32),
+                    seg_ptr.to(torch.int32),
+                    seg_start.to(torch.int32),
+                    seg_var.to(torch.int64),
                 )
             )
             if wave and s == primary:
                 slots[-1].piece_end = ends
             elif wave and dense:
                 occ, rounds = _occurrences(sorted_idx[s], tile_start)
-                if rounds.size and int(rounds.max()) <= MAX_ROUNDS:
-                    slots[-1].rank = occ.astype(np.uint16)
+                if rounds.numel() and int(rounds.max()) <= MAX_ROUNDS:
+                    slots[-1].rank = occ.to(torch.int32)
                     slots[-1].rounds = rounds
         has_dense = any(sl.mode == REDUCE_DENSE for sl in slots)
         if has_dense and ntiles:
@@ -312,7 +364,7 @@ def build_structure(
         else:
             nwg, per = ntiles, 1
         gplans.append(
-            GroupPlan(count, int(rows), primary, order, tile_start.astype(np.int32), gcap, nwg, per, slots,
+            GroupPlan(count, int(rows), primary, order, tile_start.to(torch.int32), gcap, nwg, per, slots,
                       kernel=1 if wave else 0)
         )
     # dense blocks first (ids b*count + v), then sparse partials
@@ -321,15 +373,16 @@ def build_structure(
             if sl.mode == REDUCE_DENSE:
                 sl.dense_block = types[sl.type].dense_blocks
                 types[sl.type].dense_blocks += g.nwg
-                sl.seg_target = sl.seg_var.astype(np.int32)
+                sl.seg_target = sl.seg_var.to(torch.int32)
     pairs = [[] for _ in types]
     next_id = [t.dense_blocks * t.count for t in types]
     for g in gplans:
         for sl in g.slots:
             if sl.mode == REDUCE_SPARSE and g.kernel == 0:
-                ids = next_id[sl.type] + np.arange(sl.seg_var.size, dtype=np.int64)
-                next_id[sl.type] += sl.seg_var.size
-                sl.seg_target = ids.astype(np.int32)
+                nseg = sl.seg_var.numel()
+                ids = next_id[sl.type] + _arange(nseg, device)
+                next_id[sl.type] += nseg
+                sl.seg_target = ids.to(torch.int32)
                 pairs[sl.type].append((sl.seg_var, ids))
     # wave groups: positional partial ids (one region of `count` ids per slot)
     for g in gplans:
@@ -343,28 +396,28 @@ def build_structure(
             if si == g.primary:
                 ids = sl.pos_base + sl.piece_end
             else:
-                tile_of = np.repeat(np.arange(g.ntiles, dtype=np.int64), np.diff(sl.seg_ptr))
-                ids = sl.pos_base + g.tile_start[tile_of].astype(np.int64) + sl.seg_start
-            sl.seg_target = ids.astype(np.int64)
+                tile_of = _tile_of(sl.seg_ptr.to(torch.int64), sl.seg_start.numel())
+                ids = sl.pos_base + g.tile_start[tile_of].to(torch.int64) + sl.seg_start.to(torch.int64)
+            sl.seg_target = ids.to(torch.int64)
             pairs[sl.type].append((sl.seg_var, ids))
     # a variable whose only contribution is one primary wave piece is written directly
     for t, ty in enumerate(types):
-        contrib = np.zeros(ty.count, dtype=np.int64)
+        contrib = torch.zeros(ty.count, dtype=torch.int64, device=device)
         if ty.dense_blocks:
             contrib += 2
         for var, _ in pairs[t]:
-            contrib += np.bincount(var, minlength=ty.count)
-        direct = np.zeros(ty.count, dtype=bool)
+            contrib += torch.bincount(var, minlength=ty.count)
+        direct = torch.zeros(ty.count, dtype=torch.bool, device=device)
         for g in gplans:
             if g.kernel == 1:
                 sl = g.slots[g.primary]
                 if sl.type == t:
-                    cand = np.zeros(ty.count, dtype=bool)
+                    cand = torch.zeros(ty.count, dtype=torch.bool, device=device)
                     cand[sl.seg_var] = True
                     direct |= cand & (contrib == 1)
-        if direct.any():
-            ty.direct = direct.astype(np.uint8)
-            ty.finish_list = np.flatnonzero(~direct).astype(np.int32)
+        if bool(direct.any()):
+            ty.direct = direct.to(torch.uint8)
+            ty.finish_list = _flatnonzero(~direct).to(torch.int32)
             keep = []
             for var, ids in pairs[t]:
                 m = ~direct[var]
@@ -373,25 +426,24 @@ def build_structure(
             for g in gplans:
                 if g.kernel == 1 and g.slots[g.primary].type == t:
                     sl = g.slots[g.primary]
-                    sl.seg_target = np.where(direct[sl.seg_var], -1, sl.seg_target)
+                    sl.seg_target = torch.where(direct[sl.seg_var], torch.full_like(sl.seg_target, -1), sl.seg_target)
     for g in gplans:
         for sl in g.slots:
-            if sl.seg_target is not None and sl.seg_target.dtype != np.int32:
-                sl.seg_target = sl.seg_target.astype(np.int32)
+            if sl.seg_target is not None and sl.seg_target.dtype != torch.int32:
+                sl.seg_target = sl.seg_target.to(torch.int32)
     for t, ty in enumerate(types):
         ty.sparse_partials = next_id[t] - ty.dense_blocks * ty.count
         if next_id[t] > 2**31 - 1:
             raise ValueError("partial ids exceed the 32-bit range")
         if pairs[t]:
-            var = np.concatenate([p[0] for p in pairs[t]])
-            ids = np.concatenate([p[1] for p in pairs[t]])
-            o = np.lexsort((ids, var))
-            ty.csr_idx = ids[o].astype(np.int32)
-            ptr = np.zeros(ty.count + 1, dtype=np.int64)
-            np.cumsum(np.bincount(var, minlength=ty.count), out=ptr[1:])
-            ty.csr_ptr = ptr.astype(np.int32)
+            var = torch.cat([p[0] for p in pairs[t]])
+            ids = torch.cat([p[1] for p in pairs[t]])
+            o = _stable_order([var, ids])
+            ty.csr_idx = ids[o].to(torch.int32)
+            ty.csr_ptr = _counts_ptr(var, ty.count, device).to(torch.int32)
         else:
-            ty.csr_ptr = np.zeros(ty.count + 1, dtype=np.int32)
+            ty.csr_idx = torch.zeros(0, dtype=torch.int32, device=device)
+            ty.csr_ptr = torch.zeros(ty.count + 1, dtype=torch.int32, device=device)
     return StructurePlan(types, gplans, esize, cap)
 
 
@@ -406,103 +458,26 @@ def major_eligible(variables: int, observations: int) -> bool:
         and observations < 2**31
 
 
-def major_order(index: np.ndarray, count: int):
+def major_order(index: torch.Tensor, count: int):
     """Observations sorted by variable (stable) and each variable's range: (order, ptr), int32."""
-    index = np.asarray(index, dtype=np.int64).reshape(-1)
-    order = np.argsort(index, kind="stable")
-    ptr = np.zeros(count + 1, dtype=np.int64)
-    np.cumsum(np.bincount(index, minlength=count), out=ptr[1:])
-    return order.astype(np.int32), ptr.astype(np.int32)
+    index = torch.as_tensor(index).to(torch.int64).reshape(-1)
+    order = torch.sort(index, stable=True).indices
+    return order.to(torch.int32), _counts_ptr(index, count, index.device).to(torch.int32)
 
 
-def major_edges(index_c: np.ndarray, index_p: np.ndarray, count_c: int):
+def major_edges(index_c: torch.Tensor, index_p: torch.Tensor, count_c: int):
     """Observations sorted by (camera, point, position) and their (camera, point) runs:
     (order, edge_ptr [E+1], var_edge_ptr [count_c+1], edge_point [E]), all int32."""
-    ic = np.asarray(index_c, dtype=np.int64).reshape(-1)
-    ip = np.asarray(index_p, dtype=np.int64).reshape(-1)
-    n = ic.size
-    order = np.lexsort((np.arange(n), ip, ic))
+    ic = torch.as_tensor(index_c).to(torch.int64).reshape(-1)
+    ip = torch.as_tensor(index_p).to(torch.int64).reshape(-1).to(ic.device)
+    n = ic.numel()
+    order = _stable_order([ic, ip])
     sc, sp = ic[order], ip[order]
-    head = np.ones(n, dtype=bool)
+    head = torch.ones(n, dtype=torch.bool, device=ic.device)
     if n:
         head[1:] = (sc[1:] != sc[:-1]) | (sp[1:] != sp[:-1])
-    starts = np.flatnonzero(head)
-    edge_ptr = np.append(starts, n)
-    edge_cam = sc[starts]
-    var_edge_ptr = np.zeros(count_c + 1, dtype=np.int64)
-    np.cumsum(np.bincount(edge_cam, minlength=count_c), out=var_edge_ptr[1:])
-    return (order.astype(np.int32), edge_ptr.astype(np.int32), var_edge_ptr.astype(np.int32),
-            sp[starts].astype(np.int32))
-
-
-def emulate_normal_matvec(plan: StructurePlan, packed: list, x: np.ndarray, damping: float):
-    """Execute the plan's data flow on the CPU (numpy), exactly as the kernels do.
-
-    ``packed[g]`` is the group's component-major array [components, count] in
-    sorted order.  Used by the CPU tests to validate plans; not a product path.
-    """
-    types = plan.types
-    parts = [np.zeros((t.partial_count, t.dim)) for t in types]
-    direct_out = []
-    for g, gp in zip(plan.groups, packed):
-        offs = np.cumsum([0] + [s.dim for s in g.slots])
-        R = g.rows
-        xs = []
-        for s in g.slots:
-            t = types[s.type]
-            xs.append(x[t.offset : t.offset + t.count * t.dim].reshape(t.count, t.dim)[s.index])
-        q = np.zeros((g.count, R))
-        for si, s in enumerate(g.slots):
-            J = gp[R * offs[si] : R * offs[si + 1]].T.reshape(g.count, R, s.dim)
-            q += np.einsum("nrd,nd->nr", J, xs[si])
-        dense = {}
-        for si, s in enumerate(g.slots):
-            J = gp[R * offs[si] : R * offs[si + 1]].T.reshape(g.count, R, s.dim)
-            cont = np.einsum("nrd,nr->nd", J, q)
-            t = types[s.type]
-            if s.mode == REDUCE_DENSE:
-                dense[si] = np.zeros((g.nwg, t.count, t.dim))
-            for tile in range(g.ntiles):
-                a, b = g.tile_start[tile], g.tile_start[tile + 1]
-                stage = np.zeros((b - a, s.dim))
-                rank = np.arange(b - a) if s.rank is None else s.rank[a:b].astype(np.int64)
-                stage[rank] = cont[a:b]
-                k0, k1 = s.seg_ptr[tile], s.seg_ptr[tile + 1]
-                for k in range(k0, k1):
-                    lo = s.seg_start[k]
-                    hi = s.seg_start[k + 1] if k + 1 < k1 else b - a
-                    total = stage[lo:hi].sum(axis=0)
-                    if s.mode == REDUCE_DENSE:
-                        dense[si][tile // g.tiles_per_wg, s.seg_target[k]] += total
-                    elif s.seg_target[k] < 0:
-                        v = s.seg_var[k]
-                        direct_out.append((t, v, total))
-                    else:
-                        parts[s.type][s.seg_target[k]] = total
-            if s.mode == REDUCE_DENSE:
-                for w in range(g.nwg):
-                    base = (s.dense_block + w) * t.count
-                    parts[s.type][base : base + t.count] = dense[si][w]
-    y = np.zeros_like(x)
-    for ti, t in enumerate(types):
-        acc = np.zeros((t.count, t.dim))
-        for b in range(t.dense_blocks):
-            acc += parts[ti][b * t.count : (b + 1) * t.count]
-        for v in range(t.count):
-            for k in range(t.csr_ptr[v], t.csr_ptr[v + 1]):
-                acc[v] += parts[ti][t.csr_idx[k]]
-        sl = slice(t.offset, t.offset + t.count * t.dim)
-        y[sl] = acc.reshape(-1) + damping * x[sl]
-    for t, v, total in direct_out:
-        sl = slice(t.offset + v * t.dim, t.offset + (v + 1) * t.dim)
-        y[sl] = total + damping * x[sl]
-    return y
-
-
-def pack_numpy(plan: StructurePlan, group_blocks: list) -> list:
-    """CPU twin of tlh_pack_blocks: blocks [N, r, d] per slot -> [components, N] sorted."""
-    out = []
-    for g, blocks in zip(plan.groups, group_blocks):
-        cols = [np.asarray(b)[g.order].reshape(g.count, -1).T for b in blocks]
-        out.append(np.concatenate(cols, axis=0) if cols else np.zeros((0, g.count)))
-    return out
+    starts = _flatnonzero(head)
+    edge_ptr = torch.cat((starts, torch.tensor([n], dtype=torch.int64, device=ic.device)))
+    var_edge_ptr = _counts_ptr(sc[starts], count_c, ic.device)
+    return (order.to(torch.int32), edge_ptr.to(torch.int32), var_edge_ptr.to(torch.int32),
+            sp[starts].to(torch.int32))

@@ -21,13 +21,12 @@ from __future__ import annotations
 
 import ctypes
 
-import numpy as np
 import torch
 
 from . import _lib
 from .hip_diagonal import LinAlgError
 from .hip_normal import DeviceStructure, packed_blocks
-from .plan import build_structure, major_edges, major_eligible
+from .plan import _stable_order, build_structure, major_edges, major_eligible
 
 
 class HipSchurPlan:
@@ -78,16 +77,13 @@ class HipSchurPlan:
         if self.wgroup is None:
             raise NotImplementedError("no residual group couples the retained and eliminated groups")
         self.retained_only_multi = any(len(sl) > 1 for k, (_, sl, _) in enumerate(self.groups) if k != self.wgroup)
-        specs = []
-        for gi, slots, ltypes in self.groups:
-            b = problem.batches[gi]
-            specs.append((None, ltypes, [b.indices[s].numpy() if s < len(b.indices) else None for s in slots], gi))
         info = _lib.device_info(device)
         esize = 4 if dtype == torch.float32 else 8
+        # the symbolic analysis runs on the device: index arrays are moved there once
         plan_groups = []
-        for _, ltypes, idx, gi in specs:
+        for gi, slots, ltypes in self.groups:
             b = problem.batches[gi]
-            plan_groups.append((None, ltypes, [np.asarray(i, dtype=np.int64) for i in idx]))
+            plan_groups.append((None, ltypes, [b.indices[s].to(device=device, dtype=torch.int64) for s in slots]))
         self.rows = [None] * len(self.groups)
         self._plan_groups = plan_groups
         self._problem = problem
@@ -119,13 +115,15 @@ class HipSchurPlan:
                                            lds_limit=int(self._info.lds_per_block), allow_dense=False,
                                            force_primary={0: self.eslot}, split_primary=False)
         w = self.plan.groups[self.wgroup]
-        prim = w.slots[self.eslot].index
-        cuts = w.tile_start[1:-1]
-        if cuts.size and np.any(prim[cuts] == prim[cuts - 1]):
+        prim = w.slots[self.eslot].index.to(torch.int64)
+        cuts = w.tile_start[1:-1].to(torch.int64)
+        if cuts.numel() and bool((prim[cuts] == prim[cuts - 1]).any()):
             raise NotImplementedError("an eliminated variable has more observations than one tile holds")
         # eliminated variables no coupling residual observes: back-substituted as C^-1 b_e alone
-        lonely = np.setdiff1d(np.arange(self.ecount), prim[: w.count])
-        self.lonely = (torch.from_numpy(lonely.astype(np.int32)).to(self.device) if lonely.size else None)
+        seen = torch.zeros(self.ecount, dtype=torch.bool, device=self.device)
+        seen[prim] = True
+        lonely = torch.nonzero(~seen).reshape(-1)
+        self.lonely = lonely.to(torch.int32) if lonely.numel() else None
         self.dev = DeviceStructure(self.plan, self.device)
         self.sdev = DeviceStructure(self.sparse_plan, self.device)
         # camera-order preconditioner (csrc/major.hip) for few, heavily observed retained variables
@@ -136,8 +134,7 @@ class HipSchurPlan:
         if (tuple(grp.blocks[sr].shape[1:]) == (2, 6) and tuple(grp.blocks[self.eslot].shape[1:]) == (2, 3)
                 and major_eligible(rcount, int(grp.blocks[0].shape[0]))):
             idx = self._plan_groups[self.wgroup][2]
-            arrays = major_edges(idx[sr], idx[self.eslot], rcount)
-            self.major = tuple(torch.from_numpy(a).to(self.device) for a in arrays)
+            self.major = major_edges(idx[sr], idx[self.eslot], rcount)
         nt = len(self.retained)
         self.partials = [torch.empty(max(1, t.partial_count * t.dim), dtype=self.dtype, device=self.device)
                          for t in self.plan.types]
@@ -148,35 +145,38 @@ class HipSchurPlan:
         self.ntypes_ret = nt
 
     def _dense_pairs(self):
-        """Observation pairs of every point run with cam(i) <= cam(j), grouped by camera pair."""
+        """Observation pairs of every point run with cam(i) <= cam(j), grouped by camera pair
+        (built on the device)."""
         if self.pairs is not None:
             return self.pairs
         w = self.plan.groups[self.wgroup]
         cs = 1 - self.eslot
-        cam = w.slots[cs].index.astype(np.int64)
-        prim = w.slots[self.eslot].index.astype(np.int64)
-        n = prim.size
-        starts = np.concatenate(([0], np.flatnonzero(np.diff(prim)) + 1, [n]))
-        lens = np.diff(starts)
+        cam = w.slots[cs].index.to(torch.int64)
+        prim = w.slots[self.eslot].index.to(torch.int64)
+        n = prim.numel()
+        head = torch.ones(n, dtype=torch.bool, device=self.device)
+        head[1:] = prim[1:] != prim[:-1]
+        starts = torch.nonzero(head).reshape(-1)
+        lens = torch.diff(torch.cat((starts, starts.new_tensor([n]))))
         pi, pj = [], []
-        for L in np.unique(lens):
-            s0 = starts[:-1][lens == L]
-            a = np.arange(L)
-            ii = (s0[:, None, None] + a[None, :, None]).repeat(L, axis=2).reshape(-1)
-            jj = (s0[:, None, None] + a[None, None, :]).repeat(L, axis=1).reshape(-1)
+        for L in torch.unique(lens).tolist():
+            s0 = starts[lens == L]
+            a = torch.arange(L, dtype=torch.int64, device=self.device)
+            ii = (s0[:, None, None] + a[None, :, None]).expand(-1, L, L).reshape(-1)
+            jj = (s0[:, None, None] + a[None, None, :]).expand(-1, L, L).reshape(-1)
             keep = cam[ii] <= cam[jj]
             pi.append(ii[keep])
             pj.append(jj[keep])
-        pi = np.concatenate(pi)
-        pj = np.concatenate(pj)
+        pi = torch.cat(pi)
+        pj = torch.cat(pj)
         nc = int(self.rlayout[self.plan.groups[self.wgroup].slots[cs].type][0])
         key = cam[pi] * nc + cam[pj]
-        o = np.lexsort((pj, pi, key))
+        o = _stable_order([key, pi, pj])
         pi, pj, key = pi[o], pj[o], key[o]
-        ukey, first = np.unique(key, return_index=True)
-        ptr = np.concatenate((first, [key.size])).astype(np.int32)
-        t = lambda a: torch.from_numpy(np.ascontiguousarray(a.astype(np.int32))).to(self.device)  # noqa: E731
-        self.pairs = dict(ptr=t(ptr), i=t(pi), j=t(pj), ci=t(ukey // nc), cj=t(ukey % nc), n=int(ukey.size))
+        ukey, counts = torch.unique_consecutive(key, return_counts=True)
+        ptr = torch.cat((counts.new_zeros(1), torch.cumsum(counts, 0)))
+        t = lambda a: a.to(torch.int32).contiguous()  # noqa: E731
+        self.pairs = dict(ptr=t(ptr), i=t(pi), j=t(pj), ci=t(ukey // nc), cj=t(ukey % nc), n=int(ukey.numel()))
         return self.pairs
 
     def linearize(self, jacobian, diagonal, *, dense: bool = False, col_scale: torch.Tensor | None = None):

@@ -3,12 +3,15 @@
 ``HipSchurPlan(problem, eliminate, device, dtype)`` is the symbolic analysis
 (SchurPlan, schur.py:17-161): the eliminated parameter group must be
 independent per residual (schur.py:44-49); the retained groups keep their
-layout order with re-based offsets (schur.py:27-37).  The HIP operator supports
-the bundle-adjustment structure: exactly one residual group ("W-group") that
-couples one retained slot with one eliminated slot, any number of groups on
-retained variables only (e.g. camera priors; each with one slot for the dense
+layout order with re-based offsets (schur.py:27-37).  The fused HIP kernels
+serve the bundle-adjustment structure: exactly one residual group ("W-group")
+that couples one retained slot with one eliminated slot, any number of groups
+on retained variables only (e.g. camera priors; one slot each for the dense
 path), and any groups on eliminated variables only (e.g. point priors, which
-only enter C = D_e + lambda I).
+only enter C = D_e + lambda I).  Every other structure the reference accepts
+(several retained slots per residual, retained-retained coupling, several
+coupling groups, eliminated variables with more observations than a tile) is
+served by ``schur_general``.
 
 ``plan.linearize(J, diagonal)`` packs one linearization (SchurLinearization,
 schur.py:192-257); ``.damped(lambda)`` gives the system with ``reduced_rhs``,
@@ -57,8 +60,13 @@ class HipSchurPlan:
         gidx = [torch.arange(layout[t].offset, layout[t].offset + layout[t].count * layout[t].dim)
                 for t in self.retained]
         self.global_indices = torch.cat(gidx).to(device)
+        self.tangent_dim = sum(e.count * e.dim for e in layout)
+        self.kernels = None  # LinearizationKernels of the general path
         self.wgroup = None
         self.groups = []  # (lin group index, [lin slot indices], [local types])
+        # general: a structure outside the fused kernels' (one group coupling one retained
+        # slot with one eliminated slot) -- served by schur_general
+        self.general = False
         for gi, b in enumerate(problem.batches):
             es = [s for s, t in enumerate(b.type_indices) if t == self.e]
             rs = [s for s, t in enumerate(b.type_indices) if t != self.e]
@@ -66,16 +74,15 @@ class HipSchurPlan:
                 raise ValueError("Eliminated variables must be independent: a residual connects distinct eliminated indices")
             if es and rs:
                 if self.wgroup is not None or len(es) != 1 or len(rs) != 1:
-                    raise NotImplementedError(
-                        "the HIP Schur operator supports one residual group coupling one retained slot "
-                        "with one eliminated slot")
+                    self.general = True
+                    continue
                 self.wgroup = len(self.groups)
                 self.eslot = es[0]
                 self.groups.append((gi, [0, 1], [self.local[b.type_indices[0]], self.local[b.type_indices[1]]]))
             elif rs:
                 self.groups.append((gi, list(range(len(rs))), [self.local[b.type_indices[s]] for s in rs]))
         if self.wgroup is None:
-            raise NotImplementedError("no residual group couples the retained and eliminated groups")
+            self.general = True
         self.retained_only_multi = any(len(sl) > 1 for k, (_, sl, _) in enumerate(self.groups) if k != self.wgroup)
         info = _lib.device_info(device)
         esize = 4 if dtype == torch.float32 else 8
@@ -95,6 +102,9 @@ class HipSchurPlan:
 
     def _build(self, jacobian):
         """Plans need residual row counts, known from the first linearization."""
+        if self.general:
+            self.plan = True  # built: the general path has no tile plan
+            return
         groups = []
         for k, (gi, slots, ltypes) in enumerate(self.groups):
             rows = int(jacobian.groups[gi].blocks[0].shape[1])
@@ -118,7 +128,10 @@ class HipSchurPlan:
         prim = w.slots[self.eslot].index.to(torch.int64)
         cuts = w.tile_start[1:-1].to(torch.int64)
         if cuts.numel() and bool((prim[cuts] == prim[cuts - 1]).any()):
-            raise NotImplementedError("an eliminated variable has more observations than one tile holds")
+            # an eliminated variable has more observations than one tile holds
+            self.general = True
+            self.plan = True
+            return
         # eliminated variables no coupling residual observes: back-substituted as C^-1 b_e alone
         seen = torch.zeros(self.ecount, dtype=torch.bool, device=self.device)
         seen[prim] = True
@@ -185,8 +198,6 @@ class HipSchurPlan:
         then be the scaled blocks, as the reference passes them)."""
         if self.plan is None:
             self._build(jacobian)
-        if dense and self.retained_only_multi:
-            raise NotImplementedError("dense HIP Schur supports single-slot retained-only residual groups")
         local_scale = None
         if col_scale is not None:
             total = sum(e.count * e.dim for e in self.layout)
@@ -196,6 +207,10 @@ class HipSchurPlan:
             # the plan's layout: retained types (re-based), then the eliminated type
             local_scale = torch.cat((col_scale[self.global_indices],
                                      col_scale[self.eoffset: self.eoffset + self.ecount * self.edim])).contiguous()
+        if self.general or (dense and self.retained_only_multi):
+            from .schur_general import GeneralSchurLinearization
+
+            return GeneralSchurLinearization(self, jacobian, diagonal, dense, local_scale)
         return HipSchurLinearization(self, jacobian, diagonal, dense, local_scale)
 
 

@@ -186,7 +186,7 @@ typedef struct tlh_block_diag {
 } tlh_block_diag_t;
 
 /* inverse = (blocks + damping I)^-1 through a float64 Cholesky per block
- * (dim <= 8); *status |= 1 when a block is not positive definite (the analogue
+ * (dim <= 16; joint preconditioner groups reach 9-16); *status |= 1 when a block is not positive definite (the analogue
  * of torch.linalg.cholesky's LinAlgError, solvers.py:267-272, 595-596).
  * Replaces: block_jacobi's factorisation (solvers.py:259-285). */
 int tlh_block_inverse(const double* blocks, int64_t count, int dim, double damping,

@@ -127,3 +127,100 @@ def test_general_schur_lm_reaches_the_full_system_optimum(device, name, solver):
           f"({s.termination_reason})", flush=True)
     assert s.final_cost < s.initial_cost
     assert math.isclose(s.final_cost, ref.final_cost, rel_tol=1e-8)
+
+
+def joint_problem(device, seed=6):
+    """Cameras with a per-camera 3-vector (focal and principal point) observed together."""
+    prob = generators.bundle_adjustment(10, 200, 5, seed=seed)
+    P = Problem({"cams": torch.tensor(prob.cameras_init, device=device),
+                 "intr": torch.tensor([[1.02, 0.01, -0.01]], dtype=torch.float64, device=device).repeat(prob.n_cameras, 1),
+                 "points": torch.tensor(prob.points_init, device=device)}, manifolds={"cams": SE3Var})
+    ci = torch.tensor(prob.cam_idx)
+
+    def intrinsic_reprojection(camera, intr, point, uv):
+        return intr[0] * project(camera, point) + intr[1:] - uv
+
+    P.add_residual(intrinsic_reprojection, ("cams", ci), ("intr", ci), ("points", torch.tensor(prob.pt_idx)),
+                   torch.tensor(prob.observations, device=device))
+    P.add_residual(lambda k: k - torch.tensor([1.0, 0.0, 0.0], dtype=k.dtype, device=k.device),
+                   ("intr", torch.arange(prob.n_cameras)))
+    P.add_residual(lambda c, r: (c[:3, :] - r[:3, :]).reshape(-1), ("cams", 0),
+                   torch.tensor(prob.cameras_true[0], device=device))
+    return P
+
+
+def _dense_joint_blocks(J, layout, groups_members, lam):
+    """Joint blocks of the dense J^T J (+ lam I), read off a brute-force dense matrix."""
+    A = J.to_dense()
+    H = A.T @ A + lam * np.eye(J.n)
+    out = []
+    for members in groups_members:
+        count = layout[members[0]][0]
+        blocks = []
+        for v in range(count):
+            cols = np.concatenate([layout[t][2] + v * layout[t][1] + np.arange(layout[t][1]) for t in members])
+            blocks.append(H[np.ix_(cols, cols)])
+        out.append(np.stack(blocks))
+    return out
+
+
+def test_joint_block_jacobi_against_dense_normal_matrix(device):
+    """preconditioning.py:47-121: joint blocks are the (cams, intr) diagonal blocks of J^T J."""
+    from torchls_amd.preconditioning import BlockJacobiPlan
+
+    P = joint_problem(device)
+    an = P._analyze()
+    lin = an.linearize(P.parameters)
+    diag = tuple(d.double() for d in lin.jacobian.diagonal_blocks())
+    plan = BlockJacobiPlan(an.layout, (("cams", "intr"),), device)
+    blocks = plan.normal_blocks(lin.jacobian, diag)
+    J = to_np(lin.jacobian, an.layout)
+    ref = _dense_joint_blocks(J, [(e.count, e.dim, e.offset) for e in an.layout],
+                              [b.members for b in plan.blocks], 0.0)
+    for a, b in zip(blocks, ref):
+        assert rel_err(a.cpu().numpy(), b) <= 1e-12
+    # the factor applies (B + lam I)^-1 block by block
+    pre = plan.factor(blocks, 0.5)
+    r = torch.randn(J.n, dtype=torch.float64, device=device)
+    z = pre(r).cpu().numpy()
+    rn = r.cpu().numpy()
+    for b, B in zip(plan.blocks, ref):
+        cols = b.columns.cpu().numpy()
+        for v in range(0, b.count, 3):
+            want = np.linalg.solve(B[v] + 0.5 * np.eye(b.dim), rn[cols[v]])
+            assert np.allclose(z[cols[v]], want, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("solver", ["conjugate_gradient", "schur_cg"])
+def test_joint_groups_lm(device, solver):
+    """LM with joint (cams, intr) groups reaches the optimum of the plain block-Jacobi solve;
+    for Schur-CG the joint reduced blocks equal the dense Schur complement's blocks."""
+    from torchls_amd import ConjugateGradientConfig
+
+    print(f"joint groups lm {solver}", flush=True)
+    term = TerminationConfig(max_iterations=15)
+    opts = dict(linear_solver=solver, eliminate="points") if solver == "schur_cg" else {}
+    _, ref = joint_problem(device).solve(return_summary=True, termination=term, **opts)
+    P = joint_problem(device)
+    cg = ConjugateGradientConfig(preconditioner_groups=(("cams", "intr"),))
+    _, s = P.solve(return_summary=True, termination=term, conjugate_gradient=cg, **opts)
+    print(f"  plain {ref.final_cost:.12e} pcg {sum(ref.linear_iterations_history)}; joint {s.final_cost:.12e} "
+          f"pcg {sum(s.linear_iterations_history)}", flush=True)
+    assert math.isclose(s.final_cost, ref.final_cost, rel_tol=1e-8)
+    if solver == "schur_cg":
+        an = P._analyze()
+        lin = an.linearize(P.parameters)
+        diag = tuple(d.double() for d in lin.jacobian.diagonal_blocks())
+        plan = HipSchurPlan(an, "points", device, preconditioner_groups=(("cams", "intr"),))
+        system = plan.linearize(lin.jacobian, diag).damped(1e-2)
+        pre = system.preconditioner()
+        S = SchurSystem(to_np(lin.jacobian, an.layout), [d.cpu().numpy() for d in diag], 2, 1e-2).dense_matrix()
+        # joint block of camera v: rows/columns of cams v and intr v in the reduced system
+        nc = an.layout[0].count
+        r = torch.randn(plan.dim, dtype=torch.float64, device=device)
+        z = pre(r).cpu().numpy()
+        rn = r.cpu().numpy()
+        for v in range(0, nc, 3):
+            cols = np.concatenate((6 * v + np.arange(6), 6 * nc + 3 * v + np.arange(3)))
+            want = np.linalg.solve(S[np.ix_(cols, cols)], rn[cols])
+            assert np.allclose(z[cols], want, rtol=1e-9, atol=1e-12)

@@ -677,6 +677,14 @@ extern "C" int tlh_block_inverse(const double* blocks, int64_t count, int dim, d
     TLH_INV_CASE(6)
     TLH_INV_CASE(7)
     TLH_INV_CASE(8)
+    TLH_INV_CASE(9)
+    TLH_INV_CASE(10)
+    TLH_INV_CASE(11)
+    TLH_INV_CASE(12)
+    TLH_INV_CASE(13)
+    TLH_INV_CASE(14)
+    TLH_INV_CASE(15)
+    TLH_INV_CASE(16)
 #undef TLH_INV_CASE
     default:
       return TLH_ENOTFOUND;

@@ -33,7 +33,7 @@ from .plan import _stable_order, build_structure, major_edges, major_eligible
 
 
 class HipSchurPlan:
-    def __init__(self, problem, eliminate, device, *, dtype=torch.float64):
+    def __init__(self, problem, eliminate, device, *, dtype=torch.float64, preconditioner_groups=()):
         layout = problem.layout
         matches = [i for i, e in enumerate(layout) if e.key == eliminate or e.name == eliminate]
         if len(matches) != 1:
@@ -82,6 +82,16 @@ class HipSchurPlan:
             elif rs:
                 self.groups.append((gi, list(range(len(rs))), [self.local[b.type_indices[s]] for s in rs]))
         if self.wgroup is None:
+            self.general = True
+        # joint preconditioner groups of retained variables (schur.py:137-146, 331-352): served by
+        # the general path's reduced preconditioner
+        self.joint = None
+        if preconditioner_groups:
+            from .preconditioning import BlockJacobiPlan
+
+            self.joint = BlockJacobiPlan(layout, preconditioner_groups, device)
+            if any(self.e in b.members and len(b.members) > 1 for b in self.joint.blocks):
+                raise ValueError("Unknown preconditioner key (Schur groups must be retained variables)")
             self.general = True
         self.retained_only_multi = any(len(sl) > 1 for k, (_, sl, _) in enumerate(self.groups) if k != self.wgroup)
         info = _lib.device_info(device)

@@ -156,58 +156,117 @@ class GeneralSchurSystem:
         full[self._esl()] = self._cinv(self._be - t)
         return full
 
-    def preconditioner_blocks(self):
-        """float64 D_r + lambda I - sum over (retained, eliminated) edges W C^-1 W^T."""
+    def _cinv_full(self):
+        """Full float64 C^-1 blocks [ecount, d, d] from the packed upper triangles."""
         p = self.plan
-        lin = self.lin
-        J = lin.jacobian
-        s = lin.scale
         d = p.edim
-        # full C^-1 blocks from the packed upper triangles
         tri = torch.triu_indices(d, d, device=p.device)
         cinv = torch.zeros((p.ecount, d, d), dtype=torch.float64, device=p.device)
         packed = self.cinv[: p.ecount * d * (d + 1) // 2].reshape(p.ecount, -1)
         cinv[:, tri[0], tri[1]] = packed
         cinv[:, tri[1], tri[0]] = packed
-        blocks = []
-        for k, t in enumerate(p.retained):
-            ent = p.layout[t]
-            D = lin.diagonal[t]
-            M = D + self.damping * torch.eye(ent.dim, dtype=torch.float64, device=p.device)
-            rows, keys = [], []
-            for g in J.groups:
-                es = [q for q, ty in enumerate(g.type_indices) if ty == p.e]
-                if not es:
+        return cinv
+
+    def _edges(self, t):
+        """(edge keys v * ecount + e sorted, W [E, dim_t, d]) of retained type t: each edge's
+        W = sum over its observations of J_t^T J_e, in a fixed order."""
+        p = self.plan
+        J = self.lin.jacobian
+        s = self.lin.scale
+        rows, keys = [], []
+        for g in J.groups:
+            es = [q for q, ty in enumerate(g.type_indices) if ty == p.e]
+            if not es:
+                continue
+            je = sum(g.blocks[q].to(torch.float64) for q in es)
+            ie = g.variable_indices[es[0]]
+            if s is not None:
+                je = je * s[g.columns[es[0]]][:, None, :]
+            for q, ty in enumerate(g.type_indices):
+                if ty != t:
                     continue
-                je = sum(g.blocks[q].to(torch.float64) for q in es)
-                ie = g.variable_indices[es[0]]
+                jr = g.blocks[q].to(torch.float64)
                 if s is not None:
-                    je = je * s[g.columns[es[0]]][:, None, :]
-                for q, ty in enumerate(g.type_indices):
-                    if ty != t:
-                        continue
-                    jr = g.blocks[q].to(torch.float64)
-                    if s is not None:
-                        jr = jr * s[g.columns[q]][:, None, :]
-                    rows.append(torch.bmm(jr.transpose(1, 2).contiguous(), je))
-                    keys.append(g.variable_indices[q].to(torch.int64) * p.ecount + ie.to(torch.int64))
-            if rows:
-                order, ukey, ptr = _groups_by(torch.cat(keys))
-                W = _segment_sum(torch.cat(rows), order, ptr, ukey.numel()).reshape(-1, ent.dim, d)
+                    jr = jr * s[g.columns[q]][:, None, :]
+                rows.append(torch.bmm(jr.transpose(1, 2).contiguous(), je))
+                keys.append(g.variable_indices[q].to(torch.int64) * p.ecount + ie.to(torch.int64))
+        if not rows:
+            return None
+        order, ukey, ptr = _groups_by(torch.cat(keys))
+        W = _segment_sum(torch.cat(rows), order, ptr, ukey.numel()).reshape(-1, p.layout[t].dim, p.edim)
+        return ukey, W
+
+    def _per_variable(self, P, v, count):
+        """Sum of P [E, a, b] over edges grouped by v (nondecreasing) -> [count, a, b]."""
+        p = self.plan
+        vptr = torch.zeros(count + 1, dtype=torch.int64, device=p.device)
+        vptr[1:] = torch.cumsum(torch.bincount(v, minlength=count), 0)
+        eorder = torch.arange(v.numel(), dtype=torch.int32, device=p.device)
+        return _segment_sum(P, eorder, vptr.to(torch.int32), count).reshape(count, *P.shape[1:])
+
+    def preconditioner_blocks(self):
+        """float64 D_r + lambda I - sum over (retained, eliminated) edges W C^-1 W^T."""
+        p = self.plan
+        cinv = self._cinv_full()
+        blocks = []
+        for t in p.retained:
+            ent = p.layout[t]
+            M = self.lin.diagonal[t] + self.damping * torch.eye(ent.dim, dtype=torch.float64, device=p.device)
+            edges = self._edges(t)
+            if edges is not None:
+                ukey, W = edges
                 P = torch.bmm(torch.bmm(W, cinv[ukey % p.ecount]), W.transpose(1, 2).contiguous())
-                # edges are sorted by key, so those of one retained variable are contiguous
-                v = ukey // p.ecount
-                vptr = torch.zeros(ent.count + 1, dtype=torch.int64, device=p.device)
-                vptr[1:] = torch.cumsum(torch.bincount(v, minlength=ent.count), 0)
-                eorder = torch.arange(v.numel(), dtype=torch.int32, device=p.device)
-                M = M - _segment_sum(P, eorder, vptr.to(torch.int32), ent.count).reshape(ent.count, ent.dim, ent.dim)
+                M = M - self._per_variable(P, ukey // p.ecount, ent.count)
             blocks.append(M)
         return blocks
+
+    def _joint_preconditioner(self):
+        """Joint groups of retained variables (schur.py:331-352): each block holds the members'
+        reduced diagonal blocks and, between two members a, b, the retained cross terms minus
+        the sum over eliminated e linked to both of W_ae C_e^-1 W_be^T."""
+        from .preconditioning import HipJointBlockJacobi
+
+        p = self.plan
+        jp = p.joint
+        base = jp.normal_blocks(self.lin.jacobian, self.lin.diagonal, col_scale=self.lin.scale)
+        cinv = self._cinv_full()
+        edges = {t: self._edges(t) for t in p.retained}
+        # retained coordinates of each full tangent column
+        to_retained = torch.full((p.tangent_dim,), -1, dtype=torch.int64, device=p.device)
+        to_retained[p.global_indices] = torch.arange(p.dim, device=p.device)
+        meta, values = [], []
+        for b, block in enumerate(jp.blocks):
+            if p.e in block.members:
+                continue
+            M = base[b] + self.damping * torch.eye(block.dim, dtype=torch.float64, device=p.device)
+            for ia, ta in enumerate(block.members):
+                oa = jp.locations[ta][1]
+                for tc in block.members[ia:]:
+                    oc = jp.locations[tc][1]
+                    ea, ec = edges[ta], edges[tc]
+                    if ea is None or ec is None:
+                        continue
+                    ka, Wa = ea
+                    kc, Wc = ec
+                    pos = torch.searchsorted(ka, kc).clamp_max(ka.numel() - 1)
+                    both = ka[pos] == kc
+                    kk, Wa2, Wc2 = kc[both], Wa[pos[both]], Wc[both]
+                    P = torch.bmm(torch.bmm(Wa2, cinv[kk % p.ecount]), Wc2.transpose(1, 2).contiguous())
+                    S = self._per_variable(P, kk // p.ecount, block.count)
+                    da, dc = Wa.shape[1], Wc.shape[1]
+                    M[:, oa: oa + da, oc: oc + dc] -= S
+                    if tc != ta:
+                        M[:, oc: oc + dc, oa: oa + da] -= S.transpose(1, 2)
+            meta.append((block.count, block.dim, to_retained[block.columns]))
+            values.append(M)
+        return HipJointBlockJacobi(meta, values, 0.0, status=self.status, defer_check=self.deferred)
 
     def preconditioner(self, *, graph_safe=False):
         from .sparse import VariableLayout
 
         p = self.plan
+        if p.joint is not None:
+            return self._joint_preconditioner()
         lay = tuple(VariableLayout(p.layout[t].var_type, p.layout[t].ids, off)
                     for t, (_, _, off) in zip(p.retained, p.rlayout[:-1]))
         return HipBlockJacobi(lay, self.preconditioner_blocks(), 0.0, status=self.status, defer_check=self.deferred)

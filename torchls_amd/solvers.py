@@ -66,8 +66,11 @@ class ConjugateGradientConfig:
             raise ValueError("Invalid CG tolerance or iteration limit")
         if self.preconditioner not in ("block_jacobi", None):
             raise ValueError("preconditioner must be block_jacobi or None")
-        if self.preconditioner_groups:
-            raise ValueError("joint preconditioner groups are not supported by the HIP backend")
+        from .preconditioning import normalize_groups
+
+        object.__setattr__(self, "preconditioner_groups", normalize_groups(self.preconditioner_groups))
+        if self.preconditioner_groups and self.preconditioner is None:
+            raise ValueError("Joint groups require block_jacobi preconditioning")
         if (isinstance(self.residual_check_interval, bool) or not isinstance(self.residual_check_interval, int)
                 or self.residual_check_interval < 0):
             raise ValueError("residual_check_interval must be a nonnegative integer")
@@ -187,10 +190,13 @@ def block_jacobi(jacobian, damping: float, *, diagonal=None, groups=(), graph_sa
     """float64 (D_t + damping I)^-1 per variable block (solvers.py:259-285).
 
     Raises ``LinAlgError`` when a damped block is not positive definite."""
-    if groups:
-        raise ValueError("joint preconditioner groups are not supported by the HIP backend")
     if diagonal is None:
         diagonal = HipDiagonalPlan(jacobian).diagonal_blocks(jacobian)
+    if groups:
+        from .preconditioning import BlockJacobiPlan
+
+        plan = BlockJacobiPlan(jacobian.layout, groups, jacobian.groups[0].blocks[0].device)
+        return plan.factor(plan.normal_blocks(jacobian, diagonal), damping)
     return HipBlockJacobi(jacobian.layout, tuple(d.to(torch.float64) for d in diagonal), damping)
 
 
@@ -225,6 +231,8 @@ def solve(
     is_schur = linear_solver.startswith("schur_")
     if eliminate is not None and not is_schur:
         raise ValueError("eliminate requires a Schur linear solver")
+    if conjugate_gradient.preconditioner_groups and linear_solver not in ("conjugate_gradient", "schur_cg"):
+        raise ValueError("Joint preconditioner groups require a CG linear solver")
     if is_schur and eliminate is None:
         raise ValueError("Schur solvers need eliminate=<parameter group name>")
     start = time.perf_counter()
@@ -265,13 +273,21 @@ def solve(
         from .schur import HipSchurPlan
 
         with region("analyze_schur"):
-            schur_plan = cached(("schur", eliminate),
-                                lambda: HipSchurPlan(problem, eliminate, device, dtype=mv_dtype))
+            groups = conjugate_gradient.preconditioner_groups
+            schur_plan = cached(("schur", eliminate, groups),
+                                lambda: HipSchurPlan(problem, eliminate, device, dtype=mv_dtype,
+                                                     preconditioner_groups=groups))
         if linear_solver == "schur_dense" and schur_plan.dim > 8192:
             raise ValueError("Dense Schur is limited to 8192 retained dimensions; use schur_cg")
         summary.reduced_dim = schur_plan.dim
     normal_plan = diagonal_plan = kernels = None
     column_scale = None
+    joint_plan = None
+    if conjugate_gradient.preconditioner_groups and linear_solver == "conjugate_gradient":
+        from .preconditioning import BlockJacobiPlan
+
+        joint_plan = cached(("joint", conjugate_gradient.preconditioner_groups),
+                            lambda: BlockJacobiPlan(problem.layout, conjugate_gradient.preconditioner_groups, device))
 
     def examine_device(lin):
         """float64 gradient of a linearization and its (nonfinite count, max |g|,
@@ -351,6 +367,8 @@ def solve(
                 linear_gradient = gradient
             schur_linear = None
             normal_operator = None
+            joint_diagonal = (joint_plan.normal_blocks(jacobian, diagonal, col_scale=solve_scale)
+                              if joint_plan is not None else None)
             if is_schur:
                 schur_linear = schur_plan.linearize(jacobian, diagonal, dense=linear_solver == "schur_dense",
                                                     col_scale=solve_scale)
@@ -410,10 +428,12 @@ def solve(
                     summary.linear_iterations_history.append(iterations)
                     summary.linear_converged_history.append(converged)
                 else:
-                    precondition = (
-                        HipBlockJacobi(problem.layout, diagonal, damping, defer_check=True)
-                        if conjugate_gradient.preconditioner is not None else None
-                    )
+                    if conjugate_gradient.preconditioner is None:
+                        precondition = None
+                    elif joint_plan is not None:
+                        precondition = joint_plan.factor(joint_diagonal, damping, defer_check=True)
+                    else:
+                        precondition = HipBlockJacobi(problem.layout, diagonal, damping, defer_check=True)
                     with region("linear_solve"):
                         result = pcg(partial(normal_operator.normal_matvec, damping=damping),
                                      (-linear_gradient).to(mv_dtype), preconditioner=precondition,
@@ -475,6 +495,7 @@ def solve(
                             damping = max(trust_region.lambda_min, damping / trust_region.lambda_factor)
                     residual = jacobian = original_jacobian = None
                     normal_operator = schur_linear = system = diagonal = gradient = linear_gradient = None
+                    joint_diagonal = None
                     lin = lin_next
                     head = (g_next, stats_host[0] == 0, stats_host[1], stats_host[2])
                     summary.gradient_norm = head[2]

@@ -56,6 +56,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--matvec-only", action="store_true",
                    help="only time the 5M-observation normal matvec (for counter profiles) and exit")
+    p.add_argument("--lm-large-only", action="store_true",
+                   help="only run the 5M-observation LM leg (for kernel-trace profiles) and exit")
     p.add_argument("--repeats", type=int, default=3, help="timed solves per precision (median reported)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a 1 GiB device copy 20 times (HBM counter calibration) and exit")
@@ -487,6 +489,9 @@ def main():
         sys.exit(2)
     if args.matvec_only:
         print(json.dumps(large_normal_matvec(torch.device("cuda:0"), args.kernel_reps)), flush=True)
+        return
+    if args.lm_large_only:
+        print(json.dumps(lm_large(torch.device("cuda:0"), 3)), flush=True)
         return
     from torchls_amd import generators
 

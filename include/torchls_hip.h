@@ -471,7 +471,8 @@ int tlh_cholesky_solve_blocks_f64_noalloc(const double* L, int64_t n, int64_t ld
 /* One reprojection residual batch r_i = pi(R_c p + t_c) - obs_i (pi(x,y,z) = (x/z, y/z);
  * poses world-to-camera, rows [R | t], cam_stride 12 (3x4) or 16 (4x4) elements per
  * pose; tangent [v; w] applied on the right) linearized in `dtype` (TLH_F32/TLH_F64;
- * cams, pts, obs and all outputs in that dtype):
+ * cams, pts, obs, raw and model in that dtype; jc and jp in block_dtype, which is dtype
+ * or, for float64 parameters, TLH_F32: float32 Jacobian storage with float64 residuals):
  *   raw[i]  = r_i (may be NULL),  model[i] = sqrt(w_i rho'_i) r_i,
  *   jc[i]   = sqrt(w_i rho'_i) [D R, -D R [p]x]  ([count,2,6]),
  *   jp[i]   = sqrt(w_i rho'_i) D R               ([count,2,3]),
@@ -482,7 +483,7 @@ int tlh_cholesky_solve_blocks_f64_noalloc(const double* L, int64_t n, int64_t ld
  * zeroed uint32 (left zeroed).
  * Replaces, for this residual: AnalyzedLeastSquaresProblem.linearize's vmapped
  * jacrev/jacfwd and loss rescaling (linearization.py:62-128, losses.py:56-90). */
-int tlh_reprojection_linearize(int dtype, const void* cams, int64_t cam_stride, const void* pts,
+int tlh_reprojection_linearize(int dtype, int block_dtype, const void* cams, int64_t cam_stride, const void* pts,
                                const void* obs, const int64_t* cam_idx, const int64_t* pt_idx,
                                int64_t count, double huber_delta, const double* weights, void* raw,
                                void* model, void* jc, void* jp, double* out2, double* partials,

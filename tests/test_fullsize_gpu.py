@@ -206,8 +206,10 @@ def test_pose_graph_10k(device):
         assert math.isclose(cost, s.final_cost, rel_tol=1e-9)
         out[mv] = s
     s64, s32 = out[None], out[torch.float32]
-    # float32 PCG vectors on this system (condition ~1e8) need more iterations (451,
-    # 2,208, 3,000 against 418, 998, 2,371) and leave slightly different steps: the
-    # costs agree to 1e-11 after the first iteration and to about 1.1e-6 after two
-    assert math.isclose(s32.cost_history[1], s64.cost_history[1], rel_tol=1e-9)
+    # mixed precision stores the blocks in float32 (the gradient is summed in float64 from
+    # them: 6e-8 relative after the first step) and its float32 PCG vectors on this system
+    # (condition ~1e8) need more iterations and leave slightly different steps
+    print(f"  cost rel diff per iteration {[abs(a - b) / b for a, b in zip(s32.cost_history, s64.cost_history)]}",
+          flush=True)
+    assert math.isclose(s32.cost_history[1], s64.cost_history[1], rel_tol=2e-7)
     assert np.allclose(s32.cost_history, s64.cost_history, rtol=2e-6, atol=0)

@@ -260,7 +260,7 @@ SIGNATURES = {
     "tlh_cholesky_solve_blocks_f64_noalloc": (ctypes.c_int, [c_void_p, i64, i64, c_void_p, c_void_p, c_void_p]),
     "tlh_reprojection_linearize": (
         ctypes.c_int,
-        [ctypes.c_int, c_void_p, i64, c_void_p, c_void_p, c_void_p, c_void_p, i64, ctypes.c_double, c_void_p,
+        [ctypes.c_int, ctypes.c_int, c_void_p, i64, c_void_p, c_void_p, c_void_p, c_void_p, i64, ctypes.c_double, c_void_p,
          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     ),
     "tlh_reprojection_trial": (

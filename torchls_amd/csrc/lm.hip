@@ -81,11 +81,14 @@ __global__ void __launch_bounds__(TLH_THREADS) rmatvec_tile_kernel(RmatvecArgs a
 // observation's v row and block rows are read as 16-byte vectors in one burst;
 // both slots' J^T v are formed once, then staged slot by slot for the same
 // segment sums as rmatvec_tile_kernel.
-template <int R, int D0, int D1>
+// BT: block storage type (float64, or float32 blocks of a mixed-precision
+// linearization); v and the sums stay float64.
+template <typename BT, int R, int D0, int D1>
 __global__ void __launch_bounds__(TLH_THREADS) rmatvec_pair_kernel(RmatvecArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* stage = reinterpret_cast<double*>(smem_raw);
   typedef double v2d __attribute__((ext_vector_type(2)));
+  typedef BT v2b __attribute__((ext_vector_type(2)));
   constexpr int K0 = R * D0, K1 = R * D1, DM = D0 > D1 ? D0 : D1;
   static_assert(K0 % 2 == 0 && K1 % 2 == 0 && R % 2 == 0, "16-byte rows");
   const int t = blockIdx.x;
@@ -111,21 +114,21 @@ __global__ void __launch_bounds__(TLH_THREADS) rmatvec_pair_kernel(RmatvecArgs a
 #pragma unroll
     for (int k = 0; k < K1; ++k) j1[k] = 0.0;
     if (a.blocks[0]) {
-      const v2d* p0 = reinterpret_cast<const v2d*>(static_cast<const double*>(a.blocks[0]) + o * K0);
+      const v2b* p0 = reinterpret_cast<const v2b*>(static_cast<const BT*>(a.blocks[0]) + o * K0);
 #pragma unroll
       for (int k = 0; k < K0 / 2; ++k) {
-        const v2d v = p0[k];
-        j0[2 * k] = v[0];
-        j0[2 * k + 1] = v[1];
+        const v2b v = p0[k];
+        j0[2 * k] = (double)v[0];
+        j0[2 * k + 1] = (double)v[1];
       }
     }
     if (a.blocks[1]) {
-      const v2d* p1 = reinterpret_cast<const v2d*>(static_cast<const double*>(a.blocks[1]) + o * K1);
+      const v2b* p1 = reinterpret_cast<const v2b*>(static_cast<const BT*>(a.blocks[1]) + o * K1);
 #pragma unroll
       for (int k = 0; k < K1 / 2; ++k) {
-        const v2d v = p1[k];
-        j1[2 * k] = v[0];
-        j1[2 * k + 1] = v[1];
+        const v2b v = p1[k];
+        j1[2 * k] = (double)v[0];
+        j1[2 * k + 1] = (double)v[1];
       }
     }
 #pragma unroll
@@ -390,20 +393,26 @@ extern "C" int tlh_rmatvec_f64(const tlh_operator_t* op, int group, const void* 
   a.v_dtype = v_dtype;
   const int64_t lds = (int64_t)g.tile_cap * dmax * 8;
   if (lds > lds_limit) return TLH_ELDS;
-  const bool pair = g.nslots == 2 && g.rows == 2 && block_dtype == TLH_F64 && v_dtype == TLH_F64 &&
-                    g.tile_cap <= 4 * TLH_THREADS && ((uintptr_t)v & 15) == 0 &&
-                    ((uintptr_t)blocks[0] & 15) == 0 && ((uintptr_t)blocks[1] & 15) == 0;
-  if (pair && a.dims[0] == 6 && a.dims[1] == 3) {
-    TLH_RETURN_IF_ERROR(allow_lds((const void*)rmatvec_pair_kernel<2, 6, 3>, lds));
-    hipLaunchKernelGGL((rmatvec_pair_kernel<2, 6, 3>), dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds,
-                       reinterpret_cast<hipStream_t>(stream), a);
-    TLH_RETURN_IF_LAUNCH_FAILED();
-    return 0;
-  }
-  if (pair && a.dims[0] == 3 && a.dims[1] == 6) {
-    TLH_RETURN_IF_ERROR(allow_lds((const void*)rmatvec_pair_kernel<2, 3, 6>, lds));
-    hipLaunchKernelGGL((rmatvec_pair_kernel<2, 3, 6>), dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds,
-                       reinterpret_cast<hipStream_t>(stream), a);
+  const uintptr_t bal = block_dtype == TLH_F64 ? 15 : 7;  // 2-element vector loads of the blocks
+  const bool pair = g.nslots == 2 && g.rows == 2 && (block_dtype == TLH_F64 || block_dtype == TLH_F32) &&
+                    v_dtype == TLH_F64 && g.tile_cap <= 4 * TLH_THREADS && ((uintptr_t)v & 15) == 0 &&
+                    ((uintptr_t)blocks[0] & bal) == 0 && ((uintptr_t)blocks[1] & bal) == 0;
+  if (pair && ((a.dims[0] == 6 && a.dims[1] == 3) || (a.dims[0] == 3 && a.dims[1] == 6))) {
+    const bool f64 = block_dtype == TLH_F64, c0 = a.dims[0] == 6;
+    const void* kernel = f64 ? (c0 ? (const void*)rmatvec_pair_kernel<double, 2, 6, 3>
+                                   : (const void*)rmatvec_pair_kernel<double, 2, 3, 6>)
+                             : (c0 ? (const void*)rmatvec_pair_kernel<float, 2, 6, 3>
+                                   : (const void*)rmatvec_pair_kernel<float, 2, 3, 6>);
+    TLH_RETURN_IF_ERROR(allow_lds(kernel, lds));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (f64 && c0)
+      hipLaunchKernelGGL((rmatvec_pair_kernel<double, 2, 6, 3>), dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds, s, a);
+    else if (f64)
+      hipLaunchKernelGGL((rmatvec_pair_kernel<double, 2, 3, 6>), dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds, s, a);
+    else if (c0)
+      hipLaunchKernelGGL((rmatvec_pair_kernel<float, 2, 6, 3>), dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds, s, a);
+    else
+      hipLaunchKernelGGL((rmatvec_pair_kernel<float, 2, 3, 6>), dim3(g.ntiles), dim3(TLH_THREADS), (size_t)lds, s, a);
     TLH_RETURN_IF_LAUNCH_FAILED();
     return 0;
   }

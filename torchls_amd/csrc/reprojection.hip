@@ -69,13 +69,15 @@ __device__ __forceinline__ double loss_value(double s, double delta, double& sqr
 // instead of one strided row per lane.  Padded index: one extra word every 32.
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
 
-template <typename Real>
+// BReal: storage type of the blocks (the parameters' type, or float32 blocks under
+// float64 parameters: mixed precision keeps the residuals and their sums float64).
+template <typename Real, typename BReal>
 __global__ void __launch_bounds__(TLH_THREADS)
-    linearize_kernel(ReprojArgs<Real> a, Real* __restrict__ raw, Real* __restrict__ model, Real* __restrict__ jc,
-                     Real* __restrict__ jp, double* out, double* partials, uint32_t* counter) {
+    linearize_kernel(ReprojArgs<Real> a, Real* __restrict__ raw, Real* __restrict__ model, BReal* __restrict__ jc,
+                     BReal* __restrict__ jp, double* out, double* partials, uint32_t* counter) {
   __shared__ double smem[2 * TLH_THREADS];
   constexpr int NJ = TLH_THREADS * 18;
-  __shared__ Real sj[NJ + NJ / 32];
+  __shared__ BReal sj[NJ + NJ / 32];
   double v[2] = {0.0, 0.0};
   for (int64_t base = (int64_t)blockIdx.x * TLH_THREADS; base < a.count; base += (int64_t)gridDim.x * TLH_THREADS) {
     const int rows = (int)min((int64_t)TLH_THREADS, a.count - base);
@@ -124,9 +126,9 @@ __global__ void __launch_bounds__(TLH_THREADS)
 #pragma unroll
       for (int r = 0; r < 2; ++r) {
 #pragma unroll
-        for (int j = 0; j < 6; ++j) ok = ok && isfinite(c[r][j]);
+        for (int j = 0; j < 6; ++j) ok = ok && isfinite((BReal)c[r][j]);
 #pragma unroll
-        for (int j = 0; j < 3; ++j) ok = ok && isfinite(e[r][j]);
+        for (int j = 0; j < 3; ++j) ok = ok && isfinite((BReal)e[r][j]);
       }
       v[1] += ok ? 0.0 : 1.0;
       if (raw) {
@@ -139,9 +141,9 @@ __global__ void __launch_bounds__(TLH_THREADS)
 #pragma unroll
       for (int r = 0; r < 2; ++r) {
 #pragma unroll
-        for (int j = 0; j < 6; ++j) sj[lds_pad(threadIdx.x * 12 + r * 6 + j)] = c[r][j];
+        for (int j = 0; j < 6; ++j) sj[lds_pad(threadIdx.x * 12 + r * 6 + j)] = (BReal)c[r][j];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) sj[lds_pad(12 * TLH_THREADS + threadIdx.x * 6 + r * 3 + j)] = e[r][j];
+        for (int j = 0; j < 3; ++j) sj[lds_pad(12 * TLH_THREADS + threadIdx.x * 6 + r * 3 + j)] = (BReal)e[r][j];
       }
     }
     __syncthreads();
@@ -213,23 +215,30 @@ bool valid(int dtype, const void* cams, int64_t cam_stride, const void* pts, con
 
 }  // namespace
 
-extern "C" int tlh_reprojection_linearize(int dtype, const void* cams, int64_t cam_stride, const void* pts,
-                                          const void* obs, const int64_t* cam_idx, const int64_t* pt_idx,
-                                          int64_t count, double huber_delta, const double* weights, void* raw,
-                                          void* model, void* jc, void* jp, double* out2, double* partials,
-                                          uint32_t* counter, tlh_stream_t stream) {
+extern "C" int tlh_reprojection_linearize(int dtype, int block_dtype, const void* cams, int64_t cam_stride,
+                                          const void* pts, const void* obs, const int64_t* cam_idx,
+                                          const int64_t* pt_idx, int64_t count, double huber_delta,
+                                          const double* weights, void* raw, void* model, void* jc, void* jp,
+                                          double* out2, double* partials, uint32_t* counter, tlh_stream_t stream) {
   if (!valid(dtype, cams, cam_stride, pts, obs, cam_idx, pt_idx, count) || !model || !jc || !jp || !out2 ||
       !partials || !counter)
     return TLH_EINVAL;
+  // float64 parameters may store float32 blocks; float32 parameters store float32 blocks
+  if (!(block_dtype == dtype || (dtype == TLH_F64 && block_dtype == TLH_F32))) return TLH_EINVAL;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const dim3 grid(red_grid(count)), block(TLH_THREADS);
-  if (dtype == TLH_F64)
-    hipLaunchKernelGGL(linearize_kernel<double>, grid, block, 0, s,
+  if (dtype == TLH_F64 && block_dtype == TLH_F64)
+    hipLaunchKernelGGL((linearize_kernel<double, double>), grid, block, 0, s,
                        make_args<double>(cams, cam_stride, pts, obs, cam_idx, pt_idx, count, huber_delta, weights),
                        static_cast<double*>(raw), static_cast<double*>(model), static_cast<double*>(jc),
                        static_cast<double*>(jp), out2, partials, counter);
+  else if (dtype == TLH_F64)
+    hipLaunchKernelGGL((linearize_kernel<double, float>), grid, block, 0, s,
+                       make_args<double>(cams, cam_stride, pts, obs, cam_idx, pt_idx, count, huber_delta, weights),
+                       static_cast<double*>(raw), static_cast<double*>(model), static_cast<float*>(jc),
+                       static_cast<float*>(jp), out2, partials, counter);
   else
-    hipLaunchKernelGGL(linearize_kernel<float>, grid, block, 0, s,
+    hipLaunchKernelGGL((linearize_kernel<float, float>), grid, block, 0, s,
                        make_args<float>(cams, cam_stride, pts, obs, cam_idx, pt_idx, count, huber_delta, weights),
                        static_cast<float*>(raw), static_cast<float*>(model), static_cast<float*>(jc),
                        static_cast<float*>(jp), out2, partials, counter);

@@ -74,8 +74,10 @@ class LinearizationKernels:
             keep.append(blocks)
             v = residual[offset : offset + count * rows]
             offset += count * rows
+            # the two-slot kernel reads float64 residuals and float64 or float32 blocks (vector loads)
             pair = (len(blocks) == 2 and rows == 2 and v.dtype == torch.float64 and v.data_ptr() % 16 == 0
-                    and all(b.dtype == torch.float64 and b.data_ptr() % 16 == 0 for b in blocks))
+                    and all(b.dtype in (torch.float64, torch.float32) and b.dtype == blocks[0].dtype
+                            and b.data_ptr() % 16 == 0 for b in blocks))
             ptrs = []
             for s, blk in enumerate(blocks):
                 if pair and (gi, s) in slot_of:
@@ -205,21 +207,23 @@ def _reprojection_common(operands, idx, weights):
     return cams, pts, obs, delta, n, wptr
 
 
-def reprojection_linearize(red: Reduction, slot: int, operands, idx, weights, want_raw: bool):
+def reprojection_linearize(red: Reduction, slot: int, operands, idx, weights, want_raw: bool, block_dtype=None):
     """HIP linearization of one reprojection batch (tlh_reprojection_linearize).
 
     Returns (raw residual or None, model residual, (J_c, J_p), objective), the
     objective being the float64 sum w rho(||r||^2) as a 0-d view of
-    ``red.out[slot, 0]``."""
+    ``red.out[slot, 0]``.  ``block_dtype`` (default: the parameters' dtype) may be
+    float32 under float64 parameters: float32 blocks, float64 residuals."""
     cams, pts, obs, delta, n, wptr = _reprojection_common(operands, idx, weights)
     dt, dev = cams.dtype, cams.device
+    bdt = dt if block_dtype is None else block_dtype
     raw = torch.empty((n, 2), dtype=dt, device=dev) if want_raw else None
     model = torch.empty((n, 2), dtype=dt, device=dev)
-    jc = torch.empty((n, 2, 6), dtype=dt, device=dev)
-    jp = torch.empty((n, 2, 3), dtype=dt, device=dev)
+    jc = torch.empty((n, 2, 6), dtype=bdt, device=dev)
+    jp = torch.empty((n, 2, 3), dtype=bdt, device=dev)
     lib = _lib.load()
     _lib.check(lib.tlh_reprojection_linearize(
-        _lib.dtype_code(dt), cams.data_ptr(), cams[0].numel(), pts.data_ptr(), obs.data_ptr(),
+        _lib.dtype_code(dt), _lib.dtype_code(bdt), cams.data_ptr(), cams[0].numel(), pts.data_ptr(), obs.data_ptr(),
         idx[0].data_ptr(), idx[1].data_ptr(), n, delta, wptr, None if raw is None else raw.data_ptr(),
         model.data_ptr(), jc.data_ptr(), jp.data_ptr(), red.out[slot].data_ptr(), red.partials.data_ptr(),
         red.counter.data_ptr(), _lib.stream_handle(dev)), "tlh_reprojection_linearize")

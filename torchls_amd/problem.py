@@ -329,7 +329,10 @@ class Analysis:
         terms = self._objective_terms(values, parts)
         return terms[0], terms[1]
 
-    def linearize(self, values) -> Linearization:
+    def linearize(self, values, *, block_dtype: torch.dtype | None = None) -> Linearization:
+        """Linearize every residual batch.  ``block_dtype=torch.float32`` under float64
+        parameters stores the Jacobian blocks of the HIP reprojection batches in float32
+        (mixed precision: residuals, objective and the gradient sums stay float64)."""
         from .hip_ops import between_linearize, reprojection_linearize
 
         dev, dt = _values_device_dtype(values)
@@ -345,12 +348,15 @@ class Analysis:
                     objectives.append(obj)
                 bad.append(red.out[k, 1])
                 residuals.append(r.reshape(-1))
+                if block_dtype is not None:  # every group of one Jacobian shares the block dtype
+                    blocks = tuple(x.to(block_dtype) for x in blocks)
                 groups.append(JacobianGroup(blocks, cols, b.type_indices, idx))
                 continue
             hip = b.hip_operands(values, self.layout, dev, dt)
             if hip is not None:
                 red = self._reduction(dev)
-                raw, r, blocks, obj = reprojection_linearize(red, k, hip, idx, w, self.has_loss)
+                raw, r, blocks, obj = reprojection_linearize(red, k, hip, idx, w, self.has_loss,
+                                                             block_dtype=block_dtype)
                 if self.has_loss:
                     raws.append(raw.reshape(-1))
                     objectives.append(obj)
@@ -377,6 +383,8 @@ class Analysis:
             residuals.append(r.reshape(-1))
             blocks = tuple(x.contiguous() for x in blocks)
             bad.append(sum((~torch.isfinite(x)).sum() for x in (r, *blocks)).to(torch.float64))
+            if block_dtype is not None:  # every group of one Jacobian shares the block dtype
+                blocks = tuple(x.to(block_dtype) for x in blocks)
             groups.append(JacobianGroup(blocks, cols, b.type_indices, idx))
         # one batch (the common case): its tensors are used as they are, no concatenation copies
         one = len(residuals) == 1

@@ -249,6 +249,10 @@ def solve(
     if device.type != "cuda":
         raise ValueError("the HIP backend solves on a GPU device; move the parameters to cuda")
     mv_dtype = conjugate_gradient.matvec_dtype or dtype
+    # mixed precision: the Jacobian is stored in the matvec dtype as it is linearized
+    # (float32 blocks, float64 residuals); the gradient, diagonal blocks and model
+    # decrease are still summed in float64 from those blocks
+    block_dtype = mv_dtype if mv_dtype != dtype else None
     initial_cost = float(problem.cost(vals))
     if not math.isfinite(initial_cost):
         raise ValueError("Initial residuals are nonfinite")
@@ -309,7 +313,7 @@ def solve(
         return g, bool(host[0] == 0), float(host[1]), float(host[2])
 
     with region("linearize"):
-        lin = problem.linearize(vals)
+        lin = problem.linearize(vals, block_dtype=block_dtype)
     head = None
     pcg_hint = None  # iterations of the previous PCG solve: the next one's first batch
     for iteration in range(termination.max_iterations):
@@ -388,7 +392,7 @@ def solve(
             finite_step = None
             try:
                 if linear_solver == "dense_qr":
-                    dense = jacobian.to_dense()
+                    dense = jacobian.to_dense().to(dtype)
                     if solve_scale is not None:
                         dense = dense * solve_scale.to(dense.dtype)
                     eye = torch.eye(problem.tangent_dim, dtype=dtype, device=device)
@@ -465,7 +469,7 @@ def solve(
                 # The next linearization is issued before the trial is read (used if the step
                 # is accepted, dropped otherwise): its gradient and cost ride on the same read.
                 with region("linearize"):
-                    lin_next = problem.linearize(proposed)
+                    lin_next = problem.linearize(proposed, block_dtype=block_dtype)
                 g_next, stats_next = examine_device(lin_next)
                 host = torch.stack([*parts, *stats_next.unbind()]).cpu().tolist()
                 proposed_cost, actual, dot_gd, sq_jd, step_norm = host[:5]

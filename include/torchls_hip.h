@@ -94,6 +94,9 @@ typedef struct tlh_group {
                                 other slot by on-the-fly segment heads (DENSE accumulator or
                                 partial pos_base + tile_start + head) */
   int32_t primary;           /* slot the observations are sorted by */
+  int32_t out_entries;       /* wave kernel: most primary-slot piece ends of one wavefront over
+                                the tiles of one workgroup (its y / partial writes are staged in
+                                LDS when they fit); 0 = not planned (written as they are found) */
   tlh_slot_t slot[TLH_MAX_SLOTS];
 } tlh_group_t;
 

@@ -69,6 +69,7 @@ class Group(ctypes.Structure):
         ("packed", c_void_p),
         ("kernel", i32),
         ("primary", i32),
+        ("out_entries", i32),
         ("slot", Slot * TLH_MAX_SLOTS),
     ]
 

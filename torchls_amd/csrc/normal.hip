@@ -245,6 +245,7 @@ struct WaveArgs {
   int32_t count_s;
   Real damping;
   const int32_t* rounds;  // occurrence rounds per tile (dense secondary) or NULL
+  int32_t out_cap;        // staged primary outputs per wavefront (staged-output kernel)
 };
 
 // out[k] = base[start + b + k]; a short tile (len < T) clamps to its last position.
@@ -276,7 +277,14 @@ struct WaveIds {
   static constexpr int RW = (ITEMS + 1) / 2;
   int ip[ITEMS], is[ITEMS];
   uint32_t rk[RW];
+  int rounds;  // occurrence rounds of the tile (0 without them)
 };
+
+// Loads of the pipeline are issued on every path: a load skipped on one path, or
+// a generic (flat) load, makes the compiler's later wait counts conservative,
+// down to waiting for every load in flight.  An absent array is therefore
+// replaced, through a select, by another kernel argument array (global memory,
+// always valid) and the loaded value is discarded.
 
 // Raw per-thread data of one tile: ids, ranks, block components and the
 // gathered primary x (and secondary x when it is not held in LDS).
@@ -285,6 +293,7 @@ struct WaveTile {
   static constexpr int RW = (ITEMS + 1) / 2;
   int ip[ITEMS], is[ITEMS];
   uint32_t rk[RW];
+  int rounds;
   Real jp[R][DP][ITEMS], js[R][DS][ITEMS];
   Real xp[ITEMS][DP], xs[ITEMS][XL ? 1 : DS];
   int direct[ITEMS];
@@ -298,13 +307,14 @@ struct WaveTile {
 // bounds branch; positions past the group's count are masked by the caller.
 // Ranks are read as single 16-bit loads (zero-extended, no unpacking) so that
 // nothing forces a wait between the id loads and the block loads.
-template <typename Real, int ITEMS, int NT>
+template <typename Real, int ITEMS, int NT, bool FAST>
 __device__ __forceinline__ void load_wave_ids(const WaveArgs<Real>& a, int t, int b, WaveIds<ITEMS>& d) {
   constexpr int T = NT * ITEMS;
   const int64_t start = (int64_t)t * T;
   load_items<int32_t, ITEMS>(a.idx_p, start, b, T, true, d.ip);
   load_items<int32_t, ITEMS>(a.idx_s, start, b, T, true, d.is);
-  if (a.rank_s) {
+  if constexpr (FAST) {
+    // occurrence ranks are always present; the round counts are read from LDS
     if constexpr (ITEMS == 1) {
       d.rk[0] = a.rank_s[start + b];
     } else {
@@ -312,22 +322,52 @@ __device__ __forceinline__ void load_wave_ids(const WaveArgs<Real>& a, int t, in
 #pragma unroll
       for (int w = 0; w < ITEMS / 2; ++w) d.rk[w] = r32[w];
     }
+    d.rounds = 0;
+    return;
+  }
+  // without ranks (the tile is in secondary order) the loads read the id array
+  // (valid memory of at least twice the size) and the identity ranks are selected
+  const bool ranked = a.rank_s != nullptr;
+  const uint16_t* rank = ranked ? a.rank_s : reinterpret_cast<const uint16_t*>(a.idx_s);
+  if constexpr (ITEMS == 1) {
+    const uint32_t v = rank[start + b];
+    d.rk[0] = ranked ? v : (uint32_t)b;
   } else {
-    if constexpr (ITEMS == 1) {
-      d.rk[0] = (uint32_t)b;
-    } else {
+    const uint32_t* r32 = reinterpret_cast<const uint32_t*>(rank + start + b);
 #pragma unroll
-      for (int w = 0; w < ITEMS / 2; ++w) d.rk[w] = (uint32_t)(b + 2 * w) | ((uint32_t)(b + 2 * w + 1) << 16);
+    for (int w = 0; w < ITEMS / 2; ++w) {
+      const uint32_t v = r32[w];
+      d.rk[w] = ranked ? v : ((uint32_t)(b + 2 * w) | ((uint32_t)(b + 2 * w + 1) << 16));
     }
+  }
+  const int32_t nr = *(a.rounds ? a.rounds + t : a.idx_p);
+  d.rounds = a.rounds ? nr : 0;
+}
+
+// Block loads of the tile starting at position `start` (component stride
+// `stride`): no dependence on anything loaded before them, so they are issued a
+// whole tile ahead of their use.
+template <typename Real, int R, int DP, int DS, int OP, int OS, int ITEMS, int NT, bool STREAM, bool XL>
+__device__ __forceinline__ void load_wave_blocks(const WaveArgs<Real>& a, int64_t start, int64_t stride, int b,
+                                                 WaveTile<Real, R, DP, DS, ITEMS, XL>& d) {
+  constexpr int T = NT * ITEMS;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+#pragma unroll
+    for (int c = 0; c < DP; ++c)
+      load_items<Real, ITEMS, STREAM>(a.packed + (int64_t)(OP + r * DP + c) * stride, start, b, T, true,
+                                      d.jp[r][c]);
+#pragma unroll
+    for (int c = 0; c < DS; ++c)
+      load_items<Real, ITEMS, STREAM>(a.packed + (int64_t)(OS + r * DS + c) * stride, start, b, T, true,
+                                      d.js[r][c]);
   }
 }
 
-// Block loads of tile t and the x gathers addressed by its (already loaded) ids.
-template <typename Real, int R, int DP, int DS, int OP, int OS, int ITEMS, int NT, bool STREAM, bool XL>
-__device__ __forceinline__ void load_wave_tile(const WaveArgs<Real>& a, int t, int b, const WaveIds<ITEMS>& ids,
-                                               WaveTile<Real, R, DP, DS, ITEMS, XL>& d) {
-  constexpr int T = NT * ITEMS;
-  const int64_t start = (int64_t)t * T;
+// Ids of a tile (loaded earlier) and the x gathers they address.
+template <typename Real, int R, int DP, int DS, int ITEMS, bool XL>
+__device__ __forceinline__ void load_wave_gathers(const WaveArgs<Real>& a, const WaveIds<ITEMS>& ids,
+                                                  WaveTile<Real, R, DP, DS, ITEMS, XL>& d) {
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     d.ip[k] = ids.ip[k];
@@ -335,17 +375,9 @@ __device__ __forceinline__ void load_wave_tile(const WaveArgs<Real>& a, int t, i
   }
 #pragma unroll
   for (int w = 0; w < WaveIds<ITEMS>::RW; ++w) d.rk[w] = ids.rk[w];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-#pragma unroll
-    for (int c = 0; c < DP; ++c)
-      load_items<Real, ITEMS, STREAM>(a.packed + (int64_t)(OP + r * DP + c) * a.stride, start, b, T, true,
-                                      d.jp[r][c]);
-#pragma unroll
-    for (int c = 0; c < DS; ++c)
-      load_items<Real, ITEMS, STREAM>(a.packed + (int64_t)(OS + r * DS + c) * a.stride, start, b, T, true,
-                                      d.js[r][c]);
-  }
+  d.rounds = ids.rounds;
+  const bool has_direct = a.direct_p != nullptr;
+  const uint8_t* direct = has_direct ? a.direct_p : reinterpret_cast<const uint8_t*>(a.idx_p);
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
 #pragma unroll
@@ -354,8 +386,27 @@ __device__ __forceinline__ void load_wave_tile(const WaveArgs<Real>& a, int t, i
 #pragma unroll
       for (int c = 0; c < DS; ++c) d.xs[k][c] = a.x_s[(int64_t)ids.is[k] * DS + c];
     }
-    d.direct[k] = a.direct_p ? (int)a.direct_p[ids.ip[k]] : 0;
+    // raw byte (garbage without the array): tested against direct_p where used, so
+    // that no select right after the load forces a wait for it
+    d.direct[k] = direct[has_direct ? ids.ip[k] : 0];
   }
+}
+
+// Empty asm that takes every block component of a tile as a read-write register
+// operand, with a memory clobber: memory accesses before it (the next tile's
+// block loads) are issued first, and arithmetic on the components cannot move
+// above it.
+template <typename Real, int R, int DP, int DS, int ITEMS, bool XL>
+__device__ __forceinline__ void pin_blocks(WaveTile<Real, R, DP, DS, ITEMS, XL>& d) {
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+#pragma unroll
+      for (int c = 0; c < DP; ++c) asm volatile("" : "+v"(d.jp[r][c][k])::"memory");
+#pragma unroll
+      for (int c = 0; c < DS; ++c) asm volatile("" : "+v"(d.js[r][c][k])::"memory");
+    }
 }
 
 // Wave kernel for two-slot groups (bundle adjustment, pose-graph edges).
@@ -382,7 +433,20 @@ __device__ __forceinline__ void load_wave_tile(const WaveArgs<Real>& a, int t, i
 // float32 do) has that slice copied into LDS once per workgroup, so the
 // per-observation secondary gathers are LDS reads rather than scattered L2
 // requests.
-template <typename Real, int R, int D0, int D1, int P, int ITEMS, int NT, bool STREAM, bool XL>
+// FAST (dense secondary with occurrence rounds, primary outputs that fit in
+// LDS): the workgroup's round counts are read into LDS once, and every piece
+// end is staged in its wavefront's LDS region (value and target) and written
+// to y or the partials after the last tile.  The tile loop then issues no
+// global store, so the compiler's load waits keep the next tile's block loads
+// in flight (a pending store makes them wait for every load).  out_cap, the
+// most piece ends of one wavefront over the workgroup's tiles, comes from the
+// plan; a wavefront that would exceed it writes that tile's ends directly.
+template <int DP, typename Real>
+__host__ __device__ constexpr int64_t out_region_bytes(int cap) {
+  return ((int64_t)cap * (DP * (int64_t)sizeof(Real) + 4) + 15) / 16 * 16;
+}
+
+template <typename Real, int R, int D0, int D1, int P, int ITEMS, int NT, bool STREAM, bool XL, bool FAST>
 __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
   constexpr int DP = P == 0 ? D0 : D1;
   constexpr int DS = P == 0 ? D1 : D0;
@@ -391,26 +455,50 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
   constexpr int T = NT * ITEMS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // LDS: [stage: T contributions + T ids (staged reduction only)] [dense accumulator] [XL: x slice]
+  //      FAST: [dense accumulator] [XL: x slice] [round counts] [per-wavefront output regions]
   Real* cstage = reinterpret_cast<Real*>(smem_raw);
   int32_t* cid = reinterpret_cast<int32_t*>(cstage + T * DS);
-  Real* acc = a.rounds ? reinterpret_cast<Real*>(smem_raw) : reinterpret_cast<Real*>(cid + T);
+  Real* acc = (FAST || a.rounds) ? reinterpret_cast<Real*>(smem_raw) : reinterpret_cast<Real*>(cid + T);
   Real* xl = acc + (int64_t)a.count_s * DS;
-  const bool dense = a.mode_s == TLH_REDUCE_DENSE;
+  const bool dense = FAST || a.mode_s == TLH_REDUCE_DENSE;
   if (dense) {
     for (int i = threadIdx.x; i < a.count_s * DS; i += NT) acc[i] = Real(0);
-    __syncthreads();  // the first occurrence round may add to any entry
+    if constexpr (!FAST) __syncthreads();  // the first occurrence round may add to any entry
   }
   const int lane = threadIdx.x & 63;
   const int b = ITEMS * (int)threadIdx.x;  // first tile-local position of this thread
   const int t0 = blockIdx.x * a.tiles_per_wg;
   const int t1 = min(t0 + a.tiles_per_wg, a.ntiles);
+  WaveIds<ITEMS> idn;  // ids of the tile after the one being processed
+  int32_t* nrounds = reinterpret_cast<int32_t*>(
+      smem_raw + (((int64_t)a.count_s * DS * (XL ? 2 : 1) * (int64_t)sizeof(Real) + 15) / 16) * 16);
+  unsigned char* out_base =
+      reinterpret_cast<unsigned char*>(nrounds) + ((int64_t)a.tiles_per_wg * 4 + 15) / 16 * 16;
+  Real* out_val = reinterpret_cast<Real*>(out_base + (threadIdx.x >> 6) * out_region_bytes<DP, Real>(a.out_cap));
+  int32_t* out_tgt = reinterpret_cast<int32_t*>(out_val + (int64_t)a.out_cap * DP);
+  int out_n = 0;  // piece ends staged by this wavefront (uniform over its lanes)
+  if constexpr (FAST) {
+    for (int i = threadIdx.x; i < t1 - t0; i += NT) nrounds[i] = a.rounds[t0 + i];
+  }
 
-  // One tile: gathers and contributions from `cur`, then `prefetch` (which issues
-  // the following tile's loads into the other register buffer), then the
-  // primary scan and the secondary LDS reduction.
-  auto process = [&](const WaveTile<Real, R, DP, DS, ITEMS, XL>& cur, int t, auto&& prefetch) {
+  // One tile: the next tile's block loads into the other register buffer, then
+  // this tile's contributions from `cur`, the next tile's gathers and the ids of
+  // the tile after it, then the primary scan and the secondary LDS reduction.
+  // Two tiles of block loads are thus in flight while a tile's data is awaited.
+  auto process = [&](WaveTile<Real, R, DP, DS, ITEMS, XL>& cur, WaveTile<Real, R, DP, DS, ITEMS, XL>& nxt,
+                     int t) {
+    // The loads are unconditional so that the compiler's wait counts stay exact
+    // (a skipped load on one path makes every later wait conservative): after
+    // the workgroup's last tile they re-read the first positions of component 0
+    // (stride 0), and the ids of a tile past it are those of the group's last tile.
+    const bool more = t + 1 < t1;
+    load_wave_blocks<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, more ? (int64_t)(t + 1) * T : 0,
+                                                                      more ? a.stride : 0, b, nxt);
+    pin_blocks(cur);
     const int64_t start = (int64_t)t * T;
-    const int len = (int)min((int64_t)T, a.count - start);
+    // t == t1 is the padding half of the last pair (an odd tile count): it
+    // computes on stand-in data, writes nothing and takes no rounds
+    const int len = t < t1 ? (int)min((int64_t)T, a.count - start) : 0;
     const int left = len - b;  // items of this thread inside the tile (may be <= 0)
     int ip[ITEMS], is[ITEMS], direct[ITEMS];
     Real xp[ITEMS][DP], xs[ITEMS][DS];
@@ -456,12 +544,13 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
         cs[k][c] = v;
       }
     }
-    // The next tile's loads overlap this tile's reductions.  They must be issued
-    // after every use of this tile's blocks: vmcnt waits are in issue order, so a
-    // block use sunk below them would wait for the whole prefetch.
+    // The next tile's gathers wait for its ids, which were issued after this
+    // tile's blocks (vmcnt waits are in issue order): they follow every use of
+    // this tile's blocks, and the ids of tile t+2 follow them.
     pin_before_loads(cp);
     pin_before_loads(cs);
-    prefetch();
+    load_wave_gathers<Real, R, DP, DS, ITEMS, XL>(a, idn, nxt);
+    load_wave_ids<Real, ITEMS, NT, FAST>(a, min(t + 2, a.ntiles - 1), b, idn);
     // primary slot: runs inside the thread, then a segmented scan over lanes
 #pragma unroll
     for (int k = 1; k < ITEMS; ++k)
@@ -492,12 +581,41 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
     for (int k = 0; k < ITEMS; ++k) {
       if (k > 0 && key[k] != key[k - 1]) first_run = false;
       const bool tail = k < left && (k + 1 < ITEMS ? key[k + 1] != key[k] : (lane == 63 || next_first != key[k]));
-      if (tail) {
+      const bool dk = a.direct_p != nullptr && direct[k] != 0;  // read only at piece ends
+      if constexpr (FAST) {
+        const uint64_t ends = __ballot(tail);
+        const int count = __popcll(ends);
+        if (out_n + count <= a.out_cap) {
+          if (tail) {
+            const int slot = out_n + __popcll(ends & ((1ull << lane) - 1ull));
+#pragma unroll
+            for (int c = 0; c < DP; ++c) {
+              const Real sc = first_run ? cp[k][c] + carry[c] : cp[k][c];
+              out_val[slot * DP + c] = dk ? sc + a.damping * xp[k][c] : sc;
+            }
+            out_tgt[slot] = dk ? -(ip[k] + 1) : (int32_t)(a.pos_base_p + start + b + k);
+          }
+          out_n += count;
+        } else {
+          // beyond the planned capacity (a plan that does not match the group):
+          // direct writes, then a wait for every memory operation so that no
+          // store stays pending in the tile loop
+          if (tail) {
+            Real* out = dk ? a.y_p + (int64_t)ip[k] * DP : a.part_p + (a.pos_base_p + start + b + k) * DP;
+#pragma unroll
+            for (int c = 0; c < DP; ++c) {
+              const Real sc = first_run ? cp[k][c] + carry[c] : cp[k][c];
+              out[c] = dk ? sc + a.damping * xp[k][c] : sc;
+            }
+          }
+          __builtin_amdgcn_s_waitcnt(0);
+        }
+      } else if (tail) {
         Real s[DP];
 #pragma unroll
         for (int c = 0; c < DP; ++c) s[c] = first_run ? cp[k][c] + carry[c] : cp[k][c];
         const int64_t vv = ip[k];
-        if (direct[k]) {
+        if (dk) {
 #pragma unroll
           for (int c = 0; c < DP; ++c) a.y_p[vv * DP + c] = s[c] + a.damping * xp[k][c];
         } else {
@@ -507,12 +625,12 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
         }
       }
     }
-    if (a.rounds) {
+    if (FAST || a.rounds) {
       // secondary slot, occurrence rounds: round r adds every r-th occurrence of a
       // variable in this tile, so each accumulator entry has one writer per round
       // (a plain read-modify-write; LDS float atomics measured twice as slow) and
       // receives the tile's contributions in position order
-      const int nr = a.rounds[t];
+      const int nr = t < t1 ? (FAST ? nrounds[t - t0] : cur.rounds) : 0;
       for (int r = 0; r < nr; ++r) {
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
@@ -558,34 +676,34 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
     }
     __syncthreads();
   };
-  // Two register buffers used alternately (no copies between them), so the loads
-  // of tile t+1 are in flight while tile t is reduced through LDS; the ids of
-  // tile t+2 are loaded behind them.
-  WaveIds<ITEMS> idn;
+  // Two register buffers used alternately (no copies between them): the block
+  // loads of tile t+1 are issued before tile t's data is awaited.
   WaveTile<Real, R, DP, DS, ITEMS, XL> ta, tb;
   if (t0 < t1) {
-    load_wave_ids<Real, ITEMS, NT>(a, t0, b, idn);
-    load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, t0, b, idn, ta);
-    if (t0 + 1 < t1) load_wave_ids<Real, ITEMS, NT>(a, t0 + 1, b, idn);
+    load_wave_ids<Real, ITEMS, NT, FAST>(a, t0, b, idn);
+    load_wave_blocks<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, (int64_t)t0 * T, a.stride, b, ta);
+    load_wave_gathers<Real, R, DP, DS, ITEMS, XL>(a, idn, ta);
+    load_wave_ids<Real, ITEMS, NT, FAST>(a, min(t0 + 1, a.ntiles - 1), b, idn);
   }
   if constexpr (XL) {
     for (int i = threadIdx.x; i < a.count_s * DS; i += NT) xl[i] = a.x_s[i];
-    __syncthreads();
   }
+  if constexpr (XL || FAST) __syncthreads();
+  // pairs of tiles with no branch between the halves (a conditional half would
+  // move loop-carried registers and force waits for loads in flight)
   for (int t = t0; t < t1; t += 2) {
-    process(ta, t, [&] {
-      if (t + 1 < t1) {
-        load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, t + 1, b, idn, tb);
-        if (t + 2 < t1) load_wave_ids<Real, ITEMS, NT>(a, t + 2, b, idn);
-      }
-    });
-    if (t + 1 < t1)
-      process(tb, t + 1, [&] {
-        if (t + 2 < t1) {
-          load_wave_tile<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, t + 2, b, idn, ta);
-          if (t + 3 < t1) load_wave_ids<Real, ITEMS, NT>(a, t + 3, b, idn);
-        }
-      });
+    process(ta, tb, t);
+    process(tb, ta, t + 1);
+  }
+  if constexpr (FAST) {
+    // the staged piece ends of this wavefront (the rounds' barriers have made
+    // every lane's LDS writes visible)
+    for (int e = lane; e < out_n; e += 64) {
+      const int32_t tg = out_tgt[e];
+      Real* out = tg < 0 ? a.y_p + (int64_t)(-(tg + 1)) * DP : a.part_p + (int64_t)tg * DP;
+#pragma unroll
+      for (int c = 0; c < DP; ++c) out[c] = out_val[e * DP + c];
+    }
   }
   if (dense) {
     const int64_t nn = (int64_t)a.count_s * DS;
@@ -924,22 +1042,51 @@ int launch_generic(const tlh_operator_t* op, const tlh_group_t& g, const void* x
   return 0;
 }
 
-// LDS bytes of a wave group: stage (contribution + id per position), the dense
-// secondary accumulator and, when all of that fits, the secondary x slice.
+// LDS of a wave group.  Staged reduction: stage (contribution + id per
+// position), the dense secondary accumulator and, when all of that fits, the
+// secondary x slice.  Occurrence rounds with a planned output capacity (the
+// FAST kernel): accumulator, x slice when it fits, the workgroup's round counts
+// and one output region per wavefront.
+struct WaveLds {
+  int64_t bytes;
+  bool xl;
+  bool fast;
+};
+
+inline int64_t align16(int64_t b) { return (b + 15) / 16 * 16; }
+
 template <typename Real>
-int64_t wave_lds(const tlh_group_t& g, const tlh_type_t& ts, int64_t lds_max, bool& xl) {
-  const int ds = g.slot[1 - g.primary].dim;
+WaveLds wave_lds(const tlh_group_t& g, const tlh_type_t& ts, int64_t lds_max) {
   const tlh_slot_t& ss = g.slot[1 - g.primary];
-  const bool rounds = ss.mode == TLH_REDUCE_DENSE && ss.rounds;
+  const int ds = ss.dim;
+  const int dp = g.slot[g.primary].dim;
+  const bool dense = ss.mode == TLH_REDUCE_DENSE;
+  const bool rounds = dense && ss.rounds;
+  const int64_t slice = dense ? ts.count * ds * (int64_t)sizeof(Real) : 0;
+  // positional partial ids of staged outputs are int32
+  const bool ids_fit = g.slot[g.primary].pos_base + (int64_t)g.ntiles * g.tile_cap <= INT32_MAX;
+  if (rounds && ss.rank && g.out_entries > 0 && ids_fit) {
+    const int64_t rounds_b = align16((int64_t)g.tiles_per_wg * 4);
+    const int64_t out_b = (TLH_WAVE_THREADS / 64) * align16((int64_t)g.out_entries * (dp * (int64_t)sizeof(Real) + 4));
+    for (int x = 1; x >= 0; --x) {
+      const int64_t b = align16(slice * (x ? 2 : 1)) + rounds_b + out_b;
+      if (b <= lds_max) return {b, x == 1, true};
+    }
+  }
   int64_t b = rounds ? 0 : (int64_t)g.tile_cap * (ds * (int64_t)sizeof(Real) + 4);
-  xl = false;
-  if (g.slot[1 - g.primary].mode == TLH_REDUCE_DENSE) {
-    const int64_t slice = ts.count * ds * (int64_t)sizeof(Real);
+  bool xl = false;
+  if (dense) {
     b += slice;
     xl = b + slice <= lds_max;
     if (xl) b += slice;
   }
-  return b;
+  return {b, xl, false};
+}
+
+template <typename Real, int R, int D0, int D1, int P, int ITEMS, int NT, bool STREAM, bool XL>
+auto wave_kernel_for(bool fast) {
+  return fast ? normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, STREAM, XL, true>
+              : normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, STREAM, XL, false>;
 }
 
 template <typename Real, int R, int D0, int D1, int P>
@@ -978,16 +1125,16 @@ int launch_wave_t(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
   a.count_s = (int32_t)ts.count;
   a.damping = (Real)damping;
   a.rounds = ss.mode == TLH_REDUCE_DENSE ? ss.rounds : nullptr;
-  bool xl = false;
-  const int64_t lds = wave_lds<Real>(g, ts, device_lds_max(), xl);
+  const WaveLds lds = wave_lds<Real>(g, ts, device_lds_max());
+  a.out_cap = lds.fast ? g.out_entries : 0;
   // blocks that cannot stay in the 256 MB last-level cache between matvecs are streamed
   const bool stream_blocks = (int64_t)g.stride * g.rows * (D0 + D1) * (int64_t)sizeof(Real) > (192ll << 20);
-  auto kernel = stream_blocks ? (xl ? normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, true, true>
-                                    : normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, true, false>)
-                              : (xl ? normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, false, true>
-                                    : normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, false, false>);
-  TLH_RETURN_IF_ERROR(allow_lds((const void*)kernel, lds));
-  hipLaunchKernelGGL(kernel, dim3(g.nwg), dim3(NT), (size_t)lds, stream, a);
+  auto kernel = stream_blocks ? (lds.xl ? wave_kernel_for<Real, R, D0, D1, P, ITEMS, NT, true, true>(lds.fast)
+                                        : wave_kernel_for<Real, R, D0, D1, P, ITEMS, NT, true, false>(lds.fast))
+                              : (lds.xl ? wave_kernel_for<Real, R, D0, D1, P, ITEMS, NT, false, true>(lds.fast)
+                                        : wave_kernel_for<Real, R, D0, D1, P, ITEMS, NT, false, false>(lds.fast));
+  TLH_RETURN_IF_ERROR(allow_lds((const void*)kernel, lds.bytes));
+  hipLaunchKernelGGL(kernel, dim3(g.nwg), dim3(NT), (size_t)lds.bytes, stream, a);
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }
@@ -1180,10 +1327,9 @@ extern "C" int64_t tlh_group_lds_bytes(const tlh_operator_t* op, int group) {
   if (check_op(op) || group < 0 || group >= op->ngroups) return -1;
   const tlh_group_t& g = op->groups[group];
   if (g.kernel == 1) {
-    bool xl = false;
     const tlh_type_t& ts = op->types[g.slot[1 - g.primary].type];
-    return op->dtype == TLH_F32 ? wave_lds<float>(g, ts, device_lds_max(), xl)
-                                : wave_lds<double>(g, ts, device_lds_max(), xl);
+    return op->dtype == TLH_F32 ? wave_lds<float>(g, ts, device_lds_max()).bytes
+                                : wave_lds<double>(g, ts, device_lds_max()).bytes;
   }
   int dmax = 0;
   for (int s = 0; s < g.nslots; ++s) dmax = std::max(dmax, (int)g.slot[s].dim);

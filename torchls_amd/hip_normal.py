@@ -98,6 +98,7 @@ class DeviceStructure:
             d.packed = None
             d.kernel = g.kernel
             d.primary = g.primary
+            d.out_entries = g.out_entries
             for si, s in enumerate(g.slots):
                 ds = d.slot[si]
                 ds.type = s.type

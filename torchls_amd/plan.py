@@ -87,6 +87,9 @@ class GroupPlan:
     tiles_per_wg: int
     slots: list
     kernel: int = 0  # 0: segment-array kernel, 1: wave kernel (2 slots, specialised shape)
+    # wave kernel: most primary-slot piece ends (y / partial writes) of one wavefront
+    # over the tiles of one workgroup, the capacity of its LDS output region
+    out_entries: int = 0
 
     @property
     def ntiles(self) -> int:
@@ -363,9 +366,17 @@ def build_structure(
             nwg = -(-ntiles // per)
         else:
             nwg, per = ntiles, 1
+        out_entries = 0
+        if wave and ntiles:
+            ps = slots[primary]
+            span = WAVE * wave_items(esize)
+            pieces = (ps.seg_ptr[1:] - ps.seg_ptr[:-1]).to(torch.int64)
+            tile_of_piece = torch.repeat_interleave(_arange(ntiles, device), pieces)
+            key = (tile_of_piece // per) * (WAVE_THREADS // WAVE) + ps.seg_start.to(torch.int64) // span
+            out_entries = int(torch.bincount(key).max()) if key.numel() else 0
         gplans.append(
             GroupPlan(count, int(rows), primary, order, tile_start.to(torch.int32), gcap, nwg, per, slots,
-                      kernel=1 if wave else 0)
+                      kernel=1 if wave else 0, out_entries=out_entries)
         )
     # dense blocks first (ids b*count + v), then sparse partials
     for g in gplans:

@@ -268,17 +268,21 @@ __device__ __forceinline__ void load_items(const T* __restrict__ base, int64_t s
   }
 }
 
-// Ids and ranks of one tile, loaded two tiles ahead of its reduction so that
-// the gathers they address can be issued one tile ahead.  Ranks stay packed as
-// loaded (two 16-bit ranks per word for even ITEMS) and are unpacked only
-// where they are used.
-template <int ITEMS>
-struct WaveIds {
-  static constexpr int RW = (ITEMS + 1) / 2;
-  int ip[ITEMS], is[ITEMS];
-  uint32_t rk[RW];
-  int rounds;  // occurrence rounds of the tile (0 without them)
-};
+// Wave-kernel loads: a uniform tile base plus a 32-bit byte offset per lane, so
+// that each load addresses through a scalar base and one shared offset register
+// instead of a 64-bit address computed per load.
+template <typename T, int ITEMS, bool STREAM = false>
+__device__ __forceinline__ void load_lane(const T* __restrict__ tile, uint32_t off, T (&out)[ITEMS]) {
+  const char* p = reinterpret_cast<const char*>(tile) + off;
+  if constexpr (ITEMS == 1) {
+    out[0] = STREAM ? __builtin_nontemporal_load(reinterpret_cast<const T*>(p)) : *reinterpret_cast<const T*>(p);
+  } else {
+    typedef typename VecT<T, ITEMS>::type V;
+    const V v = STREAM ? __builtin_nontemporal_load(reinterpret_cast<const V*>(p)) : *reinterpret_cast<const V*>(p);
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) out[k] = v[k];
+  }
+}
 
 // Loads of the pipeline are issued on every path: a load skipped on one path, or
 // a generic (flat) load, makes the compiler's later wait counts conservative,
@@ -296,7 +300,7 @@ struct WaveTile {
   int rounds;
   Real jp[R][DP][ITEMS], js[R][DS][ITEMS];
   Real xp[ITEMS][DP], xs[ITEMS][XL ? 1 : DS];
-  int direct[ITEMS];
+  uint8_t direct[ITEMS];  // raw flag bytes (a widened copy would force a wait)
   __device__ __forceinline__ int rank(int k) const {
     return ITEMS == 1 ? (int)rk[0] : (int)((k & 1) ? (rk[k >> 1] >> 16) : (rk[k >> 1] & 0xffffu));
   }
@@ -307,20 +311,21 @@ struct WaveTile {
 // bounds branch; positions past the group's count are masked by the caller.
 // Ranks are read as single 16-bit loads (zero-extended, no unpacking) so that
 // nothing forces a wait between the id loads and the block loads.
-template <typename Real, int ITEMS, int NT, bool FAST>
-__device__ __forceinline__ void load_wave_ids(const WaveArgs<Real>& a, int t, int b, WaveIds<ITEMS>& d) {
+template <typename Real, int ITEMS, int NT, bool FAST, typename Tile>
+__device__ __forceinline__ void load_wave_ids(const WaveArgs<Real>& a, int t, int b, Tile& d) {
   constexpr int T = NT * ITEMS;
   const int64_t start = (int64_t)t * T;
-  load_items<int32_t, ITEMS>(a.idx_p, start, b, T, true, d.ip);
-  load_items<int32_t, ITEMS>(a.idx_s, start, b, T, true, d.is);
+  const uint32_t off = (uint32_t)b * 4u;
+  load_lane<int32_t, ITEMS>(a.idx_p + start, off, d.ip);
+  load_lane<int32_t, ITEMS>(a.idx_s + start, off, d.is);
   if constexpr (FAST) {
     // occurrence ranks are always present; the round counts are read from LDS
+    const char* rp = reinterpret_cast<const char*>(a.rank_s + start) + (uint32_t)b * 2u;
     if constexpr (ITEMS == 1) {
-      d.rk[0] = a.rank_s[start + b];
+      d.rk[0] = *reinterpret_cast<const uint16_t*>(rp);
     } else {
-      const uint32_t* r32 = reinterpret_cast<const uint32_t*>(a.rank_s + start + b);
 #pragma unroll
-      for (int w = 0; w < ITEMS / 2; ++w) d.rk[w] = r32[w];
+      for (int w = 0; w < ITEMS / 2; ++w) d.rk[w] = reinterpret_cast<const uint32_t*>(rp)[w];
     }
     d.rounds = 0;
     return;
@@ -344,51 +349,46 @@ __device__ __forceinline__ void load_wave_ids(const WaveArgs<Real>& a, int t, in
   d.rounds = a.rounds ? nr : 0;
 }
 
-// Block loads of the tile starting at position `start` (component stride
-// `stride`): no dependence on anything loaded before them, so they are issued a
-// whole tile ahead of their use.
+// Block loads of tile t: no dependence on anything loaded before them, so they
+// are issued a whole tile ahead of their use.
 template <typename Real, int R, int DP, int DS, int OP, int OS, int ITEMS, int NT, bool STREAM, bool XL>
-__device__ __forceinline__ void load_wave_blocks(const WaveArgs<Real>& a, int64_t start, int64_t stride, int b,
+__device__ __forceinline__ void load_wave_blocks(const WaveArgs<Real>& a, int t, int b,
                                                  WaveTile<Real, R, DP, DS, ITEMS, XL>& d) {
   constexpr int T = NT * ITEMS;
+  const Real* tile = a.packed + (int64_t)t * T;
+  const uint32_t off = (uint32_t)b * (uint32_t)sizeof(Real);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
 #pragma unroll
     for (int c = 0; c < DP; ++c)
-      load_items<Real, ITEMS, STREAM>(a.packed + (int64_t)(OP + r * DP + c) * stride, start, b, T, true,
-                                      d.jp[r][c]);
+      load_lane<Real, ITEMS, STREAM>(tile + (int64_t)(OP + r * DP + c) * a.stride, off, d.jp[r][c]);
 #pragma unroll
     for (int c = 0; c < DS; ++c)
-      load_items<Real, ITEMS, STREAM>(a.packed + (int64_t)(OS + r * DS + c) * stride, start, b, T, true,
-                                      d.js[r][c]);
+      load_lane<Real, ITEMS, STREAM>(tile + (int64_t)(OS + r * DS + c) * a.stride, off, d.js[r][c]);
   }
 }
 
-// Ids of a tile (loaded earlier) and the x gathers they address.
+// The x gathers addressed by a tile's ids (loaded into the tile earlier: no id
+// is copied between registers, so no copy waits for a load in flight).
 template <typename Real, int R, int DP, int DS, int ITEMS, bool XL>
-__device__ __forceinline__ void load_wave_gathers(const WaveArgs<Real>& a, const WaveIds<ITEMS>& ids,
-                                                  WaveTile<Real, R, DP, DS, ITEMS, XL>& d) {
-#pragma unroll
-  for (int k = 0; k < ITEMS; ++k) {
-    d.ip[k] = ids.ip[k];
-    d.is[k] = ids.is[k];
-  }
-#pragma unroll
-  for (int w = 0; w < WaveIds<ITEMS>::RW; ++w) d.rk[w] = ids.rk[w];
-  d.rounds = ids.rounds;
+__device__ __forceinline__ void load_wave_gathers(const WaveArgs<Real>& a, WaveTile<Real, R, DP, DS, ITEMS, XL>& d) {
+  const WaveTile<Real, R, DP, DS, ITEMS, XL>& ids = d;
   const bool has_direct = a.direct_p != nullptr;
   const uint8_t* direct = has_direct ? a.direct_p : reinterpret_cast<const uint8_t*>(a.idx_p);
+  // 32-bit byte offsets: the launcher checks that each x slice is below 4 GiB
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
+    const char* xp = reinterpret_cast<const char*>(a.x_p) + (uint32_t)ids.ip[k] * (uint32_t)(DP * sizeof(Real));
 #pragma unroll
-    for (int c = 0; c < DP; ++c) d.xp[k][c] = a.x_p[(int64_t)ids.ip[k] * DP + c];
+    for (int c = 0; c < DP; ++c) d.xp[k][c] = reinterpret_cast<const Real*>(xp)[c];
     if constexpr (!XL) {
+      const char* xs = reinterpret_cast<const char*>(a.x_s) + (uint32_t)ids.is[k] * (uint32_t)(DS * sizeof(Real));
 #pragma unroll
-      for (int c = 0; c < DS; ++c) d.xs[k][c] = a.x_s[(int64_t)ids.is[k] * DS + c];
+      for (int c = 0; c < DS; ++c) d.xs[k][c] = reinterpret_cast<const Real*>(xs)[c];
     }
     // raw byte (garbage without the array): tested against direct_p where used, so
     // that no select right after the load forces a wait for it
-    d.direct[k] = direct[has_direct ? ids.ip[k] : 0];
+    d.direct[k] = direct[has_direct ? (uint32_t)ids.ip[k] : 0u];
   }
 }
 
@@ -469,7 +469,6 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
   const int b = ITEMS * (int)threadIdx.x;  // first tile-local position of this thread
   const int t0 = blockIdx.x * a.tiles_per_wg;
   const int t1 = min(t0 + a.tiles_per_wg, a.ntiles);
-  WaveIds<ITEMS> idn;  // ids of the tile after the one being processed
   int32_t* nrounds = reinterpret_cast<int32_t*>(
       smem_raw + (((int64_t)a.count_s * DS * (XL ? 2 : 1) * (int64_t)sizeof(Real) + 15) / 16) * 16);
   unsigned char* out_base =
@@ -489,18 +488,19 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
                      int t) {
     // The loads are unconditional so that the compiler's wait counts stay exact
     // (a skipped load on one path makes every later wait conservative): after
-    // the workgroup's last tile they re-read the first positions of component 0
-    // (stride 0), and the ids of a tile past it are those of the group's last tile.
-    const bool more = t + 1 < t1;
-    load_wave_blocks<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, more ? (int64_t)(t + 1) * T : 0,
-                                                                      more ? a.stride : 0, b, nxt);
+    // the workgroup's last tile they re-read a valid tile (its last, or the
+    // group's last), whose data is not used.
+    const int tn = min(t + 1 < t1 ? t + 1 : t, a.ntiles - 1);
+    load_wave_ids<Real, ITEMS, NT, FAST>(a, tn, b, nxt);
+    load_wave_blocks<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, tn, b, nxt);
     pin_blocks(cur);
     const int64_t start = (int64_t)t * T;
     // t == t1 is the padding half of the last pair (an odd tile count): it
     // computes on stand-in data, writes nothing and takes no rounds
     const int len = t < t1 ? (int)min((int64_t)T, a.count - start) : 0;
     const int left = len - b;  // items of this thread inside the tile (may be <= 0)
-    int ip[ITEMS], is[ITEMS], direct[ITEMS];
+    int ip[ITEMS], is[ITEMS];
+    uint8_t direct[ITEMS];
     Real xp[ITEMS][DP], xs[ITEMS][DS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
@@ -549,8 +549,7 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
     // this tile's blocks, and the ids of tile t+2 follow them.
     pin_before_loads(cp);
     pin_before_loads(cs);
-    load_wave_gathers<Real, R, DP, DS, ITEMS, XL>(a, idn, nxt);
-    load_wave_ids<Real, ITEMS, NT, FAST>(a, min(t + 2, a.ntiles - 1), b, idn);
+    load_wave_gathers<Real, R, DP, DS, ITEMS, XL>(a, nxt);
     // primary slot: runs inside the thread, then a segmented scan over lanes
 #pragma unroll
     for (int k = 1; k < ITEMS; ++k)
@@ -680,10 +679,9 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
   // loads of tile t+1 are issued before tile t's data is awaited.
   WaveTile<Real, R, DP, DS, ITEMS, XL> ta, tb;
   if (t0 < t1) {
-    load_wave_ids<Real, ITEMS, NT, FAST>(a, t0, b, idn);
-    load_wave_blocks<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, (int64_t)t0 * T, a.stride, b, ta);
-    load_wave_gathers<Real, R, DP, DS, ITEMS, XL>(a, idn, ta);
-    load_wave_ids<Real, ITEMS, NT, FAST>(a, min(t0 + 1, a.ntiles - 1), b, idn);
+    load_wave_ids<Real, ITEMS, NT, FAST>(a, t0, b, ta);
+    load_wave_blocks<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, t0, b, ta);
+    load_wave_gathers<Real, R, DP, DS, ITEMS, XL>(a, ta);
   }
   if constexpr (XL) {
     for (int i = threadIdx.x; i < a.count_s * DS; i += NT) xl[i] = a.x_s[i];
@@ -1103,6 +1101,9 @@ int launch_wave_t(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
   // packed stride (and the caller's id/rank arrays) to cover ntiles * tile_cap positions
   if (g.tile_cap != NT * ITEMS || g.ntiles != ceil_div(g.count, NT * ITEMS) || g.stride % ITEMS != 0 ||
       g.stride < (int64_t)g.ntiles * g.tile_cap || g.primary != P)
+    return TLH_EINVAL;
+  // the x gathers address each slot's slice with 32-bit byte offsets
+  if (tp.count * tp.dim * (int64_t)sizeof(Real) > UINT32_MAX || ts.count * ts.dim * (int64_t)sizeof(Real) > UINT32_MAX)
     return TLH_EINVAL;
   a.packed = static_cast<const Real*>(g.packed);
   a.stride = g.stride;

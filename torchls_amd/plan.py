@@ -318,7 +318,9 @@ def build_structure(
         sorted_idx = [slot_idx[s][order] for s in range(ns)]
         dmax = max(types[t].dim for t in slot_types)
         shape = (int(rows), *(types[t].dim for t in slot_types))
-        wave = ns == 2 and shape in wave_shapes and count > 0
+        # (the wave kernel addresses each slot's x slice with 32-bit byte offsets)
+        wave = (ns == 2 and shape in wave_shapes and count > 0
+                and all(types[t].count * types[t].dim * esize < 2**32 for t in slot_types))
         gcap = WAVE_THREADS * wave_items(esize) if wave else cap
         tile_start = _aligned_tiles(count, gcap, device) if wave else _cut_tiles(sorted_idx[primary], cap)
         ntiles = tile_start.numel() - 1

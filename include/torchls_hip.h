@@ -94,9 +94,15 @@ typedef struct tlh_group {
                                 other slot by on-the-fly segment heads (DENSE accumulator or
                                 partial pos_base + tile_start + head) */
   int32_t primary;           /* slot the observations are sorted by */
-  int32_t out_entries;       /* wave kernel: most primary-slot piece ends of one wavefront over
-                                the tiles of one workgroup (its y / partial writes are staged in
-                                LDS when they fit); 0 = not planned (written as they are found) */
+  int32_t out_entries;       /* wave kernel with a DENSE secondary slot that has rank and rounds:
+                                0, or the most primary-slot piece ends of one wavefront over the
+                                tiles of one workgroup (the y / partial writes are then staged in
+                                LDS, and a plan that does not fit is an error) */
+  int32_t stage_entries;     /* out_entries > 0: 0 (occurrence rounds), or the overflow encoding:
+                                the secondary rank holds 0 for the first occurrence of its
+                                variable in the tile, else 1 + its slot in the tile's (variable,
+                                position) order of later occurrences, rounds holds the number of
+                                later occurrences per tile, and stage_entries their maximum */
   tlh_slot_t slot[TLH_MAX_SLOTS];
 } tlh_group_t;
 

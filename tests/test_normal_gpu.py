@@ -38,12 +38,17 @@ def test_random_structures(device, shape, dtype, tol):
     assert rel_err(y.cpu().numpy(), ref) <= tol
 
 
-# (2000, 6000, 4): many cameras per tile, so the camera slot is reduced in
-# occurrence rounds; the others stage camera segments in LDS
+# (2000, 6000, 4) has many cameras per tile, so the camera slot is reduced in
+# occurrence rounds; with fewer cameras a camera repeats too often in a tile.
+# overflow=True: piece ends are staged in LDS, and the repetitive camera slots
+# take the overflow encoding (first occurrences added directly, later ones
+# staged and summed per camera run).  overflow=False: the repetitive camera
+# slots stage camera segments in LDS and piece ends are written directly.
+@pytest.mark.parametrize("overflow", [True, False])
 @pytest.mark.parametrize("n_cams,n_pts,k", [(16, 500, 8), (200, 50_000, 10), (2000, 6000, 4)])
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 1e-5)])
-def test_bundle_adjustment(device, n_cams, n_pts, k, dtype, tol):
-    print(f"normal matvec BA cams={n_cams} pts={n_pts} dtype={dtype}", flush=True)
+def test_bundle_adjustment(device, n_cams, n_pts, k, dtype, tol, overflow):
+    print(f"normal matvec BA cams={n_cams} pts={n_pts} dtype={dtype} overflow={overflow}", flush=True)
     prob = generators.bundle_adjustment(n_cams, n_pts, k, seed=7)
     J, _ = ba_jacobian(prob, delta=1e-2)
     if dtype == torch.float32:
@@ -53,10 +58,13 @@ def test_bundle_adjustment(device, n_cams, n_pts, k, dtype, tol):
     x = rng.standard_normal(J.n)
     ref = J.normal_matvec(x, 1e-3)
     BJ = to_block_jacobian(J, device, dtype)
-    plan = HipNormalPlan(BJ)
+    plan = HipNormalPlan(BJ, overflow=overflow)
     g = plan.plan.groups[0]
     assert g.kernel == 1
-    assert (g.slots[0].rounds is not None) == (n_cams == 2000)
+    if overflow:
+        assert g.out_entries > 0 and (g.stage_entries > 0) == (n_cams != 2000)
+    else:
+        assert g.out_entries == 0 and (g.slots[0].rounds is not None) == (n_cams == 2000)
     op = plan.bind(BJ)
     xt = torch.tensor(x, dtype=dtype, device=device)
     y1 = op.normal_matvec(xt, 1e-3)

@@ -70,6 +70,7 @@ class Group(ctypes.Structure):
         ("kernel", i32),
         ("primary", i32),
         ("out_entries", i32),
+        ("stage_entries", i32),
         ("slot", Slot * TLH_MAX_SLOTS),
     ]
 

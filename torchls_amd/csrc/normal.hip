@@ -245,7 +245,8 @@ struct WaveArgs {
   int32_t count_s;
   Real damping;
   const int32_t* rounds;  // occurrence rounds per tile (dense secondary) or NULL
-  int32_t out_cap;        // staged primary outputs per wavefront (staged-output kernel)
+  int32_t out_cap;        // staged primary outputs per wavefront (FAST kernel)
+  int32_t stage_cap;      // overflow positions of one tile (FAST kernel)
 };
 
 // out[k] = base[start + b + k]; a short tile (len < T) clamps to its last position.
@@ -433,10 +434,11 @@ __device__ __forceinline__ void pin_blocks(WaveTile<Real, R, DP, DS, ITEMS, XL>&
 // float32 do) has that slice copied into LDS once per workgroup, so the
 // per-observation secondary gathers are LDS reads rather than scattered L2
 // requests.
-// FAST (dense secondary with occurrence rounds, primary outputs that fit in
-// LDS): the workgroup's round counts are read into LDS once, and every piece
-// end is staged in its wavefront's LDS region (value and target) and written
-// to y or the partials after the last tile.  The tile loop then issues no
+// FAST (dense secondary with occurrence rounds or the overflow encoding,
+// primary outputs that fit in LDS): the workgroup's round (or overflow) counts
+// are read into LDS once, and every piece end is staged in its wavefront's LDS
+// region (value and target) and written to y or the partials after the last
+// tile.  The tile loop then issues no
 // global store, so the compiler's load waits keep the next tile's block loads
 // in flight (a pending store makes them wait for every load).  out_cap, the
 // most piece ends of one wavefront over the workgroup's tiles, comes from the
@@ -446,7 +448,7 @@ __host__ __device__ constexpr int64_t out_region_bytes(int cap) {
   return ((int64_t)cap * (DP * (int64_t)sizeof(Real) + 4) + 15) / 16 * 16;
 }
 
-template <typename Real, int R, int D0, int D1, int P, int ITEMS, int NT, bool STREAM, bool XL, bool FAST>
+template <typename Real, int R, int D0, int D1, int P, int ITEMS, int NT, bool STREAM, bool XL, bool FAST, bool OVF>
 __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
   constexpr int DP = P == 0 ? D0 : D1;
   constexpr int DS = P == 0 ? D1 : D0;
@@ -475,6 +477,11 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
       reinterpret_cast<unsigned char*>(nrounds) + ((int64_t)a.tiles_per_wg * 4 + 15) / 16 * 16;
   Real* out_val = reinterpret_cast<Real*>(out_base + (threadIdx.x >> 6) * out_region_bytes<DP, Real>(a.out_cap));
   int32_t* out_tgt = reinterpret_cast<int32_t*>(out_val + (int64_t)a.out_cap * DP);
+  // FAST overflow stage (overflow encoding only): contributions of the second
+  // and later occurrences of a variable in the tile, in (variable, position)
+  // order, and their variables
+  Real* ov_val = reinterpret_cast<Real*>(out_base + (NT / 64) * out_region_bytes<DP, Real>(a.out_cap));
+  int32_t* ov_var = reinterpret_cast<int32_t*>(ov_val + (int64_t)a.stage_cap * DS);
   int out_n = 0;  // piece ends staged by this wavefront (uniform over its lanes)
   if constexpr (FAST) {
     for (int i = threadIdx.x; i < t1 - t0; i += NT) nrounds[i] = a.rounds[t0 + i];
@@ -623,6 +630,46 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
           for (int c = 0; c < DP; ++c) out[c] = s[c];
         }
       }
+    }
+    if constexpr (FAST && OVF) {
+      // secondary slot, overflow encoding: the first occurrence of each variable
+      // in the tile adds into the accumulator (no two share a variable); every
+      // later occurrence is staged at its slot (rank - 1) of the tile's
+      // (variable, position) order.  After a barrier the head of each staged run
+      // sums the run in order and adds the sum.  Each entry thus receives, per
+      // tile, its first contribution and then the sum of the others: a fixed
+      // order, two barriers (used where occurrence rounds would be many).
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) {
+        if (k < left) {
+          const int rk = cur.rank(k);
+          if (rk == 0) {
+#pragma unroll
+            for (int c = 0; c < DS; ++c) acc[is[k] * DS + c] += cs[k][c];
+          } else if (rk <= a.stage_cap) {
+#pragma unroll
+            for (int c = 0; c < DS; ++c) ov_val[(rk - 1) * DS + c] = cs[k][c];
+            ov_var[rk - 1] = is[k];
+          }
+        }
+      }
+      __syncthreads();
+      const int nov = t < t1 ? min(nrounds[t - t0], a.stage_cap) : 0;
+      for (int p = threadIdx.x; p < nov; p += NT) {
+        const int v = ov_var[p];
+        if (p > 0 && ov_var[p - 1] == v) continue;
+        Real sum[DS];
+#pragma unroll
+        for (int c = 0; c < DS; ++c) sum[c] = ov_val[p * DS + c];
+        for (int q = p + 1; q < nov && ov_var[q] == v; ++q) {
+#pragma unroll
+          for (int c = 0; c < DS; ++c) sum[c] += ov_val[q * DS + c];
+        }
+#pragma unroll
+        for (int c = 0; c < DS; ++c) acc[v * DS + c] += sum[c];
+      }
+      __syncthreads();
+      return;
     }
     if (FAST || a.rounds) {
       // secondary slot, occurrence rounds: round r adds every r-th occurrence of a
@@ -1042,9 +1089,11 @@ int launch_generic(const tlh_operator_t* op, const tlh_group_t& g, const void* x
 
 // LDS of a wave group.  Staged reduction: stage (contribution + id per
 // position), the dense secondary accumulator and, when all of that fits, the
-// secondary x slice.  Occurrence rounds with a planned output capacity (the
-// FAST kernel): accumulator, x slice when it fits, the workgroup's round counts
-// and one output region per wavefront.
+// secondary x slice.  Occurrence rounds: the accumulator and the x slice when
+// it fits.  FAST (out_entries > 0): accumulator, x slice when it fits, the
+// workgroup's round (or overflow) counts, one output region per wavefront and,
+// with the overflow encoding, the overflow stage; a FAST plan that does not
+// fit is an error (the overflow encoding admits no other kernel).
 struct WaveLds {
   int64_t bytes;
   bool xl;
@@ -1061,15 +1110,18 @@ WaveLds wave_lds(const tlh_group_t& g, const tlh_type_t& ts, int64_t lds_max) {
   const bool dense = ss.mode == TLH_REDUCE_DENSE;
   const bool rounds = dense && ss.rounds;
   const int64_t slice = dense ? ts.count * ds * (int64_t)sizeof(Real) : 0;
-  // positional partial ids of staged outputs are int32
-  const bool ids_fit = g.slot[g.primary].pos_base + (int64_t)g.ntiles * g.tile_cap <= INT32_MAX;
-  if (rounds && ss.rank && g.out_entries > 0 && ids_fit) {
-    const int64_t rounds_b = align16((int64_t)g.tiles_per_wg * 4);
+  if (g.out_entries > 0) {
+    // positional partial ids of staged outputs are int32
+    const bool ids_fit = g.slot[g.primary].pos_base + (int64_t)g.ntiles * g.tile_cap <= INT32_MAX;
+    if (!rounds || !ss.rank || !ids_fit || g.stage_entries < 0) return {-1, false, true};
+    const int64_t counts_b = align16((int64_t)g.tiles_per_wg * 4);
     const int64_t out_b = (TLH_WAVE_THREADS / 64) * align16((int64_t)g.out_entries * (dp * (int64_t)sizeof(Real) + 4));
+    const int64_t stage_b = align16((int64_t)g.stage_entries * (ds * (int64_t)sizeof(Real) + 4));
     for (int x = 1; x >= 0; --x) {
-      const int64_t b = align16(slice * (x ? 2 : 1)) + rounds_b + out_b;
+      const int64_t b = align16(slice * (x ? 2 : 1)) + counts_b + out_b + stage_b;
       if (b <= lds_max) return {b, x == 1, true};
     }
+    return {-1, false, true};
   }
   int64_t b = rounds ? 0 : (int64_t)g.tile_cap * (ds * (int64_t)sizeof(Real) + 4);
   bool xl = false;
@@ -1081,10 +1133,14 @@ WaveLds wave_lds(const tlh_group_t& g, const tlh_type_t& ts, int64_t lds_max) {
   return {b, xl, false};
 }
 
+// mode 0: general (direct stores), 1: FAST with occurrence rounds, 2: FAST with
+// the overflow encoding (separate kernels: the other path's code in the tile
+// loop measurably slows the rounds kernel)
 template <typename Real, int R, int D0, int D1, int P, int ITEMS, int NT, bool STREAM, bool XL>
-auto wave_kernel_for(bool fast) {
-  return fast ? normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, STREAM, XL, true>
-              : normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, STREAM, XL, false>;
+auto wave_kernel_for(int mode) {
+  return mode == 2   ? normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, STREAM, XL, true, true>
+         : mode == 1 ? normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, STREAM, XL, true, false>
+                     : normal_wave_kernel<Real, R, D0, D1, P, ITEMS, NT, STREAM, XL, false, false>;
 }
 
 template <typename Real, int R, int D0, int D1, int P>
@@ -1127,13 +1183,16 @@ int launch_wave_t(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
   a.damping = (Real)damping;
   a.rounds = ss.mode == TLH_REDUCE_DENSE ? ss.rounds : nullptr;
   const WaveLds lds = wave_lds<Real>(g, ts, device_lds_max());
+  if (lds.bytes < 0) return TLH_EINVAL;
   a.out_cap = lds.fast ? g.out_entries : 0;
+  a.stage_cap = lds.fast ? g.stage_entries : 0;
   // blocks that cannot stay in the 256 MB last-level cache between matvecs are streamed
   const bool stream_blocks = (int64_t)g.stride * g.rows * (D0 + D1) * (int64_t)sizeof(Real) > (192ll << 20);
-  auto kernel = stream_blocks ? (lds.xl ? wave_kernel_for<Real, R, D0, D1, P, ITEMS, NT, true, true>(lds.fast)
-                                        : wave_kernel_for<Real, R, D0, D1, P, ITEMS, NT, true, false>(lds.fast))
-                              : (lds.xl ? wave_kernel_for<Real, R, D0, D1, P, ITEMS, NT, false, true>(lds.fast)
-                                        : wave_kernel_for<Real, R, D0, D1, P, ITEMS, NT, false, false>(lds.fast));
+  const int mode = !lds.fast ? 0 : (g.stage_entries > 0 ? 2 : 1);
+  auto kernel = stream_blocks ? (lds.xl ? wave_kernel_for<Real, R, D0, D1, P, ITEMS, NT, true, true>(mode)
+                                        : wave_kernel_for<Real, R, D0, D1, P, ITEMS, NT, true, false>(mode))
+                              : (lds.xl ? wave_kernel_for<Real, R, D0, D1, P, ITEMS, NT, false, true>(mode)
+                                        : wave_kernel_for<Real, R, D0, D1, P, ITEMS, NT, false, false>(mode));
   TLH_RETURN_IF_ERROR(allow_lds((const void*)kernel, lds.bytes));
   hipLaunchKernelGGL(kernel, dim3(g.nwg), dim3(NT), (size_t)lds.bytes, stream, a);
   TLH_RETURN_IF_LAUNCH_FAILED();

@@ -99,6 +99,7 @@ class DeviceStructure:
             d.kernel = g.kernel
             d.primary = g.primary
             d.out_entries = g.out_entries
+            d.stage_entries = g.stage_entries
             for si, s in enumerate(g.slots):
                 ds = d.slot[si]
                 ds.type = s.type
@@ -168,7 +169,8 @@ class DeviceStructure:
 WAVE_SHAPES = frozenset({(2, 6, 3), (2, 3, 6)})
 
 
-def plan_for(jacobian, dtype: torch.dtype, *, allow_dense: bool = True, tile_cap: int | None = None):
+def plan_for(jacobian, dtype: torch.dtype, *, allow_dense: bool = True, tile_cap: int | None = None,
+             overflow: bool = True):
     """Build the StructurePlan and its device copy for a BlockJacobian."""
     device = jacobian.groups[0].blocks[0].device
     info = _lib.device_info(device)
@@ -189,6 +191,7 @@ def plan_for(jacobian, dtype: torch.dtype, *, allow_dense: bool = True, tile_cap
         lds_limit=int(info.lds_per_block),
         cu_count=int(info.cu_count),
         allow_dense=allow_dense,
+        overflow=overflow,
         tile_cap=tile_cap,
         wave_shapes=WAVE_SHAPES if allow_dense else frozenset(),
     )
@@ -206,7 +209,8 @@ class HipNormalPlan:
     """
 
     def __init__(self, jacobian, *, shards=128, hot_threshold=512, max_packed_bytes=256 * 1024**2,
-                 dtype: torch.dtype | None = None, allow_dense: bool = True, tile_cap: int | None = None):
+                 dtype: torch.dtype | None = None, allow_dense: bool = True, tile_cap: int | None = None,
+                 overflow: bool = True):
         proto = jacobian.groups[0].blocks[0]
         if proto.device.type != "cuda" or proto.dtype not in (torch.float32, torch.float64):
             raise ValueError("HIP normal matvec requires GPU float32 or float64 blocks")
@@ -228,11 +232,12 @@ class HipNormalPlan:
             (tuple(b.shape for b in g.blocks), tuple(g.type_indices), tuple(g.variable_indices))
             for g in jacobian.groups
         )
-        self.plan, self.dev = plan_for(jacobian, self.dtype, allow_dense=allow_dense, tile_cap=tile_cap)
+        self.plan, self.dev = plan_for(jacobian, self.dtype, allow_dense=allow_dense, tile_cap=tile_cap,
+                                       overflow=overflow)
         for gi in range(len(self.plan.groups)):
             op = self.dev.operator(self.dtype, [None] * len(self.plan.groups), [None] * len(self.plan.types))
             need = self.lib.tlh_group_lds_bytes(ctypes.byref(op), gi)
-            if need > _lib.device_info(self.device).lds_per_block:
+            if need < 0 or need > _lib.device_info(self.device).lds_per_block:
                 raise ValueError("tile plan exceeds the device LDS limit")
 
     def check_structure(self, jacobian) -> None:

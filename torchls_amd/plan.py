@@ -87,9 +87,12 @@ class GroupPlan:
     tiles_per_wg: int
     slots: list
     kernel: int = 0  # 0: segment-array kernel, 1: wave kernel (2 slots, specialised shape)
-    # wave kernel: most primary-slot piece ends (y / partial writes) of one wavefront
-    # over the tiles of one workgroup, the capacity of its LDS output region
+    # wave kernel, overflow encoding of the dense secondary slot (0: none): most
+    # primary-slot piece ends (y / partial writes) of one wavefront over the tiles
+    # of one workgroup, the capacity of its LDS output region, and most later
+    # occurrences of one tile, the capacity of the overflow stage
     out_entries: int = 0
+    stage_entries: int = 0
 
     @property
     def ntiles(self) -> int:
@@ -250,12 +253,45 @@ def _occurrences(index_sorted: torch.Tensor, tile_start: torch.Tensor):
     first = torch.ones(n, dtype=torch.bool, device=device)
     first[1:] = (kt[1:] != kt[:-1]) | (kv[1:] != kv[:-1])
     pos = _arange(n, device)
-    start = torch.cummax(torch.where(first, pos, torch.zeros_like(pos)), 0).values
+    start = _flatnonzero(first)[torch.cumsum(first.to(torch.int64), 0) - 1]  # first position of each run
     occ = torch.empty(n, dtype=torch.int64, device=device)
     occ[order] = pos - start
     rounds = torch.zeros(ntiles, dtype=torch.int64, device=device)
     rounds.scatter_reduce_(0, tile, occ + 1, reduce="amax")
     return occ, rounds.to(torch.int32)
+
+
+def _overflow(index_sorted: torch.Tensor, tile_start: torch.Tensor):
+    """Overflow encoding of a dense secondary slot (wave kernel with staged outputs):
+    rank 0 for the first position of each variable in its tile, else 1 + the slot of
+    the position among the tile's later occurrences in (variable, position) order;
+    and the number of later occurrences of each tile."""
+    device = index_sorted.device
+    n = index_sorted.numel()
+    ntiles = tile_start.numel() - 1
+    if n == 0:
+        return torch.zeros(0, dtype=torch.int64, device=device), torch.zeros(ntiles, dtype=torch.int32, device=device)
+    tile = _tile_of(tile_start, n)
+    order = _stable_order([tile, index_sorted.to(torch.int64)])
+    kt, kv = tile[order], index_sorted.to(torch.int64)[order]
+    first = torch.ones(n, dtype=torch.bool, device=device)
+    first[1:] = (kt[1:] != kt[:-1]) | (kv[1:] != kv[:-1])
+    later = (~first).to(torch.int64)
+    counts = torch.bincount(kt[~first], minlength=ntiles)
+    before = torch.cumsum(counts, 0) - counts  # later occurrences in earlier tiles
+    slot = torch.cumsum(later, 0) - 1 - before[kt]
+    rank = torch.empty(n, dtype=torch.int64, device=device)
+    rank[order] = torch.where(first, torch.zeros_like(slot), slot + 1)
+    return rank, counts.to(torch.int32)
+
+
+def _fast_lds(count_s: int, ds: int, dp: int, esize: int, per: int, out_entries: int, stage_entries: int) -> int:
+    """LDS bytes of the staged-output wave kernel without the secondary x slice
+    (csrc/normal.hip wave_lds): accumulator, overflow counts of the workgroup's
+    tiles, one output region per wavefront and the overflow stage."""
+    a16 = lambda b: -(-b // 16) * 16  # noqa: E731
+    return (a16(count_s * ds * esize) + a16(per * 4) + (WAVE_THREADS // WAVE) * a16(out_entries * (dp * esize + 4))
+            + a16(stage_entries * (ds * esize + 4)))
 
 
 def _wave_pieces(index_sorted: torch.Tensor, tile_start: torch.Tensor, span: int):
@@ -288,6 +324,7 @@ def build_structure(
     piece: int = 32,
     tile_cap: int | None = None,
     allow_dense: bool = True,
+    overflow: bool = True,
     force_primary: dict | None = None,
     split_primary: bool = True,
     wave_shapes: frozenset = frozenset(),
@@ -297,11 +334,16 @@ def build_structure(
     ``layout``: [(count, dim, offset)] per variable type.
     ``groups``: [(rows, [type per slot], [int64 index tensor per slot])], every index
     tensor on the device the plan is built on (all groups on one device).
+    ``overflow``: a wave group's dense secondary slot takes the overflow encoding
+    (staged outputs, two barriers per tile) when it fits in LDS; otherwise, or
+    when False, occurrence rounds (at most MAX_ROUNDS) or the staged reduction.
     """
     cap = tile_cap or THREADS * items_for(esize)
     types = [TypePlan(int(c), int(d), int(o)) for c, d, o in layout]
     device = next(torch.as_tensor(ix).device for _, _, idx in groups for ix in idx)
     gplans = []
+    # staged wave outputs name positional partials with int32 ids
+    ids_fit = sum(int(torch.as_tensor(idx[0]).numel()) for _, _, idx in groups) + 2 * WAVE_TILE < 2**31
     for gi, (rows, slot_types, slot_idx) in enumerate(groups):
         ns = len(slot_types)
         if not 1 <= ns <= 3:
@@ -355,11 +397,6 @@ def build_structure(
             )
             if wave and s == primary:
                 slots[-1].piece_end = ends
-            elif wave and dense:
-                occ, rounds = _occurrences(sorted_idx[s], tile_start)
-                if rounds.numel() and int(rounds.max()) <= MAX_ROUNDS:
-                    slots[-1].rank = occ.to(torch.int32)
-                    slots[-1].rounds = rounds
         has_dense = any(sl.mode == REDUCE_DENSE for sl in slots)
         if has_dense and ntiles:
             # a wave-kernel workgroup (1024 threads) fills its compute unit's wave slots
@@ -368,17 +405,38 @@ def build_structure(
             nwg = -(-ntiles // per)
         else:
             nwg, per = ntiles, 1
-        out_entries = 0
-        if wave and ntiles:
-            ps = slots[primary]
-            span = WAVE * wave_items(esize)
-            pieces = (ps.seg_ptr[1:] - ps.seg_ptr[:-1]).to(torch.int64)
-            tile_of_piece = torch.repeat_interleave(_arange(ntiles, device), pieces)
-            key = (tile_of_piece // per) * (WAVE_THREADS // WAVE) + ps.seg_start.to(torch.int64) // span
-            out_entries = int(torch.bincount(key).max()) if key.numel() else 0
+        out_entries = stage_entries = 0
+        sec = 1 - primary
+        if wave and ntiles and slots[sec].mode == REDUCE_DENSE:
+            occ, rounds = _occurrences(sorted_idx[sec], tile_start)
+            use_rounds = bool(rounds.numel()) and int(rounds.max()) <= MAX_ROUNDS
+            if use_rounds:
+                slots[sec].rank = occ.to(torch.int32)
+                slots[sec].rounds = rounds
+            fits = False
+            if overflow and ids_fit:
+                # staged outputs: the most piece ends of one wavefront over a workgroup's tiles
+                ps = slots[primary]
+                span = WAVE * wave_items(esize)
+                pieces = (ps.seg_ptr[1:] - ps.seg_ptr[:-1]).to(torch.int64)
+                tile_of_piece = torch.repeat_interleave(_arange(ntiles, device), pieces)
+                key = (tile_of_piece // per) * (WAVE_THREADS // WAVE) + ps.seg_start.to(torch.int64) // span
+                out_entries = int(torch.bincount(key).max()) if key.numel() else 0
+                ts, tp = types[slot_types[sec]], types[slot_types[primary]]
+                fits = out_entries > 0 and _fast_lds(ts.count, ts.dim, tp.dim, esize, per, out_entries, 0) <= lds_limit
+                if fits and not use_rounds:
+                    # many repeats per tile: overflow encoding instead of the LDS stage
+                    ov_rank, ov_counts = _overflow(sorted_idx[sec], tile_start)
+                    stage_entries = max(1, int(ov_counts.max()))
+                    fits = _fast_lds(ts.count, ts.dim, tp.dim, esize, per, out_entries, stage_entries) <= lds_limit
+                    if fits:
+                        slots[sec].rank = ov_rank.to(torch.int32)
+                        slots[sec].rounds = ov_counts
+            if not fits:
+                out_entries = stage_entries = 0
         gplans.append(
             GroupPlan(count, int(rows), primary, order, tile_start.to(torch.int32), gcap, nwg, per, slots,
-                      kernel=1 if wave else 0, out_entries=out_entries)
+                      kernel=1 if wave else 0, out_entries=out_entries, stage_entries=stage_entries)
         )
     # dense blocks first (ids b*count + v), then sparse partials
     for g in gplans:

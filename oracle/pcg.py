@@ -44,6 +44,9 @@ class Config:
     # converged recurrence whose true residual is above STALL_FACTOR times the true
     # residual at the previous such restart ends the solve as "stagnated"
     stop_on_stagnation: bool = False
+    # the HIP backend's other optional exit: at most this many restarts after a
+    # converged recurrence whose true residual misses the tolerance (None: no limit)
+    max_refinements: int | None = None
 
 
 STALL_FACTOR = 0.5
@@ -159,6 +162,7 @@ def pcg_batched(matvec, rhs, apply=None, cfg: Config = Config()) -> Result:
     reason = "max_iterations"
     tn = math.nan
     converged_true = math.inf
+    refinements = 0
     while it < cfg.max_iterations:
         iv = cfg.residual_check_interval
         stop_at = min(cfg.max_iterations, ((it // iv + 1) * iv) if iv else cfg.max_iterations)
@@ -182,11 +186,15 @@ def pcg_batched(matvec, rhs, apply=None, cfg: Config = Config()) -> Result:
             break
         if it >= cfg.max_iterations:
             break
-        if status == 1 and cfg.stop_on_stagnation:
-            if tn > STALL_FACTOR * converged_true:
+        if status == 1:
+            if cfg.stop_on_stagnation and tn > STALL_FACTOR * converged_true:
+                reason = "stagnated"
+                break
+            if cfg.max_refinements is not None and refinements >= cfg.max_refinements:
                 reason = "stagnated"
                 break
             converged_true = tn
+            refinements += 1
         drift = float(np.linalg.norm(tr - r))
         if status == 1 or cfg.restart_strategy == "periodic" or drift > cfg.residual_drift_tolerance * max(tn, thr):
             r = tr

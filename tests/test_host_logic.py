@@ -1,4 +1,4 @@
-"""Host-side logic that needs no GPU: the exact-Hessian adjoint iteration."""
+"""Host-side logic that needs no GPU: the exact-Hessian adjoint iteration, config checks."""
 
 import torch
 
@@ -22,3 +22,15 @@ def test_exact_adjoint_converges_with_an_approximate_preconditioner():
     assert torch.linalg.vector_norm(w - exact) / torch.linalg.vector_norm(exact) <= 1e-10
     # zero iterations leave the Gauss-Newton solution unchanged
     assert torch.equal(_exact_adjoint(lambda v: H @ v, gn_solve, b, w0, iterations=0), w0)
+
+
+def test_conjugate_gradient_config_validates_refinement_limit():
+    import pytest
+
+    from torchls_amd.solvers import ConjugateGradientConfig
+
+    assert ConjugateGradientConfig().max_refinements is None
+    assert ConjugateGradientConfig(max_refinements=0).max_refinements == 0
+    for bad in (-1, True, 1.5):
+        with pytest.raises(ValueError, match="max_refinements"):
+            ConjugateGradientConfig(max_refinements=bad)

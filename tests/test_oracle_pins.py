@@ -268,6 +268,33 @@ def test_pcg_batched_exits_by_hand():
     assert (res.iterations, res.residual_checks, res.reason, res.converged) == (0, 0, "nonfinite", False)
 
 
+def refinement_system():
+    """Diagonal operator rounded to float32 (e * v with both factors rounded): its
+    true residual cannot fall below ~1e-7 |b|, far above rtol 1e-12, while the
+    float64 recurrence keeps converging."""
+    rng = np.random.default_rng(0)
+    e = np.linspace(1.0, 100.0, 40)
+    e32 = e.astype(np.float32)
+    return (lambda v: (e32 * v.astype(np.float32)).astype(np.float64)), e, rng.standard_normal(40)
+
+
+@pytest.mark.parametrize("limit", [0, 1, 2])
+def test_pcg_batched_refinement_limit(limit):
+    """max_refinements=k: with drift restarts disabled and checks only at a converged
+    recurrence, exactly k restarts (k + 1 checks) and the exit "stagnated"; the same
+    operator without rounding converges at the first check."""
+    matvec, e, b = refinement_system()
+    cfg = dict(rtol=1e-12, max_iterations=1000, residual_check_interval=0, residual_drift_tolerance=1e30)
+    res = pcg_batched(matvec, b, None, Config(max_refinements=limit, **cfg))
+    assert (res.restarts, res.residual_checks, res.reason, res.converged) == (limit, limit + 1, "stagnated", False)
+    assert 1e-9 < res.residual_norm / np.linalg.norm(b) < 1e-6
+    exact = pcg_batched(lambda v: e * v, b, None, Config(max_refinements=0, **cfg))
+    assert (exact.restarts, exact.residual_checks, exact.reason) == (0, 1, "residual_tolerance")
+    # without a limit (and without stop_on_stagnation) the solve restarts until max_iterations
+    free = pcg_batched(matvec, b, None, Config(**dict(cfg, max_iterations=300)))
+    assert free.reason == "max_iterations" and free.restarts > limit
+
+
 @pytest.mark.parametrize("mode", ["diagonal", "jacobi", "both"])
 def test_lm_scaling_is_marquardt_damping(mode):
     """Column scaling s and lambda I on J diag(s) is the damped system

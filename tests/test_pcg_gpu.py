@@ -190,3 +190,23 @@ def test_exits_by_hand(device):
     res = hip_pcg(lambda v: v * float("nan"), bt, preconditioner=None, config=cfg)
     assert (res.iterations, res.restarts, res.residual_checks, res.reason, res.converged) == (
         1, 0, 1, "nonfinite", False)
+
+
+@pytest.mark.parametrize("limit", [0, 2])
+def test_refinement_limit_matches_oracle(device, limit):
+    """max_refinements on a float32-rounded diagonal operator (elementwise products
+    round identically here and in numpy): the oracle's counts and exit."""
+    from tests.test_oracle_pins import refinement_system
+
+    _, e, b = refinement_system()
+    e32 = torch.tensor(e, device=device).float()
+    cfg = dict(rtol=1e-12, max_iterations=1000, residual_check_interval=0, residual_drift_tolerance=1e30,
+               max_refinements=limit)
+    res = hip_pcg(lambda v: (e32 * v.float()).double(), torch.tensor(b, device=device), preconditioner=None,
+                  config=ConjugateGradientConfig(**cfg))
+    matvec, _, _ = refinement_system()
+    ref = pcg_batched(matvec, b, None, Config(**cfg))
+    assert (res.restarts, res.residual_checks, res.reason) == (limit, limit + 1, "stagnated")
+    assert (res.iterations, res.restarts, res.residual_checks, res.reason) == (
+        ref.iterations, ref.restarts, ref.residual_checks, ref.reason)
+

@@ -81,7 +81,8 @@ SMALL_BATCH = 4
 # least halve the true residual of the previous such restart; otherwise the
 # operator's own rounding (a float32 matvec under float64 vectors) bounds the
 # attainable residual and the solve stops as "stagnated" (usable).  Without it the
-# solve keeps restarting until max_iterations, as warp_pcg does.
+# solve keeps restarting until max_iterations, as warp_pcg does.  With
+# config.max_refinements, at most that many such restarts are made.
 STALL_FACTOR = 0.5
 # Systems up to this size run each iteration's vector work in one workgroup
 # (tlh_pcg_iteration_small accepts up to _lib.PCG_SMALL_MAX_N); larger ones spread
@@ -228,6 +229,8 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None):
     reason = "max_iterations"
     true_norm = math.nan
     converged_true = math.inf  # true residual at the last check after a converged recurrence
+    refinements = 0  # restarts after a converged recurrence
+    max_refinements = getattr(config, "max_refinements", None)
     batch = max(1, int(first_batch)) if first_batch else config.graph_batch_iterations
     while True:
         for _ in range(max(0, min(batch, stop_at - iterations))):
@@ -265,11 +268,15 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None):
             break
         if iterations >= config.max_iterations:
             break
-        if status == 1 and getattr(config, "stop_on_stagnation", False):
-            if true_norm > STALL_FACTOR * converged_true:
+        if status == 1:
+            if getattr(config, "stop_on_stagnation", False) and true_norm > STALL_FACTOR * converged_true:
+                reason = "stagnated"
+                break
+            if max_refinements is not None and refinements >= max_refinements:
                 reason = "stagnated"
                 break
             converged_true = true_norm
+            refinements += 1
         drift = math.sqrt(drift2) if drift2 >= 0 else math.nan
         if (
             status == 1

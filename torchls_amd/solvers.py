@@ -50,6 +50,12 @@ class ConjugateGradientConfig:
     # step, like "max_iterations").  This is where a float32 matvec under float64
     # PCG vectors reaches its rounding floor.
     stop_on_stagnation: bool = False
+    # Not in the reference either: when set, at most this many restarts follow a
+    # converged recurrence whose true residual misses the tolerance; the next such
+    # check ends the solve as "stagnated".  0 suits a float32 operator, whose true
+    # residual floor usually lies above rtol: the outer float64 iteration corrects
+    # the step instead of repeated refinement inside the solve.
+    max_refinements: int | None = None
 
     def __post_init__(self):
         if (isinstance(self.normal_max_packed_bytes, bool) or not isinstance(self.normal_max_packed_bytes, int)
@@ -82,6 +88,10 @@ class ConjugateGradientConfig:
             raise ValueError("matvec_dtype must be None, torch.float32 or torch.float64")
         if not isinstance(self.stop_on_stagnation, bool):
             raise ValueError("stop_on_stagnation must be a bool")
+        if self.max_refinements is not None and (
+                isinstance(self.max_refinements, bool) or not isinstance(self.max_refinements, int)
+                or self.max_refinements < 0):
+            raise ValueError("max_refinements must be None or a nonnegative integer")
 
 
 @dataclass(frozen=True)

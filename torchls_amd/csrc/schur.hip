@@ -257,6 +257,198 @@ __global__ void __launch_bounds__(NTH) schur_tile_kernel(SchurArgs<Real> a) {
   }
 }
 
+// ------------------------------------------- pipelined matvec (dense cameras)
+// The matvec of the tile kernel above (same arithmetic, same order: identical
+// results) for a DENSE camera slot, restructured so that memory latency is
+// hidden: the next tile's block, id, segment and inverse loads are issued before
+// the current tile is processed, into a second register buffer.  Every load is
+// issued on every path (indices clamped, values selected afterwards) and the tile
+// loop stores nothing to global memory, so the compiler's load waits stay exact
+// and two tiles of loads are in flight.  The workgroup's tile starts and camera
+// segment ranges are read into LDS once.
+template <typename Real, int R, int DC, int DP, bool XL>
+struct SchurTileRegs {
+  static constexpr int NM = DP * (DP + 1) / 2;
+  Real jc[R][DC], jp[R][DP];
+  int vp, vprev, vnext, ic, rk, cseg, clo, chi;
+  double mk[NM];
+};
+
+template <typename Real, int R, int DC, int DP, bool CFIRST, int NTH, bool XL>
+__device__ __forceinline__ void schur_load_tile(const SchurArgs<Real>& a, const int* tstart, const int* tseg, int t,
+                                                int j, int64_t count, SchurTileRegs<Real, R, DC, DP, XL>& d) {
+  constexpr int OC = CFIRST ? 0 : R * DP;
+  constexpr int OP = CFIRST ? R * DC : 0;
+  constexpr int NM = DP * (DP + 1) / 2;
+  const int pos = (int)threadIdx.x;
+  const int start = tstart[j], len = tstart[j + 1] - start;
+  const uint32_t k = (uint32_t)max(0, min(pos, len - 1));
+  const Real* tile = a.packed + start;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+#pragma unroll
+    for (int c = 0; c < DC; ++c) d.jc[r][c] = tile[(int64_t)(OC + r * DC + c) * a.stride + k];
+#pragma unroll
+    for (int c = 0; c < DP; ++c) d.jp[r][c] = tile[(int64_t)(OP + r * DP + c) * a.stride + k];
+  }
+  const int64_t n = (int64_t)start + k;
+  d.vp = a.idx_p[n];
+  d.vprev = a.idx_p[n > 0 ? n - 1 : 0];
+  d.vnext = a.idx_p[n + 1 < count ? n + 1 : n];
+  d.ic = a.idx_c[n];
+  const bool ranked = a.rank_c != nullptr;
+  const int rv = (ranked ? a.rank_c : reinterpret_cast<const uint16_t*>(a.idx_c))[n];
+  d.rk = ranked ? rv : (int)k;
+  // this thread's camera segment (a tile of at most NTH observations has at most NTH)
+  const int cb = tseg[j], ce = tseg[j + 1];
+  const int si = max(cb, min(cb + pos, ce - 1));
+  const int tg = a.seg_target_c[si], lo = a.seg_start_c[si], hi = a.seg_start_c[min(si + 1, ce - 1)];
+  d.cseg = cb + pos < ce ? tg : -1;
+  d.clo = lo;
+  d.chi = cb + pos + 1 < ce ? hi : len;
+#pragma unroll
+  for (int e = 0; e < NM; ++e) d.mk[e] = a.cinv[(int64_t)d.vp * NM + e];
+  (void)t;
+}
+
+template <typename Real, int R, int DC, int DP, bool CFIRST, int NTH, bool XL>
+__global__ void __launch_bounds__(NTH) schur_matvec_pipe_kernel(SchurArgs<Real> a, int64_t count) {
+  constexpr int T = NTH;
+  constexpr int NW = NTH / 64;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* tail = reinterpret_cast<double*>(smem_raw);  // [NW][DP + 1]
+  int* tail_flag = reinterpret_cast<int*>(tail + NW * (DP + 1));
+  Real* stage = reinterpret_cast<Real*>(tail_flag + NW * 2);
+  Real* sstage = stage + T * DC;
+  Real* acc = sstage + T * DP;
+  Real* xl = acc + (int64_t)a.count_c * DC;
+  int* tstart = reinterpret_cast<int*>(XL ? xl + (int64_t)a.count_c * DC : xl);  // [tiles_per_wg + 2]
+  int* tseg = tstart + a.tiles_per_wg + 2;                                        // [tiles_per_wg + 2]
+  const int pos = (int)threadIdx.x;
+  const int wave = pos >> 6, lane = pos & 63;
+  const int t0 = blockIdx.x * a.tiles_per_wg;
+  const int t1 = min(t0 + a.tiles_per_wg, a.ntiles);
+  for (int i = threadIdx.x; i < a.count_c * DC; i += NTH) acc[i] = Real(0);
+  if constexpr (XL) {
+    for (int i = threadIdx.x; i < a.count_c * DC; i += NTH) xl[i] = a.xc[i];
+  }
+  // tile starts and camera segment ranges of tiles t0 .. t1 (entry t1 - t0 + 1
+  // repeats the last tile's end: the padding half of an odd tile count)
+  for (int i = threadIdx.x; i <= t1 - t0 + 1; i += NTH) {
+    const int tt = min(t0 + i, t1);
+    tstart[i] = a.tile_start[tt];
+    tseg[i] = a.seg_ptr_c[tt];
+  }
+  __syncthreads();
+
+  auto process = [&](SchurTileRegs<Real, R, DC, DP, XL>& cur, SchurTileRegs<Real, R, DC, DP, XL>& nxt, int t) {
+    // next tile's loads first (a re-read of this one, discarded, after the last tile)
+    const int jn = t + 1 < t1 ? t + 1 - t0 : t - t0;
+    schur_load_tile<Real, R, DC, DP, CFIRST, NTH, XL>(a, tstart, tseg, t + 1, min(jn, t1 - 1 - t0), count, nxt);
+    const int j = t - t0;
+    const int len = t < t1 ? tstart[j + 1] - tstart[j] : 0;
+    const bool valid = pos < len;
+    const int vp = cur.vp;
+    const bool first = !valid || pos == 0 || cur.vprev != vp;
+    const bool last = valid && (pos == len - 1 || cur.vnext != vp);
+    Real u[R];
+    {
+      const Real* x = XL ? xl + cur.ic * DC : a.xc + (int64_t)cur.ic * DC;
+      Real xv[DC];
+#pragma unroll
+      for (int c = 0; c < DC; ++c) xv[c] = x[c];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        Real s = Real(0);
+#pragma unroll
+        for (int c = 0; c < DC; ++c) s += cur.jc[r][c] * xv[c];
+        u[r] = s;
+      }
+    }
+    double w[DP + 1];
+#pragma unroll
+    for (int c = 0; c < DP; ++c) {
+      Real s = Real(0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) s += cur.jp[r][c] * u[r];
+      w[c] = valid ? (double)s : 0.0;
+    }
+    w[DP] = first ? (double)pos : 0.0;
+    int flag = first ? 1 : 0;
+    wave_segmented_scan<double, DP + 1>(flag, w);
+    if (lane == 63) {
+#pragma unroll
+      for (int c = 0; c <= DP; ++c) tail[wave * (DP + 1) + c] = w[c];
+      tail_flag[wave] = flag;
+    }
+    __syncthreads();
+    if (!flag) {
+      for (int k = wave - 1; k >= 0; --k) {
+#pragma unroll
+        for (int c = 0; c <= DP; ++c) w[c] += tail[k * (DP + 1) + c];
+        if (tail_flag[k]) break;
+      }
+    }
+    const int fp = (int)w[DP];
+    if (last) {
+      double sp[DP];
+#pragma unroll
+      for (int i = 0; i < DP; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < DP; ++jj) s += sym<DP>(cur.mk, i, jj) * w[jj];
+        sp[i] = s;
+      }
+#pragma unroll
+      for (int c = 0; c < DP; ++c) sstage[fp * DP + c] = (Real)sp[c];
+    }
+    __syncthreads();
+    if (valid) {
+      Real sv[DP];
+#pragma unroll
+      for (int c = 0; c < DP; ++c) sv[c] = sstage[fp * DP + c];
+      Real q[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        Real s = u[r];
+#pragma unroll
+        for (int c = 0; c < DP; ++c) s -= cur.jp[r][c] * sv[c];
+        q[r] = s;
+      }
+#pragma unroll
+      for (int c = 0; c < DC; ++c) {
+        Real s = Real(0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) s += cur.jc[r][c] * q[r];
+        stage[cur.rk * DC + c] = s;
+      }
+    }
+    __syncthreads();
+    if (t < t1 && cur.cseg >= 0) {
+      Real sum[DC];
+#pragma unroll
+      for (int c = 0; c < DC; ++c) sum[c] = stage[cur.clo * DC + c];
+      for (int i = cur.clo + 1; i < cur.chi; ++i)
+#pragma unroll
+        for (int c = 0; c < DC; ++c) sum[c] += stage[i * DC + c];
+#pragma unroll
+      for (int c = 0; c < DC; ++c) acc[cur.cseg * DC + c] += sum[c];
+    }
+  };
+  SchurTileRegs<Real, R, DC, DP, XL> ta, tb;
+  if (t0 < t1) schur_load_tile<Real, R, DC, DP, CFIRST, NTH, XL>(a, tstart, tseg, t0, 0, count, ta);
+  // pairs of tiles with no branch between the halves (a conditional half would
+  // move loop-carried registers and force waits for loads in flight)
+  for (int t = t0; t < t1; t += 2) {
+    process(ta, tb, t);
+    process(tb, ta, t + 1);
+  }
+  __syncthreads();
+  const int64_t nn = (int64_t)a.count_c * DC;
+  Real* out = a.part_c + (int64_t)(a.dense_block_c + blockIdx.x) * nn;
+  for (int64_t i = threadIdx.x; i < nn; i += NTH) out[i] = acc[i];
+}
+
 // ------------------------------------------------ preconditioner edge blocks
 // Per tile (all-SPARSE plan): each (camera, point) edge head forms
 // W_e = sum_{obs in edge} J_c^T J_p (float64), then Q = W_e C_p^-1 W_e^T
@@ -715,6 +907,19 @@ int launch_schur_t(const tlh_operator_t* op, const tlh_group_t& g, int sc, int s
   const int64_t slice = op->types[g.slot[sc].type].count * DC * (int64_t)sizeof(Real);
   const bool dense = MODE != SCHUR_BACKSUB && g.slot[sc].mode == TLH_REDUCE_DENSE;
   if (dense) lds += slice;
+  if (MODE == SCHUR_MATVEC && dense) {
+    // pipelined matvec: its tile starts and camera segment ranges also go to LDS
+    const int64_t info = 2 * (int64_t)(g.tiles_per_wg + 2) * sizeof(int);
+    const bool pxl = lds + slice + info <= device_lds_max();
+    const int64_t plds = lds + (pxl ? slice : 0) + info;
+    auto pk = pxl ? schur_matvec_pipe_kernel<Real, R, DC, DP, CFIRST, NTH, true>
+                  : schur_matvec_pipe_kernel<Real, R, DC, DP, CFIRST, NTH, false>;
+    TLH_RETURN_IF_ERROR(allow_lds((const void*)pk, plds));
+    hipLaunchKernelGGL(pk, dim3((unsigned)g.nwg), dim3(NTH), (size_t)plds, s,
+                       make_schur_args<Real>(op, g, sc, sp, xc, bp, outp, cinv), (int64_t)g.count);
+    TLH_RETURN_IF_LAUNCH_FAILED();
+    return 0;
+  }
   // the retained x slice goes to LDS (after the accumulator region) when it fits
   const bool xl = MODE != SCHUR_RHS && dense && lds + slice <= device_lds_max();
   if (xl) lds += slice;

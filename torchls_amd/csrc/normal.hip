@@ -353,7 +353,7 @@ __device__ __forceinline__ void load_wave_ids(const WaveArgs<Real>& a, int t, in
 // Block loads of tile t: no dependence on anything loaded before them, so they
 // are issued a whole tile ahead of their use.
 template <typename Real, int R, int DP, int DS, int OP, int OS, int ITEMS, int NT, bool STREAM, bool XL>
-__device__ __forceinline__ void load_wave_blocks(const WaveArgs<Real>& a, int t, int b,
+__device__ __forceinline__ void load_wave_blocks(const WaveArgs<Real>& a, int t, int64_t stride, int b,
                                                  WaveTile<Real, R, DP, DS, ITEMS, XL>& d) {
   constexpr int T = NT * ITEMS;
   const Real* tile = a.packed + (int64_t)t * T;
@@ -362,10 +362,10 @@ __device__ __forceinline__ void load_wave_blocks(const WaveArgs<Real>& a, int t,
   for (int r = 0; r < R; ++r) {
 #pragma unroll
     for (int c = 0; c < DP; ++c)
-      load_lane<Real, ITEMS, STREAM>(tile + (int64_t)(OP + r * DP + c) * a.stride, off, d.jp[r][c]);
+      load_lane<Real, ITEMS, STREAM>(tile + (int64_t)(OP + r * DP + c) * stride, off, d.jp[r][c]);
 #pragma unroll
     for (int c = 0; c < DS; ++c)
-      load_lane<Real, ITEMS, STREAM>(tile + (int64_t)(OS + r * DS + c) * a.stride, off, d.js[r][c]);
+      load_lane<Real, ITEMS, STREAM>(tile + (int64_t)(OS + r * DS + c) * stride, off, d.js[r][c]);
   }
 }
 
@@ -495,11 +495,11 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
                      int t) {
     // The loads are unconditional so that the compiler's wait counts stay exact
     // (a skipped load on one path makes every later wait conservative): after
-    // the workgroup's last tile they re-read a valid tile (its last, or the
-    // group's last), whose data is not used.
-    const int tn = min(t + 1 < t1 ? t + 1 : t, a.ntiles - 1);
+    // the workgroup's last tile they read the group's first tile, which every
+    // workgroup reads there (so it stays in the L2 caches), and discard it.
+    const int tn = t + 1 < t1 ? t + 1 : 0;
     load_wave_ids<Real, ITEMS, NT, FAST>(a, tn, b, nxt);
-    load_wave_blocks<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, tn, b, nxt);
+    load_wave_blocks<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, tn, a.stride, b, nxt);
     pin_blocks(cur);
     const int64_t start = (int64_t)t * T;
     // t == t1 is the padding half of the last pair (an odd tile count): it
@@ -727,7 +727,7 @@ __global__ void __launch_bounds__(NT) normal_wave_kernel(WaveArgs<Real> a) {
   WaveTile<Real, R, DP, DS, ITEMS, XL> ta, tb;
   if (t0 < t1) {
     load_wave_ids<Real, ITEMS, NT, FAST>(a, t0, b, ta);
-    load_wave_blocks<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, t0, b, ta);
+    load_wave_blocks<Real, R, DP, DS, OP, OS, ITEMS, NT, STREAM, XL>(a, t0, a.stride, b, ta);
     load_wave_gathers<Real, R, DP, DS, ITEMS, XL>(a, ta);
   }
   if constexpr (XL) {

@@ -815,7 +815,7 @@ __global__ void __launch_bounds__(TLH_THREADS)
 // shapes below; a workgroup finds its job from the jobs' block counts.
 constexpr int FIN_COLS = 32;
 constexpr int FIN_COLS_NARROW = 8;
-enum FinishRole { FIN_LIST = 0, FIN_WAVE = 1, FIN_DENSE = 2, FIN_DENSE_NARROW = 3 };
+enum FinishRole { FIN_LIST = 0, FIN_WAVE = 1, FIN_DENSE = 2, FIN_DENSE_NARROW = 3, FIN_DENSE_VEC = 4 };
 
 template <typename Real>
 struct FinishJob {
@@ -886,6 +886,55 @@ __device__ __forceinline__ void finish_dense(const FinishJob<Real>& j, Real damp
   }
 }
 
+// Dense blocks of a wide type, vectorised: a workgroup owns FIN_VCOLS
+// consecutive columns (FIN_QUADS vectors of 16 bytes); its FIN_RANGES block
+// ranges each sum a contiguous run of dense blocks with every load of the run in
+// flight (each wavefront reads FIN_RANGES / 4 rows of 256 contiguous bytes per
+// instruction), then the range sums are added in range order.  Requires the
+// type's column count to be a multiple of the vector width.
+constexpr int FIN_QUADS = 16;
+constexpr int FIN_RANGES = TLH_THREADS / FIN_QUADS;
+
+template <typename Real>
+constexpr int fin_vec() {
+  return 16 / (int)sizeof(Real);
+}
+
+template <typename Real>
+__device__ __forceinline__ void finish_dense_vec(const FinishJob<Real>& j, Real damping, int64_t bid, Real* lane_sum) {
+  constexpr int V = fin_vec<Real>();
+  constexpr int COLS = FIN_QUADS * V;
+  typedef typename VecT<Real, V>::type Vec;
+  const int quad = threadIdx.x % FIN_QUADS;
+  const int range = threadIdx.x / FIN_QUADS;
+  const int64_t total = j.count * j.dim;
+  const int64_t i0 = bid * COLS + (int64_t)quad * V;  // first of this thread's V columns
+  const int per = (j.dense_blocks + FIN_RANGES - 1) / FIN_RANGES;
+  const int b0 = range * per, b1 = min(j.dense_blocks, b0 + per);
+  Vec sum = (Vec)Real(0);
+  if (i0 < total) {
+    const Real* col = j.partials + i0;
+#pragma unroll 8
+    for (int b = b0; b < b1; ++b) sum += *reinterpret_cast<const Vec*>(col + (int64_t)b * total);
+  }
+#pragma unroll
+  for (int c = 0; c < V; ++c) lane_sum[range * COLS + quad * V + c] = sum[c];
+  __syncthreads();
+  if (threadIdx.x < COLS) {
+    const int64_t i = bid * COLS + threadIdx.x;
+    if (i < total) {
+      Real acc = lane_sum[threadIdx.x];
+#pragma unroll
+      for (int r = 1; r < FIN_RANGES; ++r) acc += lane_sum[r * COLS + threadIdx.x];
+      const int64_t v = i / j.dim;
+      const int c = (int)(i - v * j.dim);
+      const int k1 = j.csr_ptr[v + 1];
+      for (int k = j.csr_ptr[v]; k < k1; ++k) acc += j.partials[(int64_t)j.csr_idx[k] * j.dim + c];
+      j.y[i] = acc + damping * j.x[i];
+    }
+  }
+}
+
 // One wavefront per (variable, component) for types with long partial lists.
 template <typename Real>
 __device__ __forceinline__ void finish_wave(const FinishJob<Real>& j, Real damping, int64_t bid) {
@@ -905,7 +954,7 @@ __device__ __forceinline__ void finish_wave(const FinishJob<Real>& j, Real dampi
 
 template <typename Real>
 __global__ void __launch_bounds__(TLH_THREADS) finish_jobs_kernel(FinishJobs<Real> jobs) {
-  __shared__ Real lane_sum[TLH_THREADS];
+  __shared__ Real lane_sum[TLH_THREADS * fin_vec<Real>()];  // FIN_DENSE_VEC: [FIN_RANGES][FIN_QUADS * vector]
   int64_t bid = blockIdx.x;
   int k = 0;
   while (k + 1 < jobs.njobs && bid >= jobs.job[k].blocks) {
@@ -916,6 +965,7 @@ __global__ void __launch_bounds__(TLH_THREADS) finish_jobs_kernel(FinishJobs<Rea
   switch (j.role) {  // uniform over the workgroup
     case FIN_DENSE: finish_dense<Real, FIN_COLS>(j, jobs.damping, bid, lane_sum); break;
     case FIN_DENSE_NARROW: finish_dense<Real, FIN_COLS_NARROW>(j, jobs.damping, bid, lane_sum); break;
+    case FIN_DENSE_VEC: finish_dense_vec<Real>(j, jobs.damping, bid, lane_sum); break;
     case FIN_WAVE: finish_wave<Real>(j, jobs.damping, bid); break;
     default: finish_list<Real>(j, jobs.damping, bid); break;
   }
@@ -1276,8 +1326,12 @@ int finish_impl(const tlh_operator_t* op, int t0, int t1, const void* x, double 
     int64_t nb;
     if (t.dense_blocks >= 2 * (TLH_THREADS / FIN_COLS) && !t.finish_list) {
       const bool narrow = ceil_div(total, FIN_COLS) < 256;
-      j.role = narrow ? FIN_DENSE_NARROW : FIN_DENSE;
-      nb = ceil_div(total, narrow ? FIN_COLS_NARROW : FIN_COLS);
+      constexpr int vcols = FIN_QUADS * fin_vec<Real>();
+      // vector loads need 16-byte aligned rows (total * sizeof(Real) % 16 == 0);
+      // enough column groups to spread over the chip
+      const bool vec = total % fin_vec<Real>() == 0 && ceil_div(total, vcols) >= 128;
+      j.role = vec ? FIN_DENSE_VEC : (narrow ? FIN_DENSE_NARROW : FIN_DENSE);
+      nb = ceil_div(total, vec ? vcols : (narrow ? FIN_COLS_NARROW : FIN_COLS));
     } else if (csr_heavy(t.csr_entries, nlist)) {
       j.role = FIN_WAVE;
       nb = ceil_div(nlist * t.dim, TLH_THREADS / 64);

@@ -337,7 +337,9 @@ def implicit_large(device, lm_steps, probes=8):
     def loss_fn(cams):
         return se3_log(torch.linalg.solve(truth, cams)).square().sum()
 
-    opts = dict(conjugate_gradient=ConjugateGradientConfig(matvec_dtype=torch.float32, **CG_OPTIONS),
+    # the adjoint's Gauss-Newton solves keep their float32 refinement restarts (no
+    # max_refinements): the exact-Hessian refinement starts from their solution
+    opts = dict(conjugate_gradient=ConjugateGradientConfig(matvec_dtype=torch.float32, stop_on_stagnation=True),
                 termination=TerminationConfig(max_iterations=lm_steps))
     init = {k: v.clone() for k, v in P.parameters.items()}
     out = {}

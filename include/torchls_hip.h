@@ -298,6 +298,36 @@ int tlh_pcg_iteration_small(const tlh_block_diag_t* pc, int mode, void* x, void*
 int tlh_pcg_true_residual(const void* rhs, const void* ax, const void* r, void* true_r, int64_t n,
                           int dtype, const tlh_pcg_state_t* st, tlh_stream_t stream);
 
+/* Operator of a native PCG batch (tlh_pcg_iterations).
+ * TLH_PCG_OP_NORMAL: ap = J^T J p + damping p (tlh_normal_matvec of op).
+ * TLH_PCG_OP_SCHUR: the implicit Schur operator of a Schur plan: tlh_schur_apply
+ * (MATVEC) over W-group wgroup with eliminated slot eslot and inverses cinv,
+ * tlh_group_tiles of every other group of op, then tlh_finish_types of the
+ * retained types [0, type_end) with base p scaled by damping. */
+#define TLH_PCG_OP_NORMAL 0
+#define TLH_PCG_OP_SCHUR 1
+typedef struct tlh_pcg_operator {
+  int32_t kind;
+  int32_t wgroup;
+  int32_t eslot;
+  int32_t type_end;
+  const tlh_operator_t* op;
+  const double* cinv;
+  double damping;
+} tlh_pcg_operator_t;
+
+/* count PCG iterations issued back to back on stream, no host read or
+ * synchronisation: ap = A p, then the iteration update -- for n <= small_max
+ * (and n <= 16384) tlh_pcg_iteration_small, else tlh_dot(p, ap, CURVATURE) +
+ * tlh_pcg_step + tlh_pcg_direction; mode 0: block-Jacobi pc applied in the step,
+ * 1: no preconditioner.  Iterations after the control word stopped are frozen
+ * by the device (as replayed graph iterations are).  ap is an [n] workspace.
+ * Replaces: the graph replays of one batch, warp_pcg.py:184-185, with the
+ * iteration's launches issued from native code. */
+int tlh_pcg_iterations(const tlh_pcg_operator_t* A, const tlh_block_diag_t* pc, int mode, void* x, void* r,
+                       void* z, void* p, void* ap, int64_t n, int dtype, const tlh_pcg_state_t* st,
+                       int64_t small_max, int32_t count, tlh_stream_t stream);
+
 /* ---------------------------------------------------------------- LM float64 reductions */
 
 /* Partial sums of g = J^T v for one group of an all-SPARSE plan, in float64, read

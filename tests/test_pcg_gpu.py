@@ -87,6 +87,56 @@ def test_graph_without_warmup_matches_eager(device):
     assert torch.equal(a.solution, b.solution) and a.iterations == b.iterations
 
 
+@pytest.mark.parametrize("size", [(16, 500, 8), (40, 6000, 6)])
+@pytest.mark.parametrize("precondition", [True, False])
+def test_native_batches_match_graph_and_eager(device, size, precondition):
+    """tlh_pcg_iterations issues the captured iteration's launches in the same order:
+    native, graph-replayed and eager solves are bitwise equal (small one-workgroup
+    and multi-workgroup update paths, fused block Jacobi and none)."""
+    print(f"pcg native batches size={size} precondition={precondition}", flush=True)
+    from torchls_amd.hip_pcg import native_operator
+
+    J, BJ, op, diag, g, lam = _system(device, size=size)
+    pre = HipBlockJacobi(BJ.layout, diag, lam) if precondition else None
+    rhs = torch.tensor(-g, device=device)
+    cfg = ConjugateGradientConfig(residual_check_interval=10, graph_batch_iterations=3, max_iterations=40)
+    mv = partial(op.normal_matvec, damping=lam)
+    assert native_operator(mv) is not None and native_operator(lambda v: op.normal_matvec(v, lam)) is None
+    a = hip_pcg(mv, rhs, preconditioner=pre, config=cfg, native=True)
+    b = hip_pcg(mv, rhs, preconditioner=pre, config=cfg, native=False, use_graph=True)
+    c = hip_pcg(mv, rhs, preconditioner=pre, config=cfg, native=False, use_graph=False)
+    assert torch.equal(a.solution, b.solution) and torch.equal(a.solution, c.solution)
+    assert (a.iterations, a.restarts, a.residual_checks, a.reason) == (
+        b.iterations, b.restarts, b.residual_checks, b.reason) == (
+        c.iterations, c.restarts, c.residual_checks, c.reason)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_native_schur_batches_match_graph(device, dtype):
+    """The Schur system's native batches (W-group pass, other groups' tiles, finish,
+    fused update) equal its graph-replayed iterations bitwise, with priors (a second
+    retained-only group)."""
+    print(f"pcg native schur batches dtype={dtype}", flush=True)
+    from tests.test_schur_gpu import build
+    from torchls_amd.hip_pcg import native_operator
+    from torchls_amd.schur import HipSchurPlan
+
+    prob = generators.bundle_adjustment(16, 500, 8, seed=21)
+    P = build(prob, device, priors=True)
+    an = P._analyze()
+    lin = an.linearize(P.parameters)
+    diag = tuple(d.double() for d in lin.jacobian.diagonal_blocks())
+    system = HipSchurPlan(an, "points", device, dtype=dtype).linearize(lin.jacobian, diag).damped(1e-2)
+    rhs = system.reduced_rhs(lin.jacobian.rmatvec(lin.residual).to(torch.float64), scale=-1.0)
+    assert native_operator(system.matvec) is not None
+    cfg = ConjugateGradientConfig(residual_check_interval=7, graph_batch_iterations=5)
+    a = hip_pcg(system.matvec, rhs, preconditioner=system.preconditioner(), config=cfg, native=True)
+    b = hip_pcg(system.matvec, rhs, preconditioner=system.preconditioner(), config=cfg, native=False)
+    assert torch.equal(a.solution, b.solution)
+    assert (a.iterations, a.restarts, a.residual_checks, a.reason) == (
+        b.iterations, b.restarts, b.residual_checks, b.reason)
+
+
 def test_deferred_positive_definiteness_check(device):
     J, BJ, op, diag, g, lam = _system(device)
     bad = tuple(-d.double() for d in diag)

@@ -125,6 +125,22 @@ class PcgStateDesc(ctypes.Structure):
     _fields_ = [("control", c_void_p), ("scalars", c_void_p), ("partials", c_void_p), ("counter", c_void_p)]
 
 
+PCG_OP_NORMAL = 0
+PCG_OP_SCHUR = 1
+
+
+class PcgOperator(ctypes.Structure):
+    _fields_ = [
+        ("kind", i32),
+        ("wgroup", i32),
+        ("eslot", i32),
+        ("type_end", i32),
+        ("op", ctypes.POINTER(Operator)),
+        ("cinv", c_void_p),
+        ("damping", ctypes.c_double),
+    ]
+
+
 class DeviceInfo(ctypes.Structure):
     _fields_ = [("cu_count", i32), ("lds_per_block", i32), ("warp_size", i32), ("gcn_arch", i32)]
 
@@ -192,6 +208,11 @@ SIGNATURES = {
     "tlh_pcg_true_residual": (
         ctypes.c_int,
         [c_void_p, c_void_p, c_void_p, c_void_p, i64, ctypes.c_int, ctypes.POINTER(PcgStateDesc), c_void_p],
+    ),
+    "tlh_pcg_iterations": (
+        ctypes.c_int,
+        [ctypes.POINTER(PcgOperator), ctypes.POINTER(BlockDiag), ctypes.c_int, c_void_p, c_void_p, c_void_p,
+         c_void_p, c_void_p, i64, ctypes.c_int, ctypes.POINTER(PcgStateDesc), i64, i32, c_void_p],
     ),
     "tlh_rmatvec_f64": (
         ctypes.c_int,

@@ -827,3 +827,48 @@ extern "C" int tlh_block_apply(const double* inverse, int dim, const int32_t* li
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }
+
+// Native batch of PCG iterations: the launches of one iteration (operator,
+// then the fused update) issued back to back from here instead of a replayed
+// graph, so that a batch costs one host call.  The operator entry points below
+// are this library's own (normal.hip, schur.hip).
+extern "C" int tlh_pcg_iterations(const tlh_pcg_operator_t* A, const tlh_block_diag_t* pc, int mode, void* x,
+                                  void* r, void* z, void* p, void* ap, int64_t n, int dtype,
+                                  const tlh_pcg_state_t* st, int64_t small_max, int32_t count,
+                                  tlh_stream_t stream) {
+  if (!A || !A->op || !x || !r || !z || !p || !ap || n < 1 || count < 0 || !state_ok(st)) return TLH_EINVAL;
+  if (mode != 0 && mode != 1) return TLH_EINVAL;
+  if (mode == 0 && (!pc || !precond_ok(pc, n))) return TLH_EINVAL;
+  if (dtype != TLH_F32 && dtype != TLH_F64) return TLH_EINVAL;
+  if (A->kind == TLH_PCG_OP_NORMAL) {
+    if (A->op->size != n) return TLH_EINVAL;
+  } else if (A->kind == TLH_PCG_OP_SCHUR) {
+    if (!A->cinv || A->wgroup < 0 || A->wgroup >= A->op->ngroups || A->type_end < 1 ||
+        A->type_end > A->op->ntypes)
+      return TLH_EINVAL;
+  } else {
+    return TLH_EINVAL;
+  }
+  const bool small = n <= small_max && n <= SMALL_MAX_N;
+  for (int32_t it = 0; it < count; ++it) {
+    int rc = 0;
+    if (A->kind == TLH_PCG_OP_NORMAL) {
+      rc = tlh_normal_matvec(A->op, p, ap, A->damping, stream);
+    } else {
+      rc = tlh_schur_apply(A->op, A->wgroup, A->eslot, TLH_SCHUR_MATVEC, p, nullptr, nullptr, A->cinv, stream);
+      for (int32_t g = 0; rc == 0 && g < A->op->ngroups; ++g)
+        if (g != A->wgroup) rc = tlh_group_tiles(A->op, g, p, stream);
+      if (rc == 0) rc = tlh_finish_types(A->op, 0, A->type_end, p, A->damping, ap, stream);
+    }
+    if (rc != 0) return rc;
+    if (small) {
+      rc = tlh_pcg_iteration_small(pc, mode, x, r, z, p, ap, n, dtype, st, stream);
+    } else {
+      rc = tlh_dot(p, ap, n, dtype, st, TLH_PCG_CURVATURE, stream);
+      if (rc == 0) rc = tlh_pcg_step(pc, mode, x, r, z, p, ap, n, dtype, st, stream);
+      if (rc == 0) rc = tlh_pcg_direction(p, z, n, dtype, st, stream);
+    }
+    if (rc != 0) return rc;
+  }
+  return 0;
+}

@@ -270,6 +270,7 @@ class HipNormalLinearization:
     """One linearization packed into the plan layout (WarpNormalLinearization)."""
 
     hip_graph_safe = True  # normal_matvec launches only HIP kernels on the current stream
+    pcg_method = "normal_matvec"  # described by pcg_operator for native PCG batches
 
     def __init__(self, plan: HipNormalPlan, jacobian, col_scale: torch.Tensor | None = None):
         self.plan = plan
@@ -314,6 +315,15 @@ class HipNormalLinearization:
             "tlh_normal_matvec",
         )
         return out
+
+    def pcg_operator(self, damping: float = 0.0) -> "_lib.PcgOperator":
+        """Descriptor of ``normal_matvec(., damping)`` for the native PCG batch
+        (tlh_pcg_iterations)."""
+        A = _lib.PcgOperator()
+        A.kind = _lib.PCG_OP_NORMAL
+        A.op = ctypes.pointer(self.op)
+        A.damping = float(damping)
+        return A
 
     def normal_matvec(self, vector: torch.Tensor, damping: float = 0.0) -> torch.Tensor:
         self._check_vector(vector)

@@ -12,12 +12,19 @@ follows warp_pcg._run line by line.
 
 Fast paths: a ``HipBlockJacobi`` preconditioner is applied inside the step
 kernel; any other callable is called (and captured) like in the reference.
-The matvec is any graph-capturable callable returning a new tensor.
+The matvec is any graph-capturable callable returning a new tensor.  When it
+is one of this library's operators (an object with ``pcg_operator``: the
+Schur system's ``matvec``, a normal linearization's ``normal_matvec``, bare or
+as a ``functools.partial`` binding ``damping``) and the preconditioner is
+fused or absent, a batch is issued natively by ``tlh_pcg_iterations`` (the
+same launches as the captured iteration, in the same order, one host call per
+batch) instead of graph replays.
 """
 
 from __future__ import annotations
 
 import ctypes
+import functools
 import math
 import threading
 
@@ -91,12 +98,31 @@ STALL_FACTOR = 0.5
 SMALL_PREFERRED_N = 4096
 
 
-def hip_pcg(matvec, rhs, *, preconditioner, config, use_graph: bool = True, first_batch: int | None = None):
+def native_operator(matvec):
+    """The ``_lib.PcgOperator`` of a library operator's matvec, or None: a bound
+    method named by its owner's ``pcg_method``, bare or as a partial that binds
+    only ``damping``."""
+    fn, keywords = matvec, {}
+    if isinstance(matvec, functools.partial):
+        if matvec.args or set(matvec.keywords) - {"damping"}:
+            return None
+        fn, keywords = matvec.func, matvec.keywords
+    owner = getattr(fn, "__self__", None)
+    method = getattr(owner, "pcg_method", None)
+    if method is None or getattr(fn, "__name__", None) != method:
+        return None
+    return owner.pcg_operator(float(keywords.get("damping", 0.0)))
+
+
+def hip_pcg(matvec, rhs, *, preconditioner, config, use_graph: bool = True, first_batch: int | None = None,
+            native: bool = True):
     """Solve A x = rhs by PCG on the GPU; returns a PCGResult like warp_pcg.
 
     ``first_batch`` (optional) is the number of iterations replayed before the
     first host read, e.g. the previous solve's count; it changes how often the
-    host reads the control word, never the iterates.
+    host reads the control word, never the iterates.  ``native``: issue the
+    batches of a library operator by ``tlh_pcg_iterations`` (else graph replays
+    when ``use_graph``, else eager launches); all three give identical iterates.
     """
     if rhs.device.type != "cuda" or rhs.dtype not in (torch.float32, torch.float64):
         raise ValueError("HIP PCG requires GPU float32 or float64 tensors")
@@ -105,13 +131,13 @@ def hip_pcg(matvec, rhs, *, preconditioner, config, use_graph: bool = True, firs
     stream.wait_stream(caller)
     with torch.cuda.stream(stream):
         result = _run(matvec, rhs, preconditioner=preconditioner, config=config, use_graph=use_graph,
-                      first_batch=first_batch)
+                      first_batch=first_batch, native=native)
     caller.wait_stream(stream)
     result.solution.record_stream(caller)
     return result
 
 
-def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None):
+def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None, native=True):
     from .solvers import PCGResult
 
     lib = _lib.load()
@@ -196,8 +222,18 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None):
     # library's kernels (hip_graph_safe) need no warm-up: nothing is initialised
     # lazily, and the capture records without executing, so the state set by
     # the reset above is still intact.
+    op_desc = native_operator(matvec) if native and (fused or apply is None) else None
+    if op_desc is not None:
+        use_graph = False
+        ap_work = torch.empty_like(rhs)
+        op_ref = ctypes.byref(op_desc)
+
+        def run_batch(count):
+            _lib.check(lib.tlh_pcg_iterations(op_ref, pc, 0 if fused else 1, x.data_ptr(), r.data_ptr(),
+                                              z.data_ptr(), p.data_ptr(), ap_work.data_ptr(), n, code, sd,
+                                              SMALL_PREFERRED_N, count, sh), "tlh_pcg_iterations")
     owner = getattr(getattr(matvec, "func", matvec), "__self__", None)
-    warm = not (use_graph and getattr(owner, "hip_graph_safe", False) and (fused or apply is None))
+    warm = op_desc is None and not (use_graph and getattr(owner, "hip_graph_safe", False) and (fused or apply is None))
     if warm:
         iteration()
     graph = None
@@ -233,11 +269,16 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None):
     max_refinements = getattr(config, "max_refinements", None)
     batch = max(1, int(first_batch)) if first_batch else config.graph_batch_iterations
     while True:
-        for _ in range(max(0, min(batch, stop_at - iterations))):
-            if graph is not None:
-                graph.replay()
-            else:
-                iteration()
+        count = max(0, min(batch, stop_at - iterations))
+        if op_desc is not None:
+            if count:
+                run_batch(count)
+        else:
+            for _ in range(count):
+                if graph is not None:
+                    graph.replay()
+                else:
+                    iteration()
         batch = min(SMALL_BATCH, config.graph_batch_iterations)
         rhs2, true2, drift2, it, status = check_and_read()
         if rhs_norm is None:

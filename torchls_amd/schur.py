@@ -281,6 +281,7 @@ class HipSchurSystem:
     iteration (the reference raises from torch.linalg.cholesky at this point)."""
 
     hip_graph_safe = True  # matvec launches only HIP kernels on the current stream
+    pcg_method = "matvec"  # described by pcg_operator for native PCG batches
 
     def __init__(self, lin: HipSchurLinearization, damping: float, *, defer_check: bool = False):
         self.lin, self.plan, self.damping = lin, lin.plan, float(damping)
@@ -355,6 +356,18 @@ class HipSchurSystem:
         self._apply(_lib_mode("MATVEC"), x, None, None)
         self._others(x)
         return self._finish(x, self.damping)
+
+    def pcg_operator(self, damping=None) -> "_lib.PcgOperator":
+        """Descriptor of ``matvec`` for the native PCG batch (tlh_pcg_iterations); the
+        system's own damping applies (``damping`` is ignored)."""
+        p = self.plan
+        A = _lib.PcgOperator()
+        A.kind = _lib.PCG_OP_SCHUR
+        A.wgroup, A.eslot, A.type_end = p.wgroup, p.eslot, p.ntypes_ret
+        A.op = ctypes.pointer(self.lin.op)
+        A.cinv = self.cinv.data_ptr()
+        A.damping = self.damping
+        return A
 
     def backsubstitute(self, retained: torch.Tensor) -> torch.Tensor:
         p = self.plan

@@ -462,6 +462,16 @@ int tlh_finish_types(const tlh_operator_t* op, int type_begin, int type_end, con
  * J^T J x contributions to its types' partials. */
 int tlh_group_tiles(const tlh_operator_t* op, int group, const void* x, tlh_stream_t stream);
 
+/* out[0..3] = {a.b, b.b, max|b|, max|a|} of two [n] vectors (float32 or float64 each;
+ * a or b may be NULL, its terms are then 0), float64 sums in a fixed order, maxima
+ * NaN-propagating (a NaN entry makes the maximum NaN, an infinite one infinite), in
+ * one launch.  partials: 2 * TLH_PCG_MAX_BLOCKS doubles; counter: one zeroed uint32
+ * (left at zero).
+ * Replaces: torch.dot(gradient, delta), the step norm, the isfinite(delta).all()
+ * step check and max |g| of the LM loop (solvers.py:386-397, 541, 591-593, 612-613). */
+int tlh_vector_stats(const void* a, int a_dtype, const void* b, int b_dtype, int64_t n, double* out,
+                     double* partials, uint32_t* counter, tlh_stream_t stream);
+
 /* dst[di(i)] = scale * src[si(i)] for i < count, si(i) = src_index ? src_index[i]
  * : src_offset + i (likewise di), converted through float64 (float32/float64 on
  * either side): the gathers and scatters of a Schur step in one launch each.

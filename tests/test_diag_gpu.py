@@ -7,7 +7,7 @@ import torch
 from oracle.ba import huber_reduction, huber_value
 from oracle.sparse import random_jacobian
 from tests.helpers import ba_jacobian, rel_err, to_block_jacobian
-from torchls_amd import generators
+from torchls_amd import _lib, generators
 from torchls_amd.hip_diagonal import HipBlockJacobi, HipDiagonalPlan, LinAlgError
 from torchls_amd.hip_ops import LinearizationKernels, Reduction, trial_objective
 
@@ -102,3 +102,40 @@ def test_trial_objective(device):
         got = red.out[0].cpu().numpy()
         assert abs(got[0] - val.sum()) <= 1e-12 * abs(val.sum())
         assert abs(got[1] - chg.sum()) <= 1e-12 * abs(chg).sum()
+
+
+@pytest.mark.parametrize("n", [1, 1000, 300_001])
+def test_vector_stats(device, n):
+    """tlh_vector_stats = (a.b, b.b, max|b|, max|a|) against float64 numpy, float64 a and
+    float32 b; a NaN entry makes a maximum NaN, an infinite one infinite; bitwise repeatable."""
+    print(f"vector stats n={n}", flush=True)
+    from torchls_amd.hip_ops import Reduction
+
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal(n)
+    b = rng.standard_normal(n).astype(np.float32)
+    red = Reduction(device)
+    lib = _lib.load()
+
+    def stats(x, y):
+        out = torch.empty(4, dtype=torch.float64, device=device)
+        _lib.check(lib.tlh_vector_stats(None if x is None else x.data_ptr(), 0 if x is None else _lib.dtype_code(x.dtype),
+                                        None if y is None else y.data_ptr(), 0 if y is None else _lib.dtype_code(y.dtype),
+                                        n, out.data_ptr(), red.partials.data_ptr(), red.counter.data_ptr(),
+                                        _lib.stream_handle(device)), "tlh_vector_stats")
+        return out.cpu().numpy()
+
+    ta, tb = torch.tensor(a, device=device), torch.tensor(b, device=device)
+    got = stats(ta, tb)
+    b64 = b.astype(np.float64)
+    assert abs(got[0] - a @ b64) <= 1e-12 * (np.abs(a) @ np.abs(b64))
+    assert abs(got[1] - b64 @ b64) <= 1e-12 * (b64 @ b64)
+    assert got[2] == np.abs(b64).max() and got[3] == np.abs(a).max()
+    assert np.array_equal(got, stats(ta, tb))
+    only = stats(None, tb)
+    assert only[1] == got[1] and only[2] == got[2] and only[3] == 0.0
+    bad_b, bad_a = b.copy(), a.copy()
+    bad_b[n // 2] = np.nan
+    bad_a[n - 1] = np.inf
+    got = stats(torch.tensor(bad_a, device=device), torch.tensor(bad_b, device=device))
+    assert np.isnan(got[1]) and np.isnan(got[2]) and got[3] == np.inf

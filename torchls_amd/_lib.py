@@ -245,6 +245,10 @@ SIGNATURES = {
         [ctypes.POINTER(Operator), ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, ctypes.c_double, c_void_p, i64,
          c_void_p],
     ),
+    "tlh_vector_stats": (
+        ctypes.c_int,
+        [c_void_p, ctypes.c_int, c_void_p, ctypes.c_int, i64, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
     "tlh_copy_cast": (
         ctypes.c_int,
         [c_void_p, ctypes.c_int, c_void_p, i64, c_void_p, ctypes.c_int, c_void_p, i64, i64, ctypes.c_double,

@@ -346,6 +346,73 @@ __global__ void __launch_bounds__(TLH_THREADS)
   grid_sum2(v, partials, counter, out, smem);
 }
 
+// |x| folded into a running maximum that turns NaN for good once a NaN is seen
+// (comparisons with NaN are false), like torch's max of abs.
+__device__ __forceinline__ double max_abs_nan(double m, double x) {
+  const double a = fabs(x);
+  return (a > m || a != a) ? a : m;
+}
+
+// Four statistics of two vectors in one pass (a or b may be NULL): out = {a.b,
+// b.b, max|b|, max|a|}; sums in a fixed order (thread strides, tree, partials in
+// block order by the last block), maxima NaN-propagating.  At most RED_BLOCKS/2
+// blocks, so the four partials of a block fit the two-sum partials buffer.
+__global__ void __launch_bounds__(TLH_THREADS)
+    vector_stats_kernel(const void* a, int a_dtype, const void* b, int b_dtype, int64_t n, double* out,
+                        double* partials, uint32_t* counter) {
+  __shared__ double smem[4][TLH_THREADS];
+  __shared__ int last;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * TLH_THREADS) {
+    const double x = a ? load_any(a, i, a_dtype) : 0.0;
+    const double y = b ? load_any(b, i, b_dtype) : 0.0;
+    v[0] += x * y;
+    v[1] += y * y;
+    v[2] = max_abs_nan(v[2], y);
+    v[3] = max_abs_nan(v[3], x);
+  }
+  auto tree = [&](double (&w)[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) smem[k][threadIdx.x] = w[k];
+    __syncthreads();
+    for (int h = TLH_THREADS / 2; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) {
+        smem[0][threadIdx.x] += smem[0][threadIdx.x + h];
+        smem[1][threadIdx.x] += smem[1][threadIdx.x + h];
+        smem[2][threadIdx.x] = max_abs_nan(smem[2][threadIdx.x], smem[2][threadIdx.x + h]);
+        smem[3][threadIdx.x] = max_abs_nan(smem[3][threadIdx.x], smem[3][threadIdx.x + h]);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = smem[k][0];
+    __syncthreads();
+  };
+  tree(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) partials[4 * blockIdx.x + k] = v[k];
+    const uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = t == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  double w[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int blk = threadIdx.x; blk < (int)gridDim.x; blk += TLH_THREADS) {
+    w[0] += __hip_atomic_load(&partials[4 * blk + 0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    w[1] += __hip_atomic_load(&partials[4 * blk + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    w[2] = max_abs_nan(w[2], __hip_atomic_load(&partials[4 * blk + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    w[3] = max_abs_nan(w[3], __hip_atomic_load(&partials[4 * blk + 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+  tree(w);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = w[k];
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // dst[di(i)] = scale * src[si(i)], si(i) = sidx ? sidx[i] : soff + i (likewise di),
 // converted through float64.
 template <typename S, typename D>
@@ -529,6 +596,19 @@ extern "C" int tlh_copy_cast(const void* src, int src_dtype, const int64_t* src_
   else
     hipLaunchKernelGGL((copy_cast_kernel<float, float>), grid, block, 0, s, static_cast<const float*>(src),
                        src_index, src_offset, static_cast<float*>(dst), dst_index, dst_offset, count, scale);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}
+
+extern "C" int tlh_vector_stats(const void* a, int a_dtype, const void* b, int b_dtype, int64_t n, double* out,
+                                double* partials, uint32_t* counter, tlh_stream_t stream) {
+  if (n < 1 || !out || !partials || !counter || (!a && !b)) return TLH_EINVAL;
+  if ((a && a_dtype != TLH_F32 && a_dtype != TLH_F64) || (b && b_dtype != TLH_F32 && b_dtype != TLH_F64))
+    return TLH_EINVAL;
+  int64_t g = ceil_div(n, TLH_THREADS);
+  g = g < 1 ? 1 : (g > RED_BLOCKS / 2 ? RED_BLOCKS / 2 : g);
+  hipLaunchKernelGGL(vector_stats_kernel, dim3((unsigned)g), dim3(TLH_THREADS), 0,
+                     reinterpret_cast<hipStream_t>(stream), a, a_dtype, b, b_dtype, n, out, partials, counter);
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }

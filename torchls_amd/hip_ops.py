@@ -122,7 +122,23 @@ class LinearizationKernels:
                 ),
                 "tlh_squared_matvec_norm",
             )
+        if len(jacobian.groups) == 1:
+            return self.red.out[0, 0]  # a view, valid until the next call
         return self.red.out[: len(jacobian.groups), 0].sum()
+
+    def vector_stats(self, a: torch.Tensor | None, b: torch.Tensor | None) -> torch.Tensor:
+        """float64 device tensor (a.b, b.b, max|b|, max|a|) of two equal-length vectors
+        (either may be None), one launch (tlh_vector_stats)."""
+        ref = a if a is not None else b
+        out = torch.empty(4, dtype=torch.float64, device=ref.device)
+        a = None if a is None else a.contiguous()
+        b = None if b is None else b.contiguous()
+        _lib.check(self.lib.tlh_vector_stats(
+            None if a is None else a.data_ptr(), 0 if a is None else _lib.dtype_code(a.dtype),
+            None if b is None else b.data_ptr(), 0 if b is None else _lib.dtype_code(b.dtype), ref.numel(),
+            out.data_ptr(), self.red.partials.data_ptr(), self.red.counter.data_ptr(),
+            _lib.stream_handle(ref.device)), "tlh_vector_stats")
+        return out
 
     def matvec(self, jacobian, x: torch.Tensor) -> torch.Tensor:
         x = x.contiguous()

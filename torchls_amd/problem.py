@@ -313,6 +313,8 @@ class Analysis:
                     val, chg = val * w, chg * w
                 red.out[k, 0] = val.sum()
                 red.out[k, 1] = chg.sum()
+        if len(self.batches) == 1:
+            return 0.5 * red.out[0]
         return 0.5 * red.out[: len(self.batches)].sum(0)
 
     def cost(self, values) -> Tensor:

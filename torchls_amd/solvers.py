@@ -305,7 +305,8 @@ def solve(
 
     def examine_device(lin):
         """float64 gradient of a linearization and its (nonfinite count, max |g|,
-        0.5 sum rho) as a device tensor, read by the caller with other values."""
+        0.5 sum rho) as three one-element device tensors, read by the caller with
+        other values."""
         nonlocal diagonal_plan, kernels
         jac = lin.jacobian
         if kernels is None:
@@ -314,12 +315,12 @@ def solve(
         g = kernels.gradient(jac, lin.residual)
         cost = lin.objective if lin.objective is not None else 0.5 * lin.residual.to(torch.float64).square().sum()
         bad = lin.nonfinite if lin.nonfinite is not None else (~torch.isfinite(lin.residual)).sum().double()
-        return g, torch.stack((bad.to(torch.float64), g.abs().max(), cost.to(torch.float64)))
+        return g, [bad.to(torch.float64).reshape(1), kernels.vector_stats(g, None)[3:], cost.to(torch.float64).reshape(1)]
 
     def examine(lin):
         """examine_device plus its host read: (gradient, finite, max |g|, cost)."""
         g, stats = examine_device(lin)
-        host = stats.cpu()
+        host = torch.cat(stats).cpu()
         return g, bool(host[0] == 0), float(host[1]), float(host[2])
 
     with region("linearize"):
@@ -431,8 +432,7 @@ def solve(
                             retained, iterations, converged = result.solution, result.iterations, result.converged
                             # the step's finiteness is read with the trial (one host read per trial)
                             usable = result.reason in ("residual_tolerance", "max_iterations", "stagnated")
-                            finite_step = torch.linalg.vector_norm(retained, ord=math.inf,
-                                                                   dtype=torch.float64)
+                            finite_step = kernels.vector_stats(None, retained)[2:3]  # max |retained|
                             summary.linear_restarts += result.restarts
                             summary.linear_residual_checks += result.residual_checks
                             summary.linear_relative_residual_history.append(
@@ -462,7 +462,6 @@ def solve(
                         result.residual_norm / result.rhs_norm if result.rhs_norm else result.residual_norm)
                     delta = result.solution
                     usable = result.reason in ("residual_tolerance", "max_iterations", "stagnated")
-                    finite_step = torch.linalg.vector_norm(delta, ord=math.inf, dtype=torch.float64)
             except (LinAlgError, torch.linalg.LinAlgError):
                 usable = False
             if usable:
@@ -473,16 +472,20 @@ def solve(
                 with region("trial_residual"):
                     objective, reduction = problem.evaluate_trial(proposed, lin)
                 sq = kernels.squared_matvec_norm(original_jacobian, delta)
-                gd = torch.dot(gradient, delta)
-                parts = [objective, reduction, gd, sq, torch.linalg.vector_norm(delta),
-                         finite_step if finite_step is not None else gd.new_zeros(())]
+                # (g.delta, delta.delta, max |delta|, -) in one launch; the full-system step's
+                # finiteness is that of max |delta| (the solution times a finite positive scale)
+                step = kernels.vector_stats(gradient, delta)
+                if finite_step is None:  # dense solves checked their step before the trial
+                    finite_step = step[2:3] if linear_solver == "conjugate_gradient" else step.new_zeros(1)
+                parts = [objective.reshape(1), reduction.reshape(1), sq.reshape(1), step[:2], finite_step]
                 # The next linearization is issued before the trial is read (used if the step
                 # is accepted, dropped otherwise): its gradient and cost ride on the same read.
                 with region("linearize"):
                     lin_next = problem.linearize(proposed, block_dtype=block_dtype)
                 g_next, stats_next = examine_device(lin_next)
-                host = torch.stack([*parts, *stats_next.unbind()]).cpu().tolist()
-                proposed_cost, actual, dot_gd, sq_jd, step_norm = host[:5]
+                host = torch.cat(parts + stats_next).cpu().tolist()
+                proposed_cost, actual, sq_jd, dot_gd, step_sq = host[:5]
+                step_norm = math.sqrt(step_sq) if step_sq >= 0 else math.nan
                 predicted = -dot_gd - 0.5 * sq_jd  # the model decrease, in float64 on the host
                 # a nonfinite step is unusable: rejected without a trial record
                 usable = math.isfinite(host[5])

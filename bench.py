@@ -399,17 +399,24 @@ def dense_schur_cholesky(device, reps):
     work = M.clone()
     status = torch.zeros(1, dtype=torch.int32, device=device)
     lib = _lib.load()
+    tile_bytes = int(lib.tlh_cholesky_tiles_workspace(n))
+    tile_work = torch.empty(tile_bytes, dtype=torch.uint8, device=device)
 
-    def factor():
+    def factor():  # the persistent tile launch (the dense Schur path's factorisation)
+        work.copy_(M)
+        _lib.check(lib.tlh_cholesky_tiles_f64(work.data_ptr(), n, n, tile_work.data_ptr(), tile_bytes,
+                                              status.data_ptr(), _lib.stream_handle(device)), "tlh_cholesky_tiles_f64")
+
+    def factor_panels():  # the panel launches (fallback)
         work.copy_(M)
         _lib.check(lib.tlh_cholesky_f64(work.data_ptr(), n, n, status.data_ptr(), _lib.stream_handle(device)),
                    "tlh_cholesky_f64")
 
-    t_all = event_time(factor, reps)
     t_copy = event_time(lambda: work.copy_(M), reps)
-    t = max(t_all - t_copy, 1e-9)
+    t_panels = max(event_time(factor_panels, reps) - t_copy, 1e-9)
+    t = max(event_time(factor, reps) - t_copy, 1e-9)
     if int(status.item()) != 0:
-        raise RuntimeError("dense Schur matrix is not positive definite")
+        raise RuntimeError(f"dense Schur factorisation status {int(status.item())}")
     rhs = torch.randn(n, dtype=torch.float64, device=device)
     x = rhs.clone()
     swork = torch.empty(2 * n, dtype=torch.float64, device=device)
@@ -425,6 +432,8 @@ def dense_schur_cholesky(device, reps):
         raise RuntimeError("dense Schur triangular solve did not complete")
     flops = n ** 3 / 3
     return {"workload": "ba_dense_schur", "cameras": prob.n_cameras, "n": n, "us_per_factorisation": round(t * 1e6, 1),
+            "kernel": "chol_tiles_kernel (one persistent cooperative launch over 64x64 tiles)",
+            "us_per_factorisation_panels": round(t_panels * 1e6, 1),
             "tflops": round(flops / t / 1e12, 3), "peak_tflops_fp64_mfma": 78.6,
             "frac": round(flops / t / 1e12 / 78.6, 4), "us_per_assembly": round(t_assembly * 1e6, 1),
             "us_per_solve": round(t_solve * 1e6, 1)}

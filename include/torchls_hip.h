@@ -490,6 +490,22 @@ int tlh_copy_cast(const void* src, int src_dtype, const int64_t* src_index, int6
  * non-positive pivot.  Replaces: torch.linalg.cholesky (solvers.py:517). */
 int tlh_cholesky_f64(double* A, int64_t n, int64_t ld, int32_t* status, tlh_stream_t stream);
 
+/* In-place lower Cholesky A = L L^T (row-major, leading dimension ld, float64,
+ * n <= 8192) as one persistent cooperative launch over 64x64 tiles: a
+ * workgroup factors the diagonal tiles (16x16 blocks, MFMA updates) and solves
+ * the subdiagonal ones, the others own the remaining tiles, apply their MFMA
+ * updates as soon as the L tiles they need are published (agent-scope
+ * release/acquire of per-tile states) and solve with the published inverse
+ * 16x16 diagonal blocks.  Only the lower triangle is written.  work:
+ * tlh_cholesky_tiles_workspace(n) bytes (tile states and inverse blocks; the
+ * launch clears what it needs).  *status |= 1 on a non-positive or non-finite
+ * pivot; |= 4 when a bounded wait gave up (the factor is then unusable: factor a
+ * fresh copy with tlh_cholesky_f64).  Returns TLH_ELDS when the device cannot
+ * keep two workgroups resident.  Replaces: torch.linalg.cholesky (solvers.py:517). */
+int64_t tlh_cholesky_tiles_workspace(int64_t n);
+int tlh_cholesky_tiles_f64(double* A, int64_t n, int64_t ld, void* work, int64_t work_bytes, int32_t* status,
+                           tlh_stream_t stream);
+
 /* b <- (L L^T)^-1 b with the factor from tlh_cholesky_f64 (float64, in place).
  * Replaces: torch.cholesky_solve (solvers.py:518). */
 int tlh_cholesky_solve_f64(const double* L, int64_t n, int64_t ld, double* b, tlh_stream_t stream);

@@ -75,17 +75,18 @@ def test_schur_operators(device, priors, dtype, tol):
     assert rel_err(np.tril(M), np.tril(Md)) <= tol
 
 
-@pytest.mark.parametrize("n", [5, 64, 70, 129, 200, 1000, 1536])
-def test_cholesky(device, n):
-    print(f"cholesky n={n}", flush=True)
+@pytest.mark.parametrize("method", ["tiles", "panels"])
+@pytest.mark.parametrize("n", [5, 64, 65, 70, 128, 129, 200, 1000, 1536])
+def test_cholesky(device, n, method):
+    print(f"cholesky n={n} method={method}", flush=True)
     rng = np.random.default_rng(n)
     A = rng.standard_normal((n, n + 3))
     M = A @ A.T + n * 1e-3 * np.eye(n)
-    L = cholesky_f64(torch.tensor(M, device=device)).cpu().numpy()
+    L = cholesky_f64(torch.tensor(M, device=device), method=method).cpu().numpy()
     Lr = np.linalg.cholesky(M)
     assert rel_err(np.tril(L), Lr) <= 1e-12
     b = rng.standard_normal(n)
-    x = cholesky_solve_f64(torch.tensor(M, device=device), torch.tensor(b, device=device)).cpu().numpy()
+    x = cholesky_solve_f64(torch.tensor(M, device=device), torch.tensor(b, device=device), method=method).cpu().numpy()
     assert rel_err(x, np.linalg.solve(M, b)) <= 1e-10
     # the one-workgroup substitution gives the same solution
     xs = torch.tensor(b, device=device)
@@ -94,9 +95,37 @@ def test_cholesky(device, n):
                "tlh_cholesky_solve_f64")
     assert rel_err(x, xs.cpu().numpy()) <= 1e-12
     with pytest.raises(LinAlgError):
-        cholesky_f64(torch.tensor(-M, device=device))
+        cholesky_f64(torch.tensor(-M, device=device), method=method)
     with pytest.raises(LinAlgError):
-        cholesky_solve_f64(torch.tensor(-M, device=device), torch.tensor(b, device=device))
+        cholesky_solve_f64(torch.tensor(-M, device=device), torch.tensor(b, device=device), method=method)
+    # a matrix that fails only at its last pivot
+    bad = M.copy()
+    bad[n - 1, n - 1] = -1.0
+    with pytest.raises(LinAlgError):
+        cholesky_f64(torch.tensor(bad, device=device), method=method)
+
+
+@pytest.mark.parametrize("n", [64, 130, 1536, 4096])
+def test_cholesky_tiles_complete(device, n):
+    """The persistent tile launch completes without a wait giving up (status 0, so no
+    fallback) and writes only the lower triangle; its factor matches LAPACK."""
+    print(f"cholesky tiles n={n}", flush=True)
+    from torchls_amd.cholesky import _factor
+
+    rng = np.random.default_rng(n + 1)
+    A = rng.standard_normal((n, n + 3))
+    M = A @ A.T + n * 1e-3 * np.eye(n)
+    Mt = torch.tensor(M, device=device)
+    L = Mt.clone()
+    status = torch.zeros(1, dtype=torch.int32, device=device)
+    _factor(L, status, "tiles")
+    assert int(status.item()) == 0
+    Lh = L.cpu().numpy()
+    assert rel_err(np.tril(Lh), np.linalg.cholesky(M)) <= 1e-12
+    assert np.array_equal(np.triu(Lh, 1), np.triu(M, 1))
+    L2 = Mt.clone()
+    _factor(L2, status, "tiles")
+    assert torch.equal(L, L2)  # bitwise repeatable
 
 
 def test_cholesky_solve_sentinel_rhs(device):

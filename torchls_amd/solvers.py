@@ -17,6 +17,7 @@ import time
 from contextlib import nullcontext
 from dataclasses import dataclass, field
 from functools import partial
+from types import SimpleNamespace
 
 import torch
 from torch import Tensor
@@ -323,9 +324,59 @@ def solve(
         host = torch.cat(stats).cpu()
         return g, bool(host[0] == 0), float(host[1]), float(host[2])
 
+    def prepare(jacobian, gradient):
+        """The damping-independent part of a linear system: diagonal blocks, column scale,
+        scaled gradient and the packed Schur or normal operator of one linearization."""
+        nonlocal column_scale, normal_plan
+        diagonal_damping = trust_region is not None and trust_region.damping_mode == "diagonal"
+        needs_diagonal = is_schur or (linear_solver == "conjugate_gradient"
+                                      and conjugate_gradient.preconditioner is not None)
+        diagonal = None
+        if needs_diagonal or diagonal_damping or (jacobi_scaling and column_scale is None):
+            packed_unscaled = diagonal_plan._pack(jacobian)
+            diagonal = diagonal_plan.diagonal_blocks_f64(packed_unscaled, jacobian)
+        if jacobi_scaling and column_scale is None:
+            column_scale = torch.cat(
+                [(1 / (1 + b.diagonal(dim1=-2, dim2=-1).clamp_min(0).sqrt())).reshape(-1) for b in diagonal]
+            )
+        solve_scale = column_scale
+        if diagonal_damping:
+            nd = torch.cat([b.diagonal(dim1=-2, dim2=-1).reshape(-1) for b in diagonal])
+            if column_scale is not None:
+                nd = nd * column_scale.square()
+            inv_diag = nd.clamp(trust_region.diagonal_min, trust_region.diagonal_max).rsqrt()
+            solve_scale = inv_diag if column_scale is None else column_scale * inv_diag
+        if solve_scale is not None:
+            # J diag(s) is never copied: the pack kernels scale the blocks as they
+            # lay them out for the normal / Schur operators (tlh_pack_blocks_scaled)
+            solve_scale = solve_scale.to(torch.float64).contiguous()
+            linear_gradient = gradient * solve_scale
+            if diagonal is not None:
+                diagonal = tuple(
+                    b * solve_scale[e.offset: e.offset + e.count * e.dim].reshape(e.count, e.dim, 1)
+                    * solve_scale[e.offset: e.offset + e.count * e.dim].reshape(e.count, 1, e.dim)
+                    for e, b in zip(problem.layout, diagonal)
+                )
+        else:
+            linear_gradient = gradient
+        schur_linear = normal_operator = None
+        joint_diagonal = (joint_plan.normal_blocks(jacobian, diagonal, col_scale=solve_scale)
+                          if joint_plan is not None else None)
+        if is_schur:
+            schur_linear = schur_plan.linearize(jacobian, diagonal, dense=linear_solver == "schur_dense",
+                                                col_scale=solve_scale)
+        elif linear_solver == "conjugate_gradient":
+            if normal_plan is None:
+                normal_plan = cached(("normal",), lambda: HipNormalPlan(jacobian, dtype=mv_dtype))
+            normal_operator = normal_plan.bind(jacobian, col_scale=solve_scale)
+        return SimpleNamespace(jacobian=jacobian, diagonal=diagonal, solve_scale=solve_scale,
+                               linear_gradient=linear_gradient, schur_linear=schur_linear,
+                               normal_operator=normal_operator, joint_diagonal=joint_diagonal)
+
     with region("linearize"):
         lin = problem.linearize(vals, block_dtype=block_dtype)
     head = None
+    prepared = None  # the prepared system of `lin`, when issued ahead (speculatively, before the trial read)
     pcg_hint = None  # iterations of the previous PCG solve: the next one's first batch
     for iteration in range(termination.max_iterations):
         if timed_out():
@@ -347,53 +398,16 @@ def solve(
             summary.termination_reason = "gradient_tolerance"
             break
         original_jacobian = jacobian
-        with region("prepare_linear_system"):
-            diagonal_damping = trust_region is not None and trust_region.damping_mode == "diagonal"
-            needs_diagonal = is_schur or (linear_solver == "conjugate_gradient"
-                                          and conjugate_gradient.preconditioner is not None)
-            diagonal = None
-            packed_unscaled = None
-            if needs_diagonal or diagonal_damping or (jacobi_scaling and column_scale is None):
-                packed_unscaled = diagonal_plan._pack(jacobian)
-                diagonal = diagonal_plan.diagonal_blocks_f64(packed_unscaled, jacobian)
-            if jacobi_scaling and column_scale is None:
-                column_scale = torch.cat(
-                    [(1 / (1 + b.diagonal(dim1=-2, dim2=-1).clamp_min(0).sqrt())).reshape(-1) for b in diagonal]
-                )
-            solve_scale = column_scale
-            if diagonal_damping:
-                nd = torch.cat([b.diagonal(dim1=-2, dim2=-1).reshape(-1) for b in diagonal])
-                if column_scale is not None:
-                    nd = nd * column_scale.square()
-                inv_diag = nd.clamp(trust_region.diagonal_min, trust_region.diagonal_max).rsqrt()
-                solve_scale = inv_diag if column_scale is None else column_scale * inv_diag
-            if solve_scale is not None:
-                # J diag(s) is never copied: the pack kernels scale the blocks as they
-                # lay them out for the normal / Schur operators (tlh_pack_blocks_scaled)
-                solve_scale = solve_scale.to(torch.float64).contiguous()
-                linear_gradient = gradient * solve_scale
-                if diagonal is not None:
-                    diagonal = tuple(
-                        b * solve_scale[e.offset: e.offset + e.count * e.dim].reshape(e.count, e.dim, 1)
-                        * solve_scale[e.offset: e.offset + e.count * e.dim].reshape(e.count, 1, e.dim)
-                        for e, b in zip(problem.layout, diagonal)
-                    )
-            else:
-                linear_gradient = gradient
-            schur_linear = None
-            normal_operator = None
-            joint_diagonal = (joint_plan.normal_blocks(jacobian, diagonal, col_scale=solve_scale)
-                              if joint_plan is not None else None)
-            if is_schur:
-                schur_linear = schur_plan.linearize(jacobian, diagonal, dense=linear_solver == "schur_dense",
-                                                    col_scale=solve_scale)
-            elif linear_solver == "conjugate_gradient":
-                if normal_plan is None:
-                    normal_plan = cached(("normal",), lambda: HipNormalPlan(jacobian, dtype=mv_dtype))
-                normal_operator = normal_plan.bind(jacobian, col_scale=solve_scale)
+        if prepared is None or prepared.jacobian is not jacobian:
+            with region("prepare_linear_system"):
+                prepared = prepare(jacobian, gradient)
+        diagonal, solve_scale, linear_gradient = prepared.diagonal, prepared.solve_scale, prepared.linear_gradient
+        schur_linear, normal_operator, joint_diagonal = (prepared.schur_linear, prepared.normal_operator,
+                                                         prepared.joint_diagonal)
+        prepared = None
         accepted = False
         trial_limit = trust_region.max_trials if trust_region else 1
-        for _ in range(trial_limit):
+        for trial in range(trial_limit):
             if timed_out():
                 summary.termination_reason = "time_limit"
                 break
@@ -483,6 +497,13 @@ def solve(
                 with region("linearize"):
                     lin_next = problem.linearize(proposed, block_dtype=block_dtype)
                 g_next, stats_next = examine_device(lin_next)
+                # The first trial of an iteration also issues the next system's damping-
+                # independent part (kept if the step is accepted); later trials, which
+                # follow a rejection, do not.
+                prep_next = None
+                if trial == 0 and iteration + 1 < termination.max_iterations and linear_solver != "dense_qr":
+                    with region("prepare_linear_system"):
+                        prep_next = prepare(lin_next.jacobian, g_next)
                 host = torch.cat(parts + stats_next).cpu().tolist()
                 proposed_cost, actual, sq_jd, dot_gd, step_sq = host[:5]
                 step_norm = math.sqrt(step_sq) if step_sq >= 0 else math.nan
@@ -514,6 +535,7 @@ def solve(
                     normal_operator = schur_linear = system = diagonal = gradient = linear_gradient = None
                     joint_diagonal = None
                     lin = lin_next
+                    prepared = prep_next
                     head = (g_next, stats_host[0] == 0, stats_host[1], stats_host[2])
                     summary.gradient_norm = head[2]
                     if termination.target_cost is not None and proposed_cost <= termination.target_cost:

@@ -213,130 +213,149 @@ __device__ __forceinline__ double row_bcast(double v, int c) {
 
 // Cholesky of the 64x64 tile D (LDS, lower triangle) in place; the inverses of
 // its four 16x16 diagonal blocks go to Dinv (LDS).  Returns false (every
-// thread) on a non-positive or non-finite pivot.  Per 16-column block p:
-//   A. wavefront 0 factors block (p, p), lane r of each 16-lane row holding row
-//      r, in 4-column panels: inside a panel the pivot column reaches the other
-//      rows by DPP broadcasts; after it, the panel's rows go to LDS (P) and every
-//      later column takes its rank-4 update from broadcast reads of P (entries
-//      above the diagonal are updated too and never read);
-//   B. wavefront 1 solves the rows below, x L_pp^T = a, a row per lane (left-
-//      looking: row c of L_pp read from P), while wavefront 2 inverts L_pp (a
-//      column per lane) for the other solves;
-//   C. the trailing blocks are updated with MFMA, block (p+1, p+1) by wavefront 0.
+// thread) on a non-positive or non-finite pivot.  Blocked by 8 columns; per
+// 8-column block p (rows and columns o = 8 p .. o + 7):
+//   A. wavefront 0 factors block (p, p), lane r of each 8-lane group holding row
+//      r (the pivot column reaches the other rows by DPP row broadcasts; entries
+//      above the diagonal are updated too and never read); its rows go to LDS (P);
+//   B. wavefront 1 solves the rows below, x L_pp^T = a, a row per lane (row c of
+//      L_pp read from P), while wavefront 2 inverts L_pp (a column per lane);
+//   C. the trailing part (rows and columns >= o + 8) takes the rank-8 update with
+//      MFMA on the 16x16 tiles that cover it (rows of a tile above o + 8 get a
+//      zero operand); the tile holding block (p+1, p+1) goes to wavefront 0,
+//      which goes on to step A of p+1 without waiting for the other tiles.
+// Finally each wavefront q forms the inverse of the 16x16 diagonal block q from
+// its two 8x8 inverses: [[A^-1, 0], [-C^-1 B A^-1, C^-1]].
+constexpr int FB = 8;  // factorisation block
+
+// 1 / sqrt(d) for a positive finite d: the hardware estimate (about 2^-29 relative
+// error) refined by one Newton step, y (3 - d y^2) / 2, to float64 accuracy.
+__device__ __forceinline__ double rsqrt_refined(double d) {
+  const double y = __builtin_amdgcn_rsq(d);
+  return y * fma(-0.5 * d * y, y, 1.5);
+}
+
 __device__ __forceinline__ bool factor_tile(double* D, double* Dinv, int* flag) {
-  __shared__ double P[SB][SB + 2];  // L_pp rows (16-byte aligned rows)
-  __shared__ double idiag[SB];      // 1 / L[r][r] of the current block
+  __shared__ double P[FB][FB];      // L_pp rows
+  __shared__ double idiag[FB];      // 1 / L[r][r] of the current block
+  __shared__ double I8[NB / FB][FB][FB + 1];  // inverses of the 8x8 diagonal blocks
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   bool ok = true;
 #pragma unroll 1
-  for (int p = 0; p < 4; ++p) {
-    const int o = SB * p;
+  for (int p = 0; p < NB / FB; ++p) {
+    const int o = FB * p;
     if (w == 0) {
-      const int r = l & 15;
-      double v[SB];
+      const int r = l & 7;
+      double v[FB];
 #pragma unroll
-      for (int c = 0; c < SB; ++c) v[c] = D[(o + r) * LDT + o + c];
+      for (int c = 0; c < FB; ++c) v[c] = D[(o + r) * LDT + o + c];
       double invd = 1.0;
 #pragma unroll
-      for (int g = 0; g < SB / 4; ++g) {
+      for (int j = 0; j < FB; ++j) {
+        const double dj = row_bcast(v[j], j);
+        const bool good = dj > 0.0 && isfinite(dj);
+        if (!good) ok = false;
+        const double inv = rsqrt_refined(good ? dj : 1.0);
+        const double piv = (good ? dj : 1.0) * inv;
+        v[j] = r == j ? piv : v[j] * inv;
+        invd = r == j ? inv : invd;
 #pragma unroll
-        for (int j = 4 * g; j < 4 * g + 4; ++j) {
-          const double dj = row_bcast(v[j], j);
-          const bool good = dj > 0.0 && isfinite(dj);
-          if (!good) ok = false;
-          const double inv = rsqrt(good ? dj : 1.0);
-          const double piv = (good ? dj : 1.0) * inv;
-          v[j] = r == j ? piv : v[j] * inv;
-          invd = r == j ? inv : invd;
-#pragma unroll
-          for (int c = j + 1; c < 4 * g + 4; ++c) v[c] = fma(-v[j], row_bcast(v[j], c), v[c]);
-        }
-        if (l < SB) {
-#pragma unroll
-          for (int k = 0; k < 4; ++k) P[l][4 * g + k] = v[4 * g + k];
-        }
-        wave_lds_fence();
-        if (g < 3) {
-#pragma unroll
-          for (int c = 4 * g + 4; c < SB; ++c) {
-            double t = v[c];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) t = fma(-v[4 * g + k], P[c][4 * g + k], t);
-            v[c] = t;
-          }
-        }
+        for (int c = j + 1; c < FB; ++c) v[c] = fma(-v[j], row_bcast(v[j], c), v[c]);
       }
-      if (l < SB) {
+      if (l < FB) {
 #pragma unroll
-        for (int c = 0; c < SB; ++c)
+        for (int c = 0; c < FB; ++c) {
+          P[l][c] = v[c];
           if (c <= l) D[(o + l) * LDT + o + c] = v[c];
+        }
         idiag[l] = invd;
       }
     }
     __syncthreads();
-    if (w == 1 && o + SB + l < NB) {
-      // row o + 16 + l below the block: x L^T = a, column by column
-      const int R = o + SB + l;
-      double x[SB];
+    if (w == 1 && o + FB + l < NB) {
+      // row o + 8 + l below the block: x L^T = a, column by column
+      const int R = o + FB + l;
+      double x[FB];
 #pragma unroll
-      for (int c = 0; c < SB; ++c) x[c] = D[R * LDT + o + c];
+      for (int c = 0; c < FB; ++c) x[c] = D[R * LDT + o + c];
 #pragma unroll
-      for (int c = 0; c < SB; ++c) {
-        double t0 = x[c], t1 = 0.0;
+      for (int c = 0; c < FB; ++c) {
+        double t = x[c];
 #pragma unroll
-        for (int m = 0; m < c; ++m) {
-          if (m & 1)
-            t1 = fma(-x[m], P[c][m], t1);
-          else
-            t0 = fma(-x[m], P[c][m], t0);
-        }
-        x[c] = (t0 + t1) * idiag[c];
+        for (int m = 0; m < c; ++m) t = fma(-x[m], P[c][m], t);
+        x[c] = t * idiag[c];
       }
 #pragma unroll
-      for (int c = 0; c < SB; ++c) D[R * LDT + o + c] = x[c];
-    } else if (w == 2 && l < SB) {
+      for (int c = 0; c < FB; ++c) D[R * LDT + o + c] = x[c];
+    } else if (w == 2 && l < FB) {
       // column l of L_pp^-1: x[c] = -(sum_{m<c} L[c][m] x[m]) / L[c][c], x[l] = 1 / L[l][l]
-      double x[SB];
+      double x[FB];
 #pragma unroll
-      for (int c = 0; c < SB; ++c) {
-        double t0 = c == l ? 1.0 : 0.0, t1 = 0.0;
+      for (int c = 0; c < FB; ++c) {
+        double t = c == l ? 1.0 : 0.0;
 #pragma unroll
-        for (int m = 0; m < c; ++m) {
-          if (m & 1)
-            t1 = fma(-x[m], P[c][m], t1);
-          else
-            t0 = fma(-x[m], P[c][m], t0);
-        }
-        x[c] = (t0 + t1) * idiag[c];
+        for (int m = 0; m < c; ++m) t = fma(-x[m], P[c][m], t);
+        x[c] = t * idiag[c];
       }
 #pragma unroll
-      for (int c = 0; c < SB; ++c) Dinv[(SB * p + c) * LDI + l] = x[c];
+      for (int c = 0; c < FB; ++c) I8[p][c][l] = x[c];
     }
     __syncthreads();
-    if (p < 3) {
-      // trailing blocks (a2, b2), p < b2 <= a2 < 4: D_ab -= L_ap L_bp^T; block (p+1, p+1)
-      // on wavefront 0, the others round-robin over wavefronts 1..3
+    if (p < NB / FB - 1) {
+      // trailing update of the rows and columns >= e = o + 8 on the 16x16 tiles (a2, b2),
+      // t0 <= b2 <= a2 < 4, t0 = e / 16; operand rows below e are zero
+      const int e = o + FB, t0 = e >> 4;
       int job = 0;
 #pragma unroll 1
-      for (int a2 = p + 1; a2 < 4; ++a2) {
+      for (int a2 = t0; a2 < 4; ++a2) {
 #pragma unroll 1
-        for (int b2 = p + 1; b2 <= a2; ++b2) {
-          const bool first = a2 == p + 1 && b2 == p + 1;
+        for (int b2 = t0; b2 <= a2; ++b2) {
+          const bool first = a2 == t0 && b2 == t0;
           const int owner = first ? 0 : 1 + (job++ % 3);
           if (owner != w) continue;
+          const int ra = SB * a2 + (l & 15), rb = SB * b2 + (l & 15);
           v4d c;
 #pragma unroll
           for (int v = 0; v < 4; ++v) c[v] = D[(SB * a2 + 4 * v + (l >> 4)) * LDT + SB * b2 + (l & 15)];
 #pragma unroll
-          for (int k0 = 0; k0 < SB; k0 += 4)
-            c = mfma(-D[(SB * a2 + (l & 15)) * LDT + o + k0 + (l >> 4)],
-                     D[(SB * b2 + (l & 15)) * LDT + o + k0 + (l >> 4)], c);
+          for (int k0 = 0; k0 < FB; k0 += 4) {
+            const double xa = ra >= e ? D[ra * LDT + o + k0 + (l >> 4)] : 0.0;
+            const double xb = rb >= e ? D[rb * LDT + o + k0 + (l >> 4)] : 0.0;
+            c = mfma(-xa, xb, c);
+          }
 #pragma unroll
           for (int v = 0; v < 4; ++v) D[(SB * a2 + 4 * v + (l >> 4)) * LDT + SB * b2 + (l & 15)] = c[v];
         }
       }
-      __syncthreads();
+      // No barrier: wavefront 0 updated the tile holding block (p+1, p+1), which is all
+      // that step A of p+1 reads (and only wavefront 0 writes there); the other
+      // wavefronts' tiles are read after the barrier that ends that step A.
+      wave_lds_fence();
     }
+  }
+  __syncthreads();
+  // 16x16 inverses: wavefront q, lane (i, j) = (l / 8, l % 8) forms entry (8 + i, j) of
+  // block q's inverse, -(C^-1 B A^-1)[i][j], with A = L_{2q}, B = L rows 16q + 8.., columns
+  // 16q.., C = L_{2q+1}; the two diagonal 8x8 inverses are copied as they are
+  {
+    const int q = w;
+    const int i = l >> 3, j = l & 7;
+    const int r0 = SB * q;
+    double bt = 0.0;  // (B A^-1)[i][j]
+#pragma unroll
+    for (int m = 0; m < FB; ++m) bt = fma(D[(r0 + FB + i) * LDT + r0 + m], I8[2 * q][m][j], bt);
+    // -(C^-1 (B A^-1))[i][j] needs row i of C^-1 and column j of B A^-1: exchange through LDS
+    __shared__ double BT[4][FB][FB + 1];
+    BT[q][i][j] = bt;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int m = 0; m < FB; ++m) t = fma(I8[2 * q + 1][i][m], BT[q][m][j], t);
+    double* inv = Dinv + (SB * q) * LDI;
+    inv[(FB + i) * LDI + j] = -t;
+    inv[i * LDI + j] = I8[2 * q][i][j];
+    inv[(FB + i) * LDI + FB + j] = I8[2 * q + 1][i][j];
+    inv[i * LDI + FB + j] = 0.0;
   }
   if (threadIdx.x == 0) *flag = 0;
   __syncthreads();

@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(TLH_THREADS)
                              const int64_t* __restrict__ jj, const Real* __restrict__ meas, int64_t meas_stride,
                              int64_t count, Real* __restrict__ res, Real* __restrict__ ji, Real* __restrict__ jjac,
                              double* out, double* partials, uint32_t* counter) {
-  __shared__ double smem[2 * TLH_THREADS];
+  __shared__ double smem[RED_SMEM];
   double acc[2] = {0.0, 0.0};
   for (int64_t e = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; e < count * 12;
        e += (int64_t)gridDim.x * TLH_THREADS) {

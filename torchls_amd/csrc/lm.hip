@@ -216,7 +216,7 @@ struct RawGroup {
 __global__ void __launch_bounds__(TLH_THREADS)
     sqnorm_kernel(RawGroup g, int block_dtype, const void* x, int x_dtype, double* out, double* partials,
                   uint32_t* counter) {
-  __shared__ double smem[2 * TLH_THREADS];
+  __shared__ double smem[RED_SMEM];
   double v[2] = {0.0, 0.0};
   const int R = g.rows;
   for (int64_t o = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; o < g.count;
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(TLH_THREADS)
 template <int R, int D0, int D1, typename BT, typename XT>
 __global__ void __launch_bounds__(TLH_THREADS)
     sqnorm_pair_kernel(RawGroup g, const void* x, double* out, double* partials, uint32_t* counter) {
-  __shared__ double smem[2 * TLH_THREADS];
+  __shared__ double smem[RED_SMEM];
   const BT* __restrict__ b0 = static_cast<const BT*>(g.blocks[0]);
   const BT* __restrict__ b1 = static_cast<const BT*>(g.blocks[1]);
   const XT* __restrict__ xv = static_cast<const XT*>(x);
@@ -315,7 +315,7 @@ __global__ void __launch_bounds__(TLH_THREADS)
     trial_kernel(const void* before, int before_dtype, const void* after, int after_dtype, int64_t count, int rows,
                  int loss, double delta, const double* weights, int weight_stride, double* out, double* partials,
                  uint32_t* counter) {
-  __shared__ double smem[2 * TLH_THREADS];
+  __shared__ double smem[RED_SMEM];
   double v[2] = {0.0, 0.0};
   for (int64_t o = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; o < count;
        o += (int64_t)gridDim.x * TLH_THREADS) {
@@ -356,11 +356,13 @@ __device__ __forceinline__ double max_abs_nan(double m, double x) {
 // Four statistics of two vectors in one pass (a or b may be NULL): out = {a.b,
 // b.b, max|b|, max|a|}; sums in a fixed order (thread strides, tree, partials in
 // block order by the last block), maxima NaN-propagating.  At most RED_BLOCKS/2
-// blocks, so the four partials of a block fit the two-sum partials buffer.
+// blocks (vector_grid gives at most 128), so the four partials of a block fit
+// the two-sum partials buffer.
 __global__ void __launch_bounds__(TLH_THREADS)
     vector_stats_kernel(const void* a, int a_dtype, const void* b, int b_dtype, int64_t n, double* out,
                         double* partials, uint32_t* counter) {
-  __shared__ double smem[4][TLH_THREADS];
+  constexpr int NWAVES = TLH_THREADS / 64;
+  __shared__ double smem[4][NWAVES];
   __shared__ int last;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * TLH_THREADS) {
@@ -371,21 +373,28 @@ __global__ void __launch_bounds__(TLH_THREADS)
     v[2] = max_abs_nan(v[2], y);
     v[3] = max_abs_nan(v[3], x);
   }
+  // xor-shuffle tree inside each wavefront, then the wavefronts in order
   auto tree = [&](double (&w)[4]) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) smem[k][threadIdx.x] = w[k];
-    __syncthreads();
-    for (int h = TLH_THREADS / 2; h > 0; h >>= 1) {
-      if ((int)threadIdx.x < h) {
-        smem[0][threadIdx.x] += smem[0][threadIdx.x + h];
-        smem[1][threadIdx.x] += smem[1][threadIdx.x + h];
-        smem[2][threadIdx.x] = max_abs_nan(smem[2][threadIdx.x], smem[2][threadIdx.x + h]);
-        smem[3][threadIdx.x] = max_abs_nan(smem[3][threadIdx.x], smem[3][threadIdx.x + h]);
-      }
-      __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) {
+      w[0] += __shfl_xor(w[0], off);
+      w[1] += __shfl_xor(w[1], off);
+      w[2] = max_abs_nan(w[2], __shfl_xor(w[2], off));
+      w[3] = max_abs_nan(w[3], __shfl_xor(w[3], off));
     }
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) smem[k][threadIdx.x >> 6] = w[k];
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < 4; ++k) w[k] = smem[k][0];
+#pragma unroll
+    for (int wv = 1; wv < NWAVES; ++wv) {
+      w[0] += smem[0][wv];
+      w[1] += smem[1][wv];
+      w[2] = max_abs_nan(w[2], smem[2][wv]);
+      w[3] = max_abs_nan(w[3], smem[3][wv]);
+    }
     __syncthreads();
   };
   tree(v);
@@ -605,8 +614,7 @@ extern "C" int tlh_vector_stats(const void* a, int a_dtype, const void* b, int b
   if (n < 1 || !out || !partials || !counter || (!a && !b)) return TLH_EINVAL;
   if ((a && a_dtype != TLH_F32 && a_dtype != TLH_F64) || (b && b_dtype != TLH_F32 && b_dtype != TLH_F64))
     return TLH_EINVAL;
-  int64_t g = ceil_div(n, TLH_THREADS);
-  g = g < 1 ? 1 : (g > RED_BLOCKS / 2 ? RED_BLOCKS / 2 : g);
+  const int g = vector_grid(n);  // at most 128 <= RED_BLOCKS / 2 workgroups
   hipLaunchKernelGGL(vector_stats_kernel, dim3((unsigned)g), dim3(TLH_THREADS), 0,
                      reinterpret_cast<hipStream_t>(stream), a, a_dtype, b, b_dtype, n, out, partials, counter);
   TLH_RETURN_IF_LAUNCH_FAILED();

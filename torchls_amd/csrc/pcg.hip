@@ -17,6 +17,7 @@
 #include <math.h>
 
 #include "common.hpp"
+#include "reduce.hpp"
 
 using namespace tlh;
 
@@ -30,18 +31,10 @@ __device__ __forceinline__ bool frozen(const int32_t* control) {
   return control[1] != 0 || control[0] >= control[2];
 }
 
-// Block-wide fixed-order tree sum of NV values; result valid in thread 0.
+// Block-wide fixed-order sum of NV values (wave_block_sum); every thread gets the totals.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* smem) {
-  for (int k = 0; k < NV; ++k) smem[k * TLH_THREADS + threadIdx.x] = v[k];
-  __syncthreads();
-  for (int w = TLH_THREADS / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int k = 0; k < NV; ++k) smem[k * TLH_THREADS + threadIdx.x] += smem[k * TLH_THREADS + threadIdx.x + w];
-    __syncthreads();
-  }
-  for (int k = 0; k < NV; ++k) v[k] = smem[k * TLH_THREADS];
-  __syncthreads();
+  wave_block_sum<NV>(v, smem);
 }
 
 // Publish this block's NV sums; the last block returns true with the grid totals in v.
@@ -70,10 +63,9 @@ __device__ __forceinline__ bool grid_sum(double (&v)[NV], double* partials, uint
   return true;
 }
 
-inline int grid_for(int64_t work) {
-  const int64_t g = ceil_div(work, TLH_THREADS);
-  return (int)(g < 1 ? 1 : (g > MAX_BLOCKS ? MAX_BLOCKS : g));
-}
+// Vector kernels reduce over a few workgroups (vector_grid): their tickets cost
+// more than their streaming at hundreds of workgroups.
+inline int grid_for(int64_t n, int64_t work = -1) { return vector_grid(n, work); }
 
 // ---------------------------------------------------------------- block inverse
 template <int D>
@@ -202,7 +194,7 @@ template <typename Real>
 __global__ void __launch_bounds__(TLH_THREADS)
     dot_kernel(const Real* __restrict__ a, const Real* __restrict__ b, int64_t n, double* scalars, int slot,
                double* partials, uint32_t* counter) {
-  __shared__ double smem[TLH_THREADS];
+  __shared__ double smem[TLH_THREADS / 64];
   double v[1] = {0.0};
   for (int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * TLH_THREADS)
     v[0] += (double)a[i] * (double)b[i];
@@ -217,11 +209,13 @@ __global__ void __launch_bounds__(TLH_THREADS)
     step_kernel(tlh_block_diag_t pc, int mode, Real* __restrict__ x, Real* __restrict__ r, Real* __restrict__ z,
                 const Real* __restrict__ p, const Real* __restrict__ ap, int64_t n, const int32_t* control,
                 double* scalars, double* partials, uint32_t* counter) {
-  __shared__ double smem[2 * TLH_THREADS];
+  __shared__ double smem[2 * (TLH_THREADS / 64)];
+  // The entries are loaded before the scalars are awaited, one memory round trip
+  // fewer: the stores are unconditional selects (a frozen state rewrites the same
+  // values), so no load is sunk below the branch on the scalars.
   const double c = scalars[TLH_PCG_CURVATURE];
   const double rz = scalars[TLH_PCG_RZ];
-  const bool move = !frozen(control) && finite(c) && finite(rz) && c > 0.0 && rz > 0.0;
-  const double alpha = move ? rz / c : 0.0;
+  const bool stopped = frozen(control);
   double v[2] = {0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * TLH_THREADS;
   if (mode == 0) {
@@ -232,7 +226,25 @@ __global__ void __launch_bounds__(TLH_THREADS)
       const int t = find_type(pc, bidx, local);
       const int d = pc.dim[t];
       const int64_t e0 = pc.offset[t] + local * d;
-      if (move) {
+      const bool move = !stopped && finite(c) && finite(rz) && c > 0.0 && rz > 0.0;
+      const double alpha = move ? rz / c : 0.0;
+      if (d == 6 || d == 3) {
+        Real xv[6], rv[6], pv[6], av[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+          if (k < d) {
+            xv[k] = x[e0 + k];
+            rv[k] = r[e0 + k];
+            pv[k] = p[e0 + k];
+            av[k] = ap[e0 + k];
+          }
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+          if (k < d) {
+            x[e0 + k] = move ? (Real)((double)xv[k] + alpha * (double)pv[k]) : xv[k];
+            r[e0 + k] = move ? (Real)((double)rv[k] - alpha * (double)av[k]) : rv[k];
+          }
+      } else if (move) {
         for (int k = 0; k < d; ++k) {
           x[e0 + k] = (Real)((double)x[e0 + k] + alpha * (double)p[e0 + k]);
           r[e0 + k] = (Real)((double)r[e0 + k] - alpha * (double)ap[e0 + k]);
@@ -241,16 +253,18 @@ __global__ void __launch_bounds__(TLH_THREADS)
       apply_any<Real>(pc.inv[t] + local * (d * (d + 1) / 2), d, r + e0, z + e0, v[0], v[1]);
     }
   } else {
+    const bool move = !stopped && finite(c) && finite(rz) && c > 0.0 && rz > 0.0;
+    const double alpha = move ? rz / c : 0.0;
     for (int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; i < n; i += stride) {
-      if (move) {
-        x[i] = (Real)((double)x[i] + alpha * (double)p[i]);
-        r[i] = (Real)((double)r[i] - alpha * (double)ap[i]);
-      }
+      const Real xi = x[i], ri0 = r[i], pi = p[i], ai = ap[i];
+      const Real ri = move ? (Real)((double)ri0 - alpha * (double)ai) : ri0;
+      x[i] = move ? (Real)((double)xi + alpha * (double)pi) : xi;
+      r[i] = ri;
       if (mode == 1) {
-        const double ri = (double)r[i];
-        z[i] = r[i];
-        v[0] += ri * ri;
-        v[1] += ri * ri;
+        const double rd = (double)ri;
+        z[i] = ri;
+        v[0] += rd * rd;
+        v[1] += rd * rd;
       }
     }
   }
@@ -267,16 +281,17 @@ template <typename Real>
 __global__ void __launch_bounds__(TLH_THREADS)
     direction_kernel(Real* __restrict__ p, const Real* __restrict__ z, int64_t n, int32_t* control,
                      double* scalars, double* partials, uint32_t* counter) {
-  __shared__ double smem[TLH_THREADS];
+  __shared__ double smem[TLH_THREADS / 64];
   const bool stopped = frozen(control);
   const double c = scalars[TLH_PCG_CURVATURE];
   const double rz = scalars[TLH_PCG_RZ];
   const double nrz = scalars[TLH_PCG_NEW_RZ];
   const bool move = !stopped && finite(c) && finite(rz) && finite(nrz) && c > 0.0 && rz > 0.0;
   const double beta = move ? nrz / rz : 0.0;
-  if (move) {
-    for (int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * TLH_THREADS)
-      p[i] = (Real)((double)z[i] + beta * (double)p[i]);
+  // loads issued before the scalars are awaited; a frozen state rewrites p unchanged
+  for (int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * TLH_THREADS) {
+    const Real zi = z[i], pi = p[i];
+    p[i] = move ? (Real)((double)zi + beta * (double)pi) : pi;
   }
   double v[1] = {0.0};
   if (grid_sum<1>(v, partials, counter, smem) && threadIdx.x == 0 && !stopped) {
@@ -544,7 +559,7 @@ __global__ void __launch_bounds__(TLH_THREADS)
     reset_kernel(tlh_block_diag_t pc, int identity, const Real* __restrict__ src, Real* __restrict__ x,
                  Real* __restrict__ r, Real* __restrict__ z, Real* __restrict__ p, int64_t n, double* scalars,
                  double* partials, uint32_t* counter) {
-  __shared__ double smem[2 * TLH_THREADS];
+  __shared__ double smem[2 * (TLH_THREADS / 64)];
   double v[2] = {0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * TLH_THREADS;
   if (!identity) {
@@ -581,7 +596,7 @@ __global__ void __launch_bounds__(TLH_THREADS)
     true_residual_kernel(const Real* __restrict__ rhs, const Real* __restrict__ ax, const Real* __restrict__ r,
                          Real* __restrict__ true_r, int64_t n, const int32_t* control, double* scalars,
                          double* partials, uint32_t* counter) {
-  __shared__ double smem[2 * TLH_THREADS];
+  __shared__ double smem[2 * (TLH_THREADS / 64)];
   double v[2] = {0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * TLH_THREADS) {
     const Real t = rhs[i] - ax[i];
@@ -614,7 +629,7 @@ int step_impl(const tlh_block_diag_t* pc, int mode, void* x, void* r, void* z, c
               int64_t n, const tlh_pcg_state_t* st, hipStream_t s) {
   tlh_block_diag_t empty{};
   const int64_t work = mode == 0 ? precond_blocks(pc) : n;
-  hipLaunchKernelGGL(step_kernel<Real>, dim3(grid_for(work)), dim3(TLH_THREADS), 0, s, mode == 0 ? *pc : empty,
+  hipLaunchKernelGGL(step_kernel<Real>, dim3(grid_for(n, work)), dim3(TLH_THREADS), 0, s, mode == 0 ? *pc : empty,
                      mode, (Real*)x, (Real*)r, (Real*)z, (const Real*)p, (const Real*)ap, n, st->control,
                      st->scalars, st->partials, st->counter);
   TLH_RETURN_IF_LAUNCH_FAILED();
@@ -626,7 +641,7 @@ int reset_impl(const tlh_block_diag_t* pc, const void* src, void* x, void* r, vo
                const tlh_pcg_state_t* st, hipStream_t s) {
   tlh_block_diag_t empty{};
   const int64_t work = pc ? precond_blocks(pc) : n;
-  hipLaunchKernelGGL(reset_kernel<Real>, dim3(grid_for(work)), dim3(TLH_THREADS), 0, s, pc ? *pc : empty,
+  hipLaunchKernelGGL(reset_kernel<Real>, dim3(grid_for(n, work)), dim3(TLH_THREADS), 0, s, pc ? *pc : empty,
                      pc ? 0 : 1, (const Real*)src, (Real*)x, (Real*)r, (Real*)z, (Real*)p, n, st->scalars,
                      st->partials, st->counter);
   TLH_RETURN_IF_LAUNCH_FAILED();

@@ -10,21 +10,43 @@ namespace tlh {
 
 constexpr int RED_BLOCKS = TLH_PCG_MAX_BLOCKS;
 
-__device__ __forceinline__ void block_sum2(double (&v)[2], double* smem) {
-  smem[threadIdx.x] = v[0];
-  smem[TLH_THREADS + threadIdx.x] = v[1];
+// Workgroups of a fixed-order grid reduction.  Each workgroup publishes its
+// partial with an agent-scope release and takes a ticket; on MI355X that costs
+// about 20 ns per workgroup at the end of the launch (a 1.2M-entry float64 dot:
+// 28 us on 1,024 workgroups, 10 us on 128-256, 3.9 us without the tickets), so
+// grids are capped and each thread strides over more entries.
+// 256 workgroups keep the per-observation LM kernels (linearization, trial,
+// squared norms) at full rate; 128 measured slower at 5M observations.
+constexpr int RED_GRID_CAP = 256;
+
+// Fixed-order sum of NV values over the workgroup (TLH_THREADS threads): an
+// xor-shuffle tree inside each wavefront (every lane ends with the same sum),
+// then the wavefront sums in wavefront order.  Every thread returns the total.
+template <int NV>
+__device__ __forceinline__ void wave_block_sum(double (&v)[NV], double* smem) {
+  constexpr int NWAVES = TLH_THREADS / 64;
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) smem[k * NWAVES + (threadIdx.x >> 6)] = v[k];
   __syncthreads();
-  for (int w = TLH_THREADS / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      smem[threadIdx.x] += smem[threadIdx.x + w];
-      smem[TLH_THREADS + threadIdx.x] += smem[TLH_THREADS + threadIdx.x + w];
-    }
-    __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double sum = smem[k * NWAVES];
+#pragma unroll
+    for (int w = 1; w < NWAVES; ++w) sum += smem[k * NWAVES + w];
+    v[k] = sum;
   }
-  v[0] = smem[0];
-  v[1] = smem[TLH_THREADS];
-  __syncthreads();
+  __syncthreads();  // the next sum rewrites smem
 }
+
+// LDS doubles the two-sum reductions need (one per wavefront and sum).
+constexpr int RED_SMEM = 2 * (TLH_THREADS / 64);
+
+__device__ __forceinline__ void block_sum2(double (&v)[2], double* smem) { wave_block_sum<2>(v, smem); }
 
 // Fixed-order grid reduction of two sums; the last block writes out[0..1].
 __device__ __forceinline__ void grid_sum2(double (&v)[2], double* partials, uint32_t* counter, double* out,
@@ -55,7 +77,18 @@ __device__ __forceinline__ void grid_sum2(double (&v)[2], double* partials, uint
 
 inline int red_grid(int64_t work) {
   int64_t g = ceil_div(work, TLH_THREADS);
-  return (int)(g < 1 ? 1 : (g > RED_BLOCKS ? RED_BLOCKS : g));
+  return (int)(g < 1 ? 1 : (g > RED_GRID_CAP ? RED_GRID_CAP : g));
+}
+
+// Grid of a reduction over a vector of n entries with little work per entry
+// (PCG vector updates, dot products, vector statistics), whose threads each
+// take one of `work` items (entries, or blocks of entries; default n): 64
+// workgroups up to 2^17 entries, 128 up to 2^20, then 256 -- the measured
+// balance of the ticket cost against the streaming rate.
+inline int vector_grid(int64_t n, int64_t work = -1) {
+  const int64_t cap = n <= (int64_t(1) << 17) ? 64 : (n <= (int64_t(1) << 20) ? 128 : 256);
+  const int64_t g = ceil_div(work < 0 ? n : work, TLH_THREADS);
+  return (int)(g < 1 ? 1 : (g > cap ? cap : g));
 }
 
 template <typename T>

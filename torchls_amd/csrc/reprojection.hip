@@ -75,7 +75,7 @@ template <typename Real, typename BReal>
 __global__ void __launch_bounds__(TLH_THREADS)
     linearize_kernel(ReprojArgs<Real> a, Real* __restrict__ raw, Real* __restrict__ model, BReal* __restrict__ jc,
                      BReal* __restrict__ jp, double* out, double* partials, uint32_t* counter) {
-  __shared__ double smem[2 * TLH_THREADS];
+  __shared__ double smem[RED_SMEM];
   constexpr int NJ = TLH_THREADS * 18;
   __shared__ BReal sj[NJ + NJ / 32];
   double v[2] = {0.0, 0.0};
@@ -158,7 +158,7 @@ template <typename Real>
 __global__ void __launch_bounds__(TLH_THREADS)
     trial_kernel(ReprojArgs<Real> a, const void* before, int before_dtype, double* out, double* partials,
                  uint32_t* counter) {
-  __shared__ double smem[2 * TLH_THREADS];
+  __shared__ double smem[RED_SMEM];
   double v[2] = {0.0, 0.0};
   for (int64_t o = (int64_t)blockIdx.x * TLH_THREADS + threadIdx.x; o < a.count;
        o += (int64_t)gridDim.x * TLH_THREADS) {

@@ -1097,10 +1097,9 @@ constexpr int items_for() {
   return sizeof(Real) == 4 ? 4 : 2;
 }
 
-template <typename Real, int R, int D0, int D1, int D2>
-int launch_special(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
-                   hipStream_t stream) {
-  constexpr int ITEMS = items_for<Real>();
+template <typename Real, int R, int D0, int D1, int D2, int ITEMS>
+int launch_special_items(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
+                         hipStream_t stream) {
   if (g.tile_cap > TLH_THREADS * ITEMS) return TLH_EINVAL;
   const int64_t lds = (int64_t)TLH_THREADS * ITEMS * Dims<D0, D1, D2>::max * sizeof(Real) +
                       dense_bytes<Real>(op, g);
@@ -1110,6 +1109,17 @@ int launch_special(const tlh_operator_t* op, const tlh_group_t& g, const void* x
                      make_args<Real>(op, g, x));
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
+}
+
+// Tiles of at most one observation per thread (plans of groups too small to
+// fill the chip with full tiles) take the one-item kernel: the items of a
+// thread are processed one after another, so a short tile on the multi-item
+// kernel would still pay for every item.
+template <typename Real, int R, int D0, int D1, int D2>
+int launch_special(const tlh_operator_t* op, const tlh_group_t& g, const void* x,
+                   hipStream_t stream) {
+  if (g.tile_cap <= TLH_THREADS) return launch_special_items<Real, R, D0, D1, D2, 1>(op, g, x, stream);
+  return launch_special_items<Real, R, D0, D1, D2, items_for<Real>()>(op, g, x, stream);
 }
 
 template <typename Real>

@@ -363,8 +363,11 @@ def build_structure(
         # (the wave kernel addresses each slot's x slice with 32-bit byte offsets)
         wave = (ns == 2 and shape in wave_shapes and count > 0
                 and all(types[t].count * types[t].dim * esize < 2**32 for t in slot_types))
-        gcap = WAVE_THREADS * wave_items(esize) if wave else cap
-        tile_start = _aligned_tiles(count, gcap, device) if wave else _cut_tiles(sorted_idx[primary], cap)
+        # a group whose full tiles would leave most compute units idle is cut into
+        # tiles of one observation per thread (the one-item tile kernels)
+        small = tile_cap is None and not wave and -(-count // cap) < cu_count
+        gcap = WAVE_THREADS * wave_items(esize) if wave else (THREADS if small else cap)
+        tile_start = _aligned_tiles(count, gcap, device) if wave else _cut_tiles(sorted_idx[primary], gcap)
         ntiles = tile_start.numel() - 1
         stage = gcap * dmax * esize + (gcap * 4 if wave else 0)
         dense_budget = lds_limit - stage

@@ -47,6 +47,9 @@ class Config:
     # the HIP backend's other optional exit: at most this many restarts after a
     # converged recurrence whose true residual misses the tolerance (None: no limit)
     max_refinements: int | None = None
+    # the HIP backend's option to never restart on drift (checks then restart only
+    # after a converged recurrence, or at every check with the periodic strategy)
+    drift_restarts: bool = True
 
 
 STALL_FACTOR = 0.5
@@ -196,7 +199,8 @@ def pcg_batched(matvec, rhs, apply=None, cfg: Config = Config()) -> Result:
             converged_true = tn
             refinements += 1
         drift = float(np.linalg.norm(tr - r))
-        if status == 1 or cfg.restart_strategy == "periodic" or drift > cfg.residual_drift_tolerance * max(tn, thr):
+        drifted = cfg.drift_restarts and drift > cfg.residual_drift_tolerance * max(tn, thr)
+        if status == 1 or cfg.restart_strategy == "periodic" or drifted:
             r = tr
             p = apply(r).copy()
             state[0] = float(r @ p)

@@ -192,7 +192,11 @@ def test_pose_graph_10k(device):
     out = {}
     for mv in (None, torch.float32):
         P = pose_graph_problem(pg, device)
-        cg = ConjugateGradientConfig(matvec_dtype=mv, max_iterations=3000, stop_on_stagnation=mv is not None)
+        # mixed: float32 PCG vectors end at their first converged recurrence, without
+        # restarts (their true residual's rounding floor, above rtol here, would fire the
+        # drift test at every check); the float64 LM iteration corrects the step
+        mixed = dict(stop_on_stagnation=True, max_refinements=0, drift_restarts=False) if mv is not None else {}
+        cg = ConjugateGradientConfig(matvec_dtype=mv, max_iterations=3000, **mixed)
         vals, s = P.solve(return_summary=True, conjugate_gradient=cg,
                           termination=TerminationConfig(max_iterations=steps))
         print(f"  matvec {mv}: {s.termination_reason}, costs {[f'{c:.9e}' for c in s.cost_history]}, "
@@ -207,9 +211,11 @@ def test_pose_graph_10k(device):
         out[mv] = s
     s64, s32 = out[None], out[torch.float32]
     # mixed precision stores the blocks in float32 (the gradient is summed in float64 from
-    # them: 6e-8 relative after the first step) and its float32 PCG vectors on this system
-    # (condition ~1e8) need more iterations and leave slightly different steps
+    # them: 6e-8 relative after the first step) and runs float32 PCG vectors, whose
+    # recurrence on this system (condition ~1e8) takes about as many iterations as
+    # float64's and leaves slightly different steps
     print(f"  cost rel diff per iteration {[abs(a - b) / b for a, b in zip(s32.cost_history, s64.cost_history)]}",
           flush=True)
     assert math.isclose(s32.cost_history[1], s64.cost_history[1], rel_tol=2e-7)
-    assert np.allclose(s32.cost_history, s64.cost_history, rtol=2e-6, atol=0)
+    assert np.allclose(s32.cost_history, s64.cost_history, rtol=1e-6, atol=0)
+    assert sum(s32.linear_iterations_history) <= 1.1 * sum(s64.linear_iterations_history)

@@ -295,6 +295,23 @@ def test_pcg_batched_refinement_limit(limit):
     assert free.reason == "max_iterations" and free.restarts > limit
 
 
+def test_pcg_batched_drift_restarts_off():
+    """drift_restarts=False removes exactly the drift clause of the restart test: on
+    the float32-rounded operator (its true residual floors near 1e-7 |b|, so checks
+    see drift) the solve equals the one with an unbounded drift tolerance, bit for
+    bit, and restarts only at converged recurrences; with drift restarts it restarts
+    at more checks."""
+    matvec, _, b = refinement_system()
+    cfg = dict(rtol=1e-12, max_iterations=200, residual_check_interval=10)
+    off = pcg_batched(matvec, b, None, Config(drift_restarts=False, **cfg))
+    unbounded = pcg_batched(matvec, b, None, Config(residual_drift_tolerance=1e30, **cfg))
+    assert (off.iterations, off.restarts, off.residual_checks, off.reason) == (
+        unbounded.iterations, unbounded.restarts, unbounded.residual_checks, unbounded.reason)
+    assert np.array_equal(off.solution, unbounded.solution)
+    drifting = pcg_batched(matvec, b, None, Config(**cfg))
+    assert drifting.restarts > off.restarts
+
+
 @pytest.mark.parametrize("mode", ["diagonal", "jacobi", "both"])
 def test_lm_scaling_is_marquardt_damping(mode):
     """Column scaling s and lambda I on J diag(s) is the damped system

@@ -260,3 +260,19 @@ def test_refinement_limit_matches_oracle(device, limit):
     assert (res.iterations, res.restarts, res.residual_checks, res.reason) == (
         ref.iterations, ref.restarts, ref.residual_checks, ref.reason)
 
+
+
+def test_drift_restarts_off_matches_oracle(device):
+    """drift_restarts=False on the float32-rounded diagonal operator, checks every 10
+    iterations: the oracle's counts and exit (restarts only at converged recurrences)."""
+    from tests.test_oracle_pins import refinement_system
+
+    _, e, b = refinement_system()
+    e32 = torch.tensor(e, device=device).float()
+    cfg = dict(rtol=1e-12, max_iterations=200, residual_check_interval=10, drift_restarts=False)
+    res = hip_pcg(lambda v: (e32 * v.float()).double(), torch.tensor(b, device=device), preconditioner=None,
+                  config=ConjugateGradientConfig(**cfg))
+    matvec, _, _ = refinement_system()
+    ref = pcg_batched(matvec, b, None, Config(**cfg))
+    assert (res.iterations, res.restarts, res.residual_checks, res.reason) == (
+        ref.iterations, ref.restarts, ref.residual_checks, ref.reason)

@@ -319,11 +319,9 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None, na
             converged_true = true_norm
             refinements += 1
         drift = math.sqrt(drift2) if drift2 >= 0 else math.nan
-        if (
-            status == 1
-            or config.restart_strategy == "periodic"
-            or drift > config.residual_drift_tolerance * max(true_norm, threshold)
-        ):
+        drifted = getattr(config, "drift_restarts", True) and drift > config.residual_drift_tolerance * max(
+            true_norm, threshold)
+        if status == 1 or config.restart_strategy == "periodic" or drifted:
             reset(true_r, False)
             restarts += 1
         stop_at = window_end(iterations)

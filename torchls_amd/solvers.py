@@ -57,6 +57,14 @@ class ConjugateGradientConfig:
     # residual floor usually lies above rtol: the outer float64 iteration corrects
     # the step instead of repeated refinement inside the solve.
     max_refinements: int | None = None
+    # Not in the reference either: when False, a residual check restarts only after a
+    # converged recurrence (or at every check with the periodic strategy), never on
+    # drift.  With float32 PCG vectors the computed true residual carries a rounding
+    # floor that the drift test cannot tell from drift of the recurrence: on an
+    # ill-conditioned system (a long pose-graph chain) it then restarts at every check
+    # and discards the Krylov space, while the float64 outer iteration corrects the
+    # step anyway.  Float64 solves rarely drift, so False usually changes nothing there.
+    drift_restarts: bool = True
 
     def __post_init__(self):
         if (isinstance(self.normal_max_packed_bytes, bool) or not isinstance(self.normal_max_packed_bytes, int)
@@ -89,6 +97,8 @@ class ConjugateGradientConfig:
             raise ValueError("matvec_dtype must be None, torch.float32 or torch.float64")
         if not isinstance(self.stop_on_stagnation, bool):
             raise ValueError("stop_on_stagnation must be a bool")
+        if not isinstance(self.drift_restarts, bool):
+            raise ValueError("drift_restarts must be a bool")
         if self.max_refinements is not None and (
                 isinstance(self.max_refinements, bool) or not isinstance(self.max_refinements, int)
                 or self.max_refinements < 0):

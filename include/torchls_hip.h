@@ -506,6 +506,13 @@ int64_t tlh_cholesky_tiles_workspace(int64_t n);
 int tlh_cholesky_tiles_f64(double* A, int64_t n, int64_t ld, void* work, int64_t work_bytes, int32_t* status,
                            tlh_stream_t stream);
 
+/* Poll limit of the bounded cross-workgroup waits of tlh_cholesky_tiles_f64 and
+ * tlh_cholesky_solve_blocks_f64 (and its no-allocation variant): polls >= 1 caps
+ * them at that many polls, 0 restores each kernel's own limit.  Process-wide, not
+ * synchronised with launches in flight.  No counterpart in the reference: it lets
+ * tests force the give-up paths (status bits 4 and 2) and the callers' fallbacks. */
+int tlh_set_wait_limit(int64_t polls);
+
 /* b <- (L L^T)^-1 b with the factor from tlh_cholesky_f64 (float64, in place).
  * Replaces: torch.cholesky_solve (solvers.py:518). */
 int tlh_cholesky_solve_f64(const double* L, int64_t n, int64_t ld, double* b, tlh_stream_t stream);

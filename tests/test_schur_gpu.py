@@ -128,6 +128,50 @@ def test_cholesky_tiles_complete(device, n):
     assert torch.equal(L, L2)  # bitwise repeatable
 
 
+def test_wait_give_up_fallbacks(device):
+    """With the bounded waits capped at one poll (tlh_set_wait_limit), the tile
+    factorisation gives up (status bit 4) and the blocks solve gives up (bit 2); both
+    launches still end, and cholesky_f64 / cholesky_solve_f64 fall back to the panel
+    launches and the one-workgroup substitution with results equal to LAPACK's."""
+    print("cholesky wait give-up fallbacks n=1000", flush=True)
+    from torchls_amd.cholesky import _factor
+
+    lib = _lib.load()
+    n = 1000
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((n, n + 3))
+    M = A @ A.T + n * 1e-3 * np.eye(n)
+    b = rng.standard_normal(n)
+    Mt, bt = torch.tensor(M, device=device), torch.tensor(b, device=device)
+    _lib.check(lib.tlh_set_wait_limit(1), "tlh_set_wait_limit")
+    try:
+        status = torch.zeros(1, dtype=torch.int32, device=device)
+        L = Mt.clone()
+        _factor(L, status, "tiles")
+        assert int(status.item()) & 4
+        status.zero_()
+        Lp = Mt.clone()
+        _factor(Lp, status, "panels")
+        x = bt.clone()
+        work = torch.empty(2 * n, dtype=torch.float64, device=device)
+        _lib.check(lib.tlh_cholesky_solve_blocks_f64(Lp.data_ptr(), n, n, x.data_ptr(), work.data_ptr(),
+                                                     status.data_ptr(), _lib.stream_handle(device)),
+                   "tlh_cholesky_solve_blocks_f64")
+        assert int(status.item()) == 2
+        Lf = cholesky_f64(Mt).cpu().numpy()
+        xf = cholesky_solve_f64(Mt, bt).cpu().numpy()
+    finally:
+        _lib.check(lib.tlh_set_wait_limit(0), "tlh_set_wait_limit")
+    assert rel_err(np.tril(Lf), np.linalg.cholesky(M)) <= 1e-12
+    assert rel_err(xf, np.linalg.solve(M, b)) <= 1e-10
+    assert lib.tlh_set_wait_limit(-1) == _lib.TLH_EINVAL
+    # the default limits again: no wait gives up
+    status = torch.zeros(1, dtype=torch.int32, device=device)
+    L = Mt.clone()
+    _factor(L, status, "tiles")
+    assert int(status.item()) == 0
+
+
 def test_cholesky_solve_sentinel_rhs(device):
     """The multi-workgroup solve marks unsolved values with a signalling-NaN bit pattern.
     A right-hand side holding that exact pattern completes end to end (NaN result, no

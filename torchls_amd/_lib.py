@@ -283,6 +283,7 @@ SIGNATURES = {
     "tlh_group_tiles": (ctypes.c_int, [ctypes.POINTER(Operator), ctypes.c_int, c_void_p, c_void_p]),
     "tlh_cholesky_f64": (ctypes.c_int, [c_void_p, i64, i64, c_void_p, c_void_p]),
     "tlh_cholesky_tiles_workspace": (i64, [i64]),
+    "tlh_set_wait_limit": (i32, [i64]),
     "tlh_cholesky_tiles_f64": (ctypes.c_int, [c_void_p, i64, i64, c_void_p, i64, c_void_p, c_void_p]),
     "tlh_cholesky_solve_f64": (ctypes.c_int, [c_void_p, i64, i64, c_void_p, c_void_p]),
     "tlh_cholesky_solve_blocks_f64": (ctypes.c_int, [c_void_p, i64, i64, c_void_p, c_void_p, c_void_p, c_void_p]),

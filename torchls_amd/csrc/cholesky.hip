@@ -403,7 +403,8 @@ __global__ void __launch_bounds__(SOLVE_THREADS)
 // 64-step diagonal solves and the polls.  Workgroups are dispatched in index
 // order and a block waits only on blocks with lower workgroup indices, so every
 // wait is on a workgroup that is already running; a wait still gives up after
-// SOLVE_SPIN_MAX polls, setting *status |= 2.
+// SOLVE_SPIN_MAX polls (or the limit set by tlh_set_wait_limit), setting
+// *status |= 2.
 constexpr int SOLVE_WG = 256;
 constexpr int SOLVE_SPIN_MAX = 1 << 20;  // about a second of polls; a real wait is the solve's length
 constexpr long long SOLVE_EMPTY = 0x7ff4dead5eed1234ll;  // a signalling NaN
@@ -416,7 +417,7 @@ __global__ void __launch_bounds__(TLH_THREADS) solve_work_reset(long long* work,
 // Wavefront 0 polls the cnt values at src until all are solved and stages them
 // in xs (zero past cnt); the barrier also orders every earlier read of xs
 // before the new values land.
-__device__ __forceinline__ void wait_block(const double* src, int cnt, double* xs, int32_t* status) {
+__device__ __forceinline__ void wait_block(const double* src, int cnt, double* xs, int32_t* status, int spin) {
   __syncthreads();
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x;
@@ -426,7 +427,7 @@ __device__ __forceinline__ void wait_block(const double* src, int cnt, double* x
     while (true) {
       bits = lane < cnt ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
       if (__all(bits != SOLVE_EMPTY)) break;
-      if (++polls > SOLVE_SPIN_MAX) {
+      if (++polls > spin) {
         if (lane == 0) __hip_atomic_fetch_or(status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
@@ -448,7 +449,7 @@ __device__ __forceinline__ void publish_value(double* dst, double v) {
 // Forward: block i solves L_ii y_i = b_i - sum_{j<i} L_ij y_j.
 __global__ void __launch_bounds__(SOLVE_WG)
     chol_forward_blocks(const double* __restrict__ L, int64_t n, int64_t ld, double* b, double* ywork,
-                        int32_t* status) {
+                        int32_t* status, int spin) {
   __shared__ double D[NB][NB + 1];
   __shared__ double xs[NB], acc[NB];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -478,7 +479,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
   for (int j = 0; j < i; ++j) {
     double nxt[16];
     if (j + 1 < i) load(j + 1, nxt);
-    wait_block(ywork + (int64_t)j * NB, NB, xs, status);
+    wait_block(ywork + (int64_t)j * NB, NB, xs, status, spin);
     double sacc = 0.0;
 #pragma unroll
     for (int c = 0; c < 16; ++c) sacc += cur[c] * xs[q * 16 + c];
@@ -508,7 +509,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
 // Backward: workgroup w solves block k = nb - 1 - w: L_kk^T x_k = y_k - sum_{j>k} L_jk^T x_j.
 __global__ void __launch_bounds__(SOLVE_WG)
     chol_backward_blocks(const double* __restrict__ L, int64_t n, int64_t ld, double* b, double* xwork,
-                         int32_t* status) {
+                         int32_t* status, int spin) {
   __shared__ double D[NB][NB + 1];
   __shared__ double xs[NB], acc[NB];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -544,7 +545,7 @@ __global__ void __launch_bounds__(SOLVE_WG)
     const int mj = (int)min((int64_t)NB, n - j0);
     double nxt[16];
     if (jb - 1 > k) load(jb - 1, nxt);
-    wait_block(xwork + j0, mj, xs, status);
+    wait_block(xwork + j0, mj, xs, status, spin);
     double sacc = 0.0;
 #pragma unroll
     for (int rr = 0; rr < 16; ++rr) sacc += cur[rr] * xs[q * 16 + rr];
@@ -621,9 +622,12 @@ int solve_blocks(const double* L, int64_t n, int64_t ld, double* b, double* work
   hipLaunchKernelGGL(solve_work_reset, dim3((unsigned)ceil_div(2 * n, TLH_THREADS)), dim3(TLH_THREADS), 0, s,
                      reinterpret_cast<long long*>(work), 2 * n);
   TLH_RETURN_IF_LAUNCH_FAILED();
-  hipLaunchKernelGGL(chol_forward_blocks, dim3((unsigned)nb), dim3(SOLVE_WG), 0, s, L, n, ld, b, work, status);
+  const int spin = wait_limit > 0 ? (int)std::min<int64_t>(wait_limit, SOLVE_SPIN_MAX) : SOLVE_SPIN_MAX;
+  hipLaunchKernelGGL(chol_forward_blocks, dim3((unsigned)nb), dim3(SOLVE_WG), 0, s, L, n, ld, b, work, status,
+                     spin);
   TLH_RETURN_IF_LAUNCH_FAILED();
-  hipLaunchKernelGGL(chol_backward_blocks, dim3((unsigned)nb), dim3(SOLVE_WG), 0, s, L, n, ld, b, work + n, status);
+  hipLaunchKernelGGL(chol_backward_blocks, dim3((unsigned)nb), dim3(SOLVE_WG), 0, s, L, n, ld, b, work + n, status,
+                     spin);
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }
@@ -667,4 +671,14 @@ extern "C" int tlh_cholesky_solve_blocks_f64_noalloc(const double* L, int64_t n,
   double* work = nullptr;
   if (int rc = scratch_for_current_device(&work)) return rc;
   return solve_blocks(L, n, ld, b, work, status, reinterpret_cast<hipStream_t>(stream));
+}
+
+namespace tlh {
+int64_t wait_limit = 0;
+}  // namespace tlh
+
+extern "C" int tlh_set_wait_limit(int64_t polls) {
+  if (polls < 0) return TLH_EINVAL;
+  tlh::wait_limit = polls;
+  return 0;
 }

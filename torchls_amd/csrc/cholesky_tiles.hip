@@ -587,7 +587,7 @@ extern "C" int tlh_cholesky_tiles_f64(double* A, int64_t n, int64_t ld, void* wo
   a.abort = a.state + ntiles;
   a.dinv = reinterpret_cast<double*>(base + state_bytes(T));
   a.status = status;
-  a.spin = (int64_t)1 << 18;
+  a.spin = wait_limit > 0 ? std::min<int64_t>(wait_limit, (int64_t)1 << 18) : (int64_t)1 << 18;
   TLH_RETURN_IF_ERROR(hipFuncSetAttribute((const void*)chol_tiles_kernel,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_BYTES));
   int dev = 0, cus = 0, per_cu = 0;

@@ -77,6 +77,10 @@ inline hipError_t allow_lds(const void* kernel, int64_t bytes) {
   return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+// Poll limit of the bounded cross-workgroup waits (tile Cholesky, blocks solve)
+// set by tlh_set_wait_limit; 0 keeps each kernel's own limit.  Process-wide.
+extern int64_t wait_limit;
+
 }  // namespace tlh
 
 namespace tlh {

@@ -472,6 +472,41 @@ int tlh_group_tiles(const tlh_operator_t* op, int group, const void* x, tlh_stre
 int tlh_vector_stats(const void* a, int a_dtype, const void* b, int b_dtype, int64_t n, double* out,
                      double* partials, uint32_t* counter, tlh_stream_t stream);
 
+/* Trust region of tlh_lm_decide (TrustRegionConfig): enabled = 0 without one (every
+ * finite trial is accepted, the damping stays), adaptive = 1 for the "adaptive"
+ * strategy, 0 for the factor rule. */
+typedef struct tlh_lm_trust {
+  int32_t enabled;
+  int32_t adaptive;
+  double lambda_min;
+  double lambda_max;
+  double lambda_factor;
+  double quality_min;
+} tlh_lm_trust_t;
+
+/* Where a trial's float64 values are on the device: the trial objective, the actual
+ * reduction, ||J delta||^2, the step statistics {g.delta, delta.delta, ...} of
+ * tlh_vector_stats, and a value finite exactly when every step entry is. */
+typedef struct tlh_lm_trial {
+  const double* cost;
+  const double* actual;
+  const double* sq;
+  const double* stats;
+  const double* finite;
+} tlh_lm_trial_t;
+
+/* Gain ratio, acceptance and damping update of one LM trial, on the device (one
+ * thread).  trial == NULL: a trial without a usable step (rejected).  state[0..1] =
+ * {damping, rejection factor} (float64, device) is read and updated in place.
+ * out[0..8] (float64, device) = {predicted reduction -g.delta - ||J delta||^2 / 2,
+ * gain ratio, accepted, usable, ||delta||, the trial's damping, stop (rejected at
+ * lambda_max, or rejected without a trust region), damping and rejection factor
+ * after the update}; flags are 0 or 1.
+ * Replaces: the predicted reduction, gain ratio, acceptance and lambda update of
+ * solvers.py:607-617, 633-655, 683-693. */
+int tlh_lm_decide(const tlh_lm_trial_t* trial, double* state, const tlh_lm_trust_t* trust, double* out,
+                  tlh_stream_t stream);
+
 /* dst[di(i)] = scale * src[si(i)] for i < count, si(i) = src_index ? src_index[i]
  * : src_offset + i (likewise di), converted through float64 (float32/float64 on
  * either side): the gathers and scatters of a Schur step in one launch each.

@@ -57,6 +57,31 @@ class Summary:
     linear_iterations_history: list = field(default_factory=list)
 
 
+def trial_decision(obj, red, pred, lam, rej, tr, usable=True):
+    """Gain ratio, acceptance and damping update of one trial (solvers.py:607-617,
+    633-655, 683-693): (quality, accepted, damping, rejection factor, stop), where
+    stop means the trial loop ends on a rejection (no trust region, or lambda_max
+    reached).  usable=False: a trial without a usable step, rejected."""
+    q = (red / pred if pred > 0 else -math.inf) if usable else math.nan
+    accepted = usable and math.isfinite(obj) and (tr is None or (red > 0 and q >= tr.step_quality_min))
+    if accepted:
+        if tr:
+            if tr.strategy == "adaptive":
+                mult = max(1 / 3, 1 - (2 * min(q, 1.0) - 1) ** 3)
+                lam = max(tr.lambda_min, min(tr.lambda_max, lam * mult))
+                rej = 2.0
+            elif q > 0.75:
+                lam = max(tr.lambda_min, lam / tr.lambda_factor)
+        return q, True, lam, rej, False
+    if tr is None or lam >= tr.lambda_max:
+        return q, False, lam, rej, True
+    if tr.strategy == "adaptive":
+        lam, rej = min(tr.lambda_max, lam * rej), min(rej * 2, tr.lambda_max / tr.lambda_min)
+    else:
+        lam = min(tr.lambda_max, lam * tr.lambda_factor)
+    return q, False, lam, rej, False
+
+
 def _solve_scale(diag, column_scale, jacobi_scaling, tr):
     """(column_scale, solve_scale) of one iteration (solvers.py:430-460): the Jacobi
     column scale 1 / (1 + sqrt(diag(J^T J))) fixed at the first iteration, and with
@@ -166,20 +191,13 @@ def solve(problem, vals, *, linear_solver="conjugate_gradient", cg: Config = Con
                 prop = problem.retract(vals, delta)
                 obj, red = problem.evaluate_trial(prop, lin)
                 pred = float(-(g0 @ delta) - 0.5 * J0.squared_matvec_norm(delta))
-                q = red / pred if pred > 0 else -math.inf
-                accepted = math.isfinite(obj) and (tr is None or (red > 0 and q >= tr.step_quality_min))
+                q, accepted, lam_next, rej_next, stop = trial_decision(obj, red, pred, lam, rej, tr)
                 if accepted:
                     vals = prop
                     s.accepted_steps += 1
                     s.final_cost = obj
                     s.cost_history.append(obj)
-                    if tr:
-                        if tr.strategy == "adaptive":
-                            mult = max(1 / 3, 1 - (2 * min(q, 1.0) - 1) ** 3)
-                            lam = max(tr.lambda_min, min(tr.lambda_max, lam * mult))
-                            rej = 2.0
-                        elif q > 0.75:
-                            lam = max(tr.lambda_min, lam / tr.lambda_factor)
+                    lam, rej = lam_next, rej_next
                     lin = problem.linearize(vals)
                     s.gradient_norm = float(np.abs(lin["J"].rmatvec(lin["residual"])).max())
                     if s.gradient_norm <= term.gradient_tolerance:
@@ -190,13 +208,11 @@ def solve(problem, vals, *, linear_solver="conjugate_gradient", cg: Config = Con
                         s.termination_reason = "parameter_tolerance"
                     break
             s.rejected_steps += 1
-            if tr is None or lam >= tr.lambda_max:
+            if not usable:
+                q, accepted, lam_next, rej_next, stop = trial_decision(math.nan, 0.0, 0.0, lam, rej, tr, usable=False)
+            if stop:
                 break
-            if tr.strategy == "adaptive":
-                lam = min(tr.lambda_max, lam * rej)
-                rej = min(rej * 2, tr.lambda_max / tr.lambda_min)
-            else:
-                lam = min(tr.lambda_max, lam * tr.lambda_factor)
+            lam, rej = lam_next, rej_next
         if not accepted:
             s.termination_reason = "no_acceptable_step"
             break

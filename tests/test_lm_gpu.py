@@ -155,3 +155,50 @@ def test_column_scaling_matches_oracle(device, solver, mode):
     assert s.linear_iterations_history == ref.linear_iterations_history
     assert s.lambda_history == pytest.approx(ref.lambda_history, rel=1e-12)
     assert math.isclose(s.final_cost, ref.final_cost, rel_tol=1e-9)
+
+
+def test_lm_decide_matches_oracle(device):
+    """tlh_lm_decide (gain ratio, acceptance, damping update on the device) equals the
+    oracle's trial_decision on hand-picked edge cases and 300 random trials per
+    trust-region rule, with and without a usable step: bit for bit, except that an
+    accepted adaptive step's damping may differ by up to 4 ulps (the multiplier's cube
+    is rounded once on the device, the host's libm pow to about half an ulp; one ulp of
+    a cube of magnitude <= 1 moves 1 - cube by <= 2.2e-16, <= 3 ulps of a multiplier
+    >= 1/3, and the product rounds once more)."""
+    print("lm decide vs oracle", flush=True)
+    import random
+
+    from oracle.lm import TrustRegion, trial_decision
+    from torchls_amd import _lib
+    from torchls_amd.hip_ops import lm_decide
+    rules = [TrustRegion(lambda_min=1e-9, lambda_max=1e6, lambda_factor=4.0, step_quality_min=1e-4),
+             TrustRegion(lambda_min=1e-9, lambda_max=1e6, step_quality_min=1e-4, strategy="adaptive"), None]
+    rng = random.Random(11)
+    cases = [(1.0, 0.9, 1.0), (1.0, 1.0, 1.0), (1.0, -1.0, -2.0), (1.0, 0.0, 1.0), (math.nan, 1.0, 1.0),
+             (math.inf, 1.0, 1.0), (1.0, 5.0, 1.0), (1.0, 1e-5, 1.0)]
+    cases += [(rng.uniform(0, 2), rng.uniform(-0.5, 2.0), rng.uniform(-0.2, 2.0)) for _ in range(300)]
+    for tr in rules:
+        trust = _lib.LmTrust(1 if tr else 0, 1 if tr and tr.strategy == "adaptive" else 0,
+                             tr.lambda_min if tr else 0.0, tr.lambda_max if tr else 0.0,
+                             tr.lambda_factor if tr else 0.0, tr.step_quality_min if tr else 0.0)
+        for k, (obj, red, pred) in enumerate(cases):
+            lam = (0.0 if tr is None else [1e-3, 1e6, 3.7e-2][k % 3])
+            rej = [2.0, 8.0][k % 2]
+            usable = k % 11 != 5
+            state = torch.tensor([lam, rej], dtype=torch.float64, device=device)
+            if usable:
+                t = [torch.tensor([v], dtype=torch.float64, device=device) for v in (obj, red, 0.0)]
+                stats = torch.tensor([-pred, 4.0, 0.0, 0.0], dtype=torch.float64, device=device)  # -g.d = pred
+                out = lm_decide((t[0], t[1], t[2], stats, torch.zeros(1, dtype=torch.float64, device=device)),
+                                        state, trust).cpu().tolist()
+            else:
+                out = lm_decide(None, state, trust).cpu().tolist()
+            q, acc, lam_n, rej_n, stop = trial_decision(obj, red, pred, lam, rej, tr, usable=usable)
+            st = state.cpu().tolist()
+            assert (out[2] == 1.0) == acc and (out[3] == 1.0) == usable and (out[6] == 1.0) == stop
+            tol = 4 * math.ulp(lam_n) if (acc and tr is not None and tr.strategy == "adaptive") else 0.0
+            assert abs(out[7] - lam_n) <= tol and abs(st[0] - lam_n) <= tol and out[7] == st[0]
+            assert out[8] == rej_n and st[1] == rej_n and out[5] == lam
+            if usable:
+                assert out[0] == pred and out[4] == 2.0
+                assert out[1] == q or (math.isnan(out[1]) and math.isnan(q))

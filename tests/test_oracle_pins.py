@@ -7,6 +7,8 @@ a wrong sign or a missing damping term) fails at least one assertion.
 
 from fractions import Fraction
 
+import math
+
 import numpy as np
 import pytest
 import scipy.linalg
@@ -345,3 +347,39 @@ def test_lm_scaling_is_marquardt_damping(mode):
     expect = o.retract(vals0, delta)
     for a, b in zip(vals1, expect):
         assert np.allclose(a, b, rtol=1e-9, atol=1e-12)
+
+
+def test_trial_decision_by_hand():
+    """oracle.lm.trial_decision (solvers.py:607-617, 633-655, 683-693) on cases whose
+    outcome follows from the rules by hand: the factor rule divides lambda by the
+    factor above a gain ratio of 0.75 and multiplies it on a rejection; the adaptive
+    rule scales lambda by max(1/3, 1 - (2 min(q, 1) - 1)^3) (1/3 at q = 1, 1 at q = 1/2,
+    7/8 at q = 3/4) and multiplies it by a rejection factor that doubles up to
+    lambda_max / lambda_min; a nonpositive predicted or actual reduction, a gain ratio
+    below the threshold or a nonfinite cost rejects; a rejection at lambda_max, or
+    without a trust region, stops the trials."""
+    from oracle.lm import TrustRegion, trial_decision
+
+    fixed = TrustRegion(lambda_min=1e-6, lambda_max=1.0, lambda_factor=4.0, step_quality_min=1e-4)
+    assert trial_decision(1.0, 0.9, 1.0, 0.5, 2.0, fixed) == (0.9, True, 0.125, 2.0, False)
+    assert trial_decision(1.0, 0.5, 1.0, 0.5, 2.0, fixed) == (0.5, True, 0.5, 2.0, False)
+    assert trial_decision(1.0, 1e-5, 1.0, 0.5, 2.0, fixed) == (1e-5, False, 1.0, 2.0, False)
+    assert trial_decision(1.0, 1e-5, 1.0, 1.0, 2.0, fixed)[1:] == (False, 1.0, 2.0, True)  # at lambda_max
+    assert trial_decision(1.0, -1.0, -2.0, 0.5, 2.0, fixed)[:2] == (-math.inf, False)  # predicted <= 0
+    assert trial_decision(1.0, -0.5, 1.0, 0.5, 2.0, fixed)[1] is False  # actual <= 0
+    assert trial_decision(math.nan, 0.9, 1.0, 0.5, 2.0, fixed)[1] is False  # nonfinite cost
+    q, acc, lam, rej, stop = trial_decision(0.0, 0.0, 0.0, 0.25, 2.0, fixed, usable=False)
+    assert math.isnan(q) and (acc, lam, rej, stop) == (False, 1.0, 2.0, False)
+
+    adaptive = TrustRegion(lambda_min=1e-6, lambda_max=1.0, strategy="adaptive")
+    assert trial_decision(1.0, 2.0, 2.0, 0.75, 8.0, adaptive) == (1.0, True, 0.25, 2.0, False)  # x 1/3
+    assert trial_decision(1.0, 1.0, 2.0, 0.75, 8.0, adaptive) == (0.5, True, 0.75, 2.0, False)  # x 1
+    assert trial_decision(1.0, 1.5, 2.0, 0.5, 8.0, adaptive) == (0.75, True, 0.4375, 2.0, False)  # x 7/8
+    assert trial_decision(1.0, 4.0, 2.0, 0.75, 8.0, adaptive)[2] == 0.25  # the gain ratio counts up to 1
+    assert trial_decision(1.0, 1e-5, 1.0, 1e-3, 2.0, adaptive)[2:] == (2e-3, 4.0, False)
+    assert trial_decision(1.0, 1e-5, 1.0, 1e-3, 8.0, adaptive)[2:] == (8e-3, 16.0, False)
+    capped = TrustRegion(lambda_min=0.5, lambda_max=1.0, lambda_initial=0.5, strategy="adaptive")
+    assert trial_decision(1.0, 1e-5, 1.0, 0.5, 2.0, capped)[2:] == (1.0, 2.0, False)  # rej <= max/min = 2
+
+    assert trial_decision(1.0, -3.0, 1.0, 0.0, 2.0, None)[1:] == (True, 0.0, 2.0, False)  # no trust region
+    assert trial_decision(math.inf, 1.0, 1.0, 0.0, 2.0, None)[1:] == (False, 0.0, 2.0, True)

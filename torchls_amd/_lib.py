@@ -141,6 +141,22 @@ class PcgOperator(ctypes.Structure):
     ]
 
 
+class LmTrust(ctypes.Structure):
+    _fields_ = [
+        ("enabled", i32),
+        ("adaptive", i32),
+        ("lambda_min", ctypes.c_double),
+        ("lambda_max", ctypes.c_double),
+        ("lambda_factor", ctypes.c_double),
+        ("quality_min", ctypes.c_double),
+    ]
+
+
+class LmTrial(ctypes.Structure):
+    _fields_ = [("cost", c_void_p), ("actual", c_void_p), ("sq", c_void_p), ("stats", c_void_p),
+                ("finite", c_void_p)]
+
+
 class DeviceInfo(ctypes.Structure):
     _fields_ = [("cu_count", i32), ("lds_per_block", i32), ("warp_size", i32), ("gcn_arch", i32)]
 
@@ -245,6 +261,7 @@ SIGNATURES = {
         [ctypes.POINTER(Operator), ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, ctypes.c_double, c_void_p, i64,
          c_void_p],
     ),
+    "tlh_lm_decide": (i32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "tlh_vector_stats": (
         ctypes.c_int,
         [c_void_p, ctypes.c_int, c_void_p, ctypes.c_int, i64, c_void_p, c_void_p, c_void_p, c_void_p],

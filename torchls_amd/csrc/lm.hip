@@ -620,3 +620,86 @@ extern "C" int tlh_vector_stats(const void* a, int a_dtype, const void* b, int b
   TLH_RETURN_IF_LAUNCH_FAILED();
   return 0;
 }
+
+namespace {
+
+// x^3 with both products carried as double-double and one final rounding, as the
+// correctly rounded pow(x, 3) of the host's libm gives it.
+__device__ __forceinline__ double cube_rn(double x) {
+  const double h = x * x, hl = fma(x, x, -h);
+  const double c = h * x, cl = fma(h, x, -c) + hl * x;
+  return c + cl;
+}
+
+// One thread: the trial arithmetic of the LM loop in the reference's operations and
+// order (no contraction).  Every decision and the factor rule's damping equal a
+// host computation bit for bit; the adaptive rule's multiplier differs from one
+// through libm's pow(x, 3) (accurate to about half an ulp) by at most an ulp of
+// its cube.
+__global__ void lm_decide_kernel(tlh_lm_trial_t tr, int has_trial, double* state, tlh_lm_trust_t tt, double* out) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0) return;
+  const double damping = state[0], rej = state[1];
+  const double nan = __builtin_nan("");
+  double predicted = nan, quality = nan, step_norm = nan;
+  bool usable = false, accepted = false;
+  if (has_trial) {
+    const double cost = *tr.cost, actual = *tr.actual, sq = *tr.sq;
+    const double gd = tr.stats[0], dd = tr.stats[1];
+    usable = isfinite(*tr.finite);
+    step_norm = dd >= 0.0 ? sqrt(dd) : nan;
+    predicted = -gd - 0.5 * sq;
+    if (usable) {
+      quality = predicted > 0.0 ? actual / predicted : -__builtin_inf();
+      accepted = isfinite(cost) && (!tt.enabled || (actual > 0.0 && quality >= tt.quality_min));
+    }
+  }
+  double nd = damping, nr = rej;
+  bool stop = false;
+  if (accepted) {
+    if (tt.enabled) {
+      if (tt.adaptive) {
+        const double mult = fmax(1.0 / 3.0, 1.0 - cube_rn(2.0 * fmin(quality, 1.0) - 1.0));
+        nd = fmax(tt.lambda_min, fmin(tt.lambda_max, damping * mult));
+        nr = 2.0;
+      } else if (quality > 0.75) {
+        nd = fmax(tt.lambda_min, damping / tt.lambda_factor);
+      }
+    }
+  } else {
+    stop = !tt.enabled || damping >= tt.lambda_max;
+    if (!stop) {
+      if (tt.adaptive) {
+        nd = fmin(tt.lambda_max, damping * rej);
+        nr = fmin(rej * 2.0, tt.lambda_max / tt.lambda_min);
+      } else {
+        nd = fmin(tt.lambda_max, damping * tt.lambda_factor);
+      }
+    }
+  }
+  state[0] = nd;
+  state[1] = nr;
+  out[0] = predicted;
+  out[1] = quality;
+  out[2] = accepted ? 1.0 : 0.0;
+  out[3] = usable ? 1.0 : 0.0;
+  out[4] = step_norm;
+  out[5] = damping;
+  out[6] = stop ? 1.0 : 0.0;
+  out[7] = nd;
+  out[8] = nr;
+}
+
+}  // namespace
+
+extern "C" int tlh_lm_decide(const tlh_lm_trial_t* trial, double* state, const tlh_lm_trust_t* trust, double* out,
+                             tlh_stream_t stream) {
+  if (!state || !trust || !out) return TLH_EINVAL;
+  if (trial && (!trial->cost || !trial->actual || !trial->sq || !trial->stats || !trial->finite)) return TLH_EINVAL;
+  if (trust->enabled && !(trust->lambda_min > 0.0 && trust->lambda_max >= trust->lambda_min)) return TLH_EINVAL;
+  const tlh_lm_trial_t tr = trial ? *trial : tlh_lm_trial_t{};
+  hipLaunchKernelGGL(lm_decide_kernel, dim3(1), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), tr,
+                     trial ? 1 : 0, state, *trust, out);
+  TLH_RETURN_IF_LAUNCH_FAILED();
+  return 0;
+}

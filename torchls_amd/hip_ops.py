@@ -32,6 +32,27 @@ class Reduction:
         self.out = torch.zeros((slots, 2), dtype=torch.float64, device=device)
 
 
+def lm_decide(trial, state: torch.Tensor, trust) -> torch.Tensor:
+    """Gain ratio, acceptance and damping update of one LM trial on the device
+    (tlh_lm_decide).  ``trial``: the float64 device tensors (trial objective, actual
+    reduction, ||J delta||^2, step statistics of vector_stats, a value finite exactly
+    when the step is), or None for a trial without a usable step; ``state``: float64
+    device [damping, rejection factor], updated in place; ``trust``: ``_lib.LmTrust``.
+    Returns the float64 [9] decision (include/torchls_hip.h)."""
+    out = torch.empty(9, dtype=torch.float64, device=state.device)
+    if trial is None:
+        tr = None
+    else:
+        for t in trial:
+            if t.dtype != torch.float64 or t.device != state.device:
+                raise ValueError("lm_decide expects float64 tensors on the state's device")
+        tr = ctypes.byref(_lib.LmTrial(*(t.data_ptr() for t in trial)))
+    lib = _lib.load()
+    _lib.check(lib.tlh_lm_decide(tr, state.data_ptr(), ctypes.byref(trust), out.data_ptr(),
+                                 _lib.stream_handle(state.device)), "tlh_lm_decide")
+    return out
+
+
 class LinearizationKernels:
     def __init__(self, jacobian, diag_plan: HipDiagonalPlan | None = None):
         self.lib = _lib.load()

@@ -4,8 +4,9 @@ Same interface and control flow as torchls.solvers (solvers.py:19-696) with the
 ``"hip"`` backend: every operation of the block-sparse normal-equation path --
 diagonal blocks, damped block inversion, normal matvec, PCG, the Schur
 reduce/matvec/preconditioner/back-substitution, the dense reduced Cholesky, and
-the float64 gradient, objective and predicted-reduction reductions -- runs in
-the HIP kernels of this package.  ``matvec_dtype=torch.float32`` on float64
+the float64 gradient, objective and predicted-reduction reductions, the gain
+ratio, acceptance and damping update of each trial -- runs in the HIP kernels
+of this package.  ``matvec_dtype=torch.float32`` on float64
 parameters selects the mixed-precision mode (float32 blocks, vectors and
 matvec; float64 outer residual, gradient, objective, dots and inversions).
 """
@@ -22,9 +23,10 @@ from types import SimpleNamespace
 import torch
 from torch import Tensor
 
+from . import _lib
 from .hip_diagonal import HipBlockJacobi, HipDiagonalPlan, LinAlgError
 from .hip_normal import HipNormalPlan
-from .hip_ops import LinearizationKernels
+from .hip_ops import LinearizationKernels, lm_decide
 
 
 @dataclass(frozen=True)
@@ -280,7 +282,13 @@ def solve(
     summary = SolveSummary(initial_cost, initial_cost, cost_history=[initial_cost], linear_solver=linear_solver,
                            jacobi_scaling=jacobi_scaling)
     damping = trust_region.lambda_initial if trust_region else 0.0
-    rejection_factor = 2.0
+    # the gain ratio, acceptance and damping update of every trial run on the device
+    # (tlh_lm_decide), which keeps [damping, rejection factor]; `damping` mirrors it
+    lm_trust = _lib.LmTrust(
+        1 if trust_region else 0, 1 if trust_region and trust_region.strategy == "adaptive" else 0,
+        trust_region.lambda_min if trust_region else 0.0, trust_region.lambda_max if trust_region else 0.0,
+        trust_region.lambda_factor if trust_region else 0.0, trust_region.step_quality_min if trust_region else 0.0)
+    lm_state = torch.tensor([damping, 2.0], dtype=torch.float64, device=device)
     # Symbolic plans depend only on the residual structure, which an Analysis
     # fixes: they are built on the first solve and reused by later solves.
     plan_cache = getattr(problem, "plans", None)
@@ -501,7 +509,10 @@ def solve(
                 step = kernels.vector_stats(gradient, delta)
                 if finite_step is None:  # dense solves checked their step before the trial
                     finite_step = step[2:3] if linear_solver == "conjugate_gradient" else step.new_zeros(1)
-                parts = [objective.reshape(1), reduction.reshape(1), sq.reshape(1), step[:2], finite_step]
+                # predicted reduction, gain ratio, acceptance and the damping update, on the device
+                decision = lm_decide((objective.reshape(1), reduction.reshape(1), sq.reshape(1), step,
+                                              finite_step), lm_state, lm_trust)
+                parts = [objective.reshape(1), reduction.reshape(1), decision]
                 # The next linearization is issued before the trial is read (used if the step
                 # is accepted, dropped otherwise): its gradient and cost ride on the same read.
                 with region("linearize"):
@@ -515,16 +526,14 @@ def solve(
                     with region("prepare_linear_system"):
                         prep_next = prepare(lin_next.jacobian, g_next)
                 host = torch.cat(parts + stats_next).cpu().tolist()
-                proposed_cost, actual, sq_jd, dot_gd, step_sq = host[:5]
-                step_norm = math.sqrt(step_sq) if step_sq >= 0 else math.nan
-                predicted = -dot_gd - 0.5 * sq_jd  # the model decrease, in float64 on the host
-                # a nonfinite step is unusable: rejected without a trial record
-                usable = math.isfinite(host[5])
-                stats_host = host[6:9]  # the next linearization's (nonfinite count, max |g|, cost)
-            if usable:
-                quality = actual / predicted if predicted > 0 else -math.inf
-                accepted = math.isfinite(proposed_cost) and (
-                    trust_region is None or (actual > 0 and quality >= trust_region.step_quality_min))
+                proposed_cost, actual = host[:2]
+                verdict = host[2:11]
+                stats_host = host[11:14]  # the next linearization's (nonfinite count, max |g|, cost)
+            else:  # no usable step: a rejection, the damping update still on the device
+                verdict = lm_decide(None, lm_state, lm_trust).cpu().tolist()
+            predicted, quality, accepted, usable, step_norm, _, stop, damping_next, _ = verdict
+            accepted, usable, stop = accepted != 0.0, usable != 0.0, stop != 0.0
+            if usable:  # a nonfinite step is rejected without a trial record
                 if record_trials:
                     summary.trial_history.append(dict(damping=damping, predicted_reduction=predicted,
                                                       actual_reduction=actual, gain_ratio=quality,
@@ -534,13 +543,7 @@ def solve(
                     summary.accepted_steps += 1
                     summary.final_cost = proposed_cost
                     summary.cost_history.append(proposed_cost)
-                    if trust_region:
-                        if trust_region.strategy == "adaptive":
-                            mult = max(1 / 3, 1 - (2 * min(quality, 1.0) - 1) ** 3)
-                            damping = max(trust_region.lambda_min, min(trust_region.lambda_max, damping * mult))
-                            rejection_factor = 2.0
-                        elif quality > 0.75:
-                            damping = max(trust_region.lambda_min, damping / trust_region.lambda_factor)
+                    damping = damping_next
                     residual = jacobian = original_jacobian = None
                     normal_operator = schur_linear = system = diagonal = gradient = linear_gradient = None
                     joint_diagonal = None
@@ -558,13 +561,9 @@ def solve(
                         summary.termination_reason = "parameter_tolerance"
                     break
             summary.rejected_steps += 1
-            if trust_region is None or damping >= trust_region.lambda_max:
+            if stop:  # rejected at lambda_max, or without a trust region
                 break
-            if trust_region.strategy == "adaptive":
-                damping = min(trust_region.lambda_max, damping * rejection_factor)
-                rejection_factor = min(rejection_factor * 2, trust_region.lambda_max / trust_region.lambda_min)
-            else:
-                damping = min(trust_region.lambda_max, damping * trust_region.lambda_factor)
+            damping = damping_next
         if not accepted:
             if summary.termination_reason != "time_limit":
                 summary.termination_reason = "no_acceptable_step"

@@ -126,6 +126,33 @@ def test_schur_pcg_lm_medium(device):
     assert abs(m.final_cost - ref.final_cost) <= 1e-6 * ref.final_cost
 
 
+def test_schur_pcg_lm_large(device):
+    """c3: Schur-PCG LM at 5,012,836 observations, three iterations, float64 and mixed
+    precision from the same start: the same iteration and acceptance counts, final
+    costs within 1e-6 of each other, and each final cost equal (1e-9) to the objective
+    the oracle recomputes on the CPU at the returned cameras and points."""
+    prob = workload("ba_large")
+    steps = 3
+    print(f"schur-pcg lm 5M obs, {steps} iterations", flush=True)
+    term = TerminationConfig(max_iterations=steps)
+    o = BAProblem(prob.cam_idx, prob.pt_idx, prob.observations, prob.n_cameras, prob.n_points, huber_delta=1e-2)
+    out = {}
+    for mv in (None, torch.float32):
+        opts = dict(stop_on_stagnation=True, max_refinements=0) if mv is not None else {}
+        vals, s = ba_problem(prob, device).solve(
+            return_summary=True, linear_solver="schur_cg", eliminate="points", termination=term,
+            conjugate_gradient=ConjugateGradientConfig(matvec_dtype=mv, **opts))
+        cost = o.cost((vals["cams"].cpu().numpy(), vals["points"].cpu().numpy()))
+        print(f"  matvec {mv}: {s.termination_reason}, pcg {s.linear_iterations_history}, "
+              f"final {s.final_cost:.12e}, oracle objective {cost:.12e}", flush=True)
+        assert s.iterations == steps and s.final_cost < s.initial_cost
+        assert math.isclose(cost, s.final_cost, rel_tol=1e-9)
+        out[mv] = s
+    s64, s32 = out[None], out[torch.float32]
+    assert (s32.accepted_steps, s32.rejected_steps) == (s64.accepted_steps, s64.rejected_steps)
+    assert abs(s32.final_cost - s64.final_cost) <= 1e-6 * s64.final_cost
+
+
 def dense_reduced_matrix(J, diag, lam):
     """blockdiag(U + lam I) - sum over points p and observation pairs (i, j) of p of
     W_i (V_p + lam I)^-1 W_j^T at camera block (c_i, c_j), W_i = J_c,i^T J_p,i --

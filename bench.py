@@ -235,7 +235,7 @@ def large_normal_matvec(device, reps):
     nbytes = normal_matvec_bytes(prob.n_obs, 2, (6, 3), n, 4)
     gbs = nbytes / t / 1e9
     wave = op.plan.plan.groups[0]
-    traffic = profiled_traffic(MATVEC_TRAFFIC, [("void normal_wave_kernel<float, 2, 6, 3, 1, 2, 512, true, true, true, false>", None),
+    traffic = profiled_traffic(MATVEC_TRAFFIC, [("void normal_wave_kernel<float, 2, 6, 3, 1, 4, 512, true, true, true, true>", None),
                                                 ("void finish_jobs_kernel<float>", None)])
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -43,7 +43,7 @@ typedef struct ihipStream_t* tlh_stream_t; /* == hipStream_t */
  * and every slot's index (and rank) array must cover ntiles * tile_cap positions,
  * the padding indices naming valid variables. */
 #define TLH_WAVE_THREADS 512
-#define TLH_WAVE_TILE 1024
+#define TLH_WAVE_TILE 2048
 
 #define TLH_REDUCE_SPARSE 0 /* one partial per (tile, variable) segment */
 #define TLH_REDUCE_DENSE 1  /* per-workgroup dense LDS accumulator of the whole type */

@@ -19,7 +19,7 @@ TLH_F64 = 1
 TLH_MAX_SLOTS = 3
 TLH_TILE_THREADS = 256
 TLH_WAVE_THREADS = 512
-TLH_WAVE_TILE = 1024
+TLH_WAVE_TILE = 2048
 PCG_SMALL_MAX_N = 16384
 TLH_REDUCE_SPARSE = 0
 TLH_REDUCE_DENSE = 1

@@ -48,7 +48,7 @@ def items_for(esize: int) -> int:
 
 
 WAVE_THREADS = 512  # TLH_WAVE_THREADS
-WAVE_TILE = 1024  # TLH_WAVE_TILE (float32; half for float64)
+WAVE_TILE = 2048  # TLH_WAVE_TILE (float32; half for float64)
 
 
 def wave_items(esize: int) -> int:

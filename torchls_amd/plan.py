@@ -402,7 +402,8 @@ def build_structure(
                 slots[-1].piece_end = ends
         has_dense = any(sl.mode == REDUCE_DENSE for sl in slots)
         if has_dense and ntiles:
-            # a wave-kernel workgroup (1024 threads) fills its compute unit's wave slots
+            # a wave-kernel workgroup (WAVE_THREADS = 512 threads of up to 256 VGPRs, two
+            # wavefronts per SIMD) fills its compute unit's register file: one per CU
             nwg = min(ntiles, cu_count * (1 if wave else wgs_per_cu))
             per = -(-ntiles // nwg)
             nwg = -(-ntiles // per)

@@ -79,6 +79,7 @@ class DeviceStructure:
         self.plan = plan
         self.device = device
         self.keep = []
+        self._operators = {}  # (dtype, packed pointers, partial pointers) -> Operator
         # identity of each group's sorted order: plans with equal keys share packed blocks
         self.order_keys = [order_key(g.order) for g in plan.groups]
         put = lambda a: self._keep(_to_device(a, device))  # noqa: E731
@@ -142,7 +143,15 @@ class DeviceStructure:
         return t
 
     def operator(self, dtype: torch.dtype, packed: list, partials: list) -> "_lib.Operator":
-        """Operator descriptor bound to packed blocks and partial workspaces."""
+        """Operator descriptor bound to packed blocks and partial workspaces.
+
+        Cached per (dtype, block and partial addresses): the descriptor holds only
+        those addresses and this structure's arrays, which are fixed after
+        construction, and every library entry point reads it as const."""
+        key = (dtype, tuple(_ptr(p) for p in packed), tuple(_ptr(p) for p in partials))
+        op = self._operators.get(key)
+        if op is not None:
+            return op
         groups = (_lib.Group * max(1, len(self.plan.groups)))()
         ctypes.memmove(groups, self.groups, ctypes.sizeof(self.groups))
         for gi, p in enumerate(packed):
@@ -160,6 +169,9 @@ class DeviceStructure:
         op.groups = ctypes.cast(groups, ctypes.POINTER(_lib.Group))
         op.types = ctypes.cast(types, ctypes.POINTER(_lib.Type))
         op._keep = (groups, types)  # descriptor arrays must outlive the Operator
+        if len(self._operators) >= 64:
+            self._operators.clear()
+        self._operators[key] = op
         return op
 
 

@@ -83,3 +83,27 @@ def test_major_order_and_edges_match_brute_force():
             assert np.all(ic[obs] == v) and np.all(ip[obs] == edge_point[e])
             assert np.array_equal(obs, np.flatnonzero((ic == v) & (ip == edge_point[e])))
     assert major_eligible(200, 500_000) and not major_eligible(50_000, 500_000) and not major_eligible(0, 10)
+
+
+def test_operator_descriptor_cache():
+    """DeviceStructure.operator returns one descriptor per (dtype, block and workspace
+    addresses): the same object for the same addresses, a new one carrying the new
+    addresses otherwise (the descriptor holds addresses only; no GPU call is made)."""
+    from torchls_amd.hip_normal import DeviceStructure
+
+    rng = np.random.default_rng(5)
+    layout, groups = SHAPES[2]
+    J = random_jacobian(rng, layout, groups)
+    plan = build_structure(J.layout, [(g.rows, g.types, [_t(i) for i in g.indices]) for g in J.groups], esize=4)
+    dev = DeviceStructure(plan, torch.device("cpu"))
+    a, b = torch.zeros(64), torch.zeros(64)
+    parts = [torch.zeros(8) for _ in plan.types]
+    op1 = dev.operator(torch.float32, [a], parts)
+    assert dev.operator(torch.float32, [a], parts) is op1
+    op2 = dev.operator(torch.float32, [b], parts)
+    assert op2 is not op1
+    assert op1.groups[0].packed == a.data_ptr() and op2.groups[0].packed == b.data_ptr()
+    assert op2.types[0].partials == parts[0].data_ptr()
+    op3 = dev.operator(torch.float64, [a], parts)
+    assert op3 is not op1 and op3.dtype != op1.dtype
+    assert op1.ngroups == len(plan.groups) and op1.size == plan.size

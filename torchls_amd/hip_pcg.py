@@ -126,18 +126,25 @@ def hip_pcg(matvec, rhs, *, preconditioner, config, use_graph: bool = True, firs
     """
     if rhs.device.type != "cuda" or rhs.dtype not in (torch.float32, torch.float64):
         raise ValueError("HIP PCG requires GPU float32 or float64 tensors")
+    fused = isinstance(preconditioner, HipBlockJacobi)
+    op_desc = native_operator(matvec) if native and (fused or preconditioner is None) else None
+    if op_desc is not None or not use_graph:
+        # nothing is captured: the solve runs on the caller's stream, with no
+        # cross-stream hand-off (two event waits, ~25 us of host time per solve)
+        return _run(matvec, rhs, preconditioner=preconditioner, config=config, use_graph=False,
+                    first_batch=first_batch, op_desc=op_desc)
     caller = torch.cuda.current_stream(rhs.device)
     stream = _work_stream(rhs.device)
     stream.wait_stream(caller)
     with torch.cuda.stream(stream):
         result = _run(matvec, rhs, preconditioner=preconditioner, config=config, use_graph=use_graph,
-                      first_batch=first_batch, native=native)
+                      first_batch=first_batch, op_desc=None)
     caller.wait_stream(stream)
     result.solution.record_stream(caller)
     return result
 
 
-def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None, native=True):
+def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None, op_desc=None):
     from .solvers import PCGResult
 
     lib = _lib.load()
@@ -222,7 +229,6 @@ def _run(matvec, rhs, *, preconditioner, config, use_graph, first_batch=None, na
     # library's kernels (hip_graph_safe) need no warm-up: nothing is initialised
     # lazily, and the capture records without executing, so the state set by
     # the reset above is still intact.
-    op_desc = native_operator(matvec) if native and (fused or apply is None) else None
     if op_desc is not None:
         use_graph = False
         ap_work = torch.empty_like(rhs)

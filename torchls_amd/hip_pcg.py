@@ -18,7 +18,9 @@ Schur system's ``matvec``, a normal linearization's ``normal_matvec``, bare or
 as a ``functools.partial`` binding ``damping``) and the preconditioner is
 fused or absent, a batch is issued natively by ``tlh_pcg_iterations`` (the
 same launches as the captured iteration, in the same order, one host call per
-batch) instead of graph replays.
+batch) instead of graph replays.  Native and eager solves run on the caller's
+stream; a captured solve runs on a per-device work stream (capture cannot use
+the default stream) joined to the caller's stream by events.
 """
 
 from __future__ import annotations
